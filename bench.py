@@ -1,0 +1,226 @@
+#!/usr/bin/env python3
+"""Benchmark: FP64 particle-turns/s of the modulated Henon-map tracker on one MI355X.
+
+Workload (one step): the modulated dynamic-aperture scan -- a 1000 x 1000 grid of
+initial conditions over (0, 0.6]^2 (px = py = 0, 1e6 particles) tracked for 1e5
+turns of the 4D map with mu = 0.5 and the seeded 7-line tune modulation at
+epsilon = 32 (tunes 0.168 / 0.201), loss barrier r^2 > 10. Initial conditions and
+the modulation table are resident in HBM before timing; a step restores the
+initial conditions (device-to-device) and tracks all turns.
+
+Prints ONE JSON line. `value` is nominal particle-turns/s (particles x turns per
+second). The roofline entry uses the live work (sum of survival turns) x 40
+algorithmic FP64 flops per particle-turn, over the k_track kernel time measured
+with HIP events on the launch stream.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (FMA = 2 flops)
+FLOPS_PER_PT = {0.0: 26, None: 40}  # mu == 0 / mu != 0, per live particle-turn
+METRIC = "FP64 particle-turns/s (track, tangent/MEGNO, adjoint) + % of FP64 VALU peak"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", choices=["fast", "exact"], default="fast")
+    ap.add_argument("--n-side", type=int, default=1000)
+    ap.add_argument("--turns", type=int, default=100_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-stride", type=int, default=97)
+    ap.add_argument("--compact-every", type=int, default=0)
+    return ap.parse_args()
+
+
+def flops_per_pt(mu):
+    return FLOPS_PER_PT[0.0] if mu == 0.0 else FLOPS_PER_PT[None]
+
+
+class Workload:
+    def __init__(self, n_side, turns):
+        from henon_map_cpp_amd import _capi as C
+        from henon_map_cpp_amd import workloads as W
+
+        self.C = C
+        c = W.MODULATED_DA
+        self.mu, self.barrier = c["mu"], c["barrier"]
+        self.turns = turns
+        self.x0 = W.da_grid(n_side)
+        self.n = self.x0[0].size
+        coef, freq = W.seeded_line_coefficients(), W.line_frequencies()
+        self.wx = C.lines_modulation(c["tune_x"], c["epsilon"], coef, freq, 0, turns)
+        self.wy = C.lines_modulation(c["tune_y"], c["epsilon"], coef, freq, 0, turns)
+        self.tab = C.build_table(self.wx, self.wy)
+        self.tdev = C.DeviceBuffer.from_array(self.tab)
+        self.ic = [C.DeviceBuffer.from_array(a) for a in self.x0]
+        self.parts = C.DeviceParticles(*self.x0)
+        self.zero_steps = C.DeviceBuffer.from_array(np.zeros(self.n, dtype=np.uint32))
+
+    def reset(self):
+        C = self.C
+        for src, dst in zip(self.ic, self.parts.bufs):
+            C.check(C.lib().hm_memcpy(dst.ptr, src.ptr, self.n * 8, 2, None))
+        C.check(C.lib().hm_memcpy(self.parts.steps.ptr, self.zero_steps.ptr, self.n * 4, 2, None))
+
+    def step(self, mode, cfg):
+        self.reset()
+        self.C.track(self.parts, self.tdev, self.turns, self.turns, self.mu, self.barrier, False, mode, cfg)
+
+    def live_pt(self):
+        return int(self.parts.steps.download(np.uint32, self.n).astype(np.int64).sum())
+
+
+def time_mode(w, mode, steps, warmup, cfg):
+    C = w.C
+    for _ in range(warmup):
+        w.step(mode, cfg)
+    C.check(C.lib().hm_stream_synchronize(None))
+    C.profile_reset()
+    C.profile_enable(True)
+    _sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        w.step(mode, cfg)
+    _sync()
+    C.check(C.lib().hm_stream_synchronize(None))
+    dt = time.perf_counter() - t0
+    C.profile_enable(False)
+    kms, launches = C.profile_read("k_track")
+    live = w.live_pt()
+    return dt, kms, launches, live
+
+
+def _sync():
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def cpu_baseline(stride, turns):
+    """The oracle (C++ restatement of the reference CPU path, compiled like the
+    reference's CPU build) on every `stride`-th particle of the same grid, with
+    the reference's threading model (one std::thread per core, strided)."""
+    import oracle as O
+    from henon_map_cpp_amd import workloads as W
+
+    O.build()
+    c = W.MODULATED_DA
+    x, px, y, py = (a[::stride].copy() for a in W.da_grid(1000))
+    coef, freq = W.seeded_line_coefficients(), W.line_frequencies()
+    t = O.Tracker(turns, c["tune_x"], c["tune_y"], "custom", epsilon=c["epsilon"], coef=coef, freq=freq)
+    p = O.Particles(x, px, y, py)
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16)
+    t0 = time.perf_counter()
+    t.track(p, turns, c["mu"], barrier=c["barrier"], n_threads=cores)
+    dt = time.perf_counter() - t0
+    live = int(p.steps.astype(np.int64).sum())
+    return {
+        "value": p.n * turns / dt,
+        "unit": "particle-turns/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"every {stride}th particle of the 1000x1000 grid ({p.n} particles) x {turns} turns, "
+                  f"mu=0.5, eps=32; live particle-turns/s {live / dt:.4g}; {dt:.1f} s",
+        "live_value": live / dt,
+        "seconds": dt,
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              f"(the tracker shards by particle; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    from henon_map_cpp_amd import _capi as C
+
+    if C.device_count() < 1:
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    w = Workload(a.n_side, a.turns)
+    cfg = C.LaunchCfg(0, a.compact_every, 0, 0) if a.compact_every else None
+    fpt = flops_per_pt(w.mu)
+    results = {}
+    for mode_name in ([a.mode] + [m for m in ("fast", "exact") if m != a.mode]):
+        mode = C.MODE_FAST if mode_name == "fast" else C.MODE_EXACT
+        dt, kms, launches, live = time_mode(w, mode, a.steps, a.warmup, cfg)
+        per_step_live = live
+        kernel_s_per_step = kms / 1e3 / a.steps
+        achieved = per_step_live * fpt / kernel_s_per_step / 1e12
+        results[mode_name] = dict(
+            ms_per_step=dt / a.steps * 1e3,
+            value=w.n * w.turns * a.steps / dt,
+            live_value=per_step_live * a.steps / dt,
+            kernel_ms_per_step=kernel_s_per_step * 1e3,
+            launches_per_step=launches / a.steps,
+            achieved_tflops=achieved,
+            frac=achieved / FP64_PEAK_TFLOPS,
+            live_pt=per_step_live,
+        )
+        print(f"[bench] mode={mode_name} {results[mode_name]}", file=sys.stderr, flush=True)
+    head = results[a.mode]
+    line = {
+        "metric": METRIC,
+        "value": head["value"],
+        "unit": "particle-turns/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"modulated DA scan {a.n_side}x{a.n_side} grid over (0,0.6]^2, {a.turns} turns, mu=0.5, "
+                        f"7-line tune modulation eps=32 (seeded a_k), tunes 0.168/0.201, barrier r^2>10",
+            "mode": a.mode,
+            "particles": w.n,
+            "turns": w.turns,
+            "live_particle_turns_per_step": head["live_pt"],
+            "live_value": head["live_value"],
+            "flops_per_live_particle_turn": fpt,
+            "other_mode": {k: {kk: v[kk] for kk in ("value", "live_value", "ms_per_step", "frac")}
+                           for k, v in results.items() if k != a.mode},
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": head["achieved_tflops"],
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": head["frac"],
+            "traffic": None,
+            "kernel": "k_track",
+            "kernel_ms_per_step": head["kernel_ms_per_step"],
+            "launches_per_step": head["launches_per_step"],
+        },
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu_baseline:
+        try:
+            line["cpu_baseline"] = cpu_baseline(a.cpu_sample_stride, a.turns)
+        except Exception as e:  # reported, never silently replaced
+            line["cpu_baseline"] = {"error": repr(e)}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,241 @@
+"""ctypes binding of the C ABI in include/henon_hip.h (libhenon_hip.so).
+
+This is the binding a Python caller of the C ABI uses directly; the pybind11
+module ``henon_map_engine`` is the drop-in for the reference's module and sits
+on the same C ABI. Loading fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libhenon_hip.so")
+
+HM_OK, HM_ERR_ARG, HM_ERR_RANGE, HM_ERR_HIP, HM_ERR_UNSUPPORTED, HM_ERR_ALLOC = range(6)
+MODE_EXACT, MODE_FAST = 0, 1
+OBS_RADIUS, OBS_ACTION, OBS_ACTION_X, OBS_ACTION_Y, OBS_ANGLE_X, OBS_ANGLE_Y = range(6)
+
+D = ctypes.c_double
+P = ctypes.c_void_p
+SZ = ctypes.c_size_t
+I = ctypes.c_int
+U32 = ctypes.c_uint32
+U64 = ctypes.c_uint64
+
+
+class LaunchCfg(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_int32), ("compact_every", ctypes.c_int32),
+                ("unroll", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+# name -> (restype, argtypes); mirrors include/henon_hip.h
+SIGNATURES = {
+    "hm_last_error": (ctypes.c_char_p, []),
+    "hm_version": (ctypes.c_char_p, []),
+    "hm_device_count": (I, [ctypes.POINTER(I)]),
+    "hm_malloc": (I, [ctypes.POINTER(P), SZ]),
+    "hm_free": (I, [P]),
+    "hm_memcpy": (I, [P, P, SZ, I, P]),
+    "hm_memset": (I, [P, I, SZ, P]),
+    "hm_stream_synchronize": (I, [P]),
+    "hm_take_status": (I, [ctypes.POINTER(U32), P]),
+    "hm_basic_modulation": (I, [D, D, D, I, I, P]),
+    "hm_sps_modulation": (I, [D, D, I, I, P]),
+    "hm_lines_modulation": (I, [D, D, P, P, I, I, I, P]),
+    "hm_sps_lines": (I, [P, P]),
+    "hm_gaussian_modulation": (I, [D, D, I, I, U64, P]),
+    "hm_uniform_modulation": (I, [D, D, I, I, U64, P]),
+    "hm_build_table": (I, [P, P, SZ, P]),
+    "hm_birkhoff_weights": (I, [U32, P]),
+    "hm_track": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, D, D, I, I, ctypes.POINTER(LaunchCfg), P]),
+    "hm_observable": (I, [I, P, P, P, P, SZ, P, P]),
+    "hm_tangent_matrix": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, I, P, P]),
+    "hm_profile_enable": (I, [I]),
+    "hm_profile_reset": (I, []),
+    "hm_profile_read": (I, [ctypes.c_char_p, ctypes.POINTER(D), ctypes.POINTER(U64)]),
+}
+
+_lib = None
+
+
+def _preload_runtime():
+    # torch ships its own libamdhip64 (same SONAME); importing it first makes
+    # this library bind to that runtime, so one HIP runtime serves the process.
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python -m henon_map_cpp_amd.build`")
+        _preload_runtime()
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class HMError(RuntimeError):
+    pass
+
+
+def check(rc: int) -> None:
+    if rc == HM_OK:
+        return
+    msg = lib().hm_last_error().decode(errors="replace")
+    if rc == HM_ERR_ARG:
+        raise ValueError(msg)
+    if rc == HM_ERR_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    if rc == HM_ERR_ALLOC:
+        raise MemoryError(msg)
+    raise RuntimeError(msg)
+
+
+def device_count() -> int:
+    c = I(0)
+    check(lib().hm_device_count(ctypes.byref(c)))
+    return c.value
+
+
+def ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class DeviceBuffer:
+    """A device allocation made through hm_malloc (numpy in, numpy out)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = P()
+        check(lib().hm_malloc(ctypes.byref(p), max(self.nbytes, 1)))
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, a: np.ndarray, stream=None):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a, stream)
+        return b
+
+    def upload(self, a: np.ndarray, stream=None):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib().hm_memcpy(self.ptr, ptr(a), a.nbytes, 0, stream))
+        check(lib().hm_stream_synchronize(stream))
+
+    def download(self, dtype, count, stream=None):
+        out = np.empty(count, dtype=dtype)
+        check(lib().hm_memcpy(ptr(out), self.ptr, out.nbytes, 1, stream))
+        check(lib().hm_stream_synchronize(stream))
+        return out
+
+    def free(self):
+        if self.ptr:
+            check(lib().hm_free(self.ptr))
+            self.ptr = P()
+
+    def __del__(self):
+        try:
+            if self.ptr and _lib is not None:
+                _lib.hm_free(self.ptr)
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- host helpers
+def sps_modulation(tune, epsilon, start, end):
+    out = np.empty(end - start)
+    check(lib().hm_sps_modulation(tune, epsilon, start, end, ptr(out)))
+    return out
+
+
+def basic_modulation(tune, omega, epsilon, start, end):
+    out = np.empty(end - start)
+    check(lib().hm_basic_modulation(tune, omega, epsilon, start, end, ptr(out)))
+    return out
+
+
+def lines_modulation(tune, epsilon, coef, freq, start, end):
+    coef = np.ascontiguousarray(coef, dtype=np.float64)
+    freq = np.ascontiguousarray(freq, dtype=np.float64)
+    out = np.empty(end - start)
+    check(lib().hm_lines_modulation(tune, epsilon, ptr(coef), ptr(freq), coef.size, start, end, ptr(out)))
+    return out
+
+
+def sps_lines():
+    c, f = np.empty(7), np.empty(7)
+    check(lib().hm_sps_lines(ptr(c), ptr(f)))
+    return c, f
+
+
+def build_table(omega_x, omega_y):
+    wx = np.ascontiguousarray(omega_x, dtype=np.float64)
+    wy = np.ascontiguousarray(omega_y, dtype=np.float64)
+    out = np.empty((wx.size, 4))
+    check(lib().hm_build_table(ptr(wx), ptr(wy), wx.size, ptr(out)))
+    return out
+
+
+def birkhoff_weights(n):
+    out = np.empty(n)
+    check(lib().hm_birkhoff_weights(n, ptr(out)))
+    return out
+
+
+# ---------------------------------------------------------------- device path
+class DeviceParticles:
+    """Device SoA (x, px, y, py, steps) held through the C ABI."""
+
+    def __init__(self, x, px, y, py, steps=None):
+        n = len(x)
+        self.n = n
+        self.bufs = [DeviceBuffer.from_array(np.asarray(a, dtype=np.float64)) for a in (x, px, y, py)]
+        st = np.zeros(n, dtype=np.uint32) if steps is None else np.asarray(steps, dtype=np.uint32)
+        self.steps = DeviceBuffer.from_array(st)
+
+    def ptrs(self):
+        return [b.ptr for b in self.bufs] + [self.steps.ptr]
+
+    def get(self):
+        x, px, y, py = (b.download(np.float64, self.n) for b in self.bufs)
+        return x, px, y, py, self.steps.download(np.uint32, self.n)
+
+
+def track(parts: DeviceParticles, table_dev: DeviceBuffer, table_len: int, n_turns: int, mu: float,
+          barrier: float, inverse=False, mode=MODE_EXACT, cfg: LaunchCfg | None = None, stream=None,
+          kick_module=float("nan")):
+    c = ctypes.byref(cfg) if cfg is not None else None
+    check(lib().hm_track(*parts.ptrs(), parts.n, table_dev.ptr, table_len, n_turns, mu, barrier, kick_module,
+                         int(inverse), mode, c, stream))
+
+
+def take_status(stream=None) -> int:
+    f = U32(0)
+    check(lib().hm_take_status(ctypes.byref(f), stream))
+    return f.value
+
+
+def profile_enable(on=True):
+    check(lib().hm_profile_enable(int(on)))
+
+
+def profile_reset():
+    check(lib().hm_profile_reset())
+
+
+def profile_read(name: str):
+    ms, n = D(0.0), U64(0)
+    check(lib().hm_profile_read(name.encode() if name else None, ctypes.byref(ms), ctypes.byref(n)))
+    return ms.value, n.value

@@ -1,0 +1,399 @@
+"""Oracle for the 4D modulated Henon-map path -- TEST INFRASTRUCTURE ONLY.
+
+A CPU restatement of the reference's CPU path (``oracle/oracle.cpp``), loaded
+through ctypes. Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg use it, always as the checker or the timed CPU baseline,
+never as part of the product.
+
+Functions with no reference counterpart (the fused renormalised indicators and
+the adjoint checks) live in :mod:`oracle.indicators`; their parity is pinned by
+closed forms, finite differences and tangent-map products, as stated there.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_REF_PATH = os.path.join(_HERE, "_ref", "libref_modulation.so")
+
+_lib = None
+
+D = ctypes.c_double
+P = ctypes.c_void_p
+SZ = ctypes.c_size_t
+U = ctypes.c_uint
+I = ctypes.c_int
+
+
+def build(force: bool = False) -> None:
+    """Compile liboracle.so (and oracle/_ref when the reference sources exist)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.check_call(["make", "-s", "-B", "liboracle.so"], cwd=_HERE)
+    if os.path.isdir("/root/reference/src") and (force or not os.path.exists(_REF_PATH)):
+        subprocess.check_call(["make", "-s", "-B", "ref"], cwd=_HERE)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        sigs = {
+            "or_basic_modulation": (I, [D, D, D, I, I, P]),
+            "or_sps_modulation": (I, [D, D, I, I, P]),
+            "or_lines_modulation": (I, [D, D, P, P, I, I, I, P]),
+            "or_sps_lines": (None, [P, P]),
+            "or_sin_cos": (None, [P, SZ, P, P]),
+            "or_track": (I, [P, P, P, P, P, SZ, P, P, P, P, U, D, D, D, I, I]),
+            "or_tangent_matrix": (None, [D] * 9 + [P]),
+            "or_inverse_tangent_matrix": (None, [D] * 9 + [P]),
+            "or_multiply_matrices": (None, [P, P, P]),
+            "or_get_tangent_matrix": (None, [P, P, P, P, P, SZ, P, P, P, P, U, D, I, P]),
+            "or_structured_multiply": (None, [P, P, P, P, P, SZ, P, P, P, P, U, D]),
+            "or_megno": (I, [P, P, P, P, P, SZ, P, P, P, P, U, D, D, I, P, SZ, I, I, P]),
+            "or_birkhoff_weights": (None, [U, P]),
+            "or_birkhoff_tune": (D, [P, P, SZ]),
+            "or_birkhoff_tune_vec": (I, [P, P, P, P, SZ, P, P, SZ, P]),
+            "or_fft_magnitude": (None, [P, P, SZ, I, P]),
+            "or_interp_A": (D, [D, D, D]),
+            "or_interpolation": (D, [P, SZ]),
+            "or_fft_tune": (D, [P, P, SZ]),
+            "or_fft_tune_vec": (I, [P, P, P, P, SZ, P, P, SZ, P]),
+            "or_get_tunes": (None, [P, P, SZ, P]),
+            "or_birkhoff_tunes": (I, [P, P, P, P, P, SZ, P, P, P, P, U, D, D, I, P, P, SZ, I, P]),
+            "or_all_tunes": (I, [P, P, P, P, P, SZ, P, P, P, P, U, D, D, I, P, P, SZ, I, P, P]),
+            "or_lyap_birk_add": (None, [P, P, P, P, SZ, SZ, SZ]),
+            "or_megno_construct_add": (None, [P, P, P, P, P, SZ, SZ]),
+            "or_megno_birkhoff_add": (None, [P, P, P, P, P, SZ, P, P, SZ, SZ]),
+            "or_tune_birkhoff_init": (None, [P, P, P, P, P, P, SZ]),
+            "or_tune_birkhoff_add": (None, [P] * 10 + [P, P, P, SZ, SZ, SZ]),
+            "or_matvec": (None, [P, P, SZ]),
+            "or_normalize": (None, [P, SZ]),
+            "or_nan_mean": (D, [P, SZ]),
+            "or_nan_std": (D, [P, SZ]),
+        }
+        for name, (res, args) in sigs.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """oracle/_ref: the reference's own modulation.cpp, or None if not built."""
+    if not os.path.exists(_REF_PATH):
+        return None
+    L = ctypes.CDLL(_REF_PATH)
+    L.ref_sps_modulation.argtypes = [D, D, I, I, P]
+    L.ref_sps_modulation.restype = I
+    L.ref_basic_modulation.argtypes = [D, D, D, I, I, P]
+    L.ref_basic_modulation.restype = I
+    L.ref_pick_a_modulation.argtypes = [U, D, D, ctypes.c_char_p, D, D, U, P, P]
+    L.ref_pick_a_modulation.restype = I
+    return L
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+# ---------------------------------------------------------------- modulation
+def sps_modulation(tune, epsilon, start, end):
+    out = np.empty(end - start)
+    if lib().or_sps_modulation(tune, epsilon, start, end, _p(out)):
+        raise ValueError("start must be < end")
+    return out
+
+
+def basic_modulation(tune, omega, epsilon, start, end):
+    out = np.empty(end - start)
+    if lib().or_basic_modulation(tune, omega, epsilon, start, end, _p(out)):
+        raise ValueError("start must be < end")
+    return out
+
+
+def lines_modulation(tune, epsilon, coef, freq, start, end):
+    coef, freq = _f64(coef), _f64(freq)
+    out = np.empty(end - start)
+    if lib().or_lines_modulation(tune, epsilon, _p(coef), _p(freq), len(coef), start, end, _p(out)):
+        raise ValueError("start must be < end")
+    return out
+
+
+def sps_lines():
+    c, f = np.empty(7), np.empty(7)
+    lib().or_sps_lines(_p(c), _p(f))
+    return c, f
+
+
+def sin_cos(w):
+    w = _f64(w)
+    s, c = np.empty_like(w), np.empty_like(w)
+    lib().or_sin_cos(_p(w), w.size, _p(s), _p(c))
+    return s, c
+
+
+def pick_a_modulation(n_turns, omega_x, omega_y, kind, omega_0=np.nan, epsilon=0.0, offset=0, coef=None, freq=None):
+    """modulation.cpp:98-134 for the deterministic kinds, plus the 'custom' lines kind."""
+    if kind == "sps":
+        return (sps_modulation(omega_x, epsilon, offset, offset + n_turns),
+                sps_modulation(omega_y, epsilon, offset, offset + n_turns))
+    if kind == "basic":
+        return (basic_modulation(omega_x, omega_0, epsilon, offset, offset + n_turns),
+                basic_modulation(omega_y, omega_0, epsilon, offset, offset + n_turns))
+    if kind == "custom":
+        return (lines_modulation(omega_x, epsilon, coef, freq, offset, offset + n_turns),
+                lines_modulation(omega_y, epsilon, coef, freq, offset, offset + n_turns))
+    if kind == "none":
+        # the reference fills empty vectors here; the well-defined reading is a
+        # constant 2*pi*tune table
+        return (np.full(n_turns, 2 * np.pi * omega_x), np.full(n_turns, 2 * np.pi * omega_y))
+    raise RuntimeError("Unknown modulation kind")
+
+
+class Tables:
+    """The four sin/cos tables of henon_tracker (henon.cu:2657-2694)."""
+
+    def __init__(self, wx, wy):
+        self.omega_x = _f64(wx)
+        self.omega_y = _f64(wy)
+        self.sx, self.cx = sin_cos(self.omega_x)
+        self.sy, self.cy = sin_cos(self.omega_y)
+        self.allowed_steps = self.omega_x.size
+
+    def ptrs(self):
+        return _p(self.sx), _p(self.cx), _p(self.sy), _p(self.cy)
+
+    def interleaved(self):
+        """[T][4] = (sin wx, cos wx, sin wy, cos wy): the product's table layout."""
+        return np.ascontiguousarray(np.stack([self.sx, self.cx, self.sy, self.cy], axis=1))
+
+
+# ---------------------------------------------------------------- particles + tracker
+class Particles:
+    """particles_4d state (henon.cu:699-735), host arrays."""
+
+    def __init__(self, x, px, y, py):
+        self.x, self.px, self.y, self.py = (_f64(a).copy() for a in (x, px, y, py))
+        self.x0, self.px0, self.y0, self.py0 = (a.copy() for a in (self.x, self.px, self.y, self.py))
+        self.steps = np.zeros(self.x.size, dtype=np.uint32)
+        self.valid = np.ones(self.x.size, dtype=np.uint8)
+        self.global_steps = 0
+
+    @property
+    def n(self):
+        return self.x.size
+
+    def reset(self):
+        # henon.cu:727-735: global_steps is not reset
+        self.x, self.px, self.y, self.py = (a.copy() for a in (self.x0, self.px0, self.y0, self.py0))
+        self.steps[:] = 0
+        self.valid[:] = 1
+
+    def state(self):
+        return _p(self.x), _p(self.px), _p(self.y), _p(self.py), _p(self.steps)
+
+
+class Tracker:
+    """henon_tracker (CPU): tables + track/megno/birkhoff_tunes/all_tunes."""
+
+    def __init__(self, n_turns, omega_x, omega_y, kind="sps", omega_0=np.nan, epsilon=0.0, offset=0,
+                 coef=None, freq=None):
+        wx, wy = pick_a_modulation(n_turns, omega_x, omega_y, kind, omega_0, epsilon, offset, coef, freq)
+        self.t = Tables(wx, wy)
+
+    @classmethod
+    def from_tables(cls, tables: "Tables"):
+        t = cls.__new__(cls)
+        t.t = tables
+        return t
+
+    def _check(self, p, n_turns, inverse):
+        # henon.cu:2706-2715
+        if inverse:
+            if n_turns > p.global_steps:
+                raise RuntimeError("The number of turns is too large.")
+        elif n_turns + p.global_steps > self.t.allowed_steps:
+            raise RuntimeError("The number of turns is too large.")
+
+    def _advance(self, p, n_turns, inverse):
+        p.global_steps += -n_turns if inverse else n_turns
+
+    def track(self, p, n_turns, mu, barrier=100.0, kick_module=np.nan, inverse=False, n_threads=-1):
+        self._check(p, n_turns, inverse)
+        rc = lib().or_track(*p.state(), p.n, *self.t.ptrs(), n_turns, mu, barrier, kick_module, int(inverse),
+                            n_threads)
+        if rc:
+            raise NotImplementedError("random kick is not restated in the oracle")
+        self._advance(p, n_turns, inverse)
+
+    def megno(self, p, n_turns, mu, barrier=100.0, inverse=False, turn_samples=(), n_threads=-1,
+              carry_off_diagonal=True):
+        self._check(p, n_turns, inverse)
+        ts = _u32(list(turn_samples) if len(turn_samples) else [n_turns])
+        out = np.empty((p.n, ts.size))
+        rc = lib().or_megno(*p.state(), p.n, *self.t.ptrs(), n_turns, mu, barrier, int(inverse), _p(ts), ts.size,
+                            n_threads, int(carry_off_diagonal), _p(out))
+        if rc:
+            raise ValueError("turn sample beyond n_turns")
+        self._advance(p, n_turns, inverse)
+        return out
+
+    def birkhoff_tunes(self, p, n_turns, mu, barrier=100.0, inverse=False, from_idx=(), to_idx=(), n_threads=-1):
+        self._check(p, n_turns, inverse)
+        f, t = _u32(from_idx), _u32(to_idx)
+        W = f.size + 1
+        out = np.empty((p.n, 2 * (W + 1)))
+        rc = lib().or_birkhoff_tunes(*p.state(), p.n, *self.t.ptrs(), n_turns, mu, barrier, int(inverse), _p(f),
+                                     _p(t), f.size, n_threads, _p(out))
+        if rc:
+            raise RuntimeError("compute_tune: from and to must be valid indices")
+        self._advance(p, n_turns, inverse)
+        return out
+
+    def all_tunes(self, p, n_turns, mu, barrier=100.0, inverse=False, from_idx=(), to_idx=(), n_threads=-1):
+        self._check(p, n_turns, inverse)
+        f, t = _u32(from_idx), _u32(to_idx)
+        W = f.size + 1
+        ob = np.empty((p.n, 2 * (W + 1)))
+        of = np.empty((p.n, 2 * (W + 1)))
+        rc = lib().or_all_tunes(*p.state(), p.n, *self.t.ptrs(), n_turns, mu, barrier, int(inverse), _p(f), _p(t),
+                                f.size, n_threads, _p(ob), _p(of))
+        if rc:
+            raise RuntimeError("compute_tune: from and to must be valid indices")
+        self._advance(p, n_turns, inverse)
+        return ob, of
+
+    def get_tangent_matrix(self, p, mu, reverse=False):
+        out = np.empty((p.n, 4, 4))
+        lib().or_get_tangent_matrix(_p(p.x), _p(p.px), _p(p.y), _p(p.py), _p(p.valid), p.n, *self.t.ptrs(),
+                                    p.global_steps, mu, int(reverse), _p(out))
+        return out
+
+
+# ---------------------------------------------------------------- free functions
+def tangent_matrix(x, px, y, py, sx, cx, sy, cy, mu):
+    out = np.empty((4, 4))
+    lib().or_tangent_matrix(x, px, y, py, sx, cx, sy, cy, mu, _p(out))
+    return out
+
+
+def inverse_tangent_matrix(x, px, y, py, sx, cx, sy, cy, mu):
+    out = np.empty((4, 4))
+    lib().or_inverse_tangent_matrix(x, px, y, py, sx, cx, sy, cy, mu, _p(out))
+    return out
+
+
+def multiply_matrices(a, b):
+    a, b = _f64(a), _f64(b)
+    out = np.empty((4, 4))
+    lib().or_multiply_matrices(_p(a), _p(b), _p(out))
+    return out
+
+
+def birkhoff_weights(n):
+    out = np.empty(n)
+    lib().or_birkhoff_weights(n, _p(out))
+    return out
+
+
+def birkhoff_tune(x, px):
+    x, px = _f64(x), _f64(px)
+    return lib().or_birkhoff_tune(_p(x), _p(px), x.size)
+
+
+def fft_magnitude(re, im, hanning=True):
+    re, im = _f64(re), _f64(im)
+    out = np.empty(re.size)
+    lib().or_fft_magnitude(_p(re), _p(im), re.size, int(hanning), _p(out))
+    return out
+
+
+def interpolation(mag):
+    mag = _f64(mag)
+    return lib().or_interpolation(_p(mag), mag.size)
+
+
+def fft_tune(x, px):
+    x, px = _f64(x), _f64(px)
+    return lib().or_fft_tune(_p(x), _p(px), x.size)
+
+
+def get_tunes(x, px):
+    x, px = _f64(x), _f64(px)
+    out = np.empty(2)
+    lib().or_get_tunes(_p(x), _p(px), x.size, _p(out))
+    return out
+
+
+def birkhoff_tune_vec(x, px, y, py, from_idx=(), to_idx=()):
+    x, px, y, py = (_f64(a) for a in (x, px, y, py))
+    f, t = _u32(from_idx), _u32(to_idx)
+    out = np.empty(2 * (f.size + 1))
+    if lib().or_birkhoff_tune_vec(_p(x), _p(px), _p(y), _p(py), x.size, _p(f), _p(t), f.size, _p(out)):
+        raise RuntimeError("compute_tune: from and to must be valid indices")
+    return out
+
+
+def fft_tune_vec(x, px, y, py, from_idx=(), to_idx=()):
+    x, px, y, py = (_f64(a) for a in (x, px, y, py))
+    f, t = _u32(from_idx), _u32(to_idx)
+    out = np.empty(2 * (f.size + 1))
+    if lib().or_fft_tune_vec(_p(x), _p(px), _p(y), _p(py), x.size, _p(f), _p(t), f.size, _p(out)):
+        raise RuntimeError("compute_tune: from and to must be valid indices")
+    return out
+
+
+def nan_mean(v):
+    v = _f64(v)
+    return lib().or_nan_mean(_p(v), v.size)
+
+
+def nan_std(v):
+    v = _f64(v)
+    return lib().or_nan_std(_p(v), v.size)
+
+
+def structured_multiply(mats, x, px, y, py, tables: Tables, global_steps, mu):
+    mats = np.ascontiguousarray(mats, dtype=np.float64).copy()
+    x, px, y, py = (_f64(a) for a in (x, px, y, py))
+    lib().or_structured_multiply(_p(mats), _p(x), _p(px), _p(y), _p(py), x.size, *tables.ptrs(), global_steps, mu)
+    return mats
+
+
+def lyap_birk_add(values, values_b, vectors, weights, n_weights, b_idx):
+    vectors, weights = _f64(vectors), _f64(weights)
+    lib().or_lyap_birk_add(_p(values), _p(values_b), _p(vectors), _p(weights), values.size, n_weights, b_idx)
+
+
+def megno_construct_add(l1, l2, l3, a, b, idx):
+    a, b = _f64(a), _f64(b)
+    lib().or_megno_construct_add(_p(l1), _p(l2), _p(l3), _p(a), _p(b), l1.size, idx)
+
+
+def matvec(m, v):
+    m = _f64(m)
+    v = _f64(v).copy()
+    lib().or_matvec(_p(m), _p(v), v.shape[0])
+    return v
+
+
+def normalize(v):
+    v = _f64(v).copy()
+    lib().or_normalize(_p(v), v.shape[0])
+    return v
