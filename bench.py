@@ -36,11 +36,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["fast", "exact"], default="fast")
+    ap.add_argument("--mode", choices=["fast", "exact"], default="exact")
     ap.add_argument("--n-side", type=int, default=1000)
     ap.add_argument("--turns", type=int, default=100_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-stride", type=int, default=97)
+    ap.add_argument("--cpu-sample-stride", type=int, default=13)
+    ap.add_argument("--single-mode", action="store_true", help="time only --mode (profiling runs)")
     ap.add_argument("--compact-every", type=int, default=0)
     return ap.parse_args()
 
@@ -158,7 +159,8 @@ def main():
     cfg = C.LaunchCfg(0, a.compact_every, 0, 0) if a.compact_every else None
     fpt = flops_per_pt(w.mu)
     results = {}
-    for mode_name in ([a.mode] + [m for m in ("fast", "exact") if m != a.mode]):
+    modes = [a.mode] + ([] if a.single_mode else [m for m in ("fast", "exact") if m != a.mode])
+    for mode_name in modes:
         mode = C.MODE_FAST if mode_name == "fast" else C.MODE_EXACT
         dt, kms, launches, live = time_mode(w, mode, a.steps, a.warmup, cfg)
         per_step_live = live

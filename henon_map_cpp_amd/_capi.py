@@ -28,7 +28,21 @@ U64 = ctypes.c_uint64
 
 class LaunchCfg(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int32), ("compact_every", ctypes.c_int32),
-                ("unroll", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("particles_per_lane", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+MAX_SAMPLES, MAX_BIRKHOFF = 16, 4
+
+
+class IndicatorSpec(ctypes.Structure):
+    """hm_indicator_spec (include/henon_hip.h)."""
+    _fields_ = [("renorm_every", ctypes.c_uint32), ("n_samples", ctypes.c_uint32),
+                ("samples", ctypes.c_uint32 * MAX_SAMPLES), ("fli_dir", ctypes.c_double * 4),
+                ("megno", ctypes.c_void_p), ("fli", ctypes.c_void_p), ("matrix", ctypes.c_void_p),
+                ("log_scale", ctypes.c_void_p), ("n_birkhoff", ctypes.c_uint32),
+                ("birk_from", ctypes.c_uint32 * MAX_BIRKHOFF), ("birk_to", ctypes.c_uint32 * MAX_BIRKHOFF),
+                ("birk_out", ctypes.c_void_p), ("hist_from", ctypes.c_uint32), ("hist_len", ctypes.c_uint32),
+                ("hist", ctypes.c_void_p)]
 
 
 # name -> (restype, argtypes); mirrors include/henon_hip.h
@@ -53,6 +67,21 @@ SIGNATURES = {
     "hm_track": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, D, D, I, I, ctypes.POINTER(LaunchCfg), P]),
     "hm_observable": (I, [I, P, P, P, P, SZ, P, P]),
     "hm_tangent_matrix": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, I, P, P]),
+    "hm_megno_trace": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, D, P, SZ, P, P]),
+    "hm_track_indicators": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, D, I, ctypes.POINTER(IndicatorSpec), P]),
+    "hm_tangent_apply": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, I, P]),
+    "hm_matrix_left_multiply": (I, [P, P, SZ, P]),
+    "hm_matrix_identity": (I, [P, SZ, P]),
+    "hm_matvec": (I, [P, P, P, SZ, I, P]),
+    "hm_normalize": (I, [P, SZ, P]),
+    "hm_lyapunov_add": (I, [P, P, P, P, SZ, U64, U64, P]),
+    "hm_megno_add": (I, [P, P, P, P, P, SZ, U64, P]),
+    "hm_megno_birkhoff_add": (I, [P, P, P, P, SZ, U64, P, P, P, U32, P]),
+    "hm_tune_birkhoff_add": (I, [P, P, P, P, P, P, P, P, P, P, P, P, P, U32, SZ, U64, I, P]),
+    "hm_track_history": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, D, I, P, P]),
+    "hm_history_means": (I, [P, SZ, U32, P, P]),
+    "hm_birkhoff_tunes_history": (I, [P, SZ, U32, P, P, P, SZ, P, P]),
+    "hm_fft_tunes": (I, [P, SZ, U32, P, P, P, SZ, P, P]),
     "hm_profile_enable": (I, [I]),
     "hm_profile_reset": (I, []),
     "hm_profile_read": (I, [ctypes.c_char_p, ctypes.POINTER(D), ctypes.POINTER(U64)]),
@@ -77,6 +106,9 @@ def lib():
             raise ImportError(f"{LIB_PATH} is not built; run `python -m henon_map_cpp_amd.build`")
         _preload_runtime()
         L = ctypes.CDLL(LIB_PATH)
+        missing = [name for name in SIGNATURES if not hasattr(L, name)]
+        if missing:
+            raise ImportError(f"{LIB_PATH} is stale (missing {missing[:3]}...); rebuild it")
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)
             f.restype = res
@@ -239,3 +271,94 @@ def profile_read(name: str):
     ms, n = D(0.0), U64(0)
     check(lib().hm_profile_read(name.encode() if name else None, ctypes.byref(ms), ctypes.byref(n)))
     return ms.value, n.value
+
+
+def megno_trace(parts: DeviceParticles, table_dev: DeviceBuffer, table_len: int, n_turns: int, mu: float,
+                barrier: float, samples, stream=None):
+    s = np.ascontiguousarray(samples, dtype=np.uint32)
+    out = DeviceBuffer(8 * parts.n * max(s.size, 1))
+    check(lib().hm_megno_trace(*parts.ptrs(), parts.n, table_dev.ptr, table_len, n_turns, mu, barrier, ptr(s),
+                               s.size, out.ptr, stream))
+    return out.download(np.float64, parts.n * s.size).reshape(parts.n, s.size)
+
+
+def track_indicators(parts: DeviceParticles, table_dev: DeviceBuffer, table_len: int, n_turns: int, mu: float,
+                     barrier: float, mode=MODE_EXACT, renorm_every=0, samples=(), fli_dir=(1.0, 0.0, 0.0, 0.0),
+                     matrix=None, log_scale=None, birkhoff=(), hist=None, stream=None):
+    """Fused indicators; returns numpy results (megno, fli [n_samples][n], matrix
+    [n][4][4] when `matrix` is given (array of initial matrices or True for the
+    identity), log_scale [n], birkhoff [windows][2][n], hist [len][4][n])."""
+    n = parts.n
+    spec = IndicatorSpec()
+    spec.renorm_every = renorm_every
+    spec.n_samples = len(samples)
+    for q, v in enumerate(samples):
+        spec.samples[q] = v
+    for q in range(4):
+        spec.fli_dir[q] = fli_dir[q]
+    keep = []
+    out = {}
+    if len(samples):
+        mg, fl = DeviceBuffer(8 * n * len(samples)), DeviceBuffer(8 * n * len(samples))
+        keep += [mg, fl]
+        spec.megno, spec.fli = mg.ptr, fl.ptr
+    if matrix is not None:
+        m0 = np.tile(np.eye(4), (n, 1, 1)) if matrix is True else np.asarray(matrix, dtype=np.float64)
+        md = DeviceBuffer.from_array(np.ascontiguousarray(m0))
+        ld = DeviceBuffer.from_array(np.zeros(n) if log_scale is None else np.asarray(log_scale, np.float64))
+        keep += [md, ld]
+        spec.matrix, spec.log_scale = md.ptr, ld.ptr
+    spec.n_birkhoff = len(birkhoff)
+    for q, (a, b) in enumerate(birkhoff):
+        spec.birk_from[q], spec.birk_to[q] = a, b
+    if len(birkhoff):
+        bo = DeviceBuffer(8 * n * 2 * len(birkhoff))
+        keep.append(bo)
+        spec.birk_out = bo.ptr
+    if hist is not None:
+        h0, hl = hist
+        hd = DeviceBuffer(8 * n * 4 * hl)
+        keep.append(hd)
+        spec.hist_from, spec.hist_len, spec.hist = h0, hl, hd.ptr
+    check(lib().hm_track_indicators(*parts.ptrs(), n, table_dev.ptr, table_len, n_turns, mu, barrier, mode,
+                                    ctypes.byref(spec), stream))
+    if len(samples):
+        out["megno"] = mg.download(np.float64, n * len(samples)).reshape(len(samples), n)
+        out["fli"] = fl.download(np.float64, n * len(samples)).reshape(len(samples), n)
+    if matrix is not None:
+        out["matrix"] = md.download(np.float64, n * 16).reshape(n, 4, 4)
+        out["log_scale"] = ld.download(np.float64, n)
+    if len(birkhoff):
+        out["birkhoff"] = bo.download(np.float64, n * 2 * len(birkhoff)).reshape(len(birkhoff), 2, n)
+    if hist is not None:
+        out["hist"] = hd.download(np.float64, n * 4 * hist[1]).reshape(hist[1], 4, n)
+    return out
+
+
+def track_history(parts: DeviceParticles, table_dev: DeviceBuffer, table_len: int, n_turns: int, mu: float,
+                  barrier: float, inverse=False, stream=None) -> DeviceBuffer:
+    hist = DeviceBuffer(8 * parts.n * 4 * (n_turns + 1))
+    check(lib().hm_track_history(*parts.ptrs(), parts.n, table_dev.ptr, table_len, n_turns, mu, barrier,
+                                 int(inverse), hist.ptr, stream))
+    return hist
+
+
+def history_tunes(hist: DeviceBuffer, n: int, n_samples: int, windows, kind="fft", subtract_mean=False,
+                  stream=None):
+    """Tunes of windows [(from, to)] (inclusive) of a device history; returns
+    numpy [n_windows][2][n]."""
+    means = None
+    if subtract_mean:
+        means = DeviceBuffer(8 * 4 * n)
+        check(lib().hm_history_means(hist.ptr, n, n_samples, means.ptr, stream))
+    f = np.ascontiguousarray([w[0] for w in windows], dtype=np.uint32)
+    t = np.ascontiguousarray([w[1] for w in windows], dtype=np.uint32)
+    out = DeviceBuffer(8 * n * 2 * len(windows))
+    mp = means.ptr if means is not None else None
+    if kind == "fft":
+        ln = np.ascontiguousarray(t - f + 1, dtype=np.uint32)
+        check(lib().hm_fft_tunes(hist.ptr, n, n_samples, mp, ptr(f), ptr(ln), len(windows), out.ptr, stream))
+    else:
+        check(lib().hm_birkhoff_tunes_history(hist.ptr, n, n_samples, mp, ptr(f), ptr(t), len(windows), out.ptr,
+                                              stream))
+    return out.download(np.float64, n * 2 * len(windows)).reshape(len(windows), 2, n)

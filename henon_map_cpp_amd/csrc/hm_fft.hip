@@ -1,0 +1,509 @@
+// Tunes from stored trajectories (replaces FFTW and the per-particle CPU loops
+// of src/dynamic_indicator.cpp:32-361 and src/henon.cu:2890-3078, 3221-3463).
+//  * k_history        : track n turns, writing every sample to HBM
+//  * k_means          : per-coordinate mean of each particle's samples
+//  * k_birkhoff_hist  : Birkhoff tunes of windows of the stored samples
+//  * k_fft_tune_pow2  : Hanning-windowed FFT tune, power-of-two length, radix-2 in LDS
+//  * k_fft_tune_dft   : same for any length, direct DFT with exactly reduced twiddles
+// History layout: hist[sample][coord][particle] (coord = x, px, y, py), so a
+// tracking wavefront writes one contiguous row per turn.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "hm_device.h"
+#include "hm_internal.h"
+
+namespace hm {
+
+constexpr int kMaxWindows = 64;
+
+// ---------------------------------------------------------------- history
+template <bool INV, bool OCT>
+__global__ void __launch_bounds__(256)
+k_history(double *__restrict__ gx, double *__restrict__ gpx, double *__restrict__ gy, double *__restrict__ gpy,
+          uint32_t *__restrict__ gsteps, uint32_t n, const TabEntry *__restrict__ tab, uint32_t tab_len,
+          uint32_t n_turns, double mu, double b2, double *__restrict__ hist, uint32_t *__restrict__ status)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x = gx[i], px = gpx[i], y = gy[i], py = gpy[i];
+    uint32_t s = gsteps[i];
+    const size_t nn = n;
+    hist[0 * nn + i] = x; hist[1 * nn + i] = px; hist[2 * nn + i] = y; hist[3 * nn + i] = py;
+    bool alive = !any_nan(x, px, y, py);
+    const double m3mu = -3.0 * mu;
+    for (uint32_t k = 0; k < n_turns; ++k) {
+        if (alive) {
+            const int64_t ti = INV ? (int64_t)s - 1 : (int64_t)s;
+            if (ti < 0 || ti >= (int64_t)tab_len) {
+                atomicOr(status, 1u);
+                alive = false;
+            } else {
+                map_step<kExact, INV, OCT>(x, px, y, py, mu, m3mu, tab[ti]);
+                if (radius2<kExact>(x, px, y, py) < b2) {
+                    s = INV ? s - 1 : s + 1;
+                } else {
+                    x = px = y = py = __builtin_nan("");
+                    alive = false;
+                }
+            }
+        }
+        const size_t base = (size_t)(k + 1) * 4 * nn;
+        hist[base + 0 * nn + i] = x; hist[base + 1 * nn + i] = px;
+        hist[base + 2 * nn + i] = y; hist[base + 3 * nn + i] = py;
+    }
+    gx[i] = x; gpx[i] = px; gy[i] = y; gpy[i] = py; gsteps[i] = s;
+}
+
+// std::accumulate order (henon.cu:3041-3051): sum from 0.0 in sample order, / L
+__global__ void k_means(const double *__restrict__ hist, uint32_t n, uint32_t L, double *__restrict__ means)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t nn = n;
+    for (int c = 0; c < 4; c++) {
+        double s = 0.0;
+        for (uint32_t m = 0; m < L; m++) s = s + hist[(size_t)m * 4 * nn + c * nn + i];
+        means[c * nn + i] = s / (double)L;
+    }
+}
+
+struct WindowSet {
+    uint32_t from[kMaxWindows], len[kMaxWindows], w_off[kMaxWindows];
+    uint32_t count;
+};
+
+// dynamic_indicator.cpp:81-131 per window (len samples from `from`): angles of
+// (x - mean) + i (px - mean), wrapped differences (+2pi when negative), weights
+// birkhoff_weights(len-1); NaN when a sample of the window is NaN
+// (dynamic_indicator.cpp:228-241). out: [window][plane][n].
+__global__ void k_birkhoff_hist(const double *__restrict__ hist, uint32_t n, const double *__restrict__ means,
+                                WindowSet ws, const double *__restrict__ w_flat, double *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t nn = n;
+    double mu_[4] = {0.0, 0.0, 0.0, 0.0};
+    if (means)
+        for (int c = 0; c < 4; c++) mu_[c] = means[c * nn + i];
+    for (uint32_t w = 0; w < ws.count; w++) {
+        for (int p = 0; p < 2; p++) {
+            const int cx = 2 * p, cp = 2 * p + 1;
+            double acc = 0.0, th_prev = 0.0;
+            bool bad = false;
+            for (uint32_t m = 0; m < ws.len[w]; m++) {
+                const size_t o = (size_t)(ws.from[w] + m) * 4 * nn;
+                const double xv = hist[o + cx * nn + i] - mu_[cx];
+                const double pv = hist[o + cp * nn + i] - mu_[cp];
+                // any NaN among the four coordinates of the window
+                if (__builtin_isnan(hist[o + 0 * nn + i]) || __builtin_isnan(hist[o + 1 * nn + i]) ||
+                    __builtin_isnan(hist[o + 2 * nn + i]) || __builtin_isnan(hist[o + 3 * nn + i]))
+                    bad = true;
+                const double th = atan2(pv, xv);
+                if (m > 0) {
+                    double d = th - th_prev;
+                    if (d < 0.0) d += 2.0 * M_PI;
+                    acc = acc + w_flat[ws.w_off[w] + m - 1] * d;
+                }
+                th_prev = th;
+            }
+            out[((size_t)w * 2 + p) * nn + i] = bad ? __builtin_nan("") : 1 - acc / (2 * M_PI);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- FFT tune
+struct FftArgs {
+    const double *hist;
+    uint32_t n;            // particles
+    const double *means;   // [4][n] or NULL
+    uint32_t from[kMaxWindows];
+    uint32_t count;        // windows, all of length N
+    uint32_t N;
+    const double *hann;    // [N], 2 sin^2(pi m / N) (host, glibc)
+    const double *tw;      // [N] complex: exp(-2 pi i q / N) as (cos, -sin) pairs
+    double *out;           // [window][plane][n]
+    uint32_t out_window0;  // first window index in `out`
+};
+
+__device__ __forceinline__ double interp_A(double a, double b, double c)
+{
+    // dynamic_indicator.cpp:134-137
+    return (-(a + b * c) * (a - b) + b * sqrt(c * c * ((a + b) * (a + b)) - 2 * a * b * (2 * (c * c) - c - 1))) /
+           ((a * a) + (b * b) + 2 * a * b * c);
+}
+
+// dynamic_indicator.cpp:139-176 on magnitudes mag[0..N): first maximum wins
+__device__ double interp_tune(const double *mag, uint32_t N, uint32_t index)
+{
+    const double Nd = (double)N;
+    if (index == 0) return 1.0;
+    if (index == N - 1) return 0.0;
+    uint32_t i1, i2;
+    if (mag[index - 1] > mag[index + 1]) {
+        i1 = index - 1; i2 = index; index = index - 1;
+    } else {
+        i1 = index; i2 = index + 1;
+    }
+    const double value = ((double)index / Nd) +
+                         (1.0 / (2.0 * M_PI)) * asin(interp_A(mag[i1], mag[i2], cos(2.0 * M_PI / Nd)) *
+                                                     sin(2.0 * M_PI / Nd));
+    return fabs(1.0 - value);
+}
+
+// Signal of block b: XCD-aware order, so that workgroups dispatched to the same
+// XCD (b, b+8, b+16, ...) take neighbouring particles and share the L2 lines of
+// the particle-minor history rows.
+__device__ __forceinline__ void signal_of(uint32_t b, uint32_t n_signals_per_plane_window, uint32_t &particle,
+                                          uint32_t &plane, uint32_t &window, uint32_t n)
+{
+    const uint32_t total = n_signals_per_plane_window;  // n * 2 * windows
+    const uint32_t per_xcd = (total + 7) / 8;
+    uint32_t s = (b % 8) * per_xcd + b / 8;
+    if (s >= total) s = total;  // marks an idle block
+    particle = s % n;
+    plane = (s / n) % 2;
+    window = s / (2 * n);
+    if (s == total) window = 0xffffffffu;
+}
+
+__device__ __forceinline__ void block_argmax(double v, uint32_t idx, double *sv, uint32_t *si, double &best,
+                                             uint32_t &best_i)
+{
+    // first maximum (lowest index) over the block
+    const uint32_t t = threadIdx.x;
+    sv[t] = v;
+    si[t] = idx;
+    __syncthreads();
+    for (uint32_t st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (t < st) {
+            const double ov = sv[t + st];
+            const uint32_t oi = si[t + st];
+            if (ov > sv[t] || (ov == sv[t] && oi < si[t]) || (sv[t] != sv[t] && ov == ov)) {
+                sv[t] = ov;
+                si[t] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    best = sv[0];
+    best_i = si[0];
+    __syncthreads();
+}
+
+// Radix-2 decimation-in-time, bit-reversed load, in LDS; block of 256 threads.
+__global__ void __launch_bounds__(256) k_fft_tune_pow2(FftArgs a)
+{
+    extern __shared__ double lds[];
+    double *re = lds, *im = lds + a.N;
+    __shared__ double sv[256];
+    __shared__ uint32_t si[256];
+    __shared__ int any_nan_flag;
+    uint32_t particle, plane, window;
+    signal_of(blockIdx.x, a.n * 2 * a.count, particle, plane, window, a.n);
+    if (window == 0xffffffffu) return;
+    const uint32_t N = a.N;
+    const uint32_t logn = 31 - __builtin_clz(N);
+    const size_t nn = a.n;
+    const int cx = 2 * plane, cp = 2 * plane + 1;
+    const double mx = a.means ? a.means[cx * nn + particle] : 0.0;
+    const double mp = a.means ? a.means[cp * nn + particle] : 0.0;
+    if (threadIdx.x == 0) any_nan_flag = 0;
+    __syncthreads();
+    bool nan_seen = false;
+    for (uint32_t m = threadIdx.x; m < N; m += blockDim.x) {
+        const size_t o = (size_t)(a.from[window] + m) * 4 * nn;
+        for (int c = 0; c < 4; c++) nan_seen |= __builtin_isnan(a.hist[o + c * nn + particle]);
+        const double h = a.hann[m];
+        const uint32_t r = __brev(m) >> (32 - logn);
+        re[r] = (a.hist[o + cx * nn + particle] - mx) * h;
+        im[r] = (a.hist[o + cp * nn + particle] - mp) * h;
+    }
+    if (nan_seen) atomicOr(&any_nan_flag, 1);
+    __syncthreads();
+    const size_t out_o = ((size_t)(a.out_window0 + window) * 2 + plane) * nn + particle;
+    if (any_nan_flag) {
+        if (threadIdx.x == 0) a.out[out_o] = __builtin_nan("");
+        return;
+    }
+    for (uint32_t half = 1; half < N; half <<= 1) {
+        const uint32_t stride = N / (2 * half);
+        for (uint32_t j = threadIdx.x; j < N / 2; j += blockDim.x) {
+            const uint32_t k = j & (half - 1);
+            const uint32_t p0 = ((j - k) << 1) + k, p1 = p0 + half;
+            const double wr = a.tw[2 * (k * stride)], wi = a.tw[2 * (k * stride) + 1];
+            const double br = re[p1], bi = im[p1];
+            const double tr = br * wr - bi * wi, ti = br * wi + bi * wr;
+            const double ar = re[p0], ai = im[p0];
+            re[p0] = ar + tr; im[p0] = ai + ti;
+            re[p1] = ar - tr; im[p1] = ai - ti;
+        }
+        __syncthreads();
+    }
+    // magnitudes into re[], then the first maximum
+    double best = -1.0;
+    uint32_t best_i = 0xffffffffu;
+    for (uint32_t k = threadIdx.x; k < N; k += blockDim.x) {
+        const double mg = sqrt(re[k] * re[k] + im[k] * im[k]);
+        re[k] = mg;
+        if (mg > best || best_i == 0xffffffffu) { best = mg; best_i = k; }
+    }
+    __syncthreads();
+    double bv;
+    uint32_t bi;
+    block_argmax(best, best_i, sv, si, bv, bi);
+    if (threadIdx.x == 0) a.out[out_o] = interp_tune(re, N, bi);
+}
+
+// Direct DFT for any N: X_k = sum_m z_m exp(-2 pi i k m / N), twiddle index
+// k*m mod N kept exact; the windowed signal is staged through LDS in chunks.
+__global__ void __launch_bounds__(256) k_fft_tune_dft(FftArgs a, double *__restrict__ mag_scratch)
+{
+    constexpr uint32_t CH = 2048;
+    __shared__ double zr[CH], zi[CH];
+    __shared__ double sv[256];
+    __shared__ uint32_t si[256];
+    __shared__ int any_nan_flag;
+    uint32_t particle, plane, window;
+    signal_of(blockIdx.x, a.n * 2 * a.count, particle, plane, window, a.n);
+    if (window == 0xffffffffu) return;
+    const uint32_t N = a.N;
+    const size_t nn = a.n;
+    const int cx = 2 * plane, cp = 2 * plane + 1;
+    const double mx = a.means ? a.means[cx * nn + particle] : 0.0;
+    const double mp = a.means ? a.means[cp * nn + particle] : 0.0;
+    double *mag = mag_scratch + (size_t)blockIdx.x * N;
+    if (threadIdx.x == 0) any_nan_flag = 0;
+    // each thread owns outputs k = t, t + 256, ...; at most 64 per thread here
+    constexpr int KMAX = 64;
+    double xr[KMAX], xi[KMAX];
+    uint32_t q[KMAX];
+    const uint32_t nk = (N + blockDim.x - 1) / blockDim.x;
+    for (int j = 0; j < KMAX; j++) { xr[j] = 0.0; xi[j] = 0.0; q[j] = 0; }
+    for (uint32_t c0 = 0; c0 < N; c0 += CH) {
+        const uint32_t cn = min(CH, N - c0);
+        __syncthreads();
+        bool nan_seen = false;
+        for (uint32_t m = threadIdx.x; m < cn; m += blockDim.x) {
+            const size_t o = (size_t)(a.from[window] + c0 + m) * 4 * nn;
+            for (int c = 0; c < 4; c++) nan_seen |= __builtin_isnan(a.hist[o + c * nn + particle]);
+            const double h = a.hann[c0 + m];
+            zr[m] = (a.hist[o + cx * nn + particle] - mx) * h;
+            zi[m] = (a.hist[o + cp * nn + particle] - mp) * h;
+        }
+        if (nan_seen) atomicOr(&any_nan_flag, 1);
+        __syncthreads();
+        for (uint32_t j = 0; j < nk && j < (uint32_t)KMAX; j++) {
+            const uint32_t k = threadIdx.x + j * blockDim.x;
+            if (k >= N) break;
+            uint32_t qq = (uint32_t)(((uint64_t)k * c0) % N);
+            double ar = xr[j], ai = xi[j];
+            for (uint32_t m = 0; m < cn; m++) {
+                const double wr = a.tw[2 * qq], wi = a.tw[2 * qq + 1];
+                ar += zr[m] * wr - zi[m] * wi;
+                ai += zr[m] * wi + zi[m] * wr;
+                qq += k;
+                if (qq >= N) qq -= N;
+            }
+            xr[j] = ar;
+            xi[j] = ai;
+        }
+    }
+    __syncthreads();
+    const size_t out_o = ((size_t)(a.out_window0 + window) * 2 + plane) * nn + particle;
+    if (any_nan_flag) {
+        if (threadIdx.x == 0) a.out[out_o] = __builtin_nan("");
+        return;
+    }
+    double best = -1.0;
+    uint32_t best_i = 0xffffffffu;
+    for (uint32_t j = 0; j < nk && j < (uint32_t)KMAX; j++) {
+        const uint32_t k = threadIdx.x + j * blockDim.x;
+        if (k >= N) break;
+        const double mg = sqrt(xr[j] * xr[j] + xi[j] * xi[j]);
+        mag[k] = mg;
+        if (mg > best || best_i == 0xffffffffu) { best = mg; best_i = k; }
+    }
+    __threadfence_block();
+    __syncthreads();
+    double bv;
+    uint32_t bi;
+    block_argmax(best, best_i, sv, si, bv, bi);
+    if (threadIdx.x == 0) a.out[out_o] = interp_tune(mag, N, bi);
+}
+
+// Host tables per length N: Hanning window (dynamic_indicator.cpp:47-52, glibc)
+// and twiddles exp(-2 pi i q / N) in long double, cached on the device.
+struct FftTables {
+    double *hann = nullptr, *tw = nullptr;
+};
+
+static FftTables fft_tables(uint32_t N, hipStream_t s)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, uint32_t>, FftTables> cache;
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find({dev, N});
+    if (it != cache.end()) return it->second;
+    std::vector<double> h(N), tw(2 * (size_t)N);
+    for (uint32_t i = 0; i < N; i++) h[i] = 2 * std::pow((std::sin(M_PI * i / (N))), 2);
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (uint32_t q = 0; q < N; q++) {
+        const long double ang = two_pi * (long double)q / (long double)N;
+        tw[2 * q] = (double)cosl(ang);
+        tw[2 * q + 1] = (double)-sinl(ang);
+    }
+    FftTables t;
+    hip_check(hipMalloc(&t.hann, N * sizeof(double)), "hipMalloc(hann)");
+    hip_check(hipMalloc(&t.tw, 2 * (size_t)N * sizeof(double)), "hipMalloc(twiddles)");
+    hip_check(hipMemcpyAsync(t.hann, h.data(), N * sizeof(double), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    hip_check(hipMemcpyAsync(t.tw, tw.data(), 2 * (size_t)N * sizeof(double), hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    cache[{dev, N}] = t;
+    return t;
+}
+
+static void host_birkhoff_weights(uint32_t n, double *out)
+{
+    // src/dynamic_indicator.cpp:63-79
+    for (uint32_t i = 0; i < n; i++) {
+        double t = (double)i / (double)(n);
+        out[i] = std::exp(-1 / (t * (1.0 - t)));
+    }
+    double sum = 0.0;
+    for (uint32_t i = 0; i < n; i++) sum = sum + out[i];
+    for (uint32_t i = 0; i < n; i++) out[i] = out[i] / sum;
+}
+
+}  // namespace hm
+
+using namespace hm;
+
+extern "C" int hm_track_history(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n,
+                                const double *table, size_t table_len, uint32_t n_turns, double mu, double barrier,
+                                int inverse, double *hist, void *stream)
+{
+    return guarded([&] {
+        if (n == 0) return;
+        if (!x || !px || !y || !py || !steps || !table || !hist) throw Error(HM_ERR_ARG, "history: NULL pointer");
+        hipStream_t s = as_stream(stream);
+        const uint32_t grid = (uint32_t)((n + 255) / 256);
+        const double b2 = barrier * barrier;
+        const TabEntry *tab = reinterpret_cast<const TabEntry *>(table);
+        LaunchTimer timer("k_history", s);
+        const bool oct = mu != 0.0;
+        if (inverse) {
+            if (oct) hipLaunchKernelGGL((k_history<true, true>), dim3(grid), dim3(256), 0, s, x, px, y, py, steps,
+                                        (uint32_t)n, tab, (uint32_t)table_len, n_turns, mu, b2, hist, status_word());
+            else hipLaunchKernelGGL((k_history<true, false>), dim3(grid), dim3(256), 0, s, x, px, y, py, steps,
+                                    (uint32_t)n, tab, (uint32_t)table_len, n_turns, mu, b2, hist, status_word());
+        } else {
+            if (oct) hipLaunchKernelGGL((k_history<false, true>), dim3(grid), dim3(256), 0, s, x, px, y, py, steps,
+                                        (uint32_t)n, tab, (uint32_t)table_len, n_turns, mu, b2, hist, status_word());
+            else hipLaunchKernelGGL((k_history<false, false>), dim3(grid), dim3(256), 0, s, x, px, y, py, steps,
+                                    (uint32_t)n, tab, (uint32_t)table_len, n_turns, mu, b2, hist, status_word());
+        }
+        launch_check("k_history");
+    });
+}
+
+extern "C" int hm_history_means(const double *hist, size_t n, uint32_t n_samples, double *means, void *stream)
+{
+    return guarded([&] {
+        if (n == 0) return;
+        if (!hist || !means || n_samples == 0) throw Error(HM_ERR_ARG, "means: bad arguments");
+        hipStream_t s = as_stream(stream);
+        hipLaunchKernelGGL(k_means, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, hist, (uint32_t)n, n_samples,
+                           means);
+        launch_check("k_means");
+    });
+}
+
+extern "C" int hm_birkhoff_tunes_history(const double *hist, size_t n, uint32_t n_samples, const double *means,
+                                         const uint32_t *from, const uint32_t *to, size_t n_windows, double *out,
+                                         void *stream)
+{
+    return guarded([&] {
+        if (n == 0 || n_windows == 0) return;
+        if (!hist || !out || !from || !to) throw Error(HM_ERR_ARG, "birkhoff: NULL pointer");
+        hipStream_t s = as_stream(stream);
+        for (size_t w0 = 0; w0 < n_windows; w0 += kMaxWindows) {
+            WindowSet ws{};
+            ws.count = (uint32_t)std::min<size_t>(kMaxWindows, n_windows - w0);
+            size_t total = 0;
+            for (uint32_t w = 0; w < ws.count; w++) {
+                const uint32_t f = from[w0 + w], t = to[w0 + w];
+                if (f >= n_samples || t >= n_samples) throw Error(HM_ERR_RANGE, "compute_tune: from and to must be valid indices");
+                if (t < f + 1) throw Error(HM_ERR_ARG, "birkhoff: a window needs at least two samples");
+                ws.from[w] = f;
+                ws.len[w] = t - f + 1;
+                ws.w_off[w] = (uint32_t)total;
+                total += ws.len[w] - 1;
+            }
+            std::vector<double> wts(total);
+            for (uint32_t w = 0; w < ws.count; w++) host_birkhoff_weights(ws.len[w] - 1, wts.data() + ws.w_off[w]);
+            double *dw = static_cast<double *>(workspace(std::max<size_t>(total, 1) * sizeof(double), 2));
+            hip_check(hipMemcpyAsync(dw, wts.data(), total * sizeof(double), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+            LaunchTimer timer("k_birkhoff_hist", s);
+            hipLaunchKernelGGL(k_birkhoff_hist, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, hist, (uint32_t)n,
+                               means, ws, (const double *)dw, out + w0 * 2 * n);
+            launch_check("k_birkhoff_hist");
+            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        }
+    });
+}
+
+extern "C" int hm_fft_tunes(const double *hist, size_t n, uint32_t n_samples, const double *means,
+                            const uint32_t *from, const uint32_t *len, size_t n_windows, double *out, void *stream)
+{
+    return guarded([&] {
+        if (n == 0 || n_windows == 0) return;
+        if (!hist || !out || !from || !len) throw Error(HM_ERR_ARG, "fft_tunes: NULL pointer");
+        if (n > 0x3fffffffull) throw Error(HM_ERR_ARG, "fft_tunes: too many particles");
+        hipStream_t s = as_stream(stream);
+        // group windows by length: one launch per (length, up to 64 windows)
+        std::map<uint32_t, std::vector<uint32_t>> by_len;
+        for (size_t w = 0; w < n_windows; w++) {
+            if (len[w] < 2) throw Error(HM_ERR_ARG, "fft_tunes: a window needs at least two samples");
+            if ((uint64_t)from[w] + len[w] > n_samples)
+                throw Error(HM_ERR_RANGE, "compute_tune: from and to must be valid indices");
+            by_len[len[w]].push_back((uint32_t)w);
+        }
+        for (auto &kv : by_len) {
+            const uint32_t N = kv.first;
+            const FftTables t = fft_tables(N, s);
+            const bool pow2 = (N & (N - 1)) == 0 && N <= 8192;
+            if (!pow2 && N > 64 * 256) throw Error(HM_ERR_UNSUPPORTED, "fft_tunes: window longer than 16384 samples");
+            const auto &ws = kv.second;
+            for (size_t b0 = 0; b0 < ws.size();) {
+                // consecutive output windows only (one launch writes a contiguous window range)
+                size_t b1 = b0 + 1;
+                while (b1 < ws.size() && b1 - b0 < (size_t)kMaxWindows && ws[b1] == ws[b1 - 1] + 1) ++b1;
+                FftArgs a{};
+                a.hist = hist; a.n = (uint32_t)n; a.means = means; a.count = (uint32_t)(b1 - b0); a.N = N;
+                a.hann = t.hann; a.tw = t.tw; a.out = out; a.out_window0 = ws[b0];
+                for (size_t q = b0; q < b1; q++) a.from[q - b0] = from[ws[q]];
+                const uint64_t signals = (uint64_t)n * 2 * a.count;
+                const uint32_t blocks = (uint32_t)(((signals + 7) / 8) * 8);
+                if (pow2) {
+                    LaunchTimer timer("k_fft_tune_pow2", s);
+                    hipLaunchKernelGGL(k_fft_tune_pow2, dim3(blocks), dim3(256), 2 * N * sizeof(double), s, a);
+                    launch_check("k_fft_tune_pow2");
+                } else {
+                    double *mag = static_cast<double *>(workspace((size_t)blocks * N * sizeof(double), 3));
+                    LaunchTimer timer("k_fft_tune_dft", s);
+                    hipLaunchKernelGGL(k_fft_tune_dft, dim3(blocks), dim3(256), 0, s, a, mag);
+                    launch_check("k_fft_tune_dft");
+                }
+                b0 = b1;
+            }
+        }
+    });
+}

@@ -1,0 +1,728 @@
+// Tangent-map kernels.
+//  * k_megno_trace: henon_tracker::megno (src/henon.cu:2755-2887) on the GPU, one
+//    thread per particle, the 4x4 tangent matrix in registers.
+//  * k_indicators : fused tracking + tangent matrix (structured J*M, scalar
+//    renormalisation) + MEGNO (the construct form of src/henon.cu:2253-2284) +
+//    FLI + online Birkhoff tunes + a trajectory window in HBM for FFT tunes.
+//  * per-call construct kernels (src/henon.cu:267-359, 2083-2098, 2253-2284,
+//    2366-2402, 2533-2584) for the reference's construct classes.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "hm_device.h"
+#include "hm_internal.h"
+
+namespace hm {
+
+constexpr int kMaxSamples = 16;
+constexpr int kMaxBirk = 4;
+
+// ---------------------------------------------------------------- trace MEGNO
+struct MegnoTraceArgs {
+    double *x, *px, *y, *py;
+    uint32_t *steps;
+    uint32_t n;
+    const TabEntry *tab;
+    uint32_t tab_len, n_turns;
+    double mu, b2;
+    uint32_t samples[kMaxSamples];
+    uint32_t n_samples;
+    double *out;  // [n][n_samples], the reference's per-particle row layout
+    uint32_t *status;
+};
+
+// Full 4x4 product in the reference's order (henon.cu:251-265): i-j-k, += from 0.0.
+__device__ __forceinline__ void mat_mul_full(const double *a, double *m)
+{
+    double r[16];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) acc = acc + a[i * 4 + k] * m[k * 4 + j];
+            r[i * 4 + j] = acc;
+        }
+#pragma unroll
+    for (int i = 0; i < 16; i++) m[i] = r[i];
+}
+
+// One thread per particle. Per turn k: J at the pre-step point with table
+// index k (the reference ignores the particle's turn counter here, Q4), M <- J M,
+// a_k = (k+1) log10(tr M_{k+1} / tr M_k), then the map step with table index
+// steps. The two prefix sums are accumulated online in the same order.
+// Every particle starts from the identity (the reference reuses the
+// off-diagonal entries of the previous particle its CPU thread handled).
+template <bool OCT>
+__global__ void __launch_bounds__(256) k_megno_trace(MegnoTraceArgs a)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    double x = a.x[i], px = a.px[i], y = a.y[i], py = a.py[i];
+    uint32_t s = a.steps[i];
+    double m[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) m[q] = (q % 5 == 0) ? 1.0 : 0.0;
+    double trace_1 = 4.0, trace_2 = 4.0, s1 = 0.0, s2 = 0.0;
+    double res[kMaxSamples];
+#pragma unroll
+    for (int q = 0; q < kMaxSamples; q++) res[q] = 0.0;  // slot n_turns stays 0 (Q4)
+    const double mu3 = a.mu * 3, mu6 = a.mu * 6, m3mu = -3.0 * a.mu;
+    bool alive = !any_nan(x, px, y, py);
+    for (uint32_t k = 0; k < a.n_turns; ++k) {
+        if (k >= a.tab_len) break;
+        const TabEntry tj = a.tab[k];
+        const Jac jc = jac_coeffs<kExact>(x, y, mu3, mu6);
+        double jm[16];
+        jac_dense_exact(jc, tj, jm);
+        mat_mul_full(jm, m);
+        trace_2 = trace_1;
+        trace_1 = ((m[0] + m[5]) + m[10]) + m[15];
+        const double ak = (double)(k + 1) * log10(trace_1 / trace_2);
+        s1 = s1 + ak;
+        s2 = s2 + s1 / (double)(k + 1);
+#pragma unroll
+        for (int q = 0; q < kMaxSamples; q++)
+            if (q < (int)a.n_samples && a.samples[q] == k) res[q] = s2 / (double)a.samples[q];
+        if (alive) {
+            if (s >= a.tab_len) {
+                atomicOr(a.status, 1u);
+                alive = false;
+            } else {
+                map_step<kExact, false, OCT>(x, px, y, py, a.mu, m3mu, a.tab[s]);
+                if (radius2<kExact>(x, px, y, py) < a.b2) {
+                    s += 1;
+                } else {
+                    x = px = y = py = __builtin_nan("");
+                    alive = false;
+                }
+            }
+        }
+    }
+    a.x[i] = x; a.px[i] = px; a.y[i] = y; a.py[i] = py; a.steps[i] = s;
+#pragma unroll
+    for (int q = 0; q < kMaxSamples; q++)
+        if (q < (int)a.n_samples) a.out[(size_t)i * a.n_samples + q] = res[q];
+}
+
+// ---------------------------------------------------------------- fused indicators
+struct IndicatorArgs {
+    double *x, *px, *y, *py;
+    uint32_t *steps;
+    uint32_t n;
+    const TabEntry *tab;
+    uint32_t tab_len, n_turns;
+    double mu, b2;
+    uint32_t renorm_every;
+    uint32_t samples[kMaxSamples];  // turn counts 1..n_turns, increasing
+    uint32_t n_samples;
+    double fli_dir[4];
+    double *megno, *fli;    // [n_samples][n]
+    double *matrix;         // [n][16] in/out, or NULL (identity start, not stored)
+    double *log_scale;      // [n] in/out log10 of the factored-out scale, or NULL
+    uint32_t birk_from[kMaxBirk], birk_to[kMaxBirk], birk_off[kMaxBirk];
+    uint32_t n_birk;
+    const double *birk_w;   // concatenated weight tables
+    double *birk_out;       // [n_birk][2][n] tunes
+    uint32_t hist_from, hist_len;
+    double *hist;           // [hist_len][4][n]
+    uint32_t *status;
+};
+
+// M <- J M with J's structure: rows 0-1 touch M rows 0,1,2; rows 2-3 M rows 0,2,3.
+template <int MODE>
+__device__ __forceinline__ void mat_mul_structured(const Jac &j, const TabEntry &t, double *m)
+{
+    double j00, j02, j10, j12, j20, j22, j30, j32;
+    if (MODE == kExact) {
+        j00 = t.cx + (t.sx * j.a); j02 = t.sx * j.b;
+        j10 = -t.sx + (t.cx * j.a); j12 = t.cx * j.b;
+        j20 = t.sy * j.b; j22 = t.cy + (t.sy * j.an);
+        j30 = t.cy * j.b; j32 = -t.sy + (t.cy * j.an);
+    } else {
+        j00 = fma(t.sx, j.a, t.cx); j02 = t.sx * j.b;
+        j10 = fma(t.cx, j.a, -t.sx); j12 = t.cx * j.b;
+        j20 = t.sy * j.b; j22 = fma(-t.sy, j.a, t.cy);
+        j30 = t.cy * j.b; j32 = fma(-t.cy, j.a, -t.sy);
+    }
+    double r[16];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const double m0 = m[c], m1 = m[4 + c], m2 = m[8 + c], m3 = m[12 + c];
+        if (MODE == kExact) {
+            r[c] = ((j00 * m0) + (t.sx * m1)) + (j02 * m2);
+            r[4 + c] = ((j10 * m0) + (t.cx * m1)) + (j12 * m2);
+            r[8 + c] = ((j20 * m0) + (j22 * m2)) + (t.sy * m3);
+            r[12 + c] = ((j30 * m0) + (j32 * m2)) + (t.cy * m3);
+        } else {
+            r[c] = fma(j00, m0, fma(t.sx, m1, j02 * m2));
+            r[4 + c] = fma(j10, m0, fma(t.cx, m1, j12 * m2));
+            r[8 + c] = fma(j20, m0, fma(j22, m2, t.sy * m3));
+            r[12 + c] = fma(j30, m0, fma(j32, m2, t.cy * m3));
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) m[q] = r[q];
+}
+
+template <int MODE>
+__device__ __forceinline__ double frob2(const double *m)
+{
+    double s = m[0] * m[0];
+    if (MODE == kExact) {
+#pragma unroll
+        for (int q = 1; q < 16; q++) s = s + (m[q] * m[q]);
+    } else {
+#pragma unroll
+        for (int q = 1; q < 16; q++) s = fma(m[q], m[q], s);
+    }
+    return s;
+}
+
+// One thread per particle; the table index of turn k is steps+k for both the
+// tangent matrix (evaluated at the pre-step point) and the step.
+//   MEGNO (construct form): with idx = k+1, L1 += log10(|M_{k+1}|_F / |M_k|_F) idx,
+//     L2 += L1 / idx; recorded at sample t as L2 / t.
+//   FLI: log10(|M_t v0|) / t with v0 = fli_dir / |fli_dir|, including the
+//     log10 of the scale factored out by renormalisation.
+//   Renormalisation every renorm_every turns: M <- M / |M|_F, after the MEGNO
+//     update of that turn; the next ratio uses the norm of the scaled matrix.
+//   Birkhoff windows (sample indices from < m <= to): acc += w[m-1-from] dtheta_m
+//     with dtheta wrapped into [0, 2pi) (dynamic_indicator.cpp:110-117); tune =
+//     1 - acc / 2pi (NaN once a sample of the window is NaN).
+template <int MODE, bool OCT>
+__global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in_range = i < a.n;
+    double x = __builtin_nan(""), px = x, y = x, py = x;
+    uint32_t s = 0;
+    if (in_range) {
+        x = a.x[i]; px = a.px[i]; y = a.y[i]; py = a.py[i];
+        s = a.steps[i];
+    }
+    double m[16];
+    if (a.matrix && in_range) {
+#pragma unroll
+        for (int q = 0; q < 16; q++) m[q] = a.matrix[(size_t)i * 16 + q];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; q++) m[q] = (q % 5 == 0) ? 1.0 : 0.0;
+    }
+    double lscale = (a.log_scale && in_range) ? a.log_scale[i] : 0.0;
+    double fb = sqrt(frob2<MODE>(m));
+    double l1 = 0.0, l2 = 0.0;
+    const double mu3 = a.mu * 3, mu6 = a.mu * 6, m3mu = -3.0 * a.mu;
+    const bool alive0 = in_range && !any_nan(x, px, y, py);
+    bool alive = alive0;
+    uint32_t done = a.n_turns;  // turns survived
+    double v0[4];
+    {
+        const double nn = sqrt(((a.fli_dir[0] * a.fli_dir[0] + a.fli_dir[1] * a.fli_dir[1]) +
+                                a.fli_dir[2] * a.fli_dir[2]) + a.fli_dir[3] * a.fli_dir[3]);
+#pragma unroll
+        for (int q = 0; q < 4; q++) v0[q] = a.fli_dir[q] / nn;
+    }
+    double bacc[kMaxBirk][2];
+#pragma unroll
+    for (int w = 0; w < kMaxBirk; w++) bacc[w][0] = bacc[w][1] = 0.0;
+    double thx = 0.0, thy = 0.0;
+    if (a.n_birk) {
+        thx = atan2(px, x);
+        thy = atan2(py, y);
+    }
+    if (in_range && a.hist_len && a.hist_from == 0) {
+        const size_t n = a.n;
+        a.hist[0 * n + i] = x; a.hist[1 * n + i] = px; a.hist[2 * n + i] = y; a.hist[3 * n + i] = py;
+    }
+    // wave-uniform turn index when the live particles share one (scalar table loads)
+    const uint64_t live = __ballot(alive0);
+    uint32_t s0 = 0;
+    if (live) s0 = __builtin_amdgcn_readlane(s, __builtin_ctzll(live));
+    const bool uniform = live && __all(!alive0 || s == s0) && (uint64_t)s0 + a.n_turns <= (uint64_t)a.tab_len;
+    const TabEntry *tp = a.tab + s0;
+
+    uint32_t sample_q = 0;
+    uint32_t k = 0;
+    for (; k < a.n_turns; ++k) {
+        TabEntry t;
+        if (uniform) {
+            t = tp[k];
+        } else {
+            const uint64_t ti = (uint64_t)s + k;
+            if (alive && ti >= a.tab_len) {
+                atomicOr(a.status, 1u);
+                alive = false;
+                done = k;
+            }
+            t = a.tab[ti < a.tab_len ? ti : a.tab_len - 1];
+        }
+        const Jac jc = jac_coeffs<MODE>(x, y, mu3, mu6);
+        mat_mul_structured<MODE>(jc, t, m);
+        map_step<MODE, false, OCT>(x, px, y, py, a.mu, m3mu, t);
+        if (alive && !(radius2<MODE>(x, px, y, py) < a.b2)) {
+            x = px = y = py = __builtin_nan("");
+#pragma unroll
+            for (int q = 0; q < 16; q++) m[q] = __builtin_nan("");
+            alive = false;
+            done = k;
+        }
+        const uint32_t idx = k + 1;
+        const double fa = sqrt(frob2<MODE>(m));
+        l1 = l1 + log10(fa / fb) * (double)idx;
+        l2 = l2 + (1.0 / (double)idx) * l1;
+        fb = fa;
+        if (sample_q < a.n_samples && a.samples[sample_q] == idx) {
+            const size_t o = (size_t)sample_q * a.n + i;
+            if (in_range && a.megno) a.megno[o] = (1.0 / (double)idx) * l2;
+            if (in_range && a.fli) {
+                double v[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    v[r] = (((m[r * 4] * v0[0]) + (m[r * 4 + 1] * v0[1])) + (m[r * 4 + 2] * v0[2])) +
+                           (m[r * 4 + 3] * v0[3]);
+                const double nv = sqrt((((v[0] * v[0]) + (v[1] * v[1])) + (v[2] * v[2])) + (v[3] * v[3]));
+                a.fli[o] = (lscale + log10(nv)) / (double)idx;
+            }
+            ++sample_q;
+        }
+        if (a.renorm_every && idx % a.renorm_every == 0 && alive) {
+            const double inv = 1.0 / fa;
+#pragma unroll
+            for (int q = 0; q < 16; q++) m[q] = m[q] * inv;
+            lscale = lscale + log10(fa);
+            fb = sqrt(frob2<MODE>(m));
+        }
+        if (a.n_birk) {
+            const double nx = atan2(px, x), ny = atan2(py, y);
+            double dx = nx - thx, dy = ny - thy;
+            if (dx < 0.0) dx += 2.0 * M_PI;
+            if (dy < 0.0) dy += 2.0 * M_PI;
+            thx = nx;
+            thy = ny;
+#pragma unroll
+            for (int w = 0; w < kMaxBirk; w++) {
+                if (w < (int)a.n_birk && idx > a.birk_from[w] && idx <= a.birk_to[w]) {
+                    const double wt = a.birk_w[a.birk_off[w] + (idx - 1 - a.birk_from[w])];
+                    bacc[w][0] = bacc[w][0] + wt * dx;
+                    bacc[w][1] = bacc[w][1] + wt * dy;
+                }
+            }
+        }
+        if (in_range && a.hist_len && idx >= a.hist_from && idx < a.hist_from + a.hist_len) {
+            const size_t n = a.n, base = (size_t)(idx - a.hist_from) * 4 * n;
+            a.hist[base + 0 * n + i] = x; a.hist[base + 1 * n + i] = px;
+            a.hist[base + 2 * n + i] = y; a.hist[base + 3 * n + i] = py;
+        }
+        if (!__any(alive) && !a.hist_len && !a.n_birk) break;
+    }
+    if (!in_range) return;
+    // samples not reached because every particle of the wavefront was lost
+    for (; sample_q < a.n_samples; ++sample_q) {
+        const size_t o = (size_t)sample_q * a.n + i;
+        if (a.megno) a.megno[o] = __builtin_nan("");
+        if (a.fli) a.fli[o] = __builtin_nan("");
+    }
+    if (alive0) {
+        a.x[i] = x; a.px[i] = px; a.y[i] = y; a.py[i] = py;
+        a.steps[i] = s + done;
+    }
+    if (a.matrix) {
+#pragma unroll
+        for (int q = 0; q < 16; q++) a.matrix[(size_t)i * 16 + q] = m[q];
+    }
+    if (a.log_scale) a.log_scale[i] = lscale;
+#pragma unroll
+    for (int w = 0; w < kMaxBirk; w++) {
+        if (w < (int)a.n_birk) {
+            a.birk_out[((size_t)w * 2 + 0) * a.n + i] = 1.0 - bacc[w][0] / (2 * M_PI);
+            a.birk_out[((size_t)w * 2 + 1) * a.n + i] = 1.0 - bacc[w][1] / (2 * M_PI);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- construct kernels
+// henon.cu:267-313: M <- J(table[ti]) M (multiply=1) or M <- J (multiply=0),
+// full product in the reference's order.
+__global__ void k_tangent_apply(double *__restrict__ mats, const double *__restrict__ x, const double *__restrict__ px,
+                                const double *__restrict__ y, const double *__restrict__ py, uint32_t n,
+                                const TabEntry *__restrict__ tab, uint32_t ti, double mu, int multiply)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    (void)px; (void)py;
+    const Jac jc = jac_coeffs<kExact>(x[i], y[i], mu * 3, mu * 6);
+    double jm[16], m[16];
+    jac_dense_exact(jc, tab[ti], jm);
+    if (multiply) {
+#pragma unroll
+        for (int q = 0; q < 16; q++) m[q] = mats[(size_t)i * 16 + q];
+        mat_mul_full(jm, m);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; q++) m[q] = jm[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) mats[(size_t)i * 16 + q] = m[q];
+}
+
+// matrix_4d_vector::multiply (henon.cu:1721-1741): M <- L M, L given per particle
+__global__ void k_left_multiply(double *__restrict__ mats, const double *__restrict__ left, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double l[16], m[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) { l[q] = left[(size_t)i * 16 + q]; m[q] = mats[(size_t)i * 16 + q]; }
+    mat_mul_full(l, m);
+#pragma unroll
+    for (int q = 0; q < 16; q++) mats[(size_t)i * 16 + q] = m[q];
+}
+
+__global__ void k_identity(double *__restrict__ mats, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int q = 0; q < 16; q++) mats[(size_t)i * 16 + q] = (q % 5 == 0) ? 1.0 : 0.0;
+}
+
+// henon.cu:316-338 (in place, out == vecs) and matrix_4d_vector::get_vector
+// (henon.cu:1758-1787: out = M rv, a sum of four products per row).
+__global__ void k_matvec(const double *__restrict__ mats, const double *__restrict__ vecs, double *__restrict__ out,
+                         uint32_t n, int reference_order)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *m = mats + (size_t)i * 16;
+    const double v0 = vecs[(size_t)i * 4], v1 = vecs[(size_t)i * 4 + 1], v2 = vecs[(size_t)i * 4 + 2],
+                 v3 = vecs[(size_t)i * 4 + 3];
+    double r[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (reference_order) {  // tmp = 0; tmp += m*v ... (henon.cu:328-332)
+            double t = 0.0;
+            t = t + m[q * 4] * v0; t = t + m[q * 4 + 1] * v1; t = t + m[q * 4 + 2] * v2; t = t + m[q * 4 + 3] * v3;
+            r[q] = t;
+        } else {  // m0 v0 + m1 v1 + m2 v2 + m3 v3 (henon.cu:1777-1780)
+            r[q] = (((m[q * 4] * v0) + (m[q * 4 + 1] * v1)) + (m[q * 4 + 2] * v2)) + (m[q * 4 + 3] * v3);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) out[(size_t)i * 4 + q] = r[q];
+}
+
+// henon.cu:341-359
+__global__ void k_normalize(double *__restrict__ v, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double norm = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) norm = norm + v[(size_t)i * 4 + q] * v[(size_t)i * 4 + q];
+    norm = sqrt(norm);
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[(size_t)i * 4 + q] = v[(size_t)i * 4 + q] / norm;
+}
+
+// henon.cu:2083-2098
+__global__ void k_lyap_birk(double *__restrict__ values, double *__restrict__ values_b, const double *__restrict__ v,
+                            const double *__restrict__ weights, uint32_t n, uint64_t n_weights, uint64_t b_idx)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double norm = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) norm = norm + v[(size_t)i * 4 + q] * v[(size_t)i * 4 + q];
+    norm = sqrt(norm);
+    values[i] = values[i] + log(norm) / (double)n_weights;
+    values_b[i] = values_b[i] + log(norm) * weights[b_idx];
+}
+
+__device__ __forceinline__ double frob_ref(const double *m)
+{
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; q++) t = t + m[q] * m[q];
+    return sqrt(t);
+}
+
+// henon.cu:2253-2284 (layer3 may be NULL) and :2366-2402 (layer2 per sample)
+__global__ void k_megno_add(double *__restrict__ l1, double *__restrict__ l2, double *__restrict__ l3,
+                            const double *__restrict__ ma, const double *__restrict__ mb, uint32_t n, uint64_t idx,
+                            const double *__restrict__ w_flat, const uint64_t *__restrict__ w_off,
+                            const uint64_t *__restrict__ turn_samples, uint32_t n_samples)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double ta = frob_ref(ma + (size_t)i * 16), tb = frob_ref(mb + (size_t)i * 16);
+    l1[i] = l1[i] + log10(ta / tb) * (double)idx;
+    if (!w_flat) {
+        l2[i] = l2[i] + (1.0 / (double)idx) * l1[i];
+        l3[i] = (1.0 / (double)idx) * l2[i];
+    } else {
+        for (uint32_t j = 0; j < n_samples; j++)
+            if (idx <= turn_samples[j])
+                l2[(size_t)j * n + i] = l2[(size_t)j * n + i] + l1[i] * w_flat[w_off[j] + idx - 1] / (double)idx;
+    }
+}
+
+// henon.cu:2533-2584; init=1 stores the angles only
+__global__ void k_tune_birkhoff_add(const double *__restrict__ x, const double *__restrict__ px,
+                                    const double *__restrict__ y, const double *__restrict__ py,
+                                    double *__restrict__ store_x, double *__restrict__ store_y,
+                                    double *__restrict__ t1x, double *__restrict__ t2x, double *__restrict__ t1y,
+                                    double *__restrict__ t2y, const double *__restrict__ w_flat,
+                                    const uint64_t *__restrict__ w_off, const uint64_t *__restrict__ turn_samples,
+                                    uint32_t n_samples, uint32_t n, uint64_t idx, int init)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double ax = atan2(px[i], x[i]), ay = atan2(py[i], y[i]);
+    if (init) {
+        store_x[i] = ax;
+        store_y[i] = ay;
+        return;
+    }
+    double dx = ax - store_x[i], dy = ay - store_y[i];
+    dx = dx > 0 ? dx : dx + 2 * M_PI;
+    dy = dy > 0 ? dy : dy + 2 * M_PI;
+    store_x[i] = ax;
+    store_y[i] = ay;
+    for (uint32_t j = 0; j < n_samples; j++) {
+        const size_t o = (size_t)j * n + i;
+        const uint64_t T = turn_samples[j];
+        if (idx < T) {
+            t1x[o] = t1x[o] + dx * w_flat[w_off[j] + idx];
+            t1y[o] = t1y[o] + dy * w_flat[w_off[j] + idx];
+        } else if (idx < 2 * T) {
+            t2x[o] = t2x[o] + dx * w_flat[w_off[j] + idx - T];
+            t2y[o] = t2y[o] + dy * w_flat[w_off[j] + idx - T];
+        }
+    }
+}
+
+}  // namespace hm
+using namespace hm;
+
+namespace {
+inline uint32_t grid_of(size_t n, uint32_t block = 256) { return (uint32_t)((n + block - 1) / block); }
+
+void host_birkhoff_weights(uint32_t n, double *out)
+{
+    // src/dynamic_indicator.cpp:63-79
+    for (uint32_t i = 0; i < n; i++) {
+        double t = (double)i / (double)(n);
+        out[i] = std::exp(-1 / (t * (1.0 - t)));
+    }
+    double sum = 0.0;
+    for (uint32_t i = 0; i < n; i++) sum = sum + out[i];
+    for (uint32_t i = 0; i < n; i++) out[i] = out[i] / sum;
+}
+}  // namespace
+
+extern "C" int hm_megno_trace(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n,
+                              const double *table, size_t table_len, uint32_t n_turns, double mu, double barrier,
+                              const uint32_t *turn_samples, size_t n_samples, double *out, void *stream)
+{
+    return guarded([&] {
+        if (n_samples > (size_t)kMaxSamples) throw Error(HM_ERR_ARG, "megno: at most 16 turn samples");
+        for (size_t q = 0; q < n_samples; q++)
+            if (turn_samples[q] > n_turns) throw Error(HM_ERR_ARG, "megno: turn sample beyond n_turns");
+        if (n_turns > table_len) throw Error(HM_ERR_RANGE, "megno: the table is shorter than n_turns");
+        if (n == 0) return;
+        if (!x || !px || !y || !py || !steps || !table || !out) throw Error(HM_ERR_ARG, "megno: NULL pointer");
+        MegnoTraceArgs a{};
+        a.x = x; a.px = px; a.y = y; a.py = py; a.steps = steps; a.n = (uint32_t)n;
+        a.tab = reinterpret_cast<const TabEntry *>(table); a.tab_len = (uint32_t)table_len; a.n_turns = n_turns;
+        a.mu = mu; a.b2 = barrier * barrier;
+        for (size_t q = 0; q < n_samples; q++) a.samples[q] = turn_samples[q];
+        a.n_samples = (uint32_t)n_samples; a.out = out; a.status = status_word();
+        hipStream_t s = as_stream(stream);
+        LaunchTimer timer("k_megno_trace", s);
+        if (mu != 0.0) hipLaunchKernelGGL(k_megno_trace<true>, dim3(grid_of(n)), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(k_megno_trace<false>, dim3(grid_of(n)), dim3(256), 0, s, a);
+        launch_check("k_megno_trace");
+    });
+}
+
+extern "C" int hm_track_indicators(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n,
+                                   const double *table, size_t table_len, uint32_t n_turns, double mu,
+                                   double barrier, int mode, const hm_indicator_spec *spec, void *stream)
+{
+    return guarded([&] {
+        if (!spec) throw Error(HM_ERR_ARG, "indicators: spec is NULL");
+        if (mode != HM_MODE_EXACT && mode != HM_MODE_FAST) throw Error(HM_ERR_ARG, "indicators: unknown mode");
+        if (spec->n_samples > (uint32_t)kMaxSamples) throw Error(HM_ERR_ARG, "indicators: at most 16 samples");
+        if (spec->n_birkhoff > (uint32_t)kMaxBirk) throw Error(HM_ERR_ARG, "indicators: at most 4 Birkhoff windows");
+        for (uint32_t q = 0; q < spec->n_samples; q++) {
+            if (spec->samples[q] < 1 || spec->samples[q] > n_turns)
+                throw Error(HM_ERR_ARG, "indicators: samples must lie in 1..n_turns");
+            if (q && spec->samples[q] <= spec->samples[q - 1])
+                throw Error(HM_ERR_ARG, "indicators: samples must increase");
+        }
+        if (spec->n_samples && !spec->megno && !spec->fli)
+            throw Error(HM_ERR_ARG, "indicators: samples given without a megno or fli output");
+        for (uint32_t w = 0; w < spec->n_birkhoff; w++)
+            if (spec->birk_from[w] >= spec->birk_to[w] || spec->birk_to[w] > n_turns)
+                throw Error(HM_ERR_ARG, "indicators: Birkhoff window needs from < to <= n_turns");
+        if (spec->n_birkhoff && !spec->birk_out) throw Error(HM_ERR_ARG, "indicators: birk_out is NULL");
+        if (spec->hist_len && (!spec->hist || (uint64_t)spec->hist_from + spec->hist_len > (uint64_t)n_turns + 1))
+            throw Error(HM_ERR_ARG, "indicators: trajectory window must lie in samples 0..n_turns");
+        if (n == 0 || n_turns == 0) return;
+        if (!x || !px || !y || !py || !steps || !table) throw Error(HM_ERR_ARG, "indicators: NULL pointer");
+        hipStream_t s = as_stream(stream);
+        IndicatorArgs a{};
+        a.x = x; a.px = px; a.y = y; a.py = py; a.steps = steps; a.n = (uint32_t)n;
+        a.tab = reinterpret_cast<const TabEntry *>(table); a.tab_len = (uint32_t)table_len; a.n_turns = n_turns;
+        a.mu = mu; a.b2 = barrier * barrier; a.renorm_every = spec->renorm_every;
+        for (uint32_t q = 0; q < spec->n_samples; q++) a.samples[q] = spec->samples[q];
+        a.n_samples = spec->n_samples;
+        for (int q = 0; q < 4; q++) a.fli_dir[q] = spec->fli_dir[q];
+        if (a.fli_dir[0] == 0 && a.fli_dir[1] == 0 && a.fli_dir[2] == 0 && a.fli_dir[3] == 0) a.fli_dir[0] = 1.0;
+        a.megno = spec->megno; a.fli = spec->fli; a.matrix = spec->matrix; a.log_scale = spec->log_scale;
+        a.n_birk = spec->n_birkhoff; a.birk_out = spec->birk_out;
+        if (a.n_birk) {
+            size_t total = 0;
+            for (uint32_t w = 0; w < a.n_birk; w++) {
+                a.birk_from[w] = spec->birk_from[w];
+                a.birk_to[w] = spec->birk_to[w];
+                a.birk_off[w] = (uint32_t)total;
+                total += spec->birk_to[w] - spec->birk_from[w];
+            }
+            std::vector<double> w(total);
+            for (uint32_t q = 0; q < a.n_birk; q++)
+                host_birkhoff_weights(a.birk_to[q] - a.birk_from[q], w.data() + a.birk_off[q]);
+            double *dw = static_cast<double *>(workspace(total * sizeof(double), 1));
+            hip_check(hipMemcpyAsync(dw, w.data(), total * sizeof(double), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+            a.birk_w = dw;
+        }
+        a.hist_from = spec->hist_from; a.hist_len = spec->hist_len; a.hist = spec->hist;
+        a.status = status_word();
+        const bool oct = mu != 0.0;
+        LaunchTimer timer("k_indicators", s);
+        if (mode == HM_MODE_EXACT) {
+            if (oct) hipLaunchKernelGGL((k_indicators<kExact, true>), dim3(grid_of(n)), dim3(256), 0, s, a);
+            else hipLaunchKernelGGL((k_indicators<kExact, false>), dim3(grid_of(n)), dim3(256), 0, s, a);
+        } else {
+            if (oct) hipLaunchKernelGGL((k_indicators<kFast, true>), dim3(grid_of(n)), dim3(256), 0, s, a);
+            else hipLaunchKernelGGL((k_indicators<kFast, false>), dim3(grid_of(n)), dim3(256), 0, s, a);
+        }
+        launch_check("k_indicators");
+    });
+}
+
+extern "C" int hm_tangent_apply(double *mats, const double *x, const double *px, const double *y, const double *py,
+                                size_t n, const double *table, size_t table_len, uint32_t table_index, double mu,
+                                int multiply, void *stream)
+{
+    return guarded([&] {
+        if (table_index >= table_len) throw Error(HM_ERR_RANGE, "tangent: table index out of range");
+        if (n == 0) return;
+        hipStream_t s = as_stream(stream);
+        LaunchTimer timer("k_tangent_apply", s);
+        hipLaunchKernelGGL(k_tangent_apply, dim3(grid_of(n)), dim3(256), 0, s, mats, x, px, y, py, (uint32_t)n,
+                           reinterpret_cast<const TabEntry *>(table), table_index, mu, multiply);
+        launch_check("k_tangent_apply");
+    });
+}
+
+extern "C" int hm_matrix_left_multiply(double *mats, const double *left, size_t n, void *stream)
+{
+    return guarded([&] {
+        if (n == 0) return;
+        hipStream_t s = as_stream(stream);
+        hipLaunchKernelGGL(k_left_multiply, dim3(grid_of(n)), dim3(256), 0, s, mats, left, (uint32_t)n);
+        launch_check("k_left_multiply");
+    });
+}
+
+extern "C" int hm_matrix_identity(double *mats, size_t n, void *stream)
+{
+    return guarded([&] {
+        if (n == 0) return;
+        hipStream_t s = as_stream(stream);
+        hipLaunchKernelGGL(k_identity, dim3(grid_of(n)), dim3(256), 0, s, mats, (uint32_t)n);
+        launch_check("k_identity");
+    });
+}
+
+extern "C" int hm_matvec(const double *mats, const double *vecs, double *out, size_t n, int reference_order,
+                         void *stream)
+{
+    return guarded([&] {
+        if (n == 0) return;
+        hipStream_t s = as_stream(stream);
+        hipLaunchKernelGGL(k_matvec, dim3(grid_of(n)), dim3(256), 0, s, mats, vecs, out, (uint32_t)n, reference_order);
+        launch_check("k_matvec");
+    });
+}
+
+extern "C" int hm_normalize(double *vecs, size_t n, void *stream)
+{
+    return guarded([&] {
+        if (n == 0) return;
+        hipStream_t s = as_stream(stream);
+        hipLaunchKernelGGL(k_normalize, dim3(grid_of(n)), dim3(256), 0, s, vecs, (uint32_t)n);
+        launch_check("k_normalize");
+    });
+}
+
+extern "C" int hm_lyapunov_add(double *values, double *values_b, const double *vecs, const double *weights, size_t n,
+                               uint64_t n_weights, uint64_t idx, void *stream)
+{
+    return guarded([&] {
+        if (idx >= n_weights) throw Error(HM_ERR_RANGE, "lyapunov: index beyond the weights");
+        if (n == 0) return;
+        hipStream_t s = as_stream(stream);
+        hipLaunchKernelGGL(k_lyap_birk, dim3(grid_of(n)), dim3(256), 0, s, values, values_b, vecs, weights,
+                           (uint32_t)n, n_weights, idx);
+        launch_check("k_lyap_birk");
+    });
+}
+
+extern "C" int hm_megno_add(double *l1, double *l2, double *l3, const double *ma, const double *mb, size_t n,
+                            uint64_t idx, void *stream)
+{
+    return guarded([&] {
+        if (n == 0) return;
+        hipStream_t s = as_stream(stream);
+        hipLaunchKernelGGL(k_megno_add, dim3(grid_of(n)), dim3(256), 0, s, l1, l2, l3, ma, mb, (uint32_t)n, idx,
+                           (const double *)nullptr, (const uint64_t *)nullptr, (const uint64_t *)nullptr, 0u);
+        launch_check("k_megno_add");
+    });
+}
+
+extern "C" int hm_megno_birkhoff_add(double *l1, double *l2, const double *ma, const double *mb, size_t n,
+                                     uint64_t idx, const double *w_flat, const uint64_t *w_off,
+                                     const uint64_t *turn_samples, uint32_t n_samples, void *stream)
+{
+    return guarded([&] {
+        if (n == 0) return;
+        if (!w_flat || !w_off || !turn_samples) throw Error(HM_ERR_ARG, "megno_birkhoff: NULL tables");
+        hipStream_t s = as_stream(stream);
+        hipLaunchKernelGGL(k_megno_add, dim3(grid_of(n)), dim3(256), 0, s, l1, l2, (double *)nullptr, ma, mb,
+                           (uint32_t)n, idx, w_flat, w_off, turn_samples, n_samples);
+        launch_check("k_megno_add");
+    });
+}
+
+extern "C" int hm_tune_birkhoff_add(const double *x, const double *px, const double *y, const double *py,
+                                    double *store_x, double *store_y, double *t1x, double *t2x, double *t1y,
+                                    double *t2y, const double *w_flat, const uint64_t *w_off,
+                                    const uint64_t *turn_samples, uint32_t n_samples, size_t n, uint64_t idx,
+                                    int init, void *stream)
+{
+    return guarded([&] {
+        if (n == 0) return;
+        hipStream_t s = as_stream(stream);
+        hipLaunchKernelGGL(k_tune_birkhoff_add, dim3(grid_of(n)), dim3(256), 0, s, x, px, y, py, store_x, store_y,
+                           t1x, t2x, t1y, t2y, w_flat, w_off, turn_samples, n_samples, (uint32_t)n, idx, init);
+        launch_check("k_tune_birkhoff_add");
+    });
+}

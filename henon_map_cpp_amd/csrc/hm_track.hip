@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdint>
+#include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "hm_device.h"
@@ -18,7 +20,7 @@ constexpr uint32_t kFlagRange = 1u;
 // leaves the table: each lane reads its own table record, and a lane whose
 // index would leave [0, tab_len) stops unchanged and raises kFlagRange.
 template <int MODE, bool INV, bool OCT>
-__device__ __attribute__((noinline)) void track_lanes(double &x, double &px, double &y, double &py, bool &alive,
+__device__ __forceinline__ void track_lanes(double &x, double &px, double &y, double &py, bool &alive,
                                                       uint32_t &done, uint32_t s, const TabEntry *__restrict__ tab,
                                                       uint32_t tab_len, uint32_t n_turns, double mu, double b2,
                                                       uint32_t *status)
@@ -45,53 +47,93 @@ __device__ __attribute__((noinline)) void track_lanes(double &x, double &px, dou
     }
 }
 
-// One thread per particle (optionally through a live-index list), n_turns
-// turns in registers. The first live lane's turn index s0 is read into a
-// scalar register; when every live lane of the wavefront shares it (the normal
-// case: all particles of a tracker advance together), the table record of turn
-// s0+k is the same for the whole wavefront and is fetched with a scalar load,
-// one turn ahead, straight into SGPR operands of the VALU ops. Losses are kept
-// as a lane mask, so a turn costs the map's VALU ops plus one compare.
-template <int MODE, bool INV, bool OCT>
-__global__ void __launch_bounds__(256)
-k_track(double *__restrict__ gx, double *__restrict__ gpx, double *__restrict__ gy, double *__restrict__ gpy,
-        uint32_t *__restrict__ gsteps, const uint32_t *__restrict__ idx_in, uint32_t n,
-        const TabEntry *__restrict__ tab, uint32_t tab_len, uint32_t n_turns, double mu, double b2,
-        uint32_t *__restrict__ idx_out, uint32_t *__restrict__ count_out, uint32_t *__restrict__ status)
-{
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in_range = tid < n;
-    uint32_t j = 0;
-    uint32_t s = 0;
-    double x = __builtin_nan(""), px = x, y = x, py = x;
-    if (in_range) {
-        j = idx_in ? idx_in[tid] : tid;
-        x = gx[j];
-        px = gpx[j];
-        y = gy[j];
-        py = gpy[j];
-        s = gsteps[j];
-    }
-    // henon.cu:431-434: a particle with a NaN coordinate is not stepped
-    const bool alive0 = in_range && !any_nan(x, px, y, py);
-    bool alive = alive0;
-    const double m3mu = -3.0 * mu;
-    const uint64_t live = __ballot(alive);
-    if (live == 0) return;
+// P particles per lane, n_turns turns in registers. Wavefront w owns the 64*P
+// consecutive entries [64*P*w, 64*P*(w+1)) of the particle list (or of the
+// live-index list `idx_in`), lane l the entries 64*q + l. The particle count is
+// read from device memory (`count_in`) when given, so a launch can be sized
+// from an upper bound without a host round trip.
+//
+// The first live lane's turn index s0 is read into a scalar register; when every
+// live particle of the wavefront shares it (the normal case: all particles of a
+// tracker advance together), the table record of turn s0+k is the same for the
+// whole wavefront and is fetched with a scalar load, an iteration ahead, straight
+// into SGPR operands of the VALU ops. Losses are kept as wave-uniform lane masks,
+// so a turn costs the map's VALU ops plus one compare per particle.
+struct TrackArgs {
+    double *gx, *gpx, *gy, *gpy;
+    uint32_t *gsteps;
+    const uint32_t *idx_in, *count_in;
+    uint32_t n_arg;
+    uint32_t slots;  // wavefronts launched in mixed mode (0 = one particle per lane everywhere)
+    const TabEntry *tab;
+    uint32_t tab_len, n_turns;
+    double mu, b2;
+    uint32_t *idx_out, *count_out, *status;
+};
 
-    const int first = __builtin_ctzll(live);
-    const uint32_t s0 = __builtin_amdgcn_readlane(s, first);
-    const bool same = __all(!alive || s == s0);
+template <int MODE, bool INV, bool OCT, int P>
+__device__ __forceinline__ void track_wave(const TrackArgs &A, uint32_t base, uint32_t n)
+{
+    double *__restrict__ gx = A.gx, *__restrict__ gpx = A.gpx, *__restrict__ gy = A.gy, *__restrict__ gpy = A.gpy;
+    uint32_t *__restrict__ gsteps = A.gsteps;
+    const uint32_t *__restrict__ idx_in = A.idx_in;
+    const TabEntry *__restrict__ tab = A.tab;
+    const uint32_t tab_len = A.tab_len, n_turns = A.n_turns;
+    const double mu = A.mu, b2 = A.b2;
+    uint32_t *__restrict__ idx_out = A.idx_out;
+    uint32_t *__restrict__ count_out = A.count_out;
+    uint32_t *__restrict__ status = A.status;
+    const uint32_t lane = __lane_id();
+    double x[P], px[P], y[P], py[P];
+    uint32_t j[P], s[P], done[P];
+    bool alive0[P];
+    uint64_t live[P];
+    const uint64_t my_bit = 1ull << lane;
+    uint64_t any_live = 0;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const uint32_t i = base + 64u * q + lane;
+        const bool in_range = i < n;
+        j[q] = 0;
+        s[q] = 0;
+        x[q] = px[q] = y[q] = py[q] = __builtin_nan("");
+        if (in_range) {
+            j[q] = idx_in ? idx_in[i] : i;
+            x[q] = gx[j[q]];
+            px[q] = gpx[j[q]];
+            y[q] = gy[j[q]];
+            py[q] = gpy[j[q]];
+            s[q] = gsteps[j[q]];
+        }
+        // henon.cu:431-434: a particle with a NaN coordinate is not stepped
+        alive0[q] = in_range && !any_nan(x[q], px[q], y[q], py[q]);
+        live[q] = __ballot(alive0[q]);
+        any_live |= live[q];
+        done[q] = n_turns;
+    }
+    if (!any_live) return;
+
+    const double m3mu = -3.0 * mu;
+    // turn index of the first live particle (q-major), and whether all share it
+    int qf = 0;
+#pragma unroll
+    for (int q = P - 1; q >= 0; --q)
+        if (live[q]) qf = q;
+    uint32_t sf = 0;
+#pragma unroll
+    for (int q = 0; q < P; ++q)
+        if (q == qf) sf = s[q];
+    uint64_t lf = 0;
+#pragma unroll
+    for (int q = 0; q < P; ++q)
+        if (q == qf) lf = live[q];
+    const uint32_t s0 = __builtin_amdgcn_readlane(sf, __builtin_ctzll(lf));
+    bool same = true;
+#pragma unroll
+    for (int q = 0; q < P; ++q) same = same && __all(!alive0[q] || s[q] == s0);
     const bool in_table = INV ? (s0 >= n_turns) : ((uint64_t)s0 + n_turns <= (uint64_t)tab_len);
-    uint32_t done = n_turns;  // turns survived
 
     if (same && in_table && n_turns > 0) {
-        // `live` is a wave-uniform lane mask: the loss test is one v_cmp into an
-        // SGPR pair and the rest of the bookkeeping is scalar. A lane keeps
-        // computing after its loss (on values that no longer matter) and is
-        // set to NaN once, after the loop.
-        uint64_t live_mask = live;
-        const uint64_t my_bit = 1ull << __lane_id();
         const TabEntry *tp = INV ? tab + (s0 - 1) : tab + s0;
         // Two turns per iteration: the allocator alternates the state registers
         // instead of copying them back, and the records of the next two turns
@@ -100,86 +142,155 @@ k_track(double *__restrict__ gx, double *__restrict__ gpx, double *__restrict__ 
         const uint32_t last = n_turns - 1;
         TabEntry n0 = rec(0), n1 = rec(last < 1 ? last : 1);
         uint32_t k = 0;
+        bool more = true;
+        // exact forward mode carries x*x, y*y from one turn to the next
+        constexpr bool CARRY = MODE == kExact && !INV;
+        double xx[P], yy[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            xx[q] = x[q] * x[q];
+            yy[q] = y[q] * y[q];
+        }
+        auto turn = [&](const TabEntry &t, uint32_t kk) {
+            double r2[P];
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                if (CARRY) {
+                    r2[q] = fwd_exact_carry<OCT>(x[q], px[q], y[q], py[q], xx[q], yy[q], mu, t);
+                } else {
+                    map_step<MODE, INV, OCT>(x[q], px[q], y[q], py[q], mu, m3mu, t);
+                    r2[q] = radius2<MODE>(x[q], px[q], y[q], py[q]);
+                }
+            }
+            uint64_t all = 0;
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const uint64_t lost = live[q] & ~__ballot(r2[q] < b2);
+                if (lost) {
+                    if (lost & my_bit) done[q] = kk;
+                    live[q] &= ~lost;
+                }
+                all |= live[q];
+            }
+            more = all != 0;
+        };
         for (; k + 1 < n_turns; k += 2) {
             const TabEntry t0 = n0, t1 = n1;
             n0 = rec(k + 2 <= last ? k + 2 : last);
             n1 = rec(k + 3 <= last ? k + 3 : last);
-            map_step<MODE, INV, OCT>(x, px, y, py, mu, m3mu, t0);
-            uint64_t lost = live_mask & ~__ballot(radius2<MODE>(x, px, y, py) < b2);
-            if (lost) {
-                if (lost & my_bit) done = k;
-                live_mask &= ~lost;
-                if (!live_mask) break;
-            }
-            map_step<MODE, INV, OCT>(x, px, y, py, mu, m3mu, t1);
-            lost = live_mask & ~__ballot(radius2<MODE>(x, px, y, py) < b2);
-            if (lost) {
-                if (lost & my_bit) done = k + 1;
-                live_mask &= ~lost;
-                if (!live_mask) break;
-            }
+            turn(t0, k);
+            if (!more) break;
+            turn(t1, k + 1);
+            if (!more) break;
         }
-        if (k == last && live_mask) {
-            map_step<MODE, INV, OCT>(x, px, y, py, mu, m3mu, n0);
-            const uint64_t lost = live_mask & ~__ballot(radius2<MODE>(x, px, y, py) < b2);
-            if (lost & my_bit) done = k;
-            live_mask &= ~lost;
-        }
-        alive = (live_mask & my_bit) != 0;
-        if (alive0 && !alive) x = px = y = py = __builtin_nan("");
+        if (more && k == last) turn(n0, k);
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+            if (alive0[q] && !(live[q] & my_bit)) x[q] = px[q] = y[q] = py[q] = __builtin_nan("");
     } else if (n_turns > 0) {
-        track_lanes<MODE, INV, OCT>(x, px, y, py, alive, done, s, tab, tab_len, n_turns, mu, b2, status);
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            bool a = alive0[q];
+            track_lanes<MODE, INV, OCT>(x[q], px[q], y[q], py[q], a, done[q], s[q], tab, tab_len, n_turns, mu, b2,
+                                        status);
+            live[q] = __ballot(a);
+        }
     }
 
-    if (alive0) {
-        gx[j] = x;
-        gpx[j] = px;
-        gy[j] = y;
-        gpy[j] = py;
-        gsteps[j] = INV ? s - done : s + done;
-    }
-    if (idx_out) {
-        // wavefront-aggregated append of the survivors to the next live list
-        const uint64_t m = __ballot(alive);
-        if (m) {
-            const int leader = __builtin_ctzll(m);
-            uint32_t base = 0;
-            if ((int)__lane_id() == leader) base = atomicAdd(count_out, (uint32_t)__popcll(m));
-            base = __builtin_amdgcn_readlane(base, leader);
-            if (alive) {
-                const uint64_t below = m & ((1ull << __lane_id()) - 1ull);
-                idx_out[base + __popcll(below)] = j;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        if (alive0[q]) {
+            gx[j[q]] = x[q];
+            gpx[j[q]] = px[q];
+            gy[j[q]] = y[q];
+            gpy[j[q]] = py[q];
+            gsteps[j[q]] = INV ? s[q] - done[q] : s[q] + done[q];
+        }
+        if (idx_out) {
+            // wavefront-aggregated append of the survivors to the next live list
+            const uint64_t m = live[q];
+            if (m) {
+                const int leader = __builtin_ctzll(m);
+                uint32_t slot = 0;
+                if ((int)lane == leader) slot = atomicAdd(count_out, (uint32_t)__popcll(m));
+                slot = __builtin_amdgcn_readlane(slot, leader);
+                if (m & my_bit) idx_out[slot + __popcll(m & (my_bit - 1ull))] = j[q];
             }
         }
     }
 }
 
+// Wavefront w owns 64*P consecutive list entries. In mixed mode (slots > 0,
+// exactly `slots` wavefronts launched) the first w2 wavefronts take two particles
+// per lane so that n particles fit in one resident round of wavefronts:
+// w2 = ceil((n - 64 slots) / 64) when n exceeds 64 slots.
 template <int MODE, bool INV, bool OCT>
-static void launch_track(double *x, double *px, double *y, double *py, uint32_t *steps, const uint32_t *idx_in,
-                         uint32_t n, const TabEntry *tab, uint32_t tab_len, uint32_t n_turns, double mu, double b2,
-                         uint32_t *idx_out, uint32_t *count_out, int block, hipStream_t s)
+__global__ void __launch_bounds__(256, 8) k_track(TrackArgs A)
 {
-    const uint32_t grid = (n + block - 1) / block;
+    const uint32_t n = A.count_in ? *A.count_in : A.n_arg;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    uint32_t w2 = 0;
+    if (A.slots && n > 64u * A.slots) w2 = min((n - 64u * A.slots + 63u) / 64u, A.slots);
+    if (wave < w2) {
+        track_wave<MODE, INV, OCT, 2>(A, wave * 128u, n);
+    } else {
+        const uint32_t base = w2 * 128u + (wave - w2) * 64u;
+        if (base < n) track_wave<MODE, INV, OCT, 1>(A, base, n);
+    }
+}
+
+struct TrackLaunch {
+    double *x, *px, *y, *py;
+    uint32_t *steps;
+    const uint32_t *idx_in, *count_in;
+    uint32_t n_arg, grid_bound;  // exact count when count_in is NULL; upper bound of it otherwise
+    const TabEntry *tab;
+    uint32_t tab_len, n_turns;
+    double mu, b2;
+    uint32_t *idx_out, *count_out;
+    int block;
+    uint32_t slots;  // resident wavefront slots of the device (0 = plain one-per-lane launch)
+};
+
+template <int MODE, bool INV, bool OCT>
+static void launch_track(const TrackLaunch &a, hipStream_t s)
+{
+    TrackArgs A{a.x, a.px, a.y, a.py, a.steps, a.idx_in, a.count_in, a.n_arg, 0u, a.tab, a.tab_len, a.n_turns,
+                a.mu, a.b2, a.idx_out, a.count_out, status_word()};
+    uint64_t waves = ((uint64_t)a.grid_bound + 63) / 64;
+    // mixed mode when one particle per lane would need more than one round of
+    // resident wavefronts and two per lane suffice
+    if (a.slots && waves > a.slots && (uint64_t)a.grid_bound <= 128ull * a.slots) {
+        A.slots = a.slots;
+        waves = a.slots;
+    }
+    const uint64_t grid = (waves * 64 + a.block - 1) / a.block;
     if (grid == 0) return;
     LaunchTimer timer("k_track", s);
-    hipLaunchKernelGGL((k_track<MODE, INV, OCT>), dim3(grid), dim3(block), 0, s, x, px, y, py, steps, idx_in, n,
-                       tab, tab_len, n_turns, mu, b2, idx_out, count_out, status_word());
+    hipLaunchKernelGGL((k_track<MODE, INV, OCT>), dim3((uint32_t)grid), dim3(a.block), 0, s, A);
     launch_check("k_track");
 }
 
-using TrackFn = void (*)(double *, double *, double *, double *, uint32_t *, const uint32_t *, uint32_t,
-                         const TabEntry *, uint32_t, uint32_t, double, double, uint32_t *, uint32_t *, int,
-                         hipStream_t);
+using TrackFn = void (*)(const TrackLaunch &, hipStream_t);
 
 static TrackFn pick_track(int mode, bool inv, bool oct)
 {
-    static const TrackFn table[2][2][2] = {
-        {{launch_track<kExact, false, false>, launch_track<kExact, false, true>},
-         {launch_track<kExact, true, false>, launch_track<kExact, true, true>}},
-        {{launch_track<kFast, false, false>, launch_track<kFast, false, true>},
-         {launch_track<kFast, true, false>, launch_track<kFast, true, true>}},
-    };
-    return table[mode][inv ? 1 : 0][oct ? 1 : 0];
+    if (mode == kExact) {
+        if (inv) return oct ? launch_track<kExact, true, true> : launch_track<kExact, true, false>;
+        return oct ? launch_track<kExact, false, true> : launch_track<kExact, false, false>;
+    }
+    if (inv) return oct ? launch_track<kFast, true, true> : launch_track<kFast, true, false>;
+    return oct ? launch_track<kFast, false, true> : launch_track<kFast, false, false>;
+}
+
+// Resident wavefront slots: CUs x 4 SIMDs x 8 wavefronts (the kernel keeps
+// its VGPRs within 64 so that 8 fit per SIMD).
+static uint32_t resident_slots()
+{
+    int dev = 0, cus = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    return (uint32_t)cus * 32u;
 }
 
 // ---------------------------------------------------------------- observables
@@ -245,7 +356,25 @@ __global__ void k_tangent_matrix(const double *__restrict__ x, const double *__r
 using namespace hm;
 
 static constexpr int kDefaultBlock = 256;
-static constexpr int kDefaultCompactEvery = 0;  // compaction off until tuned on hardware
+static constexpr int kDefaultCompactEvery = 1000;  // turns between compactions
+static constexpr uint32_t kFirstSegment = 125;     // first segments are short: fast losses go early
+
+// Pinned host words receiving the per-segment live counts.
+static uint32_t *pinned_counts(size_t n)
+{
+    static std::mutex mu;
+    static uint32_t *buf = nullptr;
+    static size_t cap = 0;
+    std::lock_guard<std::mutex> lk(mu);
+    if (cap < n) {
+        if (buf) hip_check(hipHostFree(buf), "hipHostFree");
+        buf = nullptr;
+        cap = 0;
+        hip_check(hipHostMalloc((void **)&buf, n * sizeof(uint32_t), hipHostMallocDefault), "hipHostMalloc");
+        cap = n;
+    }
+    return buf;
+}
 
 extern "C" int hm_track(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n,
                         const double *table, size_t table_len, uint32_t n_turns, double mu, double barrier,
@@ -257,44 +386,83 @@ extern "C" int hm_track(double *x, double *px, double *y, double *py, uint32_t *
         if (mode != HM_MODE_EXACT && mode != HM_MODE_FAST) throw Error(HM_ERR_ARG, "track: unknown mode");
         if (n == 0 || n_turns == 0) return;
         if (!x || !px || !y || !py || !steps || !table) throw Error(HM_ERR_ARG, "track: NULL pointer");
-        if (n > 0xffffffffull || table_len > 0xffffffffull) throw Error(HM_ERR_ARG, "track: size exceeds 2^32");
-        int block = cfg && cfg->block ? cfg->block : kDefaultBlock;
-        if (block < 64 || block > 1024 || block % 64) throw Error(HM_ERR_ARG, "track: block must be 64..1024, /64");
-        if (block > 256) block = 256;  // kernel is compiled for at most 256 threads
-        int every = cfg && cfg->compact_every ? cfg->compact_every : kDefaultCompactEvery;
+        if (n > 0x7fffffffull || table_len > 0xffffffffull) throw Error(HM_ERR_ARG, "track: size exceeds 2^31");
+        const int block = cfg && cfg->block ? cfg->block : kDefaultBlock;
+        if (block < 64 || block > 256 || block % 64) throw Error(HM_ERR_ARG, "track: block must be 64, 128, 192 or 256");
+        const int every = cfg && cfg->compact_every ? cfg->compact_every : kDefaultCompactEvery;
+        const int pair_mode = cfg ? cfg->particles_per_lane : 0;  // 0: mixed 1/2 per lane, 1: always one
+        if (pair_mode < 0 || pair_mode > 2) throw Error(HM_ERR_ARG, "track: particles_per_lane must be 0, 1 or 2");
         hipStream_t s = as_stream(stream);
         const double b2 = barrier * barrier;  // henon.cu:2717, :3173
         const bool oct = mu != 0.0;           // henon.cu:445-448
-        TrackFn fn = pick_track(mode, inverse != 0, oct);
         const TabEntry *tab = reinterpret_cast<const TabEntry *>(table);
         const uint32_t N = (uint32_t)n;
 
-        if (every <= 0 || (uint32_t)every >= n_turns) {
-            fn(x, px, y, py, steps, nullptr, N, tab, (uint32_t)table_len, n_turns, mu, b2, nullptr, nullptr, block, s);
+        TrackLaunch a{};
+        a.x = x; a.px = px; a.y = y; a.py = py; a.steps = steps;
+        a.tab = tab; a.tab_len = (uint32_t)table_len; a.mu = mu; a.b2 = b2; a.block = block;
+        a.slots = pair_mode == 1 ? 0u : resident_slots();
+
+        if (every < 0 || (uint32_t)every >= n_turns) {
+            a.n_arg = a.grid_bound = N;
+            a.n_turns = n_turns;
+            pick_track(mode, inverse != 0, oct)(a, s);
             return;
         }
-        // Segments of `every` turns; survivors of each segment are compacted into
-        // a dense index list so later segments run on full wavefronts.
-        uint32_t *lists = static_cast<uint32_t *>(workspace(sizeof(uint32_t) * (2 * (size_t)N + 64), 0));
-        uint32_t *count = lists + 2 * (size_t)N;
-        uint32_t *cur = nullptr, *nxt = lists;
-        uint32_t n_cur = N, remaining = n_turns;
-        while (remaining > 0 && n_cur > 0) {
-            const uint32_t turns = remaining < (uint32_t)every ? remaining : (uint32_t)every;
-            const bool last = turns == remaining;
-            if (!last) hip_check(hipMemsetAsync(count, 0, sizeof(uint32_t), s), "hipMemsetAsync");
-            fn(x, px, y, py, steps, cur, n_cur, tab, (uint32_t)table_len, turns, mu, b2, last ? nullptr : nxt,
-               last ? nullptr : count, block, s);
-            remaining -= turns;
-            if (!last) {
-                uint32_t c = 0;
-                hip_check(hipMemcpyAsync(&c, count, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
-                hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-                n_cur = c;
-                cur = nxt;
-                nxt = (nxt == lists) ? lists + N : lists;
-            }
+        // Segments: kFirstSegment turns, doubling up to `every`. Survivors of a
+        // segment are appended to a dense live-index list that the next segment
+        // reads, with its length read on the device. The host sizes segment i's
+        // grid from the count of segment i-2 (an upper bound), so it never waits
+        // for the segment that feeds the launch.
+        // Segment lengths: kFirstSegment doubling up to `every`; past 16*every
+        // turns (losses have become rare) they keep doubling up to 8*every.
+        std::vector<uint32_t> seg;
+        for (uint32_t done = 0, len = std::min<uint32_t>(kFirstSegment, (uint32_t)every); done < n_turns;) {
+            const uint32_t t = std::min(len, n_turns - done);
+            seg.push_back(t);
+            done += t;
+            const uint64_t cap = (uint64_t)done >= 16ull * (uint64_t)every ? 8ull * (uint64_t)every : (uint64_t)every;
+            len = (uint32_t)std::min<uint64_t>((uint64_t)len * 2, std::min<uint64_t>(cap, 0x7fffffffull));
         }
+        const size_t S = seg.size();
+        uint32_t *lists = static_cast<uint32_t *>(workspace(sizeof(uint32_t) * (2 * (size_t)N + S + 64), 0));
+        uint32_t *counts = lists + 2 * (size_t)N;  // counts[i]: length of segment i's input list (i >= 1)
+        uint32_t *host = pinned_counts(S + 1);
+        hip_check(hipMemsetAsync(counts, 0, sizeof(uint32_t) * (S + 1), s), "hipMemsetAsync");
+        std::vector<hipEvent_t> ev(S, nullptr);
+        auto cleanup = [&] {
+            for (auto e : ev)
+                if (e) (void)hipEventDestroy(e);
+        };
+        try {
+            uint64_t bound = N;  // upper bound of counts[i]
+            for (size_t i = 0; i < S; ++i) {
+                if (i >= 2) {
+                    hip_check(hipEventSynchronize(ev[i - 2]), "hipEventSynchronize");
+                    bound = std::min<uint64_t>(bound, host[i - 1]);
+                    if (host[i - 1] == 0) break;  // every particle is lost or NaN
+                }
+                const bool last = i + 1 == S;
+                a.n_turns = seg[i];
+                a.idx_in = i == 0 ? nullptr : (i % 2 ? lists : lists + N);
+                a.count_in = i == 0 ? nullptr : counts + i;
+                a.n_arg = N;
+                a.grid_bound = (uint32_t)bound;
+                a.idx_out = last ? nullptr : (i % 2 ? lists + N : lists);
+                a.count_out = last ? nullptr : counts + i + 1;
+                pick_track(mode, inverse != 0, oct)(a, s);
+                if (!last) {
+                    hip_check(hipMemcpyAsync(host + i + 1, counts + i + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+                              "hipMemcpyAsync");
+                    hip_check(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming), "hipEventCreate");
+                    hip_check(hipEventRecord(ev[i], s), "hipEventRecord");
+                }
+            }
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
     });
 }
 

@@ -89,9 +89,12 @@ int hm_birkhoff_weights(uint32_t n_weights, double *out);
 /* ---- tracking ------------------------------------------------------------ */
 /* Launch configuration; zero fields take the tuned defaults. */
 typedef struct hm_launch_cfg {
-    int32_t block;          /* threads per workgroup (64..1024, multiple of 64) */
-    int32_t compact_every;  /* turns between loss compactions (0 = default, <0 = never) */
-    int32_t unroll;         /* reserved */
+    int32_t block;              /* threads per workgroup: 64, 128, 192 or 256 (0 = 256) */
+    int32_t compact_every;      /* turns between loss compactions (0 = 1000, <0 = never); segments
+                                   start at 125 turns, double up to this, and past 16x this
+                                   keep doubling up to 8x this */
+    int32_t particles_per_lane; /* 0 = one, or two for the leading wavefronts when the live
+                                   particles exceed one round of resident wavefronts; 1 = one */
     int32_t reserved;
 } hm_launch_cfg;
 
@@ -137,6 +140,110 @@ int hm_observable(int kind, const double *x, const double *px, const double *y, 
 int hm_tangent_matrix(const double *x, const double *px, const double *y, const double *py,
                       const uint8_t *valid, size_t n, const double *table, size_t table_len,
                       uint32_t table_index, double mu, int reverse, double *out, void *stream);
+
+/* ---- tangent map, MEGNO, FLI, Birkhoff ----------------------------------- */
+/* henon_tracker::megno (src/henon.cu:2755-2887): per turn k, J at the pre-step
+ * point with table index k, M <- J M, a_k = (k+1) log10(tr M_{k+1}/tr M_k), then
+ * the map step (table index steps); out[i][q] = S2[turn_samples[q]] /
+ * turn_samples[q] with S2 the running sum of (running sum of a_k)/(k+1); a sample
+ * equal to n_turns reads 0 (the reference's never-written slot). Every particle
+ * starts from the identity. At most 16 samples. out: device double[n][n_samples]. */
+int hm_megno_trace(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n, const double *table,
+                   size_t table_len, uint32_t n_turns, double mu, double barrier, const uint32_t *turn_samples,
+                   size_t n_samples, double *out, void *stream);
+
+#define HM_MAX_SAMPLES 16
+#define HM_MAX_BIRKHOFF 4
+/* What hm_track_indicators records; NULL outputs are skipped. */
+typedef struct hm_indicator_spec {
+    uint32_t renorm_every;             /* M <- M/|M|_F every k turns (0 = never) */
+    uint32_t n_samples;                /* turn counts at which MEGNO / FLI are recorded */
+    uint32_t samples[HM_MAX_SAMPLES];  /* 1..n_turns, increasing */
+    double fli_dir[4];                 /* FLI tangent vector (normalised; zero = (1,0,0,0)) */
+    double *megno;                     /* device [n_samples][n] */
+    double *fli;                       /* device [n_samples][n] */
+    double *matrix;                    /* device [n][16]: initial M in (identity if NULL), final M out */
+    double *log_scale;                 /* device [n]: log10 of the scale factored out of M, in/out */
+    uint32_t n_birkhoff;               /* Birkhoff tune windows over samples (from, to] */
+    uint32_t birk_from[HM_MAX_BIRKHOFF];
+    uint32_t birk_to[HM_MAX_BIRKHOFF];
+    double *birk_out;                  /* device [n_birkhoff][2][n]: tune x, tune y */
+    uint32_t hist_from, hist_len;      /* trajectory window: samples [hist_from, hist_from+hist_len) */
+    double *hist;                      /* device [hist_len][4][n] (x, px, y, py) */
+} hm_indicator_spec;
+
+/* Fused tracking + tangent map + indicators, one thread per particle with the
+ * 4x4 tangent matrix in registers (replaces the per-turn construct launches of
+ * src/henon.cu:267-359, 2223-2299, 2428-2655 driven from Python, and the CPU
+ * loops of :2755-2967). Per turn: J at the pre-step point (table index
+ * steps+k), M <- J M, map step, loss test; MEGNO in the construct form
+ * (src/henon.cu:2259-2283: L1 += log10(|M_k+1|_F/|M_k|_F) idx, L2 += L1/idx,
+ * value L2/idx); FLI = log10|M_t v0| / t; Birkhoff tunes as
+ * dynamic_indicator.cpp:81-131 over the (n_turns+1)-sample trajectory; the
+ * trajectory window feeds hm_fft_tunes. mode: HM_MODE_EXACT or HM_MODE_FAST. */
+int hm_track_indicators(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n,
+                        const double *table, size_t table_len, uint32_t n_turns, double mu, double barrier, int mode,
+                        const hm_indicator_spec *spec, void *stream);
+
+/* src/henon.cu:267-313 (get_matrices_and_multiply / _set): M <- J M (multiply=1)
+ * or M <- J (multiply=0), J at the current points with table index table_index. */
+int hm_tangent_apply(double *mats, const double *x, const double *px, const double *y, const double *py, size_t n,
+                     const double *table, size_t table_len, uint32_t table_index, double mu, int multiply,
+                     void *stream);
+/* src/henon.cu:1721-1741 (matrix_4d_vector::multiply): M <- L M per particle. */
+int hm_matrix_left_multiply(double *mats, const double *left, size_t n, void *stream);
+/* src/henon.cu:1708-1719, 1808-1822 (reset to the identity). */
+int hm_matrix_identity(double *mats, size_t n, void *stream);
+/* src/henon.cu:316-338 (matrix_vector_multiply, reference_order=1, out may equal
+ * vecs) and :1758-1787 (get_vector, reference_order=0). */
+int hm_matvec(const double *mats, const double *vecs, double *out, size_t n, int reference_order, void *stream);
+/* src/henon.cu:341-359 (normalize_vectors). */
+int hm_normalize(double *vecs, size_t n, void *stream);
+/* src/henon.cu:2083-2098 (kernel_lyap_birk). */
+int hm_lyapunov_add(double *values, double *values_b, const double *vecs, const double *weights, size_t n,
+                    uint64_t n_weights, uint64_t idx, void *stream);
+/* src/henon.cu:2253-2284 (kernel_megno). */
+int hm_megno_add(double *l1, double *l2, double *l3, const double *ma, const double *mb, size_t n, uint64_t idx,
+                 void *stream);
+/* src/henon.cu:2366-2402 (kernel_megno_birkhoff); l2: device [n_samples][n],
+ * weights of sample j at w_flat + w_off[j] (all device). */
+int hm_megno_birkhoff_add(double *l1, double *l2, const double *ma, const double *mb, size_t n, uint64_t idx,
+                          const double *w_flat, const uint64_t *w_off, const uint64_t *turn_samples,
+                          uint32_t n_samples, void *stream);
+/* src/henon.cu:2533-2584 (kernel_tune_birkhoff_init when init=1, else
+ * kernel_tune_birkhoff); tune arrays: device [n_samples][n]. */
+int hm_tune_birkhoff_add(const double *x, const double *px, const double *y, const double *py, double *store_x,
+                         double *store_y, double *t1x, double *t2x, double *t1y, double *t2y, const double *w_flat,
+                         const uint64_t *w_off, const uint64_t *turn_samples, uint32_t n_samples, size_t n,
+                         uint64_t idx, int init, void *stream);
+
+/* ---- stored trajectories and tunes --------------------------------------- */
+/* History layout: hist[sample][4][n] (x, px, y, py), sample 0 = the state before
+ * the first turn. */
+/* Track n_turns (exact arithmetic) writing samples 0..n_turns into hist
+ * (device double[n_turns+1][4][n]); the trajectory store of
+ * henon_tracker::birkhoff_tunes / all_tunes (src/henon.cu:2913-2943, 3004-3037). */
+int hm_track_history(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n, const double *table,
+                     size_t table_len, uint32_t n_turns, double mu, double barrier, int inverse, double *hist,
+                     void *stream);
+/* Per-coordinate means over n_samples samples, summed in sample order from 0.0
+ * (std::accumulate, src/henon.cu:3041-3051). means: device double[4][n]. */
+int hm_history_means(const double *hist, size_t n, uint32_t n_samples, double *means, void *stream);
+/* Birkhoff tunes (src/dynamic_indicator.cpp:81-131, 198-253) of the windows
+ * [from[w], to[w]] (inclusive sample indices) of hist, after subtracting means
+ * (device double[4][n], or NULL); NaN when a sample of the window is NaN.
+ * from, to: host arrays. out: device double[n_windows][2][n] (tune x, tune y). */
+int hm_birkhoff_tunes_history(const double *hist, size_t n, uint32_t n_samples, const double *means,
+                              const uint32_t *from, const uint32_t *to, size_t n_windows, double *out, void *stream);
+/* FFT tunes (src/dynamic_indicator.cpp:32-61, 134-196, 255-331): Hanning window
+ * 2 sin^2(pi i/N), forward DFT of (x-mean) + i (px-mean), magnitude, first
+ * maximum, three-point interpolation, |1 - (k/N + asin(A sin(2pi/N))/2pi)|; 1.0 /
+ * 0.0 at the spectrum's edges; NaN when a sample of the window is NaN. Windows
+ * start at sample from[w] and have len[w] samples (host arrays). Power-of-two
+ * lengths up to 8192 use a radix-2 FFT in LDS, other lengths up to 16384 a
+ * direct DFT. out: device double[n_windows][2][n]. */
+int hm_fft_tunes(const double *hist, size_t n, uint32_t n_samples, const double *means, const uint32_t *from,
+                 const uint32_t *len, size_t n_windows, double *out, void *stream);
 
 /* ---- instrumentation ----------------------------------------------------- */
 /* When enabled, every kernel launch of the library is bracketed by HIP events

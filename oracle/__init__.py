@@ -77,6 +77,8 @@ def lib():
             "or_normalize": (None, [P, SZ]),
             "or_nan_mean": (D, [P, SZ]),
             "or_nan_std": (D, [P, SZ]),
+            "or_indicators": (I, [P, P, P, P, P, SZ, P, P, P, P, SZ, U, D, D, U, P, SZ, P, P, P, P, P, P, P, SZ,
+                                  P, I]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -397,3 +399,28 @@ def normalize(v):
     v = _f64(v).copy()
     lib().or_normalize(_p(v), v.shape[0])
     return v
+
+
+# ---------------------------------------------------------------- fused indicators
+def indicators(p: Particles, tables: Tables, n_turns, mu, barrier=10.0, renorm_every=0, samples=(),
+               fli_dir=(1.0, 0.0, 0.0, 0.0), matrix=None, log_scale=None, birkhoff=(), n_threads=-1):
+    """CPU restatement of the fused indicator kernel (exact mode); returns a dict
+    of megno, fli [n_samples][n], matrix [n][4][4], log_scale [n], birkhoff
+    [n_windows][2][n]. Advances p like track()."""
+    n = p.n
+    s = _u32(samples)
+    megno = np.empty((s.size, n))
+    fli = np.empty((s.size, n))
+    m = (np.tile(np.eye(4), (n, 1, 1)) if matrix is None else np.array(matrix, dtype=np.float64)).copy()
+    ls = np.zeros(n) if log_scale is None else np.array(log_scale, dtype=np.float64).copy()
+    bf = _u32([w[0] for w in birkhoff])
+    bt = _u32([w[1] for w in birkhoff])
+    bout = np.empty((bf.size, 2, n))
+    fd = _f64(fli_dir)
+    rc = lib().or_indicators(*p.state(), n, *tables.ptrs(), tables.allowed_steps, n_turns, mu, barrier,
+                             renorm_every, _p(s), s.size, _p(fd), _p(megno), _p(fli), _p(m), _p(ls), _p(bf),
+                             _p(bt), bf.size, _p(bout), n_threads)
+    if rc:
+        raise RuntimeError("table index out of range")
+    p.global_steps += n_turns
+    return dict(megno=megno, fli=fli, matrix=m, log_scale=ls, birkhoff=bout)

@@ -706,3 +706,147 @@ double or_nan_std(const double *v, size_t n)
 }
 
 }  // extern "C"
+
+// ---- fused indicators (new algorithm: no reference counterpart) ---------------
+// CPU restatement of the fused kernel's exact mode. Per turn k (table index
+// steps+k): J at the pre-step point, M <- J M with J's zero entries skipped in
+// a fixed order, map step and loss test, then the construct-form MEGNO of
+// henon.cu:2253-2284 (L1 += log10(|M_k+1|_F / |M_k|_F) idx, L2 += L1 / idx),
+// FLI = (log10 scale + log10 |M v0|) / t, scalar renormalisation every R turns,
+// and online Birkhoff sums as dynamic_indicator.cpp:81-131.
+// Pinned by: the reference construct formulas applied turn by turn to
+// reference-order products (tests/test_oracle_indicators.py), closed forms on
+// rotations, and renormalisation invariance.
+static void mat_mul_structured(const double *j, const double *t, double *m)
+{
+    // j: a, b, an ; t: sx, cx, sy, cy
+    const double sx = t[0], cx = t[1], sy = t[2], cy = t[3];
+    const double j00 = cx + (sx * j[0]), j02 = sx * j[1];
+    const double j10 = -sx + (cx * j[0]), j12 = cx * j[1];
+    const double j20 = sy * j[1], j22 = cy + (sy * j[2]);
+    const double j30 = cy * j[1], j32 = -sy + (cy * j[2]);
+    double r[16];
+    for (int c = 0; c < 4; c++) {
+        const double m0 = m[c], m1 = m[4 + c], m2 = m[8 + c], m3 = m[12 + c];
+        r[c] = ((j00 * m0) + (sx * m1)) + (j02 * m2);
+        r[4 + c] = ((j10 * m0) + (cx * m1)) + (j12 * m2);
+        r[8 + c] = ((j20 * m0) + (j22 * m2)) + (sy * m3);
+        r[12 + c] = ((j30 * m0) + (j32 * m2)) + (cy * m3);
+    }
+    std::memcpy(m, r, sizeof(r));
+}
+
+static double frob2(const double *m)
+{
+    double s = m[0] * m[0];
+    for (int q = 1; q < 16; q++) s = s + (m[q] * m[q]);
+    return s;
+}
+
+extern "C" int or_indicators(double *x, double *px, double *y, double *py, unsigned *steps, size_t n,
+                             const double *sx, const double *cx, const double *sy, const double *cy, size_t tab_len,
+                             unsigned n_turns, double mu, double barrier, unsigned renorm_every,
+                             const unsigned *samples, size_t n_samples, const double *fli_dir, double *megno,
+                             double *fli, double *matrix, double *log_scale, const unsigned *birk_from,
+                             const unsigned *birk_to, size_t n_birk, double *birk_out, int n_threads)
+{
+    const double b2 = barrier * barrier;
+    const double mu3 = mu * 3, mu6 = mu * 6;
+    std::vector<std::vector<double>> w(n_birk);
+    for (size_t q = 0; q < n_birk; q++) {
+        w[q].resize(birk_to[q] - birk_from[q]);
+        or_birkhoff_weights(birk_to[q] - birk_from[q], w[q].data());
+    }
+    double v0[4];
+    {
+        const double nn = std::sqrt(((fli_dir[0] * fli_dir[0] + fli_dir[1] * fli_dir[1]) + fli_dir[2] * fli_dir[2]) +
+                                    fli_dir[3] * fli_dir[3]);
+        for (int q = 0; q < 4; q++) v0[q] = fli_dir[q] / nn;
+    }
+    int bad = 0;
+    strided_parallel(n, resolve_threads(n_threads), [&](size_t i, unsigned) {
+        double m[16];
+        if (matrix) std::memcpy(m, matrix + i * 16, sizeof(m));
+        else for (int q = 0; q < 16; q++) m[q] = (q % 5 == 0) ? 1.0 : 0.0;
+        double lscale = log_scale ? log_scale[i] : 0.0;
+        double fb = std::sqrt(frob2(m));
+        double l1 = 0.0, l2 = 0.0;
+        bool alive = !check_nan(x[i], px[i], y[i], py[i]);
+        std::vector<double> bacc(2 * n_birk, 0.0);
+        double thx = std::atan2(px[i], x[i]), thy = std::atan2(py[i], y[i]);
+        size_t sq = 0;
+        unsigned s = steps[i];
+        for (unsigned k = 0; k < n_turns; k++) {
+            const size_t ti = (size_t)s + k;
+            if (ti >= tab_len) { bad = 1; break; }
+            const double t[4] = {sx[ti], cx[ti], sy[ti], cy[ti]};
+            const double xv = x[i], yv = y[i];
+            const double jc[3] = {((2.0 * xv) + ((mu3 * xv) * xv)) - ((mu3 * yv) * yv),
+                                  (-2.0 * yv) - ((mu6 * xv) * yv),
+                                  ((-2.0 * xv) - ((mu3 * xv) * xv)) + ((mu3 * yv) * yv)};
+            mat_mul_structured(jc, t, m);
+            if (alive) {
+                sextupole(x[i], px[i], y[i], py[i], false);
+                if (mu != 0.0) octupole(x[i], px[i], y[i], py[i], mu, false);
+                rotation(x[i], px[i], t[0], t[1], false);
+                rotation(y[i], py[i], t[2], t[3], false);
+                if (!check_barrier(x[i], px[i], y[i], py[i], b2)) {
+                    const double qn = std::numeric_limits<double>::quiet_NaN();
+                    x[i] = px[i] = y[i] = py[i] = qn;
+                    for (int q = 0; q < 16; q++) m[q] = qn;
+                    alive = false;
+                    steps[i] = s + k;
+                }
+            }
+            const unsigned idx = k + 1;
+            const double fa = std::sqrt(frob2(m));
+            l1 = l1 + std::log10(fa / fb) * (double)idx;
+            l2 = l2 + (1.0 / (double)idx) * l1;
+            fb = fa;
+            if (sq < n_samples && samples[sq] == idx) {
+                if (megno) megno[sq * n + i] = (1.0 / (double)idx) * l2;
+                if (fli) {
+                    double v[4];
+                    for (int r = 0; r < 4; r++)
+                        v[r] = (((m[r * 4] * v0[0]) + (m[r * 4 + 1] * v0[1])) + (m[r * 4 + 2] * v0[2])) +
+                               (m[r * 4 + 3] * v0[3]);
+                    const double nv = std::sqrt((((v[0] * v[0]) + (v[1] * v[1])) + (v[2] * v[2])) + (v[3] * v[3]));
+                    fli[sq * n + i] = (lscale + std::log10(nv)) / (double)idx;
+                }
+                ++sq;
+            }
+            if (renorm_every && idx % renorm_every == 0 && alive) {
+                const double inv = 1.0 / fa;
+                for (int q = 0; q < 16; q++) m[q] = m[q] * inv;
+                lscale = lscale + std::log10(fa);
+                fb = std::sqrt(frob2(m));
+            }
+            if (n_birk) {
+                const double nx = std::atan2(px[i], x[i]), ny = std::atan2(py[i], y[i]);
+                double dx = nx - thx, dy = ny - thy;
+                if (dx < 0.0) dx += 2.0 * M_PI;
+                if (dy < 0.0) dy += 2.0 * M_PI;
+                thx = nx; thy = ny;
+                for (size_t q = 0; q < n_birk; q++) {
+                    if (idx > birk_from[q] && idx <= birk_to[q]) {
+                        const double wt = w[q][idx - 1 - birk_from[q]];
+                        bacc[2 * q] = bacc[2 * q] + wt * dx;
+                        bacc[2 * q + 1] = bacc[2 * q + 1] + wt * dy;
+                    }
+                }
+            }
+        }
+        if (alive) steps[i] = s + n_turns;
+        for (; sq < n_samples; ++sq) {
+            if (megno) megno[sq * n + i] = std::numeric_limits<double>::quiet_NaN();
+            if (fli) fli[sq * n + i] = std::numeric_limits<double>::quiet_NaN();
+        }
+        if (matrix) std::memcpy(matrix + i * 16, m, sizeof(m));
+        if (log_scale) log_scale[i] = lscale;
+        for (size_t q = 0; q < n_birk; q++) {
+            birk_out[(2 * q) * n + i] = 1.0 - bacc[2 * q] / (2 * M_PI);
+            birk_out[(2 * q + 1) * n + i] = 1.0 - bacc[2 * q + 1] / (2 * M_PI);
+        }
+    });
+    return bad;
+}

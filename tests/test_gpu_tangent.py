@@ -1,0 +1,121 @@
+"""GPU parity of the tangent-map kernels through the C ABI: the reference's
+trace-form MEGNO, the fused indicator kernel (exact mode: state and tangent
+matrix bit-exact, logarithm-based outputs within 1e-13), fast mode, and the
+per-call construct kernels."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from henon_map_cpp_amd import _capi as C
+from henon_map_cpp_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+NT = min(os.cpu_count() or 1, 16)
+
+
+def tables(n, tx, ty, eps, kind="sps"):
+    if kind == "custom":
+        coef, freq = W.seeded_line_coefficients(), W.line_frequencies()
+        wx = C.lines_modulation(tx, eps, coef, freq, 0, n)
+        wy = C.lines_modulation(ty, eps, coef, freq, 0, n)
+    else:
+        wx, wy = C.sps_modulation(tx, eps, 0, n), C.sps_modulation(ty, eps, 0, n)
+    tab = C.build_table(wx, wy)
+    return O.Tables(wx, wy), tab, C.DeviceBuffer.from_array(tab)
+
+
+@pytest.mark.parametrize("mu", [0.0, 0.5])
+def test_megno_trace_matches_reference_cpu_megno(mu):
+    ot, tab, tdev = tables(600, 0.01, 0.012, 4.0)
+    x, px, y, py = W.ball(700, 0.3, seed=21)
+    samples = [0, 5, 100, 599, 600]
+    dev = C.DeviceParticles(x, px, y, py)
+    got = C.megno_trace(dev, tdev, 600, 600, mu, 10.0, samples)
+    p = O.Particles(x, px, y, py)
+    want = O.Tracker.from_tables(ot).megno(p, 600, mu, barrier=10.0, turn_samples=samples, n_threads=NT,
+                                           carry_off_diagonal=False)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-14, equal_nan=True)
+    assert np.all(got[:, -1] == 0.0)  # sample n_turns reads the never-written slot
+    gx, gpx, gy, gpy, gs = dev.get()
+    assert np.array_equal(gx, p.x, equal_nan=True) and np.array_equal(gs, p.steps)
+
+
+@pytest.mark.parametrize("mu,renorm", [(0.5, 0), (0.5, 100), (0.0, 50)])
+def test_indicators_exact_mode(mu, renorm):
+    ot, tab, tdev = tables(3000, 0.168, 0.201, 32.0, "custom")
+    x, px, y, py = W.ball(2000, 0.5, seed=22)
+    samples = [1, 10, 1000, 3000]
+    birk = [(0, 3000), (100, 2148)]
+    dev = C.DeviceParticles(x, px, y, py)
+    got = C.track_indicators(dev, tdev, 3000, 3000, mu, 10.0, C.MODE_EXACT, renorm, samples, (1, 1, 0, 0),
+                             matrix=True, birkhoff=birk)
+    p = O.Particles(x, px, y, py)
+    want = O.indicators(p, ot, 3000, mu, 10.0, renorm, samples, (1, 1, 0, 0), birkhoff=birk, n_threads=NT)
+    gx, gpx, gy, gpy, gs = dev.get()
+    # the map and the tangent matrix carry no transcendental: bit for bit
+    assert np.array_equal(gx, p.x, equal_nan=True) and np.array_equal(gs, p.steps)
+    if renorm == 0:
+        assert np.array_equal(got["matrix"], want["matrix"], equal_nan=True)
+    else:  # the scale factor goes through log10 only in log_scale; M itself is exact
+        assert np.array_equal(got["matrix"], want["matrix"], equal_nan=True)
+        np.testing.assert_allclose(got["log_scale"], want["log_scale"], rtol=1e-13, equal_nan=True)
+    for key in ("megno", "fli"):
+        np.testing.assert_allclose(got[key], want[key], rtol=1e-12, atol=1e-13, equal_nan=True)
+    np.testing.assert_allclose(got["birkhoff"], want["birkhoff"], rtol=0, atol=1e-13, equal_nan=True)
+
+
+def test_indicators_fast_mode_regular_orbits():
+    ot, tab, tdev = tables(2000, 0.168, 0.201, 32.0, "custom")
+    x, px, y, py = W.ball(1024, 0.15, seed=23)
+    samples = [100, 2000]
+    a = C.track_indicators(C.DeviceParticles(x, px, y, py), tdev, 2000, 2000, 0.5, 10.0, C.MODE_EXACT, 100,
+                           samples, birkhoff=[(0, 2000)])
+    b = C.track_indicators(C.DeviceParticles(x, px, y, py), tdev, 2000, 2000, 0.5, 10.0, C.MODE_FAST, 100,
+                           samples, birkhoff=[(0, 2000)])
+    np.testing.assert_allclose(b["megno"], a["megno"], rtol=1e-9, atol=1e-10)
+    np.testing.assert_allclose(b["fli"], a["fli"], rtol=1e-9, atol=1e-10)
+    np.testing.assert_allclose(b["birkhoff"], a["birkhoff"], atol=1e-11)
+
+
+def test_history_window():
+    ot, tab, tdev = tables(500, 0.31, 0.32, 16.0)
+    x, px, y, py = W.ball(333, 0.4, seed=24)
+    got = C.track_indicators(C.DeviceParticles(x, px, y, py), tdev, 500, 500, 0.5, 10.0, hist=(0, 501))
+    p = O.Particles(x, px, y, py)
+    t = O.Tracker.from_tables(ot)
+    h = [np.stack([p.x, p.px, p.y, p.py])]
+    for _ in range(500):
+        t.track(p, 1, 0.5, barrier=10.0)
+        h.append(np.stack([p.x, p.px, p.y, p.py]))
+    assert np.array_equal(got["hist"], np.array(h), equal_nan=True)
+
+
+def test_construct_kernels_match_reference_formulas():
+    ot, tab, tdev = tables(50, 0.168, 0.201, 0.0)
+    n = 300
+    x, px, y, py = W.ball(n, 0.3, seed=25)
+    dev = C.DeviceParticles(x, px, y, py)
+    mats = C.DeviceBuffer.from_array(np.tile(np.eye(4), (n, 1, 1)))
+    C.check(C.lib().hm_tangent_apply(mats.ptr, *dev.ptrs()[:4], n, tdev.ptr, 50, 3, 0.2, 1, None))
+    C.check(C.lib().hm_tangent_apply(mats.ptr, *dev.ptrs()[:4], n, tdev.ptr, 50, 4, 0.2, 1, None))
+    got = mats.download(np.float64, n * 16).reshape(n, 16)
+    want = O.structured_multiply(np.tile(np.eye(4), (n, 1, 1)).reshape(n, 16), x, px, y, py, ot, 3, 0.2)
+    want = O.structured_multiply(want, x, px, y, py, ot, 4, 0.2)
+    assert np.array_equal(got, want)
+    # matvec (in place, reference order) and normalize
+    v = np.random.default_rng(1).normal(size=(n, 4))
+    vd = C.DeviceBuffer.from_array(v)
+    C.check(C.lib().hm_matvec(mats.ptr, vd.ptr, vd.ptr, n, 1, None))
+    assert np.array_equal(vd.download(np.float64, 4 * n).reshape(n, 4), O.matvec(want, v))
+    C.check(C.lib().hm_normalize(vd.ptr, n, None))
+    np.testing.assert_allclose(vd.download(np.float64, 4 * n).reshape(n, 4), O.normalize(O.matvec(want, v)),
+                               rtol=2e-16, atol=1e-300)
+    # megno construct: add(M2, M1) with idx 1
+    l = [C.DeviceBuffer.from_array(np.zeros(n)) for _ in range(3)]
+    m1 = C.DeviceBuffer.from_array(np.tile(np.eye(4), (n, 1, 1)))
+    C.check(C.lib().hm_megno_add(l[0].ptr, l[1].ptr, l[2].ptr, mats.ptr, m1.ptr, n, 1, None))
+    r = [np.zeros(n) for _ in range(3)]
+    O.megno_construct_add(r[0], r[1], r[2], want, np.tile(np.eye(4), (n, 1, 1)).reshape(n, 16), 1)
+    np.testing.assert_allclose(l[2].download(np.float64, n), r[2], rtol=1e-14)

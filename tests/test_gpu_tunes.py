@@ -1,0 +1,106 @@
+"""GPU tunes through the C ABI: stored trajectories (bit-exact), Birkhoff and
+FFT tunes of windows against the oracle, and the reference notebook's printed
+all_tunes values."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from henon_map_cpp_amd import _capi as C
+from henon_map_cpp_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+NT = min(os.cpu_count() or 1, 16)
+
+
+def setup(n_turns, tx, ty, eps, x, px, y, py, mu, barrier=10.0):
+    wx, wy = C.sps_modulation(tx, eps, 0, n_turns), C.sps_modulation(ty, eps, 0, n_turns)
+    tab = C.build_table(wx, wy)
+    tdev = C.DeviceBuffer.from_array(tab)
+    dev = C.DeviceParticles(x, px, y, py)
+    hist = C.track_history(dev, tdev, n_turns, n_turns, mu, barrier)
+    return O.Tables(wx, wy), dev, hist
+
+
+def oracle_history(tables, x, px, y, py, n_turns, mu, barrier=10.0):
+    p = O.Particles(x, px, y, py)
+    t = O.Tracker.from_tables(tables)
+    h = [np.stack([p.x, p.px, p.y, p.py])]
+    for _ in range(n_turns):
+        t.track(p, 1, mu, barrier=barrier, n_threads=NT)
+        h.append(np.stack([p.x, p.px, p.y, p.py]))
+    return np.array(h), p
+
+
+def test_history_bit_exact():
+    x, px, y, py = W.ball(500, 0.45, seed=31)
+    tables, dev, hist = setup(700, 0.31, 0.32, 16.0, x, px, y, py, 0.5)
+    h, p = oracle_history(tables, x, px, y, py, 700, 0.5)
+    got = hist.download(np.float64, 701 * 4 * 500).reshape(701, 4, 500)
+    assert np.array_equal(got, h, equal_nan=True)
+    assert np.array_equal(dev.get()[4], p.steps)
+
+
+@pytest.mark.parametrize("mean", [False, True])
+def test_window_tunes_match_oracle(mean):
+    n_turns = 2100
+    x, px, y, py = W.ball(300, 0.3, seed=32)
+    tables, dev, hist = setup(n_turns, 0.168, 0.201, 8.0, x, px, y, py, 0.2)
+    h, _ = oracle_history(tables, x, px, y, py, n_turns, 0.2)
+    if mean:
+        h = h - h.mean(axis=0, keepdims=True)  # replaced below by the oracle's sequential mean
+        hh, _ = oracle_history(tables, x, px, y, py, n_turns, 0.2)
+        s = np.zeros((4, 300))
+        for m in range(n_turns + 1):
+            s = s + hh[m]
+        h = hh - (s / (n_turns + 1))[None]
+    windows = [(0, 2047), (52, 2099), (0, 500), (1000, 2100), (7, 1030)]  # 2048, 2048, 501, 1101, 1024
+    fft = C.history_tunes(hist, 300, n_turns + 1, windows, "fft", subtract_mean=mean)
+    birk = C.history_tunes(hist, 300, n_turns + 1, windows, "birkhoff", subtract_mean=mean)
+    for i in range(300):
+        want_b = O.birkhoff_tune_vec(h[:, 0, i], h[:, 1, i], h[:, 2, i], h[:, 3, i], [w[0] for w in windows],
+                                     [w[1] for w in windows])
+        np.testing.assert_allclose(birk[:, :, i].ravel(), want_b[:-2], rtol=0, atol=2e-15, equal_nan=True)
+        for q, (a, b) in enumerate(windows):
+            if np.any(np.isnan(h[a:b + 1, :, i])):
+                assert np.all(np.isnan(fft[q, :, i]))
+                continue
+            want = [O.fft_tune(h[a:b + 1, 0, i], h[a:b + 1, 1, i]), O.fft_tune(h[a:b + 1, 2, i], h[a:b + 1, 3, i])]
+            np.testing.assert_allclose(fft[q, :, i], want, rtol=0, atol=1e-12)
+
+
+def test_all_tunes_golden_on_gpu(golden):
+    """tests/new_tune.ipynb: (0.1, 0, 0.1, 0), 1e4 turns, sps tunes .168/.201, mu = 0:
+    the full-signal FFT tune (samples 1..n, N = 10000: direct DFT kernel), the
+    FFT tune of the 10001-sample window and the Birkhoff tunes, after the
+    per-coordinate mean removal of henon.cu:3041-3051."""
+    e = golden["new_tune_all_tunes"]["exact"]
+    x = np.array([0.1, 0.1])
+    z = np.zeros(2)
+    tables, dev, hist = setup(10000, 0.168, 0.201, 0.0, x, z, x, z, 0.0)
+    windows = [(0, 500), (500, 1000), (0, 10000), (1, 10000)]
+    fft = C.history_tunes(hist, 2, 10001, windows, "fft", subtract_mean=True)
+    birk = C.history_tunes(hist, 2, 10001, windows, "birkhoff", subtract_mean=True)
+    assert fft[3, 0, 0] == pytest.approx(e["fft_full_x"], abs=1e-14)
+    assert fft[3, 1, 0] == pytest.approx(e["fft_full_y"], abs=1e-14)
+    assert fft[2, 0, 0] == pytest.approx(e["fft_window_0_10000_x"], abs=1e-14)
+    assert fft[2, 1, 0] == pytest.approx(e["fft_window_0_10000_y"], abs=1e-14)
+    assert birk[0, 0, 0] == pytest.approx(e["birkhoff_window_0_500_x"], abs=1e-14)
+    assert birk[1, 0, 0] == pytest.approx(e["birkhoff_window_500_1000_x"], abs=1e-14)
+    assert birk[2, 0, 0] == pytest.approx(e["birkhoff_full_x"], abs=1e-14)
+
+
+def test_single_tone_pow2_and_edges():
+    # analytic tone z_m = exp(2 pi i f m): peak at f N, interpolation returns |1 - f|
+    for N, f in [(1024, 0.2371), (2048, 0.0913), (777, 0.3333)]:
+        m = np.arange(N)
+        z = np.exp(2j * np.pi * f * m) * 0.01
+        h = np.zeros((N, 4, 1))
+        h[:, 0, 0], h[:, 1, 0] = z.real, z.imag
+        h[:, 2, 0], h[:, 3, 0] = z.real, -z.imag
+        d = C.DeviceBuffer.from_array(np.ascontiguousarray(h))
+        got = C.history_tunes(d, 1, N, [(0, N - 1)], "fft")
+        want = [O.fft_tune(h[:, 0, 0], h[:, 1, 0]), O.fft_tune(h[:, 2, 0], h[:, 3, 0])]
+        np.testing.assert_allclose(got[0, :, 0], want, atol=1e-12)
+        assert got[0, 0, 0] == pytest.approx(1 - f, abs=1e-6)
