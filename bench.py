@@ -43,7 +43,24 @@ def parse():
     ap.add_argument("--cpu-sample-stride", type=int, default=13)
     ap.add_argument("--single-mode", action="store_true", help="time only --mode (profiling runs)")
     ap.add_argument("--compact-every", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run `--steps` device-to-device copies of MIB MiB (the known-size copy that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters), then exit")
     return ap.parse_args()
+
+
+def copy_calibration(mib, steps):
+    """Plain device-to-device copies of a known size, for counter calibration."""
+    from henon_map_cpp_amd import _capi as C
+
+    nbytes = mib << 20
+    src, dst = C.DeviceBuffer(nbytes), C.DeviceBuffer(nbytes)
+    C.check(C.lib().hm_memset(src.ptr, 1, nbytes, None))
+    C.check(C.lib().hm_stream_synchronize(None))
+    for _ in range(steps):
+        C.check(C.lib().hm_memcpy(dst.ptr, src.ptr, nbytes, 2, None))
+    C.check(C.lib().hm_stream_synchronize(None))
+    print(json.dumps({"copy_calibration": {"bytes_per_copy": nbytes, "copies": steps}}), flush=True)
 
 
 def flops_per_pt(mu):
@@ -148,13 +165,16 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              f"(the tracker shards by particle; see DESIGN.md)", file=sys.stderr)
+              f"(every GPU would track its own share of the particles; only one is driven here)", file=sys.stderr)
         sys.exit(2)
     from henon_map_cpp_amd import _capi as C
 
     if C.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration, a.steps)
+        return
     w = Workload(a.n_side, a.turns)
     cfg = C.LaunchCfg(0, a.compact_every, 0, 0) if a.compact_every else None
     fpt = flops_per_pt(w.mu)

@@ -27,12 +27,12 @@ HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     # exact mode relies on no contraction; the fast kernels place fma() by hand
     "-ffp-contract=off",
-    # packed FP32 ops are disabled for all HIP code on this hardware pool
+    # no packed FP32 instructions in any HIP object
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
-HIP_SOURCES = ["hm_runtime.hip", "hm_track.hip", "hm_tangent.hip", "hm_fft.hip"]
+HIP_SOURCES = ["hm_runtime.hip", "hm_track.hip", "hm_tangent.hip", "hm_fft.hip", "hm_adjoint.hip"]
 
 LIB_NAME = "libhenon_hip.so"
 ENGINE_SOURCES = ["engine.cpp"]

@@ -91,10 +91,10 @@ int hm_birkhoff_weights(uint32_t n_weights, double *out);
 typedef struct hm_launch_cfg {
     int32_t block;              /* threads per workgroup: 64, 128, 192 or 256 (0 = 256) */
     int32_t compact_every;      /* turns between loss compactions (0 = 1000, <0 = never); segments
-                                   start at 125 turns, double up to this, and past 16x this
-                                   keep doubling up to 8x this */
-    int32_t particles_per_lane; /* 0 = one, or two for the leading wavefronts when the live
-                                   particles exceed one round of resident wavefronts; 1 = one */
+                                   start at 125 turns and double up to this */
+    int32_t particles_per_lane; /* 1 = one per lane; 2 = one, or two for the leading wavefronts
+                                   when the live particles exceed one round of resident
+                                   wavefronts; 0 = 1 in exact mode, 2 in fast mode */
     int32_t reserved;
 } hm_launch_cfg;
 
@@ -244,6 +244,43 @@ int hm_birkhoff_tunes_history(const double *hist, size_t n, uint32_t n_samples, 
  * direct DFT. out: device double[n_windows][2][n]. */
 int hm_fft_tunes(const double *hist, size_t n, uint32_t n_samples, const double *means, const uint32_t *from,
                  const uint32_t *len, size_t n_windows, double *out, void *stream);
+
+/* ---- reverse-mode derivatives ---------------------------------------------- */
+/* No counterpart in the reference, whose only derivative is the one-turn tangent
+ * matrix (src/henon.cu:175-265); these extend it to whole-orbit gradients.
+ * Loss model: L = L(s_T, B) with s_T the state after n_turns forward turns from
+ * x0 (table rows 0..n_turns-1) and B = sum_{k<n_turns} w[k] x_{k+1}^2 the
+ * weighted time average of x^2 (w = hm_birkhoff_weights(n_turns) or any
+ * weights). */
+/* Forward sweep: writes the state before turns 0, every, 2 every, ... into
+ * checkpoints (device double[ceil(n_turns/every)][4][n]), the final state
+ * (double[4][n], NaN when lost), B (double[n]) and alive (uint8[n], 1 when the
+ * particle stayed inside the barrier for all n_turns). Same arithmetic as
+ * hm_track for the same mode: in exact mode final_state equals hm_track's. */
+int hm_adjoint_forward(const double *x0, const double *px0, const double *y0, const double *py0, size_t n,
+                       const double *table, size_t table_len, uint32_t n_turns, double mu, double barrier,
+                       uint32_t every, const double *weights, double *checkpoints, double *final_state, double *birk,
+                       uint8_t *alive, int mode, void *stream);
+/* Bytes of the scratch buffer hm_adjoint_backward needs for n particles and
+ * checkpoint interval every. */
+int hm_adjoint_scratch_bytes(size_t n, uint32_t every, size_t *bytes);
+/* Backward sweep: given seed_state = dL/ds_T (double[4][n]) and seed_birk =
+ * dL/dB (double[n], or NULL for 0), writes grad_x0 = dL/dx0 (double[4][n]) and
+ * grad_params = (dL/d epsilon, dL/d mu, dL/d omega_x0, dL/d omega_y0)
+ * (double[4][n]), with dtable[k] = (d omega_x(k)/d omega_x0, d omega_x(k)/d eps,
+ * d omega_y(k)/d omega_y0, d omega_y(k)/d eps) (device double[n_turns][4], from
+ * hm_lines_modulation_derivatives). Lost particles get zero gradients. Each
+ * K-turn segment is recomputed from its checkpoint and each 8-turn chunk of it
+ * from a sub-checkpoint in scratch; the map is never inverted. */
+int hm_adjoint_backward(const double *checkpoints, size_t n, const double *table, const double *dtable,
+                        size_t table_len, uint32_t n_turns, double mu, uint32_t every, const double *weights,
+                        const double *seed_state, const double *seed_birk, const uint8_t *alive, double *scratch,
+                        double *grad_x0, double *grad_params, int mode, void *stream);
+/* Host: for omega(k) = 2 pi tune (1 + eps sum_j coef_j cos(freq_j (k+start)))
+ * (hm_lines_modulation, hm_sps_modulation with hm_sps_lines' set), out[2k] =
+ * d omega(k)/d omega_0 with omega_0 = 2 pi tune and out[2k+1] = d omega(k)/d eps. */
+int hm_lines_modulation_derivatives(double tune, double epsilon, const double *coef, const double *freq,
+                                    int n_lines, int start, int end, double *out);
 
 /* ---- instrumentation ----------------------------------------------------- */
 /* When enabled, every kernel launch of the library is bracketed by HIP events

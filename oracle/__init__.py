@@ -5,9 +5,9 @@ through ctypes. Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
 ``cpu_baseline`` leg use it, always as the checker or the timed CPU baseline,
 never as part of the product.
 
-Functions with no reference counterpart (the fused renormalised indicators and
-the adjoint checks) live in :mod:`oracle.indicators`; their parity is pinned by
-closed forms, finite differences and tangent-map products, as stated there.
+Functions with no reference counterpart (the fused renormalised indicators,
+``indicators()`` below) are pinned by closed forms, finite differences and
+tangent-map products, as stated there.
 """
 from __future__ import annotations
 
@@ -30,11 +30,13 @@ U = ctypes.c_uint
 I = ctypes.c_int
 
 
-def build(force: bool = False) -> None:
-    """Compile liboracle.so (and oracle/_ref when the reference sources exist)."""
+def build(force: bool = False, with_ref: bool = False) -> None:
+    """Compile liboracle.so; with `with_ref`, also oracle/_ref from the
+    reference's own sources (only where the reference checkout exists: the
+    build step, never a test or benchmark run)."""
     if force or not os.path.exists(_LIB_PATH):
         subprocess.check_call(["make", "-s", "-B", "liboracle.so"], cwd=_HERE)
-    if os.path.isdir("/root/reference/src") and (force or not os.path.exists(_REF_PATH)):
+    if with_ref and os.path.isdir("/root/reference/src") and (force or not os.path.exists(_REF_PATH)):
         subprocess.check_call(["make", "-s", "-B", "ref"], cwd=_HERE)
 
 

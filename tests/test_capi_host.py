@@ -80,3 +80,24 @@ def test_table_layout_and_values():
 def test_birkhoff_weights_bit_exact():
     for n in (1, 2, 10, 2047, 100000):
         assert np.array_equal(C.birkhoff_weights(n), O.birkhoff_weights(n), equal_nan=True)
+
+
+def test_lines_modulation_derivatives():
+    # d omega_k/d omega_0 and d omega_k/d eps of the line modulation against
+    # central differences of the modulation itself (which is linear in eps and
+    # in tune, so the differences are exact up to rounding)
+    from henon_map_cpp_amd.adjoint import lines_derivatives
+    c, f = C.sps_lines()
+    tune, eps, start, end = 0.201, 12.0, 7, 2007
+    d = lines_derivatives(tune, eps, c, f, start, end)
+    h = 1.0  # omega is linear in eps and in tune: wide steps only cut rounding
+    de = (C.lines_modulation(tune, eps + h, c, f, start, end) - C.lines_modulation(tune, eps - h, c, f, start, end))
+    np.testing.assert_allclose(d[:, 1], de / (2 * h), rtol=0, atol=1e-12 * np.abs(d[:, 1]).max())
+    h = 0.01
+    dq = (C.lines_modulation(tune + h, eps, c, f, start, end) - C.lines_modulation(tune - h, eps, c, f, start, end))
+    np.testing.assert_allclose(d[:, 0], dq / (2 * h) / (2 * np.pi), rtol=1e-12)
+    # without modulation omega_k = omega_0
+    d0 = lines_derivatives(tune, 0.0, c, f, 0, 10)
+    assert np.all(d0[:, 0] == 1.0)
+    with pytest.raises(ValueError):
+        lines_derivatives(tune, eps, c, f, 5, 5)

@@ -137,6 +137,28 @@ def test_loss_compaction_is_bit_identical(every):
     assert_bit_exact(ref, got)
 
 
+@pytest.mark.parametrize("mode", [C.MODE_EXACT, C.MODE_FAST])
+def test_one_and_two_particles_per_lane_agree(mode):
+    """775^2 = 600625 particles exceed one round of resident wavefronts (256 CUs
+    x 32 waves x 64 lanes), so particles_per_lane=2 runs the mixed launch; both
+    launch shapes must give identical results, and exact mode the oracle's."""
+    c = W.MODULATED_DA
+    x, px, y, py = W.da_grid(775)
+    tables, tab = make_tables(400, c["tune_x"], c["tune_y"], "custom", c["epsilon"])
+    tdev = C.DeviceBuffer.from_array(tab)
+    outs = []
+    for pair in (1, 2):
+        dev = C.DeviceParticles(x, px, y, py)
+        C.track(dev, tdev, len(tab), 400, c["mu"], c["barrier"], False, mode, C.LaunchCfg(0, 100, pair, 0))
+        outs.append(dev.get())
+    assert C.take_status() == 0
+    assert_bit_exact(outs[0], outs[1])
+    if mode == C.MODE_EXACT:
+        p = O.Particles(x, px, y, py)
+        O.lib().or_track(*p.state(), p.n, *tables.ptrs(), 400, c["mu"], c["barrier"], np.nan, 0, NT)
+        assert_bit_exact((p.x, p.px, p.y, p.py, p.steps), outs[0])
+
+
 @pytest.mark.parametrize("block", [64, 128, 256])
 def test_block_sizes(block):
     x, px, y, py = W.da_grid(40)
