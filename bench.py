@@ -28,6 +28,9 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (FMA = 2 flops)
 FLOPS_PER_PT = {0.0: 26, None: 40}  # mu == 0 / mu != 0, per live particle-turn
+FLOPS_TANGENT = 103  # J entries + structured J.M, per live particle-turn
+FLOPS_MEGNO = 35     # Frobenius norm ratio construct, per live particle-turn
+TRACK_TRAFFIC_BYTES = 110.3e6
 METRIC = "FP64 particle-turns/s (track, tangent/MEGNO, adjoint) + % of FP64 VALU peak"
 
 
@@ -43,6 +46,9 @@ def parse():
     ap.add_argument("--cpu-sample-stride", type=int, default=13)
     ap.add_argument("--single-mode", action="store_true", help="time only --mode (profiling runs)")
     ap.add_argument("--compact-every", type=int, default=0)
+    ap.add_argument("--legs", default="indicators,adjoint,batch",
+                    help="extra measurements after the headline: indicators, adjoint, batch (comma list, or none)")
+    ap.add_argument("--batch-jobs", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run `--steps` device-to-device copies of MIB MiB (the known-size copy that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters), then exit")
@@ -129,6 +135,158 @@ def _sync():
             torch.cuda.synchronize()
     except Exception:
         pass
+
+
+def _lines(tune, eps, turns):
+    from henon_map_cpp_amd import _capi as C
+    from henon_map_cpp_amd import workloads as W
+
+    return C.lines_modulation(tune, eps, W.seeded_line_coefficients(), W.line_frequencies(), 0, turns)
+
+
+def indicators_leg(steps, warmup):
+    """Fused tangent map + MEGNO + FLI with renormalisation every 100 turns: the
+    chaos-indicator workload (256x256 grid + 65536 ball points, 1e5 turns)."""
+    from henon_map_cpp_amd import _capi as C
+    from henon_map_cpp_amd import workloads as W
+
+    c = W.CHAOS_INDICATORS
+    x0 = W.indicator_points()
+    T = c["n_turns"]
+    tdev = C.DeviceBuffer.from_array(C.build_table(_lines(c["tune_x"], c["epsilon"], T),
+                                                   _lines(c["tune_y"], c["epsilon"], T)))
+    n = x0[0].size
+    kms = 0.0
+    live = 0
+    for it in range(warmup + steps):
+        parts = C.DeviceParticles(*x0)
+        if it == warmup:
+            C.profile_reset()
+            C.profile_enable(True)
+        C.track_indicators(parts, tdev, T, T, c["mu"], c["barrier"], mode=C.MODE_EXACT,
+                           renorm_every=c["renorm_every"], samples=c["samples"], matrix=True)
+        if it >= warmup:
+            live += int(parts.get()[4].astype(np.int64).sum())
+    kms, launches = C.profile_read("k_indicators")
+    C.profile_enable(False)
+    fpt = FLOPS_PER_PT[None] + FLOPS_TANGENT + FLOPS_MEGNO
+    ks = kms / 1e3
+    return {
+        "workload": c["name"],
+        "kernel": "k_indicators",
+        "particles": n,
+        "turns": T,
+        "value": n * T * steps / ks,
+        "live_value": live / ks,
+        "unit": "particle-turns/s (kernel time)",
+        "flops_per_live_particle_turn": fpt,
+        "achieved": live * fpt / ks / 1e12,
+        "frac": live * fpt / ks / 1e12 / FP64_PEAK_TFLOPS,
+        "kernel_ms_per_step": kms / steps,
+    }
+
+
+def adjoint_leg():
+    """Whole-orbit gradients: 65536 points (r ~ U[0.05, 0.4]), 1e4 turns, eps=16,
+    mu=0.5, losses L1 = mean final r^2 and L2 = mean Birkhoff-averaged x^2 over the
+    survivors; cost of forward + backward sweeps against forward-only tracking."""
+    import torch
+
+    from henon_map_cpp_amd import _capi as C
+    from henon_map_cpp_amd import adjoint as A
+    from henon_map_cpp_amd import workloads as W
+
+    c = W.ADJOINT_GRADIENTS
+    T = c["n_turns"]
+    x0 = np.stack(W.shell_uniform_radius(c["n"], c["r_min"], c["r_max"], c["seed"]))
+    coef, freq = W.seeded_line_coefficients(), W.line_frequencies()
+    wx, wy = _lines(c["tune_x"], c["epsilon"], T), _lines(c["tune_y"], c["epsilon"], T)
+    tdev = C.DeviceBuffer.from_array(C.build_table(wx, wy))
+    track_ms = []
+    for it in range(3):
+        parts = C.DeviceParticles(*x0)
+        C.check(C.lib().hm_stream_synchronize(None))
+        C.profile_reset()
+        C.profile_enable(True)
+        C.track(parts, tdev, T, T, c["mu"], c["barrier"], False, C.MODE_EXACT)
+        track_ms.append(C.profile_read("k_track")[0])
+        C.profile_enable(False)
+    t_track = min(track_ms)
+    x0t = torch.as_tensor(x0, device="cuda")
+    out = {"workload": c["name"], "forward_only_track_ms": t_track, "every": {}}
+    for K in c["every"]:
+        adj = A.OrbitAdjoint(T, c["tune_x"], c["tune_y"], c["epsilon"], c["mu"], c["barrier"], coef=coef,
+                             freq=freq, every=K)
+        best = None
+        for it in range(2):
+            torch.cuda.synchronize()
+            C.profile_reset()
+            C.profile_enable(True)
+            st, birk, alive, ck = adj.forward(x0t)
+            ok = alive.bool()
+            ns = max(int(ok.sum()), 1)
+            seed_s = torch.where(ok[None, :], 2.0 * st / ns, torch.zeros_like(st))   # dL1/ds_T
+            seed_b = torch.where(ok, torch.full_like(birk, 1.0 / ns), torch.zeros_like(birk))  # dL2/dB
+            gx, gp = adj.backward(ck, alive, seed_s, seed_b)
+            torch.cuda.synchronize()
+            f_ms = C.profile_read("k_adjoint_forward")[0]
+            b_ms = C.profile_read("k_adjoint_backward")[0]
+            C.profile_enable(False)
+            if best is None or f_ms + b_ms < best[0] + best[1]:
+                best = (f_ms, b_ms)
+        f_ms, b_ms = best
+        out["every"][str(K)] = {"forward_ms": f_ms, "backward_ms": b_ms,
+                                "cost_vs_tracking": (f_ms + b_ms) / t_track,
+                                "particle_turns_per_s": x0.shape[1] * T / ((f_ms + b_ms) / 1e3)}
+    out["survivors"] = int(ok.sum())
+    return out
+
+
+def batch_leg(n_jobs):
+    """Uneven independent jobs in one batched launch sequence against one plain
+    hm_track call per job (same GPU, same exact arithmetic), wall time each."""
+    from henon_map_cpp_amd import _capi as C
+    from henon_map_cpp_amd import workloads as W
+
+    c = W.BATCHED_JOBS
+    params = W.batch_job_params(n_jobs)
+    coef, freq = W.seeded_line_coefficients(), W.line_frequencies()
+    ics, tabs = [], []
+    for (n, T, qx, qy, eps, mu, seed) in params:
+        ics.append(np.stack(W.ball(n, c["r_max"], seed=seed)))
+        wx = C.lines_modulation(qx, eps, coef, freq, 0, T)
+        wy = C.lines_modulation(qy, eps, coef, freq, 0, T)
+        tabs.append(C.DeviceBuffer.from_array(C.build_table(wx, wy)))
+    pt = sum(p[0] * p[1] for p in params)
+
+    def fresh():
+        return [C.DeviceParticles(*x) for x in ics]
+
+    def batched():
+        parts = fresh()
+        jobs = [(pp, tb, p[1], p[1], p[5], c["barrier"]) for pp, tb, p in zip(parts, tabs, params)]
+        _sync()
+        t0 = time.perf_counter()
+        C.track_batch(jobs, C.MODE_EXACT)
+        C.check(C.lib().hm_stream_synchronize(None))
+        return time.perf_counter() - t0, parts
+
+    def plain():
+        parts = fresh()
+        _sync()
+        t0 = time.perf_counter()
+        for pp, tb, p in zip(parts, tabs, params):
+            C.track(pp, tb, p[1], p[1], p[5], c["barrier"], False, C.MODE_EXACT)
+        C.check(C.lib().hm_stream_synchronize(None))
+        return time.perf_counter() - t0, parts
+
+    batched()
+    tb, pb = min((batched() for _ in range(2)), key=lambda r: r[0])
+    tp, pp = plain()
+    identical = all(np.array_equal(u, v, equal_nan=True) for a, b in zip(pb, pp) for u, v in zip(a.get(), b.get()))
+    return {"workload": c["name"], "jobs": n_jobs, "nominal_particle_turns": pt,
+            "batched_s": tb, "plain_s": tp, "batched_value": pt / tb, "plain_value": pt / tp,
+            "speedup": tp / tb, "identical_to_plain": identical}
 
 
 def cpu_baseline(stride, turns):
@@ -229,13 +387,27 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": head["frac"],
-            "traffic": None,
+            # HBM bytes per k_track launch: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
+            # (factors from a plain copy of known size), 2e4-turn run of this scan;
+            # profiles/k_track_hbm_traffic_pmc.txt
+            "traffic": TRACK_TRAFFIC_BYTES,
+            "traffic_unit": "bytes per launch",
             "kernel": "k_track",
             "kernel_ms_per_step": head["kernel_ms_per_step"],
             "launches_per_step": head["launches_per_step"],
         },
         "cpu_baseline": None,
     }
+    legs = [x for x in a.legs.split(",") if x and x != "none"]
+    if "indicators" in legs:
+        line["tangent_megno"] = indicators_leg(2, 1)
+        print(f"[bench] tangent/MEGNO {line['tangent_megno']}", file=sys.stderr, flush=True)
+    if "adjoint" in legs:
+        line["adjoint"] = adjoint_leg()
+        print(f"[bench] adjoint {line['adjoint']}", file=sys.stderr, flush=True)
+    if "batch" in legs:
+        line["batch"] = batch_leg(a.batch_jobs)
+        print(f"[bench] batch {line['batch']}", file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline(a.cpu_sample_stride, a.turns)

@@ -31,6 +31,20 @@ class LaunchCfg(ctypes.Structure):
                 ("particles_per_lane", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+class Job(ctypes.Structure):
+    """hm_job (include/henon_hip.h)."""
+    _fields_ = [("x", ctypes.c_void_p), ("px", ctypes.c_void_p), ("y", ctypes.c_void_p), ("py", ctypes.c_void_p),
+                ("steps", ctypes.c_void_p), ("n", ctypes.c_size_t), ("table", ctypes.c_void_p),
+                ("table_len", ctypes.c_size_t), ("n_turns", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("mu", ctypes.c_double), ("barrier", ctypes.c_double), ("kick_module", ctypes.c_double)]
+
+
+class BatchCfg(ctypes.Structure):
+    """hm_batch_cfg (include/henon_hip.h)."""
+    _fields_ = [("block", ctypes.c_int32), ("compact_every", ctypes.c_int32), ("waves", ctypes.c_int32),
+                ("use_graph", ctypes.c_int32)]
+
+
 MAX_SAMPLES, MAX_BIRKHOFF = 16, 4
 
 
@@ -65,6 +79,10 @@ SIGNATURES = {
     "hm_build_table": (I, [P, P, SZ, P]),
     "hm_birkhoff_weights": (I, [U32, P]),
     "hm_track": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, D, D, I, I, ctypes.POINTER(LaunchCfg), P]),
+    "hm_track_batch": (I, [ctypes.POINTER(Job), SZ, I, ctypes.POINTER(BatchCfg), P]),
+    "hm_autotune_track": (I, [P, P, P, P, P, SZ, P, SZ, U32, U32, D, D, I, ctypes.POINTER(LaunchCfg),
+                              ctypes.POINTER(D), P]),
+    "hm_autotune_clear": (I, []),
     "hm_observable": (I, [I, P, P, P, P, SZ, P, P]),
     "hm_tangent_matrix": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, I, P, P]),
     "hm_megno_trace": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, D, P, SZ, P, P]),
@@ -255,6 +273,25 @@ def track(parts: DeviceParticles, table_dev: DeviceBuffer, table_len: int, n_tur
     c = ctypes.byref(cfg) if cfg is not None else None
     check(lib().hm_track(*parts.ptrs(), parts.n, table_dev.ptr, table_len, n_turns, mu, barrier, kick_module,
                          int(inverse), mode, c, stream))
+
+
+def track_batch(jobs, mode=MODE_EXACT, cfg: BatchCfg | None = None, stream=None):
+    """jobs: list of (DeviceParticles, table DeviceBuffer, table_len, n_turns, mu, barrier)."""
+    arr = (Job * max(len(jobs), 1))()
+    for k, (parts, tdev, tlen, n_turns, mu, barrier) in enumerate(jobs):
+        x, px, y, py, st = parts.ptrs()
+        arr[k] = Job(x, px, y, py, st, parts.n, tdev.ptr, tlen, n_turns, 0, mu, barrier, float("nan"))
+    c = ctypes.byref(cfg) if cfg is not None else None
+    check(lib().hm_track_batch(arr, len(jobs), mode, c, stream))
+
+
+def autotune_track(parts: DeviceParticles, table_dev: DeviceBuffer, table_len: int, mu: float, barrier: float,
+                   mode=MODE_EXACT, warm_turns=2000, trial_turns=4000, stream=None):
+    """Returns (LaunchCfg, trial ms); hm_track with cfg=None then uses it."""
+    best, ms = LaunchCfg(), D(0.0)
+    check(lib().hm_autotune_track(*parts.ptrs(), parts.n, table_dev.ptr, table_len, warm_turns, trial_turns, mu,
+                                  barrier, mode, ctypes.byref(best), ctypes.byref(ms), stream))
+    return best, ms.value
 
 
 def take_status(stream=None) -> int:

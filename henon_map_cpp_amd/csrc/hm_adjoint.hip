@@ -85,6 +85,57 @@ struct AdjointBwdArgs {
     double *grad_p;           // [4][n]: d/d eps, d/d mu, d/d omega_x0, d/d omega_y0
 };
 
+// One turn of the adjoint, turn k: pre-step point `pre`, post-step point
+// `post`, post-step adjoint (lx, lpx, ly, lpy) -> pre-step adjoint, with the
+// parameter terms accumulated.
+struct AdjAcc {
+    double lx, lpx, ly, lpy;
+    double g_eps, g_mu, g_wx, g_wy;
+};
+
+template <bool OCT>
+__device__ __forceinline__ void adjoint_turn(const double (&pre)[4], const double (&post)[4], const TabEntry &t,
+                                             double bw, const double4 &dt, double mu3, double mu6, AdjAcc &a)
+{
+    // Birkhoff seed of sample k+1: d/dx_{k+1} of c w_k x_{k+1}^2 (bw = 2 c w_k)
+    a.lx = fma(bw, post[0], a.lx);
+    // rotation angles: d x'/d w = p', d p'/d w = -x'
+    const double dwx = fma(a.lx, post[1], -a.lpx * post[0]);
+    const double dwy = fma(a.ly, post[3], -a.lpy * post[2]);
+    a.g_wx = fma(dwx, dt.x, a.g_wx);
+    a.g_wy = fma(dwy, dt.z, a.g_wy);
+    a.g_eps = fma(dwx, dt.y, fma(dwy, dt.w, a.g_eps));
+    // through the rotations
+    const double lxt = fma(t.cx, a.lx, -t.sx * a.lpx), lpt = fma(t.sx, a.lx, t.cx * a.lpx);
+    const double lyt = fma(t.cy, a.ly, -t.sy * a.lpy), lqt = fma(t.sy, a.ly, t.cy * a.lpy);
+    // through the kicks, at the pre-step point
+    const double x = pre[0], y = pre[2], xx = x * x, yy = y * y;
+    if (OCT) a.g_mu = fma(lpt, x * fma(-3.0, yy, xx), fma(lqt, y * fma(-3.0, xx, yy), a.g_mu));
+    const double ja = fma(mu3, xx - yy, 2.0 * x);
+    const double jb = fma(-mu6 * x, y, -2.0 * y);
+    a.lx = fma(ja, lpt, fma(jb, lqt, lxt));
+    a.ly = fma(jb, lpt, fma(-ja, lqt, lyt));
+    a.lpx = lpt;
+    a.lpy = lqt;
+}
+
+template <int MODE, bool OCT>
+__device__ __forceinline__ void step4(double (&v)[4], double mu, double m3mu, const TabEntry &t)
+{
+    map_step<MODE, false, OCT>(v[0], v[1], v[2], v[3], mu, m3mu, t);
+}
+
+__device__ __forceinline__ void copy4(double (&d)[4], const double (&s)[4])
+{
+    d[0] = s[0]; d[1] = s[1]; d[2] = s[2]; d[3] = s[3];
+}
+
+// Backward sweep. Per K-segment (in reverse): recompute it from its checkpoint,
+// writing the state every kChunk = 8 turns to scratch; the state after the
+// segment becomes the post-step point of its last turn. Per 8-turn chunk (in
+// reverse), in two halves so that at most five states are held in registers:
+// turns 4..7 from S4..S7 (S4 reached from S0 by four turns), then turns 0..3
+// from S0..S3; `nxt` carries the state after the turns being processed.
 template <int MODE, bool OCT>
 __global__ void __launch_bounds__(256) k_adjoint_backward(AdjointBwdArgs a)
 {
@@ -95,74 +146,80 @@ __global__ void __launch_bounds__(256) k_adjoint_backward(AdjointBwdArgs a)
         for (int c = 0; c < 4; c++) { a.grad_x0[c * n + i] = 0.0; a.grad_p[c * n + i] = 0.0; }
         return;
     }
-    double lx = a.seed_s[i], lpx = a.seed_s[n + i], ly = a.seed_s[2 * n + i], lpy = a.seed_s[3 * n + i];
-    const double sb = a.seed_b ? a.seed_b[i] : 0.0;
-    double g_eps = 0.0, g_mu = 0.0, g_wx = 0.0, g_wy = 0.0;
+    AdjAcc acc{a.seed_s[i], a.seed_s[n + i], a.seed_s[2 * n + i], a.seed_s[3 * n + i], 0.0, 0.0, 0.0, 0.0};
+    const double bseed = 2.0 * (a.seed_b ? a.seed_b[i] : 0.0);
     const double mu = a.mu, m3mu = -3.0 * mu, mu3 = 3.0 * mu, mu6 = 6.0 * mu;
     const uint32_t T = a.n_turns, K = a.every;
     const uint32_t n_seg = (T + K - 1) / K;
+    // tables and weights are read-only here and indexed by the (wave-uniform)
+    // turn: scalar loads
+    const ConstPtr<TabEntry> tab = as_const(a.tab);
+    const ConstPtr<double> dtab = as_const(a.dtab), weights = as_const(a.weights);
+    auto rec = [&](uint32_t k) -> TabEntry { return load_record(tab + k); };
+    auto turn_k = [&](uint32_t k, const double (&pre)[4], const double (&post)[4]) {
+        const ConstPtr<double> d = dtab + 4 * (size_t)k;
+        const double4 dt{d[0], d[1], d[2], d[3]};
+        adjoint_turn<OCT>(pre, post, rec(k), bseed * weights[k], dt, mu3, mu6, acc);
+    };
     for (int seg = (int)n_seg - 1; seg >= 0; --seg) {
         const uint32_t s0 = (uint32_t)seg * K, s1 = min(s0 + K, T);
-        // level 1: recompute the segment, sub-checkpoint every kChunk turns
+        double nxt[4];
         {
             const size_t o = (size_t)seg * 4 * n;
-            double x = a.ckpt[o + i], px = a.ckpt[o + n + i], y = a.ckpt[o + 2 * n + i], py = a.ckpt[o + 3 * n + i];
+            double v[4] = {a.ckpt[o + i], a.ckpt[o + n + i], a.ckpt[o + 2 * n + i], a.ckpt[o + 3 * n + i]};
             for (uint32_t k = s0; k < s1; ++k) {
                 if ((k - s0) % kChunk == 0) {
                     const size_t q = (size_t)((k - s0) / kChunk) * 4 * n;
-                    a.sub[q + i] = x; a.sub[q + n + i] = px; a.sub[q + 2 * n + i] = y; a.sub[q + 3 * n + i] = py;
+                    a.sub[q + i] = v[0]; a.sub[q + n + i] = v[1]; a.sub[q + 2 * n + i] = v[2];
+                    a.sub[q + 3 * n + i] = v[3];
                 }
-                map_step<MODE, false, OCT>(x, px, y, py, mu, m3mu, a.tab[k]);
+                step4<MODE, OCT>(v, mu, m3mu, rec(k));
             }
+            copy4(nxt, v);
         }
         const uint32_t n_chunk = (s1 - s0 + kChunk - 1) / kChunk;
         for (int ch = (int)n_chunk - 1; ch >= 0; --ch) {
             const uint32_t c0 = s0 + (uint32_t)ch * kChunk;
             const uint32_t len = min((uint32_t)kChunk, s1 - c0);
-            // level 2: the chunk's states S[0..len] in registers
-            double S[kChunk + 1][4];
             const size_t q = (size_t)ch * 4 * n;
-            S[0][0] = a.sub[q + i]; S[0][1] = a.sub[q + n + i]; S[0][2] = a.sub[q + 2 * n + i];
-            S[0][3] = a.sub[q + 3 * n + i];
+            const double S0[4] = {a.sub[q + i], a.sub[q + n + i], a.sub[q + 2 * n + i], a.sub[q + 3 * n + i]};
+            double W[5][4];
+            if (len > 4) {
+                // W[j] = S(4 + j) for 4 + j < len, W[len - 4] = nxt
+                copy4(W[0], S0);
 #pragma unroll
-            for (int m = 0; m < kChunk; ++m) {
-                S[m + 1][0] = S[m][0]; S[m + 1][1] = S[m][1]; S[m + 1][2] = S[m][2]; S[m + 1][3] = S[m][3];
-                if ((uint32_t)m < len)
-                    map_step<MODE, false, OCT>(S[m + 1][0], S[m + 1][1], S[m + 1][2], S[m + 1][3], mu, m3mu,
-                                               a.tab[c0 + m]);
+                for (int m = 0; m < 4; ++m) step4<MODE, OCT>(W[0], mu, m3mu, rec(c0 + m));
+#pragma unroll
+                for (int j = 1; j < 4; ++j) {
+                    copy4(W[j], W[j - 1]);
+                    if (4u + j < len) step4<MODE, OCT>(W[j], mu, m3mu, rec(c0 + 3 + j));
+                }
+#pragma unroll
+                for (int j = 3; j >= 0; --j) {
+                    if (4u + j >= len) continue;
+                    if (4u + j + 1 < len) turn_k(c0 + 4 + j, W[j], W[j + 1]);
+                    else turn_k(c0 + 4 + j, W[j], nxt);
+                }
+                copy4(nxt, W[0]);
+            }
+            const uint32_t h = min(len, 4u);
+            copy4(W[0], S0);
+#pragma unroll
+            for (int j = 1; j < 4; ++j) {
+                copy4(W[j], W[j - 1]);
+                if ((uint32_t)j < h) step4<MODE, OCT>(W[j], mu, m3mu, rec(c0 + j - 1));
             }
 #pragma unroll
-            for (int m = kChunk - 1; m >= 0; --m) {
-                if ((uint32_t)m >= len) continue;
-                const uint32_t k = c0 + m;
-                const TabEntry t = a.tab[k];
-                const double x = S[m][0], y = S[m][2];
-                const double xn = S[m + 1][0], pxn = S[m + 1][1], yn = S[m + 1][2], pyn = S[m + 1][3];
-                // Birkhoff seed of sample k+1: d/dx_{k+1} of w_k x_{k+1}^2
-                lx = fma(2.0 * sb * a.weights[k], xn, lx);
-                // rotation parameters (post-step point)
-                const double dwx = fma(lx, pxn, -lpx * xn), dwy = fma(ly, pyn, -lpy * yn);
-                const double4 dt = *reinterpret_cast<const double4 *>(a.dtab + 4 * (size_t)k);
-                g_wx = fma(dwx, dt.x, g_wx);
-                g_wy = fma(dwy, dt.z, g_wy);
-                g_eps = fma(dwx, dt.y, fma(dwy, dt.w, g_eps));
-                // through the rotations
-                const double lxt = fma(t.cx, lx, -t.sx * lpx), lpt = fma(t.sx, lx, t.cx * lpx);
-                const double lyt = fma(t.cy, ly, -t.sy * lpy), lqt = fma(t.sy, ly, t.cy * lpy);
-                // through the kicks (pre-step point)
-                const double xx = x * x, yy = y * y;
-                g_mu = fma(lpt, x * fma(-3.0, yy, xx), fma(lqt, y * fma(-3.0, xx, yy), g_mu));
-                const double ja = fma(mu3, xx - yy, 2.0 * x);
-                const double jb = fma(-mu6 * x, y, -2.0 * y);
-                lx = fma(ja, lpt, fma(jb, lqt, lxt));
-                ly = fma(jb, lpt, fma(-ja, lqt, lyt));
-                lpx = lpt;
-                lpy = lqt;
+            for (int j = 3; j >= 0; --j) {
+                if ((uint32_t)j >= h) continue;
+                if ((uint32_t)j + 1 < h) turn_k(c0 + j, W[j], W[j + 1]);
+                else turn_k(c0 + j, W[j], nxt);
             }
+            copy4(nxt, S0);
         }
     }
-    a.grad_x0[i] = lx; a.grad_x0[n + i] = lpx; a.grad_x0[2 * n + i] = ly; a.grad_x0[3 * n + i] = lpy;
-    a.grad_p[i] = g_eps; a.grad_p[n + i] = g_mu; a.grad_p[2 * n + i] = g_wx; a.grad_p[3 * n + i] = g_wy;
+    a.grad_x0[i] = acc.lx; a.grad_x0[n + i] = acc.lpx; a.grad_x0[2 * n + i] = acc.ly; a.grad_x0[3 * n + i] = acc.lpy;
+    a.grad_p[i] = acc.g_eps; a.grad_p[n + i] = acc.g_mu; a.grad_p[2 * n + i] = acc.g_wx; a.grad_p[3 * n + i] = acc.g_wy;
 }
 
 }  // namespace hm

@@ -1,0 +1,271 @@
+// Batched tracking of independent jobs (each with its own particles, table,
+// turn count, mu and barrier) in one stream of persistent launches fed by a
+// device work queue. No reference counterpart: the reference tracks one
+// particle set per call (src/henon.cu:2701-2752, :3155-3185).
+//
+// The turns are cut into segments (125, 250, 500, then `compact_every` turns).
+// Per segment, two launches:
+//   k_batch_plan   one workgroup: per job, the live count (the job's particle
+//                  count in the first segment, the survivors appended by the
+//                  previous segment otherwise) -> 64-particle chunks; an
+//                  exclusive scan over jobs gives each job's first chunk id; the
+//                  queue head and the jobs' next-segment counters are cleared.
+//   k_batch_track  a persistent grid of resident wavefronts; each wavefront
+//                  pops chunk ids from the queue head (one atomic per chunk),
+//                  finds the chunk's job by binary search over the scan, and
+//                  tracks those 64 particles for the job's turns of the segment
+//                  (the same wave-level code as hm_track), appending survivors
+//                  to the job's next live list. A wavefront leaves when the queue
+//                  is drained, so every wavefront exits.
+// Nothing is read back by the host between segments, so the whole launch
+// sequence can be recorded once in a HIP graph and replayed.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "hm_device.h"
+#include "hm_internal.h"
+#include "hm_wave.h"
+
+namespace hm {
+
+struct JobDesc {
+    double *x, *px, *y, *py;
+    uint32_t *steps;
+    const TabEntry *tab;
+    uint32_t n, n_turns, tab_len, list_off;
+    double mu, b2;
+};
+
+struct BatchQueue {
+    uint32_t total;  // chunks of the current segment
+    uint32_t head;   // next chunk id to hand out
+};
+
+struct BatchSeg {
+    const JobDesc *jobs;
+    uint32_t n_jobs;
+    uint32_t seg_start, seg_len;
+    uint32_t first;                      // 1 in the first segment (identity lists)
+    const uint32_t *lists_cur;           // live lists read by this segment
+    uint32_t *lists_next;                // survivors appended by this segment
+    const uint32_t *count_cur;           // [n_jobs]
+    uint32_t *count_next;                // [n_jobs]
+    uint32_t *chunk_off;                 // [n_jobs + 1] exclusive scan of chunk counts
+    BatchQueue *queue;
+    uint32_t *status;
+};
+
+constexpr int kPlanThreads = 1024;
+
+__global__ void __launch_bounds__(kPlanThreads) k_batch_plan(BatchSeg S)
+{
+    __shared__ uint32_t wave_sum[kPlanThreads / 64];
+    __shared__ uint32_t carry;
+    const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
+    if (t == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < S.n_jobs; base += kPlanThreads) {
+        const uint32_t j = base + t;
+        uint32_t nch = 0;
+        if (j < S.n_jobs) {
+            const JobDesc &J = S.jobs[j];
+            if (S.seg_start < J.n_turns) {
+                const uint32_t live = S.first ? J.n : S.count_cur[j];
+                nch = (live + 63u) / 64u;
+            }
+            S.count_next[j] = 0;
+        }
+        // inclusive scan within the wavefront, then across wavefronts
+        uint32_t v = nch;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t u = __shfl_up(v, d, 64);
+            if ((int)lane >= d) v += u;
+        }
+        if (lane == 63) wave_sum[wv] = v;
+        __syncthreads();
+        uint32_t before = carry;
+        for (uint32_t w = 0; w < wv; ++w) before += wave_sum[w];
+        if (j < S.n_jobs) S.chunk_off[j] = before + v - nch;
+        __syncthreads();
+        if (t == kPlanThreads - 1) carry = before + v;
+        __syncthreads();
+    }
+    if (t == 0) {
+        S.chunk_off[S.n_jobs] = carry;
+        S.queue->total = carry;
+        S.queue->head = 0;
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256, 8) k_batch_track(BatchSeg S)
+{
+    const uint32_t lane = __lane_id();
+    // written by the plan launch, read-only here: scalar loads
+    const ConstPtr<uint32_t> chunk_off = as_const(S.chunk_off);
+    const ConstPtr<uint32_t> count_cur = as_const(S.count_cur);
+    const ConstPtr<JobDesc> jobs = as_const(S.jobs);
+    const uint32_t total = *as_const(&S.queue->total);
+    for (;;) {
+        uint32_t c = 0;
+        if (lane == 0) c = atomicAdd(&S.queue->head, 1u);
+        c = __builtin_amdgcn_readfirstlane(c);
+        if (c >= total) break;
+        // job = the last j with chunk_off[j] <= c (jobs with no chunk are skipped)
+        uint32_t lo = 0, hi = S.n_jobs;  // chunk_off[lo] <= c < chunk_off[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (chunk_off[mid] <= c) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t job = lo;
+        const ConstPtr<JobDesc> J = jobs + job;
+        const uint32_t n = S.first ? J->n : count_cur[job];
+        const uint32_t base = (c - chunk_off[job]) * 64u;
+        const uint32_t n_turns = J->n_turns, list_off = J->list_off;
+        const uint32_t turns = min(S.seg_len, n_turns - S.seg_start);
+        const bool last = S.seg_start + S.seg_len >= n_turns;
+        const double mu = J->mu;
+        TrackArgs A{J->x, J->px, J->y, J->py, J->steps,
+                    S.first ? nullptr : S.lists_cur + list_off, nullptr, n, 0u,
+                    J->tab, J->tab_len, turns, mu, J->b2,
+                    last ? nullptr : S.lists_next + list_off, last ? nullptr : S.count_next + job, S.status};
+        if (mu != 0.0) track_wave<MODE, false, true, 1>(A, base, n);
+        else track_wave<MODE, false, false, 1>(A, base, n);
+    }
+}
+
+static uint32_t batch_resident_waves()
+{
+    int dev = 0, cus = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    return (uint32_t)cus * 32u;  // 4 SIMDs x 8 wavefronts (the kernel stays within 64 VGPRs)
+}
+
+}  // namespace hm
+
+using namespace hm;
+
+static constexpr uint32_t kBatchFirstSegment = 125;
+
+extern "C" int hm_track_batch(const hm_job *jobs, size_t n_jobs, int mode, const hm_batch_cfg *cfg, void *stream)
+{
+    return guarded([&] {
+        if (mode != HM_MODE_EXACT && mode != HM_MODE_FAST) throw Error(HM_ERR_ARG, "batch: unknown mode");
+        if (n_jobs == 0) return;
+        if (!jobs) throw Error(HM_ERR_ARG, "batch: NULL job array");
+        if (n_jobs > (1u << 24)) throw Error(HM_ERR_ARG, "batch: too many jobs");
+        const int every = cfg && cfg->compact_every ? cfg->compact_every : 1000;
+        if (every <= 0) throw Error(HM_ERR_ARG, "batch: compact_every must be positive");
+        const int block = cfg && cfg->block ? cfg->block : 256;
+        if (block < 64 || block > 256 || block % 64) throw Error(HM_ERR_ARG, "batch: block must be 64, 128, 192 or 256");
+        uint32_t waves = cfg && cfg->waves > 0 ? (uint32_t)cfg->waves : batch_resident_waves();
+        const bool graph = cfg && cfg->use_graph;
+
+        std::vector<JobDesc> desc(n_jobs);
+        uint64_t total_n = 0;
+        uint32_t max_turns = 0;
+        for (size_t k = 0; k < n_jobs; ++k) {
+            const hm_job &j = jobs[k];
+            if (j.n > 0x7fffffffull) throw Error(HM_ERR_ARG, "batch: a job exceeds 2^31 particles");
+            if (j.n && (!j.x || !j.px || !j.y || !j.py || !j.steps || !j.table))
+                throw Error(HM_ERR_ARG, "batch: NULL pointer in job " + std::to_string(k));
+            if (j.table_len > 0xffffffffull) throw Error(HM_ERR_ARG, "batch: table exceeds 2^32 turns");
+            if (!std::isnan(j.kick_module)) throw Error(HM_ERR_UNSUPPORTED, "batch: the random kick is not available");
+            JobDesc &d = desc[k];
+            d.x = j.x; d.px = j.px; d.y = j.y; d.py = j.py; d.steps = j.steps;
+            d.tab = reinterpret_cast<const TabEntry *>(j.table);
+            d.n = (uint32_t)j.n;
+            d.n_turns = j.n ? j.n_turns : 0;
+            d.tab_len = (uint32_t)j.table_len;
+            d.list_off = (uint32_t)total_n;
+            d.mu = j.mu;
+            d.b2 = j.barrier * j.barrier;  // henon.cu:2717
+            total_n += j.n;
+            if (total_n > 0xffffffffull) throw Error(HM_ERR_ARG, "batch: more than 2^32 particles in total");
+            max_turns = std::max(max_turns, d.n_turns);
+        }
+        if (max_turns == 0) return;
+
+        // segment schedule
+        std::vector<std::pair<uint32_t, uint32_t>> seg;  // (start, length)
+        for (uint64_t done = 0, len = std::min<uint32_t>(kBatchFirstSegment, (uint32_t)every); done < max_turns;) {
+            seg.emplace_back((uint32_t)done, (uint32_t)len);
+            done += len;
+            len = std::min<uint64_t>(len * 2, (uint64_t)every);
+        }
+
+        // device buffers (workspace slot 4): descriptors | lists x2 | counts x2 | scan | queue
+        auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
+        const size_t b_desc = align(sizeof(JobDesc) * n_jobs);
+        const size_t b_list = align(sizeof(uint32_t) * std::max<uint64_t>(total_n, 1));
+        const size_t b_cnt = align(sizeof(uint32_t) * n_jobs);
+        const size_t b_off = align(sizeof(uint32_t) * (n_jobs + 1));
+        const size_t b_q = align(sizeof(BatchQueue));
+        char *ws = static_cast<char *>(workspace(b_desc + 2 * b_list + 2 * b_cnt + b_off + b_q, 4));
+        JobDesc *d_desc = reinterpret_cast<JobDesc *>(ws);
+        uint32_t *lists[2] = {reinterpret_cast<uint32_t *>(ws + b_desc),
+                              reinterpret_cast<uint32_t *>(ws + b_desc + b_list)};
+        uint32_t *counts[2] = {reinterpret_cast<uint32_t *>(ws + b_desc + 2 * b_list),
+                               reinterpret_cast<uint32_t *>(ws + b_desc + 2 * b_list + b_cnt)};
+        uint32_t *d_off = reinterpret_cast<uint32_t *>(ws + b_desc + 2 * b_list + 2 * b_cnt);
+        BatchQueue *d_q = reinterpret_cast<BatchQueue *>(ws + b_desc + 2 * b_list + 2 * b_cnt + b_off);
+
+        hipStream_t s = as_stream(stream);
+        hip_check(hipMemcpyAsync(d_desc, desc.data(), sizeof(JobDesc) * n_jobs, hipMemcpyHostToDevice, s),
+                  "hipMemcpyAsync(jobs)");
+        // the descriptors are read from pageable memory: make sure the copy has
+        // consumed them before `desc` goes out of scope
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+
+        waves = std::max<uint32_t>(waves, (uint32_t)block / 64u);
+        const uint32_t grid = (waves * 64u + (uint32_t)block - 1) / (uint32_t)block;
+        uint32_t *status = status_word();
+        auto enqueue = [&](hipStream_t q) {
+            for (size_t i = 0; i < seg.size(); ++i) {
+                BatchSeg S{d_desc, (uint32_t)n_jobs, seg[i].first, seg[i].second, i == 0 ? 1u : 0u,
+                           lists[i % 2], lists[(i + 1) % 2], counts[i % 2], counts[(i + 1) % 2], d_off, d_q, status};
+                hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(kPlanThreads), 0, q, S);
+                launch_check("k_batch_plan");
+                if (mode == HM_MODE_EXACT)
+                    hipLaunchKernelGGL(k_batch_track<kExact>, dim3(grid), dim3(block), 0, q, S);
+                else
+                    hipLaunchKernelGGL(k_batch_track<kFast>, dim3(grid), dim3(block), 0, q, S);
+                launch_check("k_batch_track");
+            }
+        };
+        if (!graph) {
+            LaunchTimer timer("k_batch_track", s);
+            enqueue(s);
+            return;
+        }
+        // record the whole segment sequence once and replay it as one graph
+        hipStream_t cs = nullptr;
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        hip_check(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate");
+        try {
+            hip_check(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            enqueue(cs);
+            hip_check(hipStreamEndCapture(cs, &g), "hipStreamEndCapture");
+            hip_check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+            {
+                LaunchTimer timer("k_batch_track", s);
+                hip_check(hipGraphLaunch(ge, s), "hipGraphLaunch");
+            }
+            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        } catch (...) {
+            if (ge) (void)hipGraphExecDestroy(ge);
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipStreamDestroy(cs);
+            throw;
+        }
+        (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
+        (void)hipStreamDestroy(cs);
+    });
+}

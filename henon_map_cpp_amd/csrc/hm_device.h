@@ -22,6 +22,22 @@ struct __attribute__((aligned(32))) TabEntry {
 
 enum Mode : int { kExact = 0, kFast = 1 };
 
+// Pointer into the constant address space: for data that no wavefront writes
+// during the launch. Wave-uniform loads through it become scalar loads.
+template <class T>
+using ConstPtr = const __attribute__((address_space(4))) T *;
+
+template <class T>
+__device__ __forceinline__ ConstPtr<T> as_const(const T *p)
+{
+    return (ConstPtr<T>)p;
+}
+
+__device__ __forceinline__ TabEntry load_record(ConstPtr<TabEntry> r)
+{
+    return TabEntry{r->sx, r->cx, r->sy, r->cy};
+}
+
 // ---------------------------------------------------------------- forward map
 // henon.cu:131-173 (sextupole, octupole, rotation), forward direction.
 template <bool OCT>

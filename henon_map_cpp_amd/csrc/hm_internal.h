@@ -66,7 +66,11 @@ inline void launch_check(const char *name)
 }
 
 // Library-owned device words, per device: the status flags word (see
-// hm_take_status) and growable scratch used for compaction lists.
+// hm_take_status) and growable scratch buffers, one per slot: 0 tracking live
+// lists, 1 Birkhoff weights of the indicator kernels, 2 tune-window weights,
+// 3 DFT magnitudes, 4 batched-job lists and plans. Growing a slot synchronises
+// the device. Calls that overlap in time on one device share these buffers.
+constexpr int kWorkspaceSlots = 8;
 uint32_t *status_word();
 void *workspace(size_t bytes, int slot);
 

@@ -78,8 +78,8 @@ namespace {
 std::mutex g_ws_mu;
 struct DeviceScratch {
     uint32_t *status = nullptr;
-    void *slots[4] = {nullptr, nullptr, nullptr, nullptr};
-    size_t sizes[4] = {0, 0, 0, 0};
+    void *slots[kWorkspaceSlots] = {};
+    size_t sizes[kWorkspaceSlots] = {};
 };
 std::map<int, DeviceScratch> g_scratch;
 }  // namespace
@@ -99,7 +99,7 @@ uint32_t *status_word()
 
 void *workspace(size_t bytes, int slot)
 {
-    if (slot < 0 || slot >= 4) throw Error(HM_ERR_ARG, "workspace slot out of range");
+    if (slot < 0 || slot >= kWorkspaceSlots) throw Error(HM_ERR_ARG, "workspace slot out of range");
     int dev = 0;
     hip_check(hipGetDevice(&dev), "hipGetDevice");
     std::lock_guard<std::mutex> lk(g_ws_mu);

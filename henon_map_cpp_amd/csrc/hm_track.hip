@@ -5,229 +5,16 @@
 #include <cmath>
 #include <cstdint>
 #include <algorithm>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "hm_device.h"
 #include "hm_internal.h"
+#include "hm_wave.h"
 
 namespace hm {
-
-constexpr uint32_t kFlagRange = 1u;
-
-// Per-lane fallback used when the live lanes of a wavefront do not share one
-// turn index (particles reset while others were not) or when that index range
-// leaves the table: each lane reads its own table record, and a lane whose
-// index would leave [0, tab_len) stops unchanged and raises kFlagRange.
-template <int MODE, bool INV, bool OCT>
-__device__ __forceinline__ void track_lanes(double &x, double &px, double &y, double &py, bool &alive,
-                                                      uint32_t &done, uint32_t s, const TabEntry *__restrict__ tab,
-                                                      uint32_t tab_len, uint32_t n_turns, double mu, double b2,
-                                                      uint32_t *status)
-{
-    const double m3mu = -3.0 * mu;
-    for (uint32_t k = 0; k < n_turns; ++k) {
-        if (alive) {
-            const int64_t ti = INV ? (int64_t)s - 1 - (int64_t)k : (int64_t)s + (int64_t)k;
-            if (ti < 0 || ti >= (int64_t)tab_len) {
-                atomicOr(status, kFlagRange);
-                alive = false;
-                done = k;
-            } else {
-                const TabEntry t = tab[ti];
-                map_step<MODE, INV, OCT>(x, px, y, py, mu, m3mu, t);
-                if (!(radius2<MODE>(x, px, y, py) < b2)) {
-                    x = px = y = py = __builtin_nan("");
-                    alive = false;
-                    done = k;
-                }
-            }
-        }
-        if (!__any(alive)) break;
-    }
-}
-
-// P particles per lane, n_turns turns in registers. Wavefront w owns the 64*P
-// consecutive entries [64*P*w, 64*P*(w+1)) of the particle list (or of the
-// live-index list `idx_in`), lane l the entries 64*q + l. The particle count is
-// read from device memory (`count_in`) when given, so a launch can be sized
-// from an upper bound without a host round trip.
-//
-// The first live lane's turn index s0 is read into a scalar register; when every
-// live particle of the wavefront shares it (the normal case: all particles of a
-// tracker advance together), the table record of turn s0+k is the same for the
-// whole wavefront and is fetched with a scalar load, an iteration ahead, straight
-// into SGPR operands of the VALU ops. Losses are kept as wave-uniform lane masks,
-// so a turn costs the map's VALU ops plus one compare per particle.
-struct TrackArgs {
-    double *gx, *gpx, *gy, *gpy;
-    uint32_t *gsteps;
-    const uint32_t *idx_in, *count_in;
-    uint32_t n_arg;
-    uint32_t slots;  // wavefronts launched in mixed mode (0 = one particle per lane everywhere)
-    const TabEntry *tab;
-    uint32_t tab_len, n_turns;
-    double mu, b2;
-    uint32_t *idx_out, *count_out, *status;
-};
-
-template <int MODE, bool INV, bool OCT, int P>
-__device__ __forceinline__ void track_wave(const TrackArgs &A, uint32_t base, uint32_t n)
-{
-    double *__restrict__ gx = A.gx, *__restrict__ gpx = A.gpx, *__restrict__ gy = A.gy, *__restrict__ gpy = A.gpy;
-    uint32_t *__restrict__ gsteps = A.gsteps;
-    const uint32_t *__restrict__ idx_in = A.idx_in;
-    const TabEntry *__restrict__ tab = A.tab;
-    const uint32_t tab_len = A.tab_len, n_turns = A.n_turns;
-    const double mu = A.mu, b2 = A.b2;
-    uint32_t *__restrict__ idx_out = A.idx_out;
-    uint32_t *__restrict__ count_out = A.count_out;
-    uint32_t *__restrict__ status = A.status;
-    const uint32_t lane = __lane_id();
-    double x[P], px[P], y[P], py[P];
-    uint32_t j[P], s[P], done[P];
-    bool alive0[P];
-    uint64_t live[P];
-    const uint64_t my_bit = 1ull << lane;
-    uint64_t any_live = 0;
-#pragma unroll
-    for (int q = 0; q < P; ++q) {
-        const uint32_t i = base + 64u * q + lane;
-        const bool in_range = i < n;
-        j[q] = 0;
-        s[q] = 0;
-        x[q] = px[q] = y[q] = py[q] = __builtin_nan("");
-        if (in_range) {
-            j[q] = idx_in ? idx_in[i] : i;
-            x[q] = gx[j[q]];
-            px[q] = gpx[j[q]];
-            y[q] = gy[j[q]];
-            py[q] = gpy[j[q]];
-            s[q] = gsteps[j[q]];
-        }
-        // henon.cu:431-434: a particle with a NaN coordinate is not stepped
-        alive0[q] = in_range && !any_nan(x[q], px[q], y[q], py[q]);
-        live[q] = __ballot(alive0[q]);
-        any_live |= live[q];
-        done[q] = n_turns;
-    }
-    if (!any_live) return;
-
-    const double m3mu = -3.0 * mu;
-    // turn index of the first live particle (q-major), and whether all share it
-    int qf = 0;
-#pragma unroll
-    for (int q = P - 1; q >= 0; --q)
-        if (live[q]) qf = q;
-    uint32_t sf = 0;
-#pragma unroll
-    for (int q = 0; q < P; ++q)
-        if (q == qf) sf = s[q];
-    uint64_t lf = 0;
-#pragma unroll
-    for (int q = 0; q < P; ++q)
-        if (q == qf) lf = live[q];
-    const uint32_t s0 = __builtin_amdgcn_readlane(sf, __builtin_ctzll(lf));
-    bool same = true;
-#pragma unroll
-    for (int q = 0; q < P; ++q) same = same && __all(!alive0[q] || s[q] == s0);
-    const bool in_table = INV ? (s0 >= n_turns) : ((uint64_t)s0 + n_turns <= (uint64_t)tab_len);
-
-    if (same && in_table && n_turns > 0) {
-        const TabEntry *tp = INV ? tab + (s0 - 1) : tab + s0;
-        // Two turns per iteration: the allocator alternates the state registers
-        // instead of copying them back, and the records of the next two turns
-        // are requested one iteration ahead.
-        auto rec = [&](uint32_t k) -> TabEntry { return INV ? *(tp - k) : tp[k]; };
-        const uint32_t last = n_turns - 1;
-        TabEntry n0 = rec(0), n1 = rec(last < 1 ? last : 1);
-        uint32_t k = 0;
-        bool more = true;
-        // exact forward mode carries x*x, y*y from one turn to the next
-        constexpr bool CARRY = MODE == kExact && !INV;
-        double xx[P], yy[P];
-#pragma unroll
-        for (int q = 0; q < P; ++q) {
-            xx[q] = x[q] * x[q];
-            yy[q] = y[q] * y[q];
-        }
-        // exact mode screens losses with a fused radius (screen_radius2) and
-        // confirms candidates with the reference's expression
-        constexpr bool SCREEN = MODE == kExact;
-        const double b2s = SCREEN ? screen_bound(b2) : b2;
-        auto turn = [&](const TabEntry &t, uint32_t kk) {
-            double r2[P];
-#pragma unroll
-            for (int q = 0; q < P; ++q) {
-                if (CARRY) {
-                    r2[q] = fwd_exact_carry<OCT>(x[q], px[q], y[q], py[q], xx[q], yy[q], mu, t);
-                } else {
-                    map_step<MODE, INV, OCT>(x[q], px[q], y[q], py[q], mu, m3mu, t);
-                    r2[q] = SCREEN ? screen_radius2(x[q], px[q], y[q], py[q]) : radius2<MODE>(x[q], px[q], y[q], py[q]);
-                }
-            }
-            uint64_t all = 0;
-#pragma unroll
-            for (int q = 0; q < P; ++q) {
-                uint64_t lost = live[q] & ~__ballot(r2[q] < b2s);
-                if (SCREEN && lost) {
-                    const double e = CARRY ? exact_radius2(xx[q], px[q], yy[q], py[q])
-                                           : exact_radius2(x[q] * x[q], px[q], y[q] * y[q], py[q]);
-                    lost &= ~__ballot(e < b2);
-                }
-                if (lost) {
-                    if (lost & my_bit) done[q] = kk;
-                    live[q] &= ~lost;
-                }
-                all |= live[q];
-            }
-            more = all != 0;
-        };
-        for (; k + 1 < n_turns; k += 2) {
-            const TabEntry t0 = n0, t1 = n1;
-            n0 = rec(k + 2 <= last ? k + 2 : last);
-            n1 = rec(k + 3 <= last ? k + 3 : last);
-            turn(t0, k);
-            if (!more) break;
-            turn(t1, k + 1);
-            if (!more) break;
-        }
-        if (more && k == last) turn(n0, k);
-#pragma unroll
-        for (int q = 0; q < P; ++q)
-            if (alive0[q] && !(live[q] & my_bit)) x[q] = px[q] = y[q] = py[q] = __builtin_nan("");
-    } else if (n_turns > 0) {
-#pragma unroll
-        for (int q = 0; q < P; ++q) {
-            bool a = alive0[q];
-            track_lanes<MODE, INV, OCT>(x[q], px[q], y[q], py[q], a, done[q], s[q], tab, tab_len, n_turns, mu, b2,
-                                        status);
-            live[q] = __ballot(a);
-        }
-    }
-
-#pragma unroll
-    for (int q = 0; q < P; ++q) {
-        if (alive0[q]) {
-            gx[j[q]] = x[q];
-            gpx[j[q]] = px[q];
-            gy[j[q]] = y[q];
-            gpy[j[q]] = py[q];
-            gsteps[j[q]] = INV ? s[q] - done[q] : s[q] + done[q];
-        }
-        if (idx_out) {
-            // wavefront-aggregated append of the survivors to the next live list
-            const uint64_t m = live[q];
-            if (m) {
-                const int leader = __builtin_ctzll(m);
-                uint32_t slot = 0;
-                if ((int)lane == leader) slot = atomicAdd(count_out, (uint32_t)__popcll(m));
-                slot = __builtin_amdgcn_readlane(slot, leader);
-                if (m & my_bit) idx_out[slot + __popcll(m & (my_bit - 1ull))] = j[q];
-            }
-        }
-    }
-}
 
 // Wavefront w owns 64*P consecutive list entries. In mixed mode (slots > 0,
 // exactly `slots` wavefronts launched) the first w2 wavefronts take two particles
@@ -368,6 +155,37 @@ static constexpr int kDefaultBlock = 256;
 static constexpr int kDefaultCompactEvery = 1000;  // turns between compactions
 static constexpr uint32_t kFirstSegment = 125;     // first segments are short: fast losses go early
 
+// Launch configurations found by hm_autotune_track, per device, arithmetic
+// mode, octupole on/off and particle-count class (floor(log2 n)).
+struct TuneKey {
+    int dev, mode, oct, nbits;
+    bool operator<(const TuneKey &o) const
+    {
+        return std::tie(dev, mode, oct, nbits) < std::tie(o.dev, o.mode, o.oct, o.nbits);
+    }
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, hm_launch_cfg> g_tuned;
+
+static TuneKey tune_key(int mode, bool oct, size_t n)
+{
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    int nbits = 0;
+    while ((size_t(2) << nbits) <= n) ++nbits;
+    return TuneKey{dev, mode, oct ? 1 : 0, nbits};
+}
+
+static bool tuned_config(int mode, bool oct, size_t n, hm_launch_cfg *out)
+{
+    const TuneKey k = tune_key(mode, oct, n);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(k);
+    if (it == g_tuned.end()) return false;
+    *out = it->second;
+    return true;
+}
+
 // Pinned host words receiving the per-segment live counts.
 static uint32_t *pinned_counts(size_t n)
 {
@@ -396,6 +214,8 @@ extern "C" int hm_track(double *x, double *px, double *y, double *py, uint32_t *
         if (n == 0 || n_turns == 0) return;
         if (!x || !px || !y || !py || !steps || !table) throw Error(HM_ERR_ARG, "track: NULL pointer");
         if (n > 0x7fffffffull || table_len > 0xffffffffull) throw Error(HM_ERR_ARG, "track: size exceeds 2^31");
+        hm_launch_cfg tuned{};
+        if (!cfg && tuned_config(mode, mu != 0.0, n, &tuned)) cfg = &tuned;
         const int block = cfg && cfg->block ? cfg->block : kDefaultBlock;
         if (block < 64 || block > 256 || block % 64) throw Error(HM_ERR_ARG, "track: block must be 64, 128, 192 or 256");
         const int every = cfg && cfg->compact_every ? cfg->compact_every : kDefaultCompactEvery;
@@ -475,6 +295,93 @@ extern "C" int hm_track(double *x, double *px, double *y, double *py, uint32_t *
             throw;
         }
         cleanup();
+    });
+}
+
+extern "C" int hm_autotune_track(const double *x, const double *px, const double *y, const double *py,
+                                 const uint32_t *steps, size_t n, const double *table, size_t table_len,
+                                 uint32_t warm_turns, uint32_t trial_turns, double mu, double barrier, int mode,
+                                 hm_launch_cfg *best, double *best_ms, void *stream)
+{
+    return guarded([&] {
+        if (!best) throw Error(HM_ERR_ARG, "autotune: NULL result");
+        if (mode != HM_MODE_EXACT && mode != HM_MODE_FAST) throw Error(HM_ERR_ARG, "autotune: unknown mode");
+        if (!x || !px || !y || !py || !steps || !table) throw Error(HM_ERR_ARG, "autotune: NULL pointer");
+        if (n == 0 || n > 0x7fffffffull || trial_turns == 0) throw Error(HM_ERR_ARG, "autotune: empty trial");
+        hipStream_t s = as_stream(stream);
+        // the trial runs on copies: the caller's particles are not touched
+        std::vector<uint32_t> host_steps(n);
+        hip_check(hipMemcpyAsync(host_steps.data(), steps, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+                  "hipMemcpyAsync(steps)");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        const uint64_t max_steps = *std::max_element(host_steps.begin(), host_steps.end());
+        if (max_steps + 2 > table_len) throw Error(HM_ERR_RANGE, "autotune: no turns left in the table");
+        const uint64_t room = table_len - max_steps;
+        warm_turns = (uint32_t)std::min<uint64_t>(warm_turns, room / 2);
+        trial_turns = (uint32_t)std::min<uint64_t>(trial_turns, room - warm_turns);
+        const size_t state = n * (4 * sizeof(double) + sizeof(uint32_t));
+        char *work = static_cast<char *>(workspace(state, 5));
+        char *warm = static_cast<char *>(workspace(state, 6));
+        double *w[4];
+        for (int c = 0; c < 4; ++c) w[c] = reinterpret_cast<double *>(work + c * n * sizeof(double));
+        uint32_t *ws = reinterpret_cast<uint32_t *>(work + 4 * n * sizeof(double));
+        const double *src[4] = {x, px, y, py};
+        for (int c = 0; c < 4; ++c)
+            hip_check(hipMemcpyAsync(w[c], src[c], n * sizeof(double), hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+        hip_check(hipMemcpyAsync(ws, steps, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+        const double nan = std::nan("");
+        hm_launch_cfg base{};
+        base.particles_per_lane = mode == HM_MODE_EXACT ? 1 : 2;
+        if (warm_turns) {
+            const int rc = hm_track(w[0], w[1], w[2], w[3], ws, n, table, table_len, warm_turns, mu, barrier, nan, 0,
+                                    mode, &base, stream);
+            if (rc != HM_OK) throw Error(rc, hm_last_error());
+        }
+        hip_check(hipMemcpyAsync(warm, work, state, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync(snapshot)");
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        float best_t = -1.0f;
+        hm_launch_cfg best_c = base;
+        try {
+            for (int block : {64, 256})
+                for (int every : {500, 1000, 2000})
+                    for (int pair : {1, 2}) {
+                        hm_launch_cfg c{block, every, pair, 0};
+                        hip_check(hipMemcpyAsync(work, warm, state, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+                        hip_check(hipEventRecord(e0, s), "hipEventRecord");
+                        const int rc = hm_track(w[0], w[1], w[2], w[3], ws, n, table, table_len, trial_turns, mu,
+                                                barrier, nan, 0, mode, &c, stream);
+                        if (rc != HM_OK) throw Error(rc, hm_last_error());
+                        hip_check(hipEventRecord(e1, s), "hipEventRecord");
+                        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                        float ms = 0.0f;
+                        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                        if (best_t < 0.0f || ms < best_t) {
+                            best_t = ms;
+                            best_c = c;
+                        }
+                    }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *best = best_c;
+        if (best_ms) *best_ms = best_t;
+        const TuneKey k = tune_key(mode, mu != 0.0, n);
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[k] = best_c;
+    });
+}
+
+extern "C" int hm_autotune_clear(void)
+{
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned.clear();
     });
 }
 

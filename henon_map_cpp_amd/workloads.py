@@ -56,3 +56,47 @@ UNMODULATED_DA = dict(name="unmodulated DA scan 100x100, 1e4 turns, mu=0", n_sid
                       tune_x=0.168, tune_y=0.201, epsilon=0.0, mu=0.0, barrier=float(np.sqrt(10.0)), kind="sps")
 MODULATED_DA = dict(name="modulated DA scan 1000x1000, 1e5 turns, mu=0.5, eps=32", n_side=1000, n_turns=100_000,
                     tune_x=0.168, tune_y=0.201, epsilon=32.0, mu=0.5, barrier=float(np.sqrt(10.0)), kind="custom")
+CHAOS_INDICATORS = dict(name="chaos indicators: 256x256 grid + 65536 ball points r<0.5, 1e5 turns, mu=0.5, eps=32, "
+                             "renormalisation every 100 turns",
+                        n_side=256, n_ball=65_536, r_ball=0.5, n_turns=100_000, renorm_every=100,
+                        tune_x=0.168, tune_y=0.201, epsilon=32.0, mu=0.5, barrier=float(np.sqrt(10.0)),
+                        samples=(1_000, 10_000, 100_000), seed=3)
+ADJOINT_GRADIENTS = dict(name="adjoint gradients: 65536 points, r ~ U[0.05, 0.4], 1e4 turns, mu=0.5, eps=16",
+                         n=65_536, r_min=0.05, r_max=0.4, n_turns=10_000, tune_x=0.168, tune_y=0.201, epsilon=16.0,
+                         mu=0.5, barrier=float(np.sqrt(10.0)), every=(64, 256, 1024), seed=4)
+BATCHED_JOBS = dict(name="batched uneven jobs: N ~ logU[64, 65536], T ~ logU[1e3, 1e5], tunes U[0.15, 0.32], "
+                         "eps U[0, 64], mu U[-1, 1], r < 0.6, barrier r^2 > 10",
+                    n_range=(64, 65_536), t_range=(1_000, 100_000), tune_range=(0.15, 0.32), eps_range=(0.0, 64.0),
+                    mu_range=(-1.0, 1.0), r_max=0.6, barrier=float(np.sqrt(10.0)), seed=5)
+
+
+def indicator_points(c=CHAOS_INDICATORS):
+    """The DA grid followed by uniform points of the 4D ball."""
+    g = da_grid(c["n_side"])
+    b = ball(c["n_ball"], c["r_ball"], seed=c["seed"])
+    return tuple(np.concatenate([u, v]) for u, v in zip(g, b))
+
+
+def shell_uniform_radius(n: int, r_min: float, r_max: float, seed: int):
+    """n points with radius ~ U[r_min, r_max] and a uniform direction on S^3."""
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n, 4))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = rng.uniform(r_min, r_max, n)
+    p = d * r[:, None]
+    return p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), p[:, 3].copy()
+
+
+def batch_job_params(n_jobs: int, c=BATCHED_JOBS):
+    """Seeded per-job parameters: (n, n_turns, tune_x, tune_y, epsilon, mu, seed)."""
+    rng = np.random.default_rng(c["seed"])
+    lo_n, hi_n = np.log(c["n_range"][0]), np.log(c["n_range"][1])
+    lo_t, hi_t = np.log(c["t_range"][0]), np.log(c["t_range"][1])
+    out = []
+    for k in range(n_jobs):
+        n = int(round(np.exp(rng.uniform(lo_n, hi_n))))
+        t = int(round(np.exp(rng.uniform(lo_t, hi_t))))
+        qx, qy = rng.uniform(*c["tune_range"], 2)
+        out.append((n, t, float(qx), float(qy), float(rng.uniform(*c["eps_range"])),
+                    float(rng.uniform(*c["mu_range"])), int(rng.integers(1 << 31))))
+    return out

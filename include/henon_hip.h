@@ -111,12 +111,60 @@ typedef struct hm_launch_cfg {
  *   inverse             : 0 forward (table index steps), 1 inverse (steps-1)
  *   mode                : HM_MODE_EXACT (bit-exact with the reference CPU path)
  *                         or HM_MODE_FAST (fused multiply-adds)
- *   cfg                 : NULL for the defaults
+ *   cfg                 : NULL for the configuration hm_autotune_track chose for this
+ *                         device, mode and size class, else the defaults
  * The caller owns the global turn counter and its range check (the Python/C++
  * layer raises "The number of turns is too large."). */
 int hm_track(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n,
              const double *table, size_t table_len, uint32_t n_turns, double mu, double barrier,
              double kick_module, int inverse, int mode, const hm_launch_cfg *cfg, void *stream);
+
+/* Autotuner (no reference counterpart). Times hm_track launch configurations
+ * (block 64 / 256, compact_every 500 / 1000 / 2000, one or two particles per
+ * lane) on a copy of the given particles: first warm_turns turns with the
+ * defaults (so that the fast early losses are behind), then trial_turns turns
+ * from that state per candidate. Returns the fastest configuration in *best
+ * (its trial time in *best_ms) and remembers it for this device, mode, mu != 0
+ * and particle-count class (floor(log2 n)): hm_track with cfg == NULL uses the
+ * remembered configuration. The caller's particles are not modified; every
+ * configuration gives bit-identical results. Synchronises `stream`. */
+int hm_autotune_track(const double *x, const double *px, const double *y, const double *py,
+                      const uint32_t *steps, size_t n, const double *table, size_t table_len, uint32_t warm_turns,
+                      uint32_t trial_turns, double mu, double barrier, int mode, hm_launch_cfg *best,
+                      double *best_ms, void *stream);
+/* Forget every remembered configuration (hm_track with cfg == NULL returns to
+ * the defaults). */
+int hm_autotune_clear(void);
+
+/* ---- batched jobs ---------------------------------------------------------- */
+/* One independent tracking job: its own device particles, table, turn count,
+ * mu and barrier (the arguments of one hm_track call, forward direction). */
+typedef struct hm_job {
+    double *x, *px, *y, *py;    /* device SoA, updated in place */
+    uint32_t *steps;            /* device; table index and survival turns */
+    size_t n;                   /* particles */
+    const double *table;        /* device table[table_len][4] */
+    size_t table_len;
+    uint32_t n_turns;
+    uint32_t reserved;
+    double mu, barrier;
+    double kick_module;         /* must be NaN */
+} hm_job;
+
+typedef struct hm_batch_cfg {
+    int32_t block;         /* threads per workgroup: 64..256 (0 = 256) */
+    int32_t compact_every; /* turns per segment (0 = 1000); segments start at 125 turns */
+    int32_t waves;         /* persistent wavefronts per launch (0 = CUs x 32, one resident round) */
+    int32_t use_graph;     /* 1: record the launch sequence in a HIP graph, replay it and wait */
+} hm_batch_cfg;
+
+/* Track every job for its own n_turns, all jobs in one sequence of persistent
+ * launches fed by a device work queue of 64-particle chunks, with loss
+ * compaction per job between segments. Results are identical to one hm_track
+ * call per job with the same mode. No counterpart in the reference, which tracks
+ * one particle set per call (henon_tracker::track, src/henon.cu:2701-2752,
+ * :3155-3185). jobs: host array. Asynchronous on `stream` unless use_graph. */
+int hm_track_batch(const hm_job *jobs, size_t n_jobs, int mode, const hm_batch_cfg *cfg, void *stream);
 
 /* ---- readback observables ------------------------------------------------ */
 enum {

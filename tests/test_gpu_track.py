@@ -234,3 +234,24 @@ def test_tangent_matrix_kernel(reverse):
         np.testing.assert_allclose(got, ref, rtol=1e-15, atol=1e-16, equal_nan=True)
     else:
         assert np.array_equal(got, ref, equal_nan=True)
+
+
+def test_autotune_leaves_inputs_and_results_unchanged():
+    """hm_autotune_track works on copies; the configuration it remembers (used by
+    hm_track with cfg=None) gives the oracle's results bit for bit."""
+    c = W.MODULATED_DA
+    x, px, y, py = W.da_grid(120)
+    tables, tab = make_tables(3000, c["tune_x"], c["tune_y"], "custom", c["epsilon"])
+    tdev = C.DeviceBuffer.from_array(tab)
+    dev = C.DeviceParticles(x, px, y, py)
+    try:
+        best, ms = C.autotune_track(dev, tdev, len(tab), c["mu"], c["barrier"], warm_turns=500, trial_turns=1000)
+        assert best.block in (64, 256) and best.compact_every in (500, 1000, 2000)
+        assert best.particles_per_lane in (1, 2) and ms > 0
+        before = dev.get()
+        assert all(np.array_equal(a, b) for a, b in zip(before[:4], (x, px, y, py)))
+        assert np.all(before[4] == 0)
+        ref, got = run_both(x, px, y, py, tables, tab, 3000, c["mu"], c["barrier"])
+        assert_bit_exact(ref, got)
+    finally:
+        C.check(C.lib().hm_autotune_clear())
