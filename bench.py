@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (FMA = 2 flops)
+HBM_PEAK_GBPS = 8000.0   # MI355X HBM3E
 FLOPS_PER_PT = {0.0: 26, None: 40}  # mu == 0 / mu != 0, per live particle-turn
 FLOPS_TANGENT = 103  # J entries + structured J.M, per live particle-turn
 FLOPS_MEGNO = 35     # Frobenius norm ratio construct, per live particle-turn
@@ -46,7 +47,7 @@ def parse():
     ap.add_argument("--cpu-sample-stride", type=int, default=13)
     ap.add_argument("--single-mode", action="store_true", help="time only --mode (profiling runs)")
     ap.add_argument("--compact-every", type=int, default=0)
-    ap.add_argument("--legs", default="indicators,adjoint,batch",
+    ap.add_argument("--legs", default="indicators,tunes,adjoint,batch",
                     help="extra measurements after the headline: indicators, adjoint, batch (comma list, or none)")
     ap.add_argument("--batch-jobs", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -183,6 +184,49 @@ def indicators_leg(steps, warmup):
         "achieved": live * fpt / ks / 1e12,
         "frac": live * fpt / ks / 1e12 / FP64_PEAK_TFLOPS,
         "kernel_ms_per_step": kms / steps,
+    }
+
+
+def tunes_leg():
+    """Trajectory store + 2048-sample FFT tunes and Birkhoff tunes of the
+    chaos-indicator particle set (256x256 grid + 65536 ball points): the
+    HBM-bound stages. Bytes: 32 B per particle-sample written by k_history and
+    read by each tune kernel."""
+    from henon_map_cpp_amd import _capi as C
+    from henon_map_cpp_amd import workloads as W
+
+    c = W.CHAOS_INDICATORS
+    x0 = W.indicator_points()
+    T = 2048
+    tdev = C.DeviceBuffer.from_array(C.build_table(_lines(c["tune_x"], c["epsilon"], T),
+                                                   _lines(c["tune_y"], c["epsilon"], T)))
+    n = x0[0].size
+    best = None
+    for it in range(2):
+        parts = C.DeviceParticles(*x0)
+        C.check(C.lib().hm_stream_synchronize(None))
+        C.profile_reset()
+        C.profile_enable(True)
+        hist = C.track_history(parts, tdev, T, T - 1, c["mu"], c["barrier"])   # samples 0..2047
+        fft = C.history_tunes(hist, n, T, [(0, T - 1)], "fft")
+        birk = C.history_tunes(hist, n, T, [(0, T - 1)], "birkhoff")
+        h_ms = C.profile_read("k_history")[0]
+        f_ms = C.profile_read("k_fft_tune_pow2")[0]
+        b_ms = C.profile_read("k_birkhoff_hist")[0]
+        C.profile_enable(False)
+        hist.free()
+        if best is None or h_ms + f_ms + b_ms < sum(best):
+            best = (h_ms, f_ms, b_ms)
+    h_ms, f_ms, b_ms = best
+    nbytes = 32.0 * n * T
+    fft_flops = 2 * 5 * T * np.log2(T) * n
+    return {
+        "workload": f"{n} particles x 2048 samples (chaos-indicator set), mu=0.5, eps=32",
+        "history_ms": h_ms, "history_GBps": nbytes / (h_ms / 1e3) / 1e9,
+        "fft_ms": f_ms, "fft_GBps": nbytes / (f_ms / 1e3) / 1e9, "fft_TFLOPs": fft_flops / (f_ms / 1e3) / 1e12,
+        "birkhoff_ms": b_ms, "birkhoff_GBps": nbytes / (b_ms / 1e3) / 1e9,
+        "hbm_peak_GBps": HBM_PEAK_GBPS,
+        "finite_fft_tunes": float(np.mean(np.isfinite(fft))), "finite_birkhoff_tunes": float(np.mean(np.isfinite(birk))),
     }
 
 
@@ -402,6 +446,9 @@ def main():
     if "indicators" in legs:
         line["tangent_megno"] = indicators_leg(2, 1)
         print(f"[bench] tangent/MEGNO {line['tangent_megno']}", file=sys.stderr, flush=True)
+    if "tunes" in legs:
+        line["tunes"] = tunes_leg()
+        print(f"[bench] tunes {line['tunes']}", file=sys.stderr, flush=True)
     if "adjoint" in legs:
         line["adjoint"] = adjoint_leg()
         print(f"[bench] adjoint {line['adjoint']}", file=sys.stderr, flush=True)

@@ -35,6 +35,7 @@ try:
     from .henon_map_engine import storage_4d, vector_4d_gpu  # noqa: F401
     from .henon_map_engine import storage_4d_gpu as cpp_storage_4d_gpu
     from .henon_map_engine import tune_birkhoff_construct as cpp_tune_birkhoff
+    from .henon_map_engine import track_batch as cpp_track_batch
     _ENGINE_ERROR = None
 except (ImportError, OSError, AttributeError) as _e:  # not built (or stale): entry points raise on use
     _ENGINE_ERROR = _e
@@ -46,13 +47,21 @@ except (ImportError, OSError, AttributeError) as _e:  # not built (or stale): en
     cpp_birkhoff_weights = cpp_henon_tracker = cpp_henon_tracker_gpu = particles_4d = particles_4d_gpu = _missing
     cpp_get_tunes = lbc = lbcm = cpp_matrix_4d_vector = cpp_matrix_4d_vector_gpu = _missing
     cpp_megno_birkhoff = cpp_megno = storage_4d = vector_4d_gpu = cpp_storage_4d_gpu = cpp_tune_birkhoff = _missing
+    cpp_track_batch = _missing
 
 import pandas as pd  # noqa: E402
 
 __all__ = ["gpu_available", "birkhoff_weights", "get_tunes", "particles", "henon_tracker", "matrix_4d_vector",
            "vector_4d", "lyapunov_birkhoff_construct", "lyapunov_birkhoff_construct_multi", "megno_construct",
            "megno_birkhoff_construct", "tune_birkhoff_construct", "storage", "storage_gpu", "particles_4d",
-           "particles_4d_gpu", "storage_4d", "vector_4d_gpu"]
+           "particles_4d_gpu", "storage_4d", "vector_4d_gpu", "track_batch"]
+
+
+def track_batch(jobs, mode="exact"):
+    """Track many independent jobs in one batched launch sequence. jobs: list of
+    (henon_tracker, particles, n_turns, mu, barrier); each job advances exactly
+    as tracker.track(particles, n_turns, mu, barrier) would."""
+    cpp_track_batch([(t.tracker, p.particles, int(n), float(mu), float(b)) for t, p, n, mu, b in jobs], mode)
 
 
 def gpu_available():
@@ -183,6 +192,12 @@ class henon_tracker:
 
     def track(self, particles, n_turns, mu, barrier=10.0, kick_module=np.nan, inverse=False, mode="exact"):
         self.tracker.track(particles.particles, n_turns, mu, barrier, kick_module, inverse, mode)
+
+    def autotune(self, particles, mu, barrier=10.0, mode="exact", warm_turns=2000, trial_turns=4000):
+        """Times launch configurations on a copy of `particles` and makes later
+        track() calls of this size class use the fastest; returns it as a dict.
+        Results do not depend on the configuration."""
+        return dict(self.tracker.autotune(particles.particles, mu, barrier, mode, warm_turns, trial_turns))
 
     def megno(self, particles, n_turns, mu, barrier=10.0, kick_module=np.nan, inverse=False,
               turn_samples=np.array([]), n_threads=-1):

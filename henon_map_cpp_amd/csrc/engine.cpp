@@ -339,6 +339,46 @@ class HenonTracker {
         if (PyErr_CheckSignals() != 0) throw py::error_already_set();
     }
 
+    // Launch-configuration autotuner (hm_autotune_track) on this tracker's
+    // table and the given particles; the particles are not modified.
+    py::dict autotune(const Particles4D &p, double mu, double barrier, const std::string &mode,
+                      unsigned int warm_turns, unsigned int trial_turns)
+    {
+        const int m = mode_of(mode);
+        hm_launch_cfg best{};
+        double ms = 0.0;
+        {
+            py::gil_scoped_release nogil;
+            check(hm_autotune_track(p.dx.as<double>(), p.dpx.as<double>(), p.dy.as<double>(), p.dpy.as<double>(),
+                                    p.dsteps.as<uint32_t>(), p.n_particles, dtable.as<double>(), allowed_steps,
+                                    warm_turns, trial_turns, mu, barrier, m, &best, &ms, nullptr));
+        }
+        py::dict d;
+        d["block"] = best.block;
+        d["compact_every"] = best.compact_every;
+        d["particles_per_lane"] = best.particles_per_lane;
+        d["trial_ms"] = ms;
+        return d;
+    }
+
+    // One job of a batched launch (track_batch): this tracker's table, the
+    // particles' device arrays, the same turn-range check as track().
+    hm_job batch_job(Particles4D &p, unsigned int n_turns, double mu, double barrier) const
+    {
+        check_turns(p, n_turns, false);
+        hm_job j{};
+        j.x = p.dx.as<double>(); j.px = p.dpx.as<double>(); j.y = p.dy.as<double>(); j.py = p.dpy.as<double>();
+        j.steps = p.dsteps.as<uint32_t>();
+        j.n = p.n_particles;
+        j.table = dtable.as<double>();
+        j.table_len = allowed_steps;
+        j.n_turns = n_turns;
+        j.mu = mu;
+        j.barrier = barrier;
+        j.kick_module = std::nan("");
+        return j;
+    }
+
     py::array_t<double> get_tangent_matrix(const Particles4D &p, double mu, bool reverse) const
     {
         // src/henon.cu:3080-3112 / :3187-3219, table index global_steps
@@ -1045,6 +1085,35 @@ class Storage4DGpu {
 };
 
 // get_tunes (src/dynamic_indicator.cpp:333-361): {fft, birkhoff} of one signal
+// Many independent (tracker, particles, n_turns, mu, barrier) jobs in one
+// batched launch sequence (hm_track_batch); each job's particles advance as
+// with tracker.track(particles, n_turns, mu, barrier, inverse=False).
+void track_batch(const py::list &jobs, const std::string &mode)
+{
+    std::vector<hm_job> js;
+    std::vector<std::pair<Particles4D *, unsigned int>> adv;
+    for (const py::handle &h : jobs) {
+        const py::tuple t = h.cast<py::tuple>();
+        if (t.size() != 5) throw std::invalid_argument("track_batch: jobs are (tracker, particles, n_turns, mu, barrier)");
+        HenonTracker &tr = t[0].cast<HenonTracker &>();
+        Particles4D &p = t[1].cast<Particles4D &>();
+        const unsigned int n_turns = t[2].cast<unsigned int>();
+        for (const auto &a : adv)
+            if (a.first == &p) throw std::invalid_argument("track_batch: a particle set appears in two jobs");
+        js.push_back(tr.batch_job(p, n_turns, t[3].cast<double>(), t[4].cast<double>()));
+        adv.emplace_back(&p, n_turns);
+    }
+    const int m = mode_of(mode);
+    {
+        py::gil_scoped_release nogil;
+        check(hm_track_batch(js.data(), js.size(), m, nullptr, nullptr));
+        uint32_t flags = 0;
+        check(hm_take_status(&flags, nullptr));
+        if (flags & 1u) throw std::runtime_error("track_batch: a particle's turn index left its modulation table");
+    }
+    for (auto &a : adv) a.first->global_steps += a.second;
+}
+
 std::vector<double> get_tunes(const std::vector<double> &x, const std::vector<double> &px)
 {
     if (x.size() != px.size()) throw std::runtime_error("get_tunes: x and px must be of the same size");
@@ -1161,6 +1230,9 @@ PYBIND11_MODULE(henon_map_engine, m)
         cls.def("compute_a_modulation", &C::compute_a_modulation, "Compute a modulation", py::arg("N"),
                 py::arg("omega_x"), py::arg("omega_y"), py::arg("modulation_kind"), py::arg("omega_0"),
                 py::arg("epsilon"), py::arg("offset"))
+            .def("autotune", &C::autotune, "Pick the fastest launch configuration for these particles",
+                 py::arg("particles"), py::arg("mu"), py::arg("barrier") = 100.0, py::arg("mode") = "exact",
+                 py::arg("warm_turns") = 2000, py::arg("trial_turns") = 4000)
             .def("track", &C::track, "Track particles", py::arg("particles"), py::arg("n_turns"), py::arg("mu"),
                  py::arg("barrier") = 100.0, py::arg("kick_module") = std::numeric_limits<double>::quiet_NaN(),
                  py::arg("inverse") = false, py::arg("mode") = "exact")
@@ -1187,6 +1259,8 @@ PYBIND11_MODULE(henon_map_engine, m)
             py::arg("frequencies") = std::vector<double>());
 
     m.def("get_tunes", &get_tunes, "Get tunes", py::arg("x"), py::arg("px"));
+    m.def("track_batch", &track_batch, "Track many (tracker, particles, n_turns, mu, barrier) jobs in one batch",
+          py::arg("jobs"), py::arg("mode") = "exact");
 
     auto tracker_more = [&](auto &cls) {
         using C = typename std::decay_t<decltype(cls)>::type;

@@ -40,30 +40,54 @@ struct AdjointFwdArgs {
     uint8_t *alive;           // [n]
 };
 
+// Every lane runs every turn (a lost particle's lane carries on with values
+// that are discarded), so the table record and the weight of a turn are
+// wave-uniform scalar loads and the loss test is one compare.
 template <int MODE, bool OCT>
 __global__ void __launch_bounds__(256) k_adjoint_forward(AdjointFwdArgs a)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
+    const bool in = i < a.n;
     const size_t n = a.n;
-    double x = a.x0[i], px = a.px0[i], y = a.y0[i], py = a.py0[i];
-    bool alive = !any_nan(x, px, y, py);
+    double x = __builtin_nan(""), px = x, y = x, py = x;
+    if (in) {
+        x = a.x0[i]; px = a.px0[i]; y = a.y0[i]; py = a.py0[i];
+    }
+    bool alive = in && !any_nan(x, px, y, py);
+    if (!__any(alive)) {
+        if (in) {
+            a.final_state[i] = x; a.final_state[n + i] = px; a.final_state[2 * n + i] = y;
+            a.final_state[3 * n + i] = py;
+            a.birk[i] = __builtin_nan("");
+            a.alive[i] = 0;
+        }
+        return;
+    }
+    const ConstPtr<TabEntry> tab = as_const(a.tab);
+    const ConstPtr<double> weights = as_const(a.weights);
     double bsum = 0.0;
     const double m3mu = -3.0 * a.mu;
-    for (uint32_t k = 0; k < a.n_turns; ++k) {
-        if (k % a.every == 0) {
-            const size_t o = (size_t)(k / a.every) * 4 * n;
-            a.ckpt[o + i] = x; a.ckpt[o + n + i] = px; a.ckpt[o + 2 * n + i] = y; a.ckpt[o + 3 * n + i] = py;
-        }
-        if (alive) {
-            map_step<MODE, false, OCT>(x, px, y, py, a.mu, m3mu, a.tab[k]);
-            if (!(radius2<MODE>(x, px, y, py) < a.b2)) {
-                alive = false;
-                x = px = y = py = __builtin_nan("");
+    const double b2s = MODE == kExact ? screen_bound(a.b2) : a.b2;
+    double *ck = a.ckpt;
+    for (uint32_t k = 0, until = 0; k < a.n_turns; ++k, --until) {
+        if (until == 0) {  // checkpoint before turns 0, every, 2 every, ...
+            if (in) {
+                ck[i] = x; ck[n + i] = px; ck[2 * n + i] = y; ck[3 * n + i] = py;
             }
+            ck += 4 * n;
+            until = a.every;
         }
-        bsum = fma(a.weights[k], x * x, bsum);
+        map_step<MODE, false, OCT>(x, px, y, py, a.mu, m3mu, load_record(tab + k));
+        // exact mode: screened radius, confirmed with the reference's
+        // expression for the rare candidates (same decisions as hm_track)
+        const double r2 = MODE == kExact ? screen_radius2(x, px, y, py) : radius2<MODE>(x, px, y, py);
+        bool out = alive && !(r2 < b2s);
+        if (MODE == kExact && __any(out)) out = out && !(radius2<kExact>(x, px, y, py) < a.b2);
+        alive = alive && !out;
+        bsum = fma(weights[k], x * x, bsum);
     }
+    if (!in) return;
+    if (!alive) x = px = y = py = bsum = __builtin_nan("");
     a.final_state[i] = x; a.final_state[n + i] = px; a.final_state[2 * n + i] = y; a.final_state[3 * n + i] = py;
     a.birk[i] = bsum;
     a.alive[i] = alive ? 1 : 0;

@@ -10,6 +10,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "hm_device.h"
@@ -130,6 +132,7 @@ struct IndicatorArgs {
     double *birk_out;       // [n_birk][2][n] tunes
     uint32_t hist_from, hist_len;
     double *hist;           // [hist_len][4][n]
+    const double *inv_idx;  // [n_turns]: 1.0 / (k + 1), correctly rounded (host table)
     uint32_t *status;
 };
 
@@ -214,7 +217,7 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
         for (int q = 0; q < 16; q++) m[q] = (q % 5 == 0) ? 1.0 : 0.0;
     }
     double lscale = (a.log_scale && in_range) ? a.log_scale[i] : 0.0;
-    double fb = sqrt(frob2<MODE>(m));
+    const double f0 = frob2<MODE>(m);
     double l1 = 0.0, l2 = 0.0;
     const double mu3 = a.mu * 3, mu6 = a.mu * 6, m3mu = -3.0 * a.mu;
     const bool alive0 = in_range && !any_nan(x, px, y, py);
@@ -244,14 +247,20 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
     uint32_t s0 = 0;
     if (live) s0 = __builtin_amdgcn_readlane(s, __builtin_ctzll(live));
     const bool uniform = live && __all(!alive0 || s == s0) && (uint64_t)s0 + a.n_turns <= (uint64_t)a.tab_len;
-    const TabEntry *tp = a.tab + s0;
+    const ConstPtr<TabEntry> tp = as_const(a.tab + s0);
+    const ConstPtr<double> inv_idx = as_const(a.inv_idx);
 
     uint32_t sample_q = 0;
+    uint32_t next_sample = a.n_samples ? a.samples[0] : 0;
     uint32_t k = 0;
+    // squared Frobenius norm of M before the turn; the MEGNO ratio
+    // |M_{k+1}|/|M_k| is taken as sqrt(fa2 / fb2): one division, no square root
+    double fb2 = f0;
+    uint32_t renorm_in = a.renorm_every;
     for (; k < a.n_turns; ++k) {
         TabEntry t;
         if (uniform) {
-            t = tp[k];
+            t = load_record(tp + k);
         } else {
             const uint64_t ti = (uint64_t)s + k;
             if (alive && ti >= a.tab_len) {
@@ -272,13 +281,13 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
             done = k;
         }
         const uint32_t idx = k + 1;
-        const double fa = sqrt(frob2<MODE>(m));
-        l1 = l1 + log10(fa / fb) * (double)idx;
-        l2 = l2 + (1.0 / (double)idx) * l1;
-        fb = fa;
-        if (sample_q < a.n_samples && a.samples[sample_q] == idx) {
+        const double fa2 = frob2<MODE>(m);
+        l1 = l1 + (0.5 * log10(fa2 / fb2)) * (double)idx;
+        l2 = l2 + inv_idx[k] * l1;
+        fb2 = fa2;
+        if (idx == next_sample) {
             const size_t o = (size_t)sample_q * a.n + i;
-            if (in_range && a.megno) a.megno[o] = (1.0 / (double)idx) * l2;
+            if (in_range && a.megno) a.megno[o] = inv_idx[k] * l2;
             if (in_range && a.fli) {
                 double v[4];
 #pragma unroll
@@ -289,13 +298,18 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
                 a.fli[o] = (lscale + log10(nv)) / (double)idx;
             }
             ++sample_q;
+            next_sample = sample_q < a.n_samples ? a.samples[sample_q] : 0;
         }
-        if (a.renorm_every && idx % a.renorm_every == 0 && alive) {
-            const double inv = 1.0 / fa;
+        if (renorm_in && --renorm_in == 0) {
+            renorm_in = a.renorm_every;
+            if (alive) {
+                const double fa = sqrt(fa2);
+                const double inv = 1.0 / fa;
 #pragma unroll
-            for (int q = 0; q < 16; q++) m[q] = m[q] * inv;
-            lscale = lscale + log10(fa);
-            fb = sqrt(frob2<MODE>(m));
+                for (int q = 0; q < 16; q++) m[q] = m[q] * inv;
+                lscale = lscale + log10(fa);
+                fb2 = frob2<MODE>(m);
+            }
         }
         if (a.n_birk) {
             const double nx = atan2(px, x), ny = atan2(py, y);
@@ -550,6 +564,27 @@ extern "C" int hm_megno_trace(double *x, double *px, double *y, double *py, uint
     });
 }
 
+// 1.0 / k for k = 1..n (correctly rounded, computed on the host), kept per
+// device in workspace slot 7 and extended when a longer run needs it.
+static const double *reciprocal_table(uint32_t n, hipStream_t s)
+{
+    static std::mutex mu;
+    static std::map<int, uint32_t> filled;
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lk(mu);
+    uint32_t &have = filled[dev];
+    const uint32_t want = std::max<uint32_t>(n, 1u << 16);
+    if (have >= n) return static_cast<const double *>(workspace((size_t)have * sizeof(double), 7));
+    double *d = static_cast<double *>(workspace((size_t)want * sizeof(double), 7));
+    std::vector<double> h(want);
+    for (uint32_t k = 0; k < want; ++k) h[k] = 1.0 / (double)(k + 1);
+    hip_check(hipMemcpyAsync(d, h.data(), want * sizeof(double), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    have = want;
+    return d;
+}
+
 extern "C" int hm_track_indicators(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n,
                                    const double *table, size_t table_len, uint32_t n_turns, double mu,
                                    double barrier, int mode, const hm_indicator_spec *spec, void *stream)
@@ -603,6 +638,7 @@ extern "C" int hm_track_indicators(double *x, double *px, double *y, double *py,
             a.birk_w = dw;
         }
         a.hist_from = spec->hist_from; a.hist_len = spec->hist_len; a.hist = spec->hist;
+        a.inv_idx = reciprocal_table(n_turns, s);
         a.status = status_word();
         const bool oct = mu != 0.0;
         LaunchTimer timer("k_indicators", s);

@@ -246,3 +246,34 @@ def test_api_indicators_and_reversibility(hm):
     ot.track(q, 1000, 0.5, barrier=100.0, inverse=True)
     d = np.sqrt((q.x - x) ** 2 + (q.px - z) ** 2 + (q.y - x) ** 2 + (q.py - z) ** 2)
     np.testing.assert_array_equal(re[2], d)
+
+
+def test_engine_track_batch_and_autotune():
+    """track_batch over trackers with different tables equals per-job track();
+    autotune leaves the particles untouched."""
+    import henon_map_cpp_amd as hm
+    rng = np.random.default_rng(7)
+    jobs, ref = [], []
+    for k in range(6):
+        n = int(rng.integers(10, 400))
+        x = rng.uniform(0.05, 0.5, n)
+        y = rng.uniform(0.05, 0.5, n)
+        z = np.zeros(n)
+        t = hm.henon_tracker(3000, 0.16 + 0.02 * k, 0.2, "sps", epsilon=8.0 * k)
+        turns = int(rng.integers(100, 3000))
+        mu = 0.0 if k == 0 else 0.3
+        jobs.append((t, hm.particles(x, z, y, z), turns, mu, 10.0 ** 0.5))
+        p = hm.particles(x, z, y, z)
+        t.track(p, turns, mu, 10.0 ** 0.5)
+        ref.append(p)
+    hm.track_batch(jobs)
+    for (t, p, *_), r in zip(jobs, ref):
+        for g in ("get_x", "get_px", "get_y", "get_py", "get_steps"):
+            assert np.array_equal(getattr(p, g)(), getattr(r, g)(), equal_nan=True)
+    t, p = jobs[1][0], hm.particles(np.linspace(0.01, 0.4, 5000), np.zeros(5000), np.linspace(0.01, 0.4, 5000),
+                                    np.zeros(5000))
+    best = t.autotune(p, 0.3, warm_turns=200, trial_turns=400)
+    assert best["block"] in (64, 256) and best["trial_ms"] > 0
+    assert np.all(p.get_steps() == 0)
+    assert np.array_equal(p.get_x(), np.linspace(0.01, 0.4, 5000))
+    hm._capi.check(hm._capi.lib().hm_autotune_clear())
