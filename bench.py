@@ -145,7 +145,7 @@ def _lines(tune, eps, turns):
     return C.lines_modulation(tune, eps, W.seeded_line_coefficients(), W.line_frequencies(), 0, turns)
 
 
-def indicators_leg(steps, warmup):
+def indicators_leg(steps, warmup, mode_name="exact"):
     """Fused tangent map + MEGNO + FLI with renormalisation every 100 turns: the
     chaos-indicator workload (256x256 grid + 65536 ball points, 1e5 turns)."""
     from henon_map_cpp_amd import _capi as C
@@ -164,7 +164,8 @@ def indicators_leg(steps, warmup):
         if it == warmup:
             C.profile_reset()
             C.profile_enable(True)
-        C.track_indicators(parts, tdev, T, T, c["mu"], c["barrier"], mode=C.MODE_EXACT,
+        C.track_indicators(parts, tdev, T, T, c["mu"], c["barrier"],
+                           mode=C.MODE_FAST if mode_name == "fast" else C.MODE_EXACT,
                            renorm_every=c["renorm_every"], samples=c["samples"], matrix=True)
         if it >= warmup:
             live += int(parts.get()[4].astype(np.int64).sum())
@@ -174,6 +175,7 @@ def indicators_leg(steps, warmup):
     ks = kms / 1e3
     return {
         "workload": c["name"],
+        "mode": mode_name,
         "kernel": "k_indicators",
         "particles": n,
         "turns": T,
@@ -306,29 +308,43 @@ def batch_leg(n_jobs):
     def fresh():
         return [C.DeviceParticles(*x) for x in ics]
 
+    megno_job = [k % 10 < 3 for k in range(n_jobs)]  # 30% of the jobs also accumulate MEGNO
+
     def batched():
         parts = fresh()
-        jobs = [(pp, tb, p[1], p[1], p[5], c["barrier"]) for pp, tb, p in zip(parts, tabs, params)]
+        megs = [C.DeviceBuffer(8 * p[0]) if m else None for p, m in zip(params, megno_job)]
+        jobs = [(pp, tb, p[1], p[1], p[5], c["barrier"], mg, 100)
+                for pp, tb, p, mg in zip(parts, tabs, params, megs)]
         _sync()
         t0 = time.perf_counter()
         C.track_batch(jobs, C.MODE_EXACT)
         C.check(C.lib().hm_stream_synchronize(None))
-        return time.perf_counter() - t0, parts
+        return time.perf_counter() - t0, parts, megs
 
     def plain():
         parts = fresh()
+        megs = []
         _sync()
         t0 = time.perf_counter()
-        for pp, tb, p in zip(parts, tabs, params):
-            C.track(pp, tb, p[1], p[1], p[5], c["barrier"], False, C.MODE_EXACT)
+        for pp, tb, p, m in zip(parts, tabs, params, megno_job):
+            if m:
+                r = C.track_indicators(pp, tb, p[1], p[1], p[5], c["barrier"], mode=C.MODE_EXACT, renorm_every=100,
+                                       samples=[p[1]])
+                megs.append(r["megno"][0])
+            else:
+                C.track(pp, tb, p[1], p[1], p[5], c["barrier"], False, C.MODE_EXACT)
+                megs.append(None)
         C.check(C.lib().hm_stream_synchronize(None))
-        return time.perf_counter() - t0, parts
+        return time.perf_counter() - t0, parts, megs
 
     batched()
-    tb, pb = min((batched() for _ in range(2)), key=lambda r: r[0])
-    tp, pp = plain()
+    tb, pb, mb = min((batched() for _ in range(2)), key=lambda r: r[0])
+    tp, pp, mp = plain()
     identical = all(np.array_equal(u, v, equal_nan=True) for a, b in zip(pb, pp) for u, v in zip(a.get(), b.get()))
-    return {"workload": c["name"], "jobs": n_jobs, "nominal_particle_turns": pt,
+    identical = identical and all(
+        np.array_equal(a.download(np.float64, len(b)), b, equal_nan=True) for a, b in zip(mb, mp) if a is not None)
+    return {"workload": c["name"] + ", 30% of the jobs with MEGNO (renormalised every 100 turns)",
+            "jobs": n_jobs, "megno_jobs": int(sum(megno_job)), "nominal_particle_turns": pt,
             "batched_s": tb, "plain_s": tp, "batched_value": pt / tb, "plain_value": pt / tp,
             "speedup": tp / tb, "identical_to_plain": identical}
 
@@ -445,6 +461,9 @@ def main():
     legs = [x for x in a.legs.split(",") if x and x != "none"]
     if "indicators" in legs:
         line["tangent_megno"] = indicators_leg(2, 1)
+        fast = indicators_leg(1, 0, "fast")
+        line["tangent_megno"]["other_mode"] = {"fast": {k: fast[k] for k in ("value", "live_value", "frac",
+                                                                            "kernel_ms_per_step")}}
         print(f"[bench] tangent/MEGNO {line['tangent_megno']}", file=sys.stderr, flush=True)
     if "tunes" in legs:
         line["tunes"] = tunes_leg()

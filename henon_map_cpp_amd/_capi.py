@@ -35,8 +35,9 @@ class Job(ctypes.Structure):
     """hm_job (include/henon_hip.h)."""
     _fields_ = [("x", ctypes.c_void_p), ("px", ctypes.c_void_p), ("y", ctypes.c_void_p), ("py", ctypes.c_void_p),
                 ("steps", ctypes.c_void_p), ("n", ctypes.c_size_t), ("table", ctypes.c_void_p),
-                ("table_len", ctypes.c_size_t), ("n_turns", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
-                ("mu", ctypes.c_double), ("barrier", ctypes.c_double), ("kick_module", ctypes.c_double)]
+                ("table_len", ctypes.c_size_t), ("n_turns", ctypes.c_uint32), ("renorm_every", ctypes.c_uint32),
+                ("mu", ctypes.c_double), ("barrier", ctypes.c_double), ("kick_module", ctypes.c_double),
+                ("megno", ctypes.c_void_p)]
 
 
 class BatchCfg(ctypes.Structure):
@@ -276,11 +277,15 @@ def track(parts: DeviceParticles, table_dev: DeviceBuffer, table_len: int, n_tur
 
 
 def track_batch(jobs, mode=MODE_EXACT, cfg: BatchCfg | None = None, stream=None):
-    """jobs: list of (DeviceParticles, table DeviceBuffer, table_len, n_turns, mu, barrier)."""
+    """jobs: list of (DeviceParticles, table DeviceBuffer, table_len, n_turns, mu, barrier[, megno
+    DeviceBuffer, renorm_every]); with a megno buffer the job also accumulates the MEGNO."""
     arr = (Job * max(len(jobs), 1))()
-    for k, (parts, tdev, tlen, n_turns, mu, barrier) in enumerate(jobs):
+    for k, job in enumerate(jobs):
+        parts, tdev, tlen, n_turns, mu, barrier = job[:6]
+        megno = job[6].ptr if len(job) > 6 and job[6] is not None else None
+        renorm = job[7] if len(job) > 7 else 0
         x, px, y, py, st = parts.ptrs()
-        arr[k] = Job(x, px, y, py, st, parts.n, tdev.ptr, tlen, n_turns, 0, mu, barrier, float("nan"))
+        arr[k] = Job(x, px, y, py, st, parts.n, tdev.ptr, tlen, n_turns, renorm, mu, barrier, float("nan"), megno)
     c = ctypes.byref(cfg) if cfg is not None else None
     check(lib().hm_track_batch(arr, len(jobs), mode, c, stream))
 

@@ -376,6 +376,8 @@ class HenonTracker {
         j.mu = mu;
         j.barrier = barrier;
         j.kick_module = std::nan("");
+        j.megno = nullptr;
+        j.renorm_every = 0;
         return j;
     }
 

@@ -69,7 +69,16 @@ __global__ void __launch_bounds__(256) k_adjoint_forward(AdjointFwdArgs a)
     const double m3mu = -3.0 * a.mu;
     const double b2s = MODE == kExact ? screen_bound(a.b2) : a.b2;
     double *ck = a.ckpt;
+    // the next turn's record and weight are requested a turn ahead
+    TabEntry nt = load_record(tab);
+    double nw = weights[0];
     for (uint32_t k = 0, until = 0; k < a.n_turns; ++k, --until) {
+        const TabEntry t = nt;
+        const double w = nw;
+        if (k + 1 < a.n_turns) {
+            nt = load_record(tab + k + 1);
+            nw = weights[k + 1];
+        }
         if (until == 0) {  // checkpoint before turns 0, every, 2 every, ...
             if (in) {
                 ck[i] = x; ck[n + i] = px; ck[2 * n + i] = y; ck[3 * n + i] = py;
@@ -77,14 +86,14 @@ __global__ void __launch_bounds__(256) k_adjoint_forward(AdjointFwdArgs a)
             ck += 4 * n;
             until = a.every;
         }
-        map_step<MODE, false, OCT>(x, px, y, py, a.mu, m3mu, load_record(tab + k));
+        map_step<MODE, false, OCT>(x, px, y, py, a.mu, m3mu, t);
         // exact mode: screened radius, confirmed with the reference's
         // expression for the rare candidates (same decisions as hm_track)
         const double r2 = MODE == kExact ? screen_radius2(x, px, y, py) : radius2<MODE>(x, px, y, py);
         bool out = alive && !(r2 < b2s);
         if (MODE == kExact && __any(out)) out = out && !(radius2<kExact>(x, px, y, py) < a.b2);
         alive = alive && !out;
-        bsum = fma(weights[k], x * x, bsum);
+        bsum = fma(w, x * x, bsum);
     }
     if (!in) return;
     if (!alive) x = px = y = py = bsum = __builtin_nan("");
@@ -156,10 +165,12 @@ __device__ __forceinline__ void copy4(double (&d)[4], const double (&s)[4])
 
 // Backward sweep. Per K-segment (in reverse): recompute it from its checkpoint,
 // writing the state every kChunk = 8 turns to scratch; the state after the
-// segment becomes the post-step point of its last turn. Per 8-turn chunk (in
-// reverse), in two halves so that at most five states are held in registers:
-// turns 4..7 from S4..S7 (S4 reached from S0 by four turns), then turns 0..3
-// from S0..S3; `nxt` carries the state after the turns being processed.
+// segment is the post-step point of its last turn. Per 8-turn chunk (in
+// reverse): load its table records, recompute its states S1..S7 from S0 into
+// registers (7 turns; the post-step point of its last turn, `nxt`, is known),
+// then run the adjoint back through its turns; S0 becomes the next chunk's
+// `nxt`. Table records are requested ahead of use, because with one particle
+// per lane and few wavefronts per SIMD nothing else hides their latency.
 template <int MODE, bool OCT>
 __global__ void __launch_bounds__(256) k_adjoint_backward(AdjointBwdArgs a)
 {
@@ -179,25 +190,23 @@ __global__ void __launch_bounds__(256) k_adjoint_backward(AdjointBwdArgs a)
     // turn: scalar loads
     const ConstPtr<TabEntry> tab = as_const(a.tab);
     const ConstPtr<double> dtab = as_const(a.dtab), weights = as_const(a.weights);
-    auto rec = [&](uint32_t k) -> TabEntry { return load_record(tab + k); };
-    auto turn_k = [&](uint32_t k, const double (&pre)[4], const double (&post)[4]) {
-        const ConstPtr<double> d = dtab + 4 * (size_t)k;
-        const double4 dt{d[0], d[1], d[2], d[3]};
-        adjoint_turn<OCT>(pre, post, rec(k), bseed * weights[k], dt, mu3, mu6, acc);
-    };
     for (int seg = (int)n_seg - 1; seg >= 0; --seg) {
         const uint32_t s0 = (uint32_t)seg * K, s1 = min(s0 + K, T);
         double nxt[4];
         {
             const size_t o = (size_t)seg * 4 * n;
             double v[4] = {a.ckpt[o + i], a.ckpt[o + n + i], a.ckpt[o + 2 * n + i], a.ckpt[o + 3 * n + i]};
-            for (uint32_t k = s0; k < s1; ++k) {
-                if ((k - s0) % kChunk == 0) {
-                    const size_t q = (size_t)((k - s0) / kChunk) * 4 * n;
-                    a.sub[q + i] = v[0]; a.sub[q + n + i] = v[1]; a.sub[q + 2 * n + i] = v[2];
-                    a.sub[q + 3 * n + i] = v[3];
+            double *sub = a.sub;
+            TabEntry nt = load_record(tab + s0);
+            for (uint32_t k = s0, until = 0; k < s1; ++k, --until) {
+                const TabEntry t = nt;
+                if (k + 1 < s1) nt = load_record(tab + k + 1);
+                if (until == 0) {
+                    sub[i] = v[0]; sub[n + i] = v[1]; sub[2 * n + i] = v[2]; sub[3 * n + i] = v[3];
+                    sub += 4 * n;
+                    until = kChunk;
                 }
-                step4<MODE, OCT>(v, mu, m3mu, rec(k));
+                step4<MODE, OCT>(v, mu, m3mu, t);
             }
             copy4(nxt, v);
         }
@@ -205,41 +214,29 @@ __global__ void __launch_bounds__(256) k_adjoint_backward(AdjointBwdArgs a)
         for (int ch = (int)n_chunk - 1; ch >= 0; --ch) {
             const uint32_t c0 = s0 + (uint32_t)ch * kChunk;
             const uint32_t len = min((uint32_t)kChunk, s1 - c0);
+            TabEntry R[kChunk];
+#pragma unroll
+            for (int m = 0; m < kChunk; ++m) R[m] = load_record(tab + min(c0 + m, s1 - 1));
             const size_t q = (size_t)ch * 4 * n;
-            const double S0[4] = {a.sub[q + i], a.sub[q + n + i], a.sub[q + 2 * n + i], a.sub[q + 3 * n + i]};
-            double W[5][4];
-            if (len > 4) {
-                // W[j] = S(4 + j) for 4 + j < len, W[len - 4] = nxt
-                copy4(W[0], S0);
+            double S[kChunk][4];
+            S[0][0] = a.sub[q + i]; S[0][1] = a.sub[q + n + i]; S[0][2] = a.sub[q + 2 * n + i];
+            S[0][3] = a.sub[q + 3 * n + i];
 #pragma unroll
-                for (int m = 0; m < 4; ++m) step4<MODE, OCT>(W[0], mu, m3mu, rec(c0 + m));
-#pragma unroll
-                for (int j = 1; j < 4; ++j) {
-                    copy4(W[j], W[j - 1]);
-                    if (4u + j < len) step4<MODE, OCT>(W[j], mu, m3mu, rec(c0 + 3 + j));
-                }
-#pragma unroll
-                for (int j = 3; j >= 0; --j) {
-                    if (4u + j >= len) continue;
-                    if (4u + j + 1 < len) turn_k(c0 + 4 + j, W[j], W[j + 1]);
-                    else turn_k(c0 + 4 + j, W[j], nxt);
-                }
-                copy4(nxt, W[0]);
-            }
-            const uint32_t h = min(len, 4u);
-            copy4(W[0], S0);
-#pragma unroll
-            for (int j = 1; j < 4; ++j) {
-                copy4(W[j], W[j - 1]);
-                if ((uint32_t)j < h) step4<MODE, OCT>(W[j], mu, m3mu, rec(c0 + j - 1));
+            for (int m = 1; m < kChunk; ++m) {
+                copy4(S[m], S[m - 1]);
+                if ((uint32_t)m < len) step4<MODE, OCT>(S[m], mu, m3mu, R[m - 1]);
             }
 #pragma unroll
-            for (int j = 3; j >= 0; --j) {
-                if ((uint32_t)j >= h) continue;
-                if ((uint32_t)j + 1 < h) turn_k(c0 + j, W[j], W[j + 1]);
-                else turn_k(c0 + j, W[j], nxt);
+            for (int m = kChunk - 1; m >= 0; --m) {
+                if ((uint32_t)m >= len) continue;
+                const uint32_t k = c0 + m;
+                const ConstPtr<double> d = dtab + 4 * (size_t)k;
+                const double4 dt{d[0], d[1], d[2], d[3]};
+                const double bw = bseed * weights[k];
+                if ((uint32_t)m + 1 == len) adjoint_turn<OCT>(S[m], nxt, R[m], bw, dt, mu3, mu6, acc);
+                else adjoint_turn<OCT>(S[m], S[m + 1 < kChunk ? m + 1 : m], R[m], bw, dt, mu3, mu6, acc);
             }
-            copy4(nxt, S0);
+            copy4(nxt, S[0]);
         }
     }
     a.grad_x0[i] = acc.lx; a.grad_x0[n + i] = acc.lpx; a.grad_x0[2 * n + i] = acc.ly; a.grad_x0[3 * n + i] = acc.lpy;

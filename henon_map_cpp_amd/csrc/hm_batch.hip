@@ -27,6 +27,7 @@
 
 #include "hm_device.h"
 #include "hm_internal.h"
+#include "hm_tangent.h"
 #include "hm_wave.h"
 
 namespace hm {
@@ -37,16 +38,28 @@ struct JobDesc {
     const TabEntry *tab;
     uint32_t n, n_turns, tab_len, list_off;
     double mu, b2;
+    double *megno;    // MEGNO jobs: output [n]; NULL for tracking-only jobs
+    double *state;    // MEGNO jobs: [n][kMegnoState] tangent matrix and sums
+    uint32_t renorm;  // MEGNO jobs: renormalisation interval (0 = never)
+    uint32_t pad;
 };
 
+// Per-particle state a MEGNO job carries from one segment to the next: the
+// tangent matrix (16), L1, L2 and |M|_F^2 of the last turn.
+constexpr int kMegnoState = 19;
+
 struct BatchQueue {
-    uint32_t total;  // chunks of the current segment
-    uint32_t head;   // next chunk id to hand out
+    uint32_t total;   // chunks of the current segment
+    uint32_t split;   // chunks of tracking-only jobs come first: [0, split)
+    uint32_t head;    // next tracking chunk id to hand out
+    uint32_t head_m;  // next MEGNO chunk id to hand out (from split)
 };
 
 struct BatchSeg {
     const JobDesc *jobs;
     uint32_t n_jobs;
+    uint32_t n_plain;                    // jobs [0, n_plain) track only, the rest accumulate MEGNO
+    const double *inv_idx;               // 1.0 / k, k = 1..max turns
     uint32_t seg_start, seg_len;
     uint32_t first;                      // 1 in the first segment (identity lists)
     const uint32_t *lists_cur;           // live lists read by this segment
@@ -93,11 +106,27 @@ __global__ void __launch_bounds__(kPlanThreads) k_batch_plan(BatchSeg S)
         if (t == kPlanThreads - 1) carry = before + v;
         __syncthreads();
     }
+    __syncthreads();
     if (t == 0) {
         S.chunk_off[S.n_jobs] = carry;
+        const uint32_t split = S.n_plain < S.n_jobs ? S.chunk_off[S.n_plain] : carry;
         S.queue->total = carry;
+        S.queue->split = split;
         S.queue->head = 0;
+        S.queue->head_m = split;
     }
+}
+
+// Identity tangent matrices and zero sums for every particle of the MEGNO jobs.
+__global__ void k_megno_init(double *__restrict__ state, uint64_t n_particles)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_particles) return;
+    double *st = state + i * kMegnoState;
+    for (int q = 0; q < 16; q++) st[q] = (q % 5 == 0) ? 1.0 : 0.0;
+    st[16] = 0.0;
+    st[17] = 0.0;
+    st[18] = 4.0;  // |I|_F^2
 }
 
 template <int MODE>
@@ -108,7 +137,7 @@ __global__ void __launch_bounds__(256, 8) k_batch_track(BatchSeg S)
     const ConstPtr<uint32_t> chunk_off = as_const(S.chunk_off);
     const ConstPtr<uint32_t> count_cur = as_const(S.count_cur);
     const ConstPtr<JobDesc> jobs = as_const(S.jobs);
-    const uint32_t total = *as_const(&S.queue->total);
+    const uint32_t total = *as_const(&S.queue->split);
     for (;;) {
         uint32_t c = 0;
         if (lane == 0) c = atomicAdd(&S.queue->head, 1u);
@@ -135,6 +164,141 @@ __global__ void __launch_bounds__(256, 8) k_batch_track(BatchSeg S)
                     last ? nullptr : S.lists_next + list_off, last ? nullptr : S.count_next + job, S.status};
         if (mu != 0.0) track_wave<MODE, false, true, 1>(A, base, n);
         else track_wave<MODE, false, false, 1>(A, base, n);
+    }
+}
+
+// One chunk of a MEGNO job for the segment's turns: the fused tangent map and
+// MEGNO recursion of k_indicators (same operations in the same order, so the
+// same results), with the matrix and sums kept per particle between segments.
+template <int MODE, bool OCT>
+__device__ __forceinline__ void megno_chunk(const BatchSeg &S, ConstPtr<JobDesc> J, uint32_t job, uint32_t base,
+                                            uint32_t n)
+{
+    const uint32_t lane = __lane_id(), i = base + lane;
+    const bool in = i < n;
+    const uint32_t list_off = J->list_off, n_turns = J->n_turns;
+    const uint32_t turns = min(S.seg_len, n_turns - S.seg_start);
+    const bool last = S.seg_start + S.seg_len >= n_turns;
+    double *gx = J->x, *gpx = J->px, *gy = J->y, *gpy = J->py;
+    uint32_t *gsteps = J->steps;
+    uint32_t j = 0, s = 0;
+    double x = __builtin_nan(""), px = x, y = x, py = x;
+    double m[16], l1 = 0.0, l2 = 0.0, fb2 = 0.0;
+    double *st = nullptr;
+    if (in) {
+        j = S.first ? i : S.lists_cur[list_off + i];
+        x = gx[j]; px = gpx[j]; y = gy[j]; py = gpy[j];
+        s = gsteps[j];
+        st = J->state + (size_t)j * kMegnoState;
+#pragma unroll
+        for (int q = 0; q < 16; q++) m[q] = st[q];
+        l1 = st[16]; l2 = st[17]; fb2 = st[18];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; q++) m[q] = 0.0;
+    }
+    const bool alive0 = in && !any_nan(x, px, y, py);
+    bool alive = alive0;
+    uint32_t done = turns;
+    const double mu = J->mu, b2 = J->b2, mu3 = mu * 3, mu6 = mu * 6, m3mu = -3.0 * mu;
+    const TabEntry *tab = J->tab;
+    const uint32_t tab_len = J->tab_len;
+    const uint64_t live0 = __ballot(alive0);
+    uint32_t s0 = 0;
+    if (live0) s0 = __builtin_amdgcn_readlane(s, __builtin_ctzll(live0));
+    const bool uniform = live0 && __all(!alive0 || s == s0) && (uint64_t)s0 + turns <= (uint64_t)tab_len;
+    const ConstPtr<TabEntry> tp = as_const(tab + s0);
+    const ConstPtr<double> inv_idx = as_const(S.inv_idx);
+    const uint32_t renorm = J->renorm;
+    uint32_t renorm_in = renorm ? renorm - S.seg_start % renorm : 0;
+    for (uint32_t k = 0; k < turns && live0; ++k) {
+        TabEntry t;
+        if (uniform) {
+            t = load_record(tp + k);
+        } else {
+            const uint64_t ti = (uint64_t)s + k;
+            if (alive && ti >= tab_len) {
+                atomicOr(S.status, 1u);
+                alive = false;
+                done = k;
+            }
+            t = tab[ti < tab_len ? ti : tab_len - 1];
+        }
+        const Jac jc = jac_coeffs<MODE>(x, y, mu3, mu6);
+        mat_mul_structured<MODE>(jc, t, m);
+        map_step<MODE, false, OCT>(x, px, y, py, mu, m3mu, t);
+        if (alive && !(radius2<MODE>(x, px, y, py) < b2)) {
+            x = px = y = py = __builtin_nan("");
+#pragma unroll
+            for (int q = 0; q < 16; q++) m[q] = __builtin_nan("");
+            alive = false;
+            done = k;
+        }
+        const uint32_t idx = S.seg_start + k + 1;
+        const double fa2 = frob2_fused(m);
+        l1 = l1 + (0.5 * log10(fa2 / fb2)) * (double)idx;
+        l2 = l2 + inv_idx[idx - 1] * l1;
+        fb2 = fa2;
+        if (renorm_in && --renorm_in == 0) {
+            renorm_in = renorm;
+            if (alive) {
+                const double fa = sqrt(frob2<MODE>(m));
+                const double inv = 1.0 / fa;
+#pragma unroll
+                for (int q = 0; q < 16; q++) m[q] = m[q] * inv;
+                fb2 = frob2_fused(m);
+            }
+        }
+        if (!__any(alive)) break;
+    }
+    if (alive0) {
+        gx[j] = x; gpx[j] = px; gy[j] = y; gpy[j] = py;
+        gsteps[j] = s + done;
+#pragma unroll
+        for (int q = 0; q < 16; q++) st[q] = m[q];
+        st[16] = l1; st[17] = l2; st[18] = fb2;
+    }
+    // a particle leaves the live list when lost (or never stepped): its MEGNO
+    // is final then; survivors get theirs after the job's last turn
+    if (in && (!alive || last)) J->megno[j] = alive ? inv_idx[n_turns - 1] * l2 : __builtin_nan("");
+    if (!last) {
+        const uint64_t mlive = __ballot(alive);
+        if (mlive) {
+            const int leader = __builtin_ctzll(mlive);
+            uint32_t slot = 0;
+            if ((int)lane == leader) slot = atomicAdd(S.count_next + job, (uint32_t)__popcll(mlive));
+            slot = __builtin_amdgcn_readlane(slot, leader);
+            const uint64_t bit = 1ull << lane;
+            if (mlive & bit) S.lists_next[list_off + slot + __popcll(mlive & (bit - 1ull))] = j;
+        }
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_batch_megno(BatchSeg S)
+{
+    const uint32_t lane = __lane_id();
+    const ConstPtr<uint32_t> chunk_off = as_const(S.chunk_off);
+    const ConstPtr<uint32_t> count_cur = as_const(S.count_cur);
+    const ConstPtr<JobDesc> jobs = as_const(S.jobs);
+    const uint32_t total = *as_const(&S.queue->total);
+    for (;;) {
+        uint32_t c = 0;
+        if (lane == 0) c = atomicAdd(&S.queue->head_m, 1u);
+        c = __builtin_amdgcn_readfirstlane(c);
+        if (c >= total) break;
+        uint32_t lo = 0, hi = S.n_jobs;  // chunk_off[lo] <= c < chunk_off[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (chunk_off[mid] <= c) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t job = lo;
+        const ConstPtr<JobDesc> J = jobs + job;
+        const uint32_t n = S.first ? J->n : count_cur[job];
+        const uint32_t base = (c - chunk_off[job]) * 64u;
+        if (J->mu != 0.0) megno_chunk<MODE, true>(S, J, job, base, n);
+        else megno_chunk<MODE, false>(S, J, job, base, n);
     }
 }
 
@@ -166,18 +330,34 @@ extern "C" int hm_track_batch(const hm_job *jobs, size_t n_jobs, int mode, const
         uint32_t waves = cfg && cfg->waves > 0 ? (uint32_t)cfg->waves : batch_resident_waves();
         const bool graph = cfg && cfg->use_graph;
 
+        // tracking-only jobs first, then the MEGNO jobs (each kernel takes one range)
+        std::vector<size_t> order;
+        order.reserve(n_jobs);
+        for (size_t k = 0; k < n_jobs; ++k)
+            if (!jobs[k].megno) order.push_back(k);
+        const uint32_t n_plain = (uint32_t)order.size();
+        for (size_t k = 0; k < n_jobs; ++k)
+            if (jobs[k].megno) order.push_back(k);
         std::vector<JobDesc> desc(n_jobs);
-        uint64_t total_n = 0;
+        uint64_t total_n = 0, megno_n = 0;
         uint32_t max_turns = 0;
-        for (size_t k = 0; k < n_jobs; ++k) {
+        for (size_t r = 0; r < n_jobs; ++r) {
+            const size_t k = order[r];
             const hm_job &j = jobs[k];
             if (j.n > 0x7fffffffull) throw Error(HM_ERR_ARG, "batch: a job exceeds 2^31 particles");
             if (j.n && (!j.x || !j.px || !j.y || !j.py || !j.steps || !j.table))
                 throw Error(HM_ERR_ARG, "batch: NULL pointer in job " + std::to_string(k));
             if (j.table_len > 0xffffffffull) throw Error(HM_ERR_ARG, "batch: table exceeds 2^32 turns");
             if (!std::isnan(j.kick_module)) throw Error(HM_ERR_UNSUPPORTED, "batch: the random kick is not available");
-            JobDesc &d = desc[k];
+            JobDesc &d = desc[r];
             d.x = j.x; d.px = j.px; d.y = j.y; d.py = j.py; d.steps = j.steps;
+            d.megno = j.megno;
+            d.renorm = j.renorm_every;
+            d.state = nullptr;
+            if (j.megno) {
+                d.state = reinterpret_cast<double *>((uintptr_t)(megno_n * kMegnoState * sizeof(double)));  // offset
+                megno_n += j.n;
+            }
             d.tab = reinterpret_cast<const TabEntry *>(j.table);
             d.n = (uint32_t)j.n;
             d.n_turns = j.n ? j.n_turns : 0;
@@ -207,6 +387,12 @@ extern "C" int hm_track_batch(const hm_job *jobs, size_t n_jobs, int mode, const
         const size_t b_off = align(sizeof(uint32_t) * (n_jobs + 1));
         const size_t b_q = align(sizeof(BatchQueue));
         char *ws = static_cast<char *>(workspace(b_desc + 2 * b_list + 2 * b_cnt + b_off + b_q, 4));
+        double *mstate = megno_n ? static_cast<double *>(workspace(megno_n * kMegnoState * sizeof(double), 9))
+                                 : nullptr;
+        for (auto &d : desc)
+            if (d.megno) d.state = reinterpret_cast<double *>(reinterpret_cast<char *>(mstate) +
+                                                              reinterpret_cast<uintptr_t>(d.state));
+        const double *inv_idx = megno_n ? reciprocal_table(max_turns, as_stream(stream)) : nullptr;
         JobDesc *d_desc = reinterpret_cast<JobDesc *>(ws);
         uint32_t *lists[2] = {reinterpret_cast<uint32_t *>(ws + b_desc),
                               reinterpret_cast<uint32_t *>(ws + b_desc + b_list)};
@@ -226,11 +412,24 @@ extern "C" int hm_track_batch(const hm_job *jobs, size_t n_jobs, int mode, const
         const uint32_t grid = (waves * 64u + (uint32_t)block - 1) / (uint32_t)block;
         uint32_t *status = status_word();
         auto enqueue = [&](hipStream_t q) {
+            if (megno_n) {
+                hipLaunchKernelGGL(k_megno_init, dim3((uint32_t)((megno_n + 255) / 256)), dim3(256), 0, q, mstate,
+                                   (uint64_t)megno_n);
+                launch_check("k_megno_init");
+            }
             for (size_t i = 0; i < seg.size(); ++i) {
-                BatchSeg S{d_desc, (uint32_t)n_jobs, seg[i].first, seg[i].second, i == 0 ? 1u : 0u,
-                           lists[i % 2], lists[(i + 1) % 2], counts[i % 2], counts[(i + 1) % 2], d_off, d_q, status};
+                BatchSeg S{d_desc, (uint32_t)n_jobs, n_plain, inv_idx, seg[i].first, seg[i].second,
+                           i == 0 ? 1u : 0u, lists[i % 2], lists[(i + 1) % 2], counts[i % 2], counts[(i + 1) % 2],
+                           d_off, d_q, status};
                 hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(kPlanThreads), 0, q, S);
                 launch_check("k_batch_plan");
+                if (megno_n) {
+                    if (mode == HM_MODE_EXACT)
+                        hipLaunchKernelGGL(k_batch_megno<kExact>, dim3(grid), dim3(block), 0, q, S);
+                    else
+                        hipLaunchKernelGGL(k_batch_megno<kFast>, dim3(grid), dim3(block), 0, q, S);
+                    launch_check("k_batch_megno");
+                }
                 if (mode == HM_MODE_EXACT)
                     hipLaunchKernelGGL(k_batch_track<kExact>, dim3(grid), dim3(block), 0, q, S);
                 else

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -259,6 +260,120 @@ __global__ void __launch_bounds__(256) k_fft_tune_pow2(FftArgs a)
     if (threadIdx.x == 0) a.out[out_o] = interp_tune(re, N, bi);
 }
 
+// Corner turn for the FFT: samples from..from+L-1 of the history rows
+// hist[sample][coord][particle] (particle-minor: what tracking writes) into
+// signal rows rows[particle - p0][coord][sample] minus the particle's means, so
+// that a workgroup reads its particle's signals contiguously. 64 x 64 tiles
+// through LDS, coalesced on both sides. grid: (particles/64, L/64, 4 coords).
+__global__ void __launch_bounds__(256) k_signal_rows(const double *__restrict__ hist, uint32_t n,
+                                                     const double *__restrict__ means, uint32_t from, uint32_t L,
+                                                     uint32_t p0, uint32_t nb, double *__restrict__ rows)
+{
+    __shared__ double tile[64][65];
+    const uint32_t tp = blockIdx.x * 64, ts = blockIdx.y * 64, c = blockIdx.z;
+    const uint32_t tx = threadIdx.x & 63u, ty = threadIdx.x >> 6;
+    const size_t nn = n;
+    {
+        const uint32_t q = tp + tx;  // batch-relative particle
+        const double m = (means && q < nb) ? means[c * nn + p0 + q] : 0.0;
+        for (uint32_t r = ty; r < 64; r += 4) {
+            const uint32_t sm = ts + r;
+            if (q < nb && sm < L) tile[r][tx] = hist[(size_t)(from + sm) * 4 * nn + c * nn + p0 + q] - m;
+        }
+    }
+    __syncthreads();
+    for (uint32_t r = ty; r < 64; r += 4) {
+        const uint32_t q = tp + r, sm = ts + tx;
+        if (q < nb && sm < L) rows[((size_t)q * 4 + c) * L + sm] = tile[tx][r];
+    }
+}
+
+struct FftRowArgs {
+    const double *rows;  // [nb][4][N], means already removed
+    uint32_t nb, p0, n, N;
+    const double *hann;  // [N]
+    const double *tw;    // [N] complex (cos, -sin)
+    double *out;         // [window][plane][n]
+    uint32_t window;
+};
+
+// Both planes of one particle per workgroup, radix-2 decimation-in-time in LDS
+// (re/im of x + i px and y + i py: 4 N doubles of dynamic LDS), signal rows read
+// contiguously.
+__global__ void __launch_bounds__(256) k_fft_tune_rows(FftRowArgs a)
+{
+    extern __shared__ double lds[];
+    const uint32_t N = a.N;
+    double *re0 = lds, *im0 = lds + N, *re1 = lds + 2 * N, *im1 = lds + 3 * N;
+    __shared__ double sv[256];
+    __shared__ uint32_t si[256];
+    __shared__ int any_nan_flag;
+    const uint32_t b = blockIdx.x;
+    if (b >= a.nb) return;
+    const uint32_t logn = 31 - __builtin_clz(N);
+    const double *rx = a.rows + (size_t)b * 4 * N, *rpx = rx + N, *ry = rx + 2 * N, *rpy = rx + 3 * N;
+    if (threadIdx.x == 0) any_nan_flag = 0;
+    __syncthreads();
+    bool nan_seen = false;
+    for (uint32_t m = threadIdx.x; m < N; m += blockDim.x) {
+        const double h = a.hann[m];
+        const double vx = rx[m], vpx = rpx[m], vy = ry[m], vpy = rpy[m];
+        nan_seen |= any_nan(vx, vpx, vy, vpy);
+        const uint32_t r = __brev(m) >> (32 - logn);
+        re0[r] = vx * h; im0[r] = vpx * h;
+        re1[r] = vy * h; im1[r] = vpy * h;
+    }
+    if (nan_seen) atomicOr(&any_nan_flag, 1);
+    __syncthreads();
+    const size_t particle = a.p0 + b, nn = a.n;
+    double *o0 = a.out + ((size_t)a.window * 2 + 0) * nn + particle;
+    double *o1 = a.out + ((size_t)a.window * 2 + 1) * nn + particle;
+    if (any_nan_flag) {
+        if (threadIdx.x == 0) { *o0 = __builtin_nan(""); *o1 = __builtin_nan(""); }
+        return;
+    }
+    for (uint32_t half = 1; half < N; half <<= 1) {
+        const uint32_t stride = N / (2 * half);
+        for (uint32_t j = threadIdx.x; j < N / 2; j += blockDim.x) {
+            const uint32_t k = j & (half - 1);
+            const uint32_t q0 = ((j - k) << 1) + k, q1 = q0 + half;
+            const double wr = a.tw[2 * (k * stride)], wi = a.tw[2 * (k * stride) + 1];
+            {
+                const double br = re0[q1], bi = im0[q1];
+                const double tr = br * wr - bi * wi, ti = br * wi + bi * wr;
+                const double ar = re0[q0], ai = im0[q0];
+                re0[q0] = ar + tr; im0[q0] = ai + ti;
+                re0[q1] = ar - tr; im0[q1] = ai - ti;
+            }
+            {
+                const double br = re1[q1], bi = im1[q1];
+                const double tr = br * wr - bi * wi, ti = br * wi + bi * wr;
+                const double ar = re1[q0], ai = im1[q0];
+                re1[q0] = ar + tr; im1[q0] = ai + ti;
+                re1[q1] = ar - tr; im1[q1] = ai - ti;
+            }
+        }
+        __syncthreads();
+    }
+    double best0 = -1.0, best1 = -1.0;
+    uint32_t bi0 = 0xffffffffu, bi1 = 0xffffffffu;
+    for (uint32_t k = threadIdx.x; k < N; k += blockDim.x) {
+        const double g0 = sqrt(re0[k] * re0[k] + im0[k] * im0[k]);
+        const double g1 = sqrt(re1[k] * re1[k] + im1[k] * im1[k]);
+        re0[k] = g0;
+        re1[k] = g1;
+        if (g0 > best0 || bi0 == 0xffffffffu) { best0 = g0; bi0 = k; }
+        if (g1 > best1 || bi1 == 0xffffffffu) { best1 = g1; bi1 = k; }
+    }
+    __syncthreads();
+    double bv;
+    uint32_t i0, i1;
+    block_argmax(best0, bi0, sv, si, bv, i0);
+    block_argmax(best1, bi1, sv, si, bv, i1);
+    if (threadIdx.x == 0) *o0 = interp_tune(re0, N, i0);
+    if (threadIdx.x == 1) *o1 = interp_tune(re1, N, i1);
+}
+
 // Direct DFT for any N: X_k = sum_m z_m exp(-2 pi i k m / N), twiddle index
 // k*m mod N kept exact; the windowed signal is staged through LDS in chunks.
 __global__ void __launch_bounds__(256) k_fft_tune_dft(FftArgs a, double *__restrict__ mag_scratch)
@@ -341,6 +456,21 @@ __global__ void __launch_bounds__(256) k_fft_tune_dft(FftArgs a, double *__restr
 struct FftTables {
     double *hann = nullptr, *tw = nullptr;
 };
+
+// Signal rows staged per launch pair (bytes); particles are taken in batches that fit.
+static constexpr uint64_t kRowBudget = 4ull << 30;
+// HM_FFT_ROW_BYTES overrides the budget (small values force several batches).
+static uint64_t row_budget()
+{
+    const char *e = std::getenv("HM_FFT_ROW_BYTES");
+    if (e && *e) {
+        const unsigned long long v = std::strtoull(e, nullptr, 10);
+        if (v > 0) return v;
+    }
+    return kRowBudget;
+}
+// Longest window of the two-plane row kernel: 4 N doubles of LDS per workgroup.
+static constexpr uint32_t kRowsMaxN = 4096;
 
 static FftTables fft_tables(uint32_t N, hipStream_t s)
 {
@@ -492,7 +622,30 @@ extern "C" int hm_fft_tunes(const double *hist, size_t n, uint32_t n_samples, co
                 for (size_t q = b0; q < b1; q++) a.from[q - b0] = from[ws[q]];
                 const uint64_t signals = (uint64_t)n * 2 * a.count;
                 const uint32_t blocks = (uint32_t)(((signals + 7) / 8) * 8);
-                if (pow2) {
+                if (pow2 && N <= kRowsMaxN) {
+                    // per window: signal rows for a batch of particles (corner
+                    // turn), then one workgroup per particle reading them
+                    const uint64_t row_bytes = 4ull * N * sizeof(double);
+                    const uint32_t batch =
+                        (uint32_t)std::min<uint64_t>(n, std::max<uint64_t>(64, row_budget() / row_bytes));
+                    double *rows = static_cast<double *>(workspace((size_t)batch * row_bytes, 8));
+                    for (size_t q = b0; q < b1; q++) {
+                        for (uint32_t p0 = 0; p0 < (uint32_t)n; p0 += batch) {
+                            const uint32_t nb = std::min<uint32_t>(batch, (uint32_t)n - p0);
+                            {
+                                LaunchTimer timer("k_signal_rows", s);
+                                hipLaunchKernelGGL(k_signal_rows, dim3((nb + 63) / 64, (N + 63) / 64, 4), dim3(256), 0,
+                                                   s, hist, (uint32_t)n, means, from[ws[q]], N, p0, nb, rows);
+                                launch_check("k_signal_rows");
+                            }
+                            FftRowArgs r{rows, nb, p0, (uint32_t)n, N, t.hann, t.tw, out, ws[q]};
+                            LaunchTimer timer("k_fft_tune_pow2", s);
+                            hipLaunchKernelGGL(k_fft_tune_rows, dim3(nb), dim3(256), 4 * N * sizeof(double), s, r);
+                            launch_check("k_fft_tune_rows");
+                        }
+                    }
+                } else if (pow2) {
+                    // 8192 samples: one signal per workgroup (2 N doubles of LDS)
                     LaunchTimer timer("k_fft_tune_pow2", s);
                     hipLaunchKernelGGL(k_fft_tune_pow2, dim3(blocks), dim3(256), 2 * N * sizeof(double), s, a);
                     launch_check("k_fft_tune_pow2");

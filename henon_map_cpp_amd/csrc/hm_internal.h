@@ -68,12 +68,16 @@ inline void launch_check(const char *name)
 // Library-owned device words, per device: the status flags word (see
 // hm_take_status) and growable scratch buffers, one per slot: 0 tracking live
 // lists, 1 Birkhoff weights of the indicator kernels, 2 tune-window weights,
-// 3 DFT magnitudes, 4 batched-job lists and plans. Growing a slot synchronises
-// the device. Calls that overlap in time on one device share these buffers.
-constexpr int kWorkspaceSlots = 8;
+// 3 DFT magnitudes, 4 batched-job lists and plans, 5-6 autotuner trial state,
+// 7 reciprocal table, 8 FFT signal rows. Growing a slot synchronises the
+// device. Calls that overlap in time on one device share these buffers.
+constexpr int kWorkspaceSlots = 12;
 uint32_t *status_word();
 void *workspace(size_t bytes, int slot);
 
 constexpr size_t kTableRecord = 4;  // doubles per turn in the interleaved table
+
+// Device table of 1.0 / k for k = 1..n (at least n entries; workspace slot 7).
+const double *reciprocal_table(uint32_t n, hipStream_t s);
 
 }  // namespace hm

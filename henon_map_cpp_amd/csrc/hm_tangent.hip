@@ -16,6 +16,7 @@
 
 #include "hm_device.h"
 #include "hm_internal.h"
+#include "hm_tangent.h"
 
 namespace hm {
 
@@ -136,60 +137,14 @@ struct IndicatorArgs {
     uint32_t *status;
 };
 
-// M <- J M with J's structure: rows 0-1 touch M rows 0,1,2; rows 2-3 M rows 0,2,3.
-template <int MODE>
-__device__ __forceinline__ void mat_mul_structured(const Jac &j, const TabEntry &t, double *m)
-{
-    double j00, j02, j10, j12, j20, j22, j30, j32;
-    if (MODE == kExact) {
-        j00 = t.cx + (t.sx * j.a); j02 = t.sx * j.b;
-        j10 = -t.sx + (t.cx * j.a); j12 = t.cx * j.b;
-        j20 = t.sy * j.b; j22 = t.cy + (t.sy * j.an);
-        j30 = t.cy * j.b; j32 = -t.sy + (t.cy * j.an);
-    } else {
-        j00 = fma(t.sx, j.a, t.cx); j02 = t.sx * j.b;
-        j10 = fma(t.cx, j.a, -t.sx); j12 = t.cx * j.b;
-        j20 = t.sy * j.b; j22 = fma(-t.sy, j.a, t.cy);
-        j30 = t.cy * j.b; j32 = fma(-t.cy, j.a, -t.sy);
-    }
-    double r[16];
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const double m0 = m[c], m1 = m[4 + c], m2 = m[8 + c], m3 = m[12 + c];
-        if (MODE == kExact) {
-            r[c] = ((j00 * m0) + (t.sx * m1)) + (j02 * m2);
-            r[4 + c] = ((j10 * m0) + (t.cx * m1)) + (j12 * m2);
-            r[8 + c] = ((j20 * m0) + (j22 * m2)) + (t.sy * m3);
-            r[12 + c] = ((j30 * m0) + (j32 * m2)) + (t.cy * m3);
-        } else {
-            r[c] = fma(j00, m0, fma(t.sx, m1, j02 * m2));
-            r[4 + c] = fma(j10, m0, fma(t.cx, m1, j12 * m2));
-            r[8 + c] = fma(j20, m0, fma(j22, m2, t.sy * m3));
-            r[12 + c] = fma(j30, m0, fma(j32, m2, t.cy * m3));
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < 16; q++) m[q] = r[q];
-}
-
-template <int MODE>
-__device__ __forceinline__ double frob2(const double *m)
-{
-    double s = m[0] * m[0];
-    if (MODE == kExact) {
-#pragma unroll
-        for (int q = 1; q < 16; q++) s = s + (m[q] * m[q]);
-    } else {
-#pragma unroll
-        for (int q = 1; q < 16; q++) s = fma(m[q], m[q], s);
-    }
-    return s;
-}
 
 // One thread per particle; the table index of turn k is steps+k for both the
 // tangent matrix (evaluated at the pre-step point) and the step.
 //   MEGNO (construct form): with idx = k+1, L1 += log10(|M_{k+1}|_F / |M_k|_F) idx,
-//     L2 += L1 / idx; recorded at sample t as L2 / t.
+//     L2 += L1 / idx; recorded at sample t as L2 / t. The ratio is evaluated as
+//     0.5 log10(|M_{k+1}|^2 / |M_k|^2) with fused squared norms (one division,
+//     no square root) and 1/idx comes from a host table; the result agrees with
+//     the separate-rounding form to a few ulps per turn.
 //   FLI: log10(|M_t v0|) / t with v0 = fli_dir / |fli_dir|, including the
 //     log10 of the scale factored out by renormalisation.
 //   Renormalisation every renorm_every turns: M <- M / |M|_F, after the MEGNO
@@ -217,7 +172,7 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
         for (int q = 0; q < 16; q++) m[q] = (q % 5 == 0) ? 1.0 : 0.0;
     }
     double lscale = (a.log_scale && in_range) ? a.log_scale[i] : 0.0;
-    const double f0 = frob2<MODE>(m);
+    const double f0 = frob2_fused(m);
     double l1 = 0.0, l2 = 0.0;
     const double mu3 = a.mu * 3, mu6 = a.mu * 6, m3mu = -3.0 * a.mu;
     const bool alive0 = in_range && !any_nan(x, px, y, py);
@@ -281,7 +236,7 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
             done = k;
         }
         const uint32_t idx = k + 1;
-        const double fa2 = frob2<MODE>(m);
+        const double fa2 = frob2_fused(m);
         l1 = l1 + (0.5 * log10(fa2 / fb2)) * (double)idx;
         l2 = l2 + inv_idx[k] * l1;
         fb2 = fa2;
@@ -303,12 +258,12 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
         if (renorm_in && --renorm_in == 0) {
             renorm_in = a.renorm_every;
             if (alive) {
-                const double fa = sqrt(fa2);
+                const double fa = sqrt(frob2<MODE>(m));
                 const double inv = 1.0 / fa;
 #pragma unroll
                 for (int q = 0; q < 16; q++) m[q] = m[q] * inv;
                 lscale = lscale + log10(fa);
-                fb2 = frob2<MODE>(m);
+                fb2 = frob2_fused(m);
             }
         }
         if (a.n_birk) {
@@ -566,7 +521,7 @@ extern "C" int hm_megno_trace(double *x, double *px, double *y, double *py, uint
 
 // 1.0 / k for k = 1..n (correctly rounded, computed on the host), kept per
 // device in workspace slot 7 and extended when a longer run needs it.
-static const double *reciprocal_table(uint32_t n, hipStream_t s)
+const double *hm::reciprocal_table(uint32_t n, hipStream_t s)
 {
     static std::mutex mu;
     static std::map<int, uint32_t> filled;

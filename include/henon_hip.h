@@ -146,9 +146,14 @@ typedef struct hm_job {
     const double *table;        /* device table[table_len][4] */
     size_t table_len;
     uint32_t n_turns;
-    uint32_t reserved;
+    uint32_t renorm_every;      /* MEGNO jobs: renormalise the tangent matrix every this many
+                                   turns (0 = never) */
     double mu, barrier;
     double kick_module;         /* must be NaN */
+    double *megno;              /* NULL: tracking only. Else device double[n] receiving the
+                                   MEGNO (construct form, as hm_track_indicators) after
+                                   n_turns turns, NaN for lost particles; the tangent matrix
+                                   starts from the identity */
 } hm_job;
 
 typedef struct hm_batch_cfg {
@@ -161,7 +166,8 @@ typedef struct hm_batch_cfg {
 /* Track every job for its own n_turns, all jobs in one sequence of persistent
  * launches fed by a device work queue of 64-particle chunks, with loss
  * compaction per job between segments. Results are identical to one hm_track
- * call per job with the same mode. No counterpart in the reference, which tracks
+ * call per job with the same mode (MEGNO jobs: to hm_track_indicators with the
+ * same renormalisation and one sample at n_turns). No counterpart in the reference, which tracks
  * one particle set per call (henon_tracker::track, src/henon.cu:2701-2752,
  * :3155-3185). jobs: host array. Asynchronous on `stream` unless use_graph. */
 int hm_track_batch(const hm_job *jobs, size_t n_jobs, int mode, const hm_batch_cfg *cfg, void *stream);
@@ -289,7 +295,10 @@ int hm_birkhoff_tunes_history(const double *hist, size_t n, uint32_t n_samples, 
  * 0.0 at the spectrum's edges; NaN when a sample of the window is NaN. Windows
  * start at sample from[w] and have len[w] samples (host arrays). Power-of-two
  * lengths up to 8192 use a radix-2 FFT in LDS, other lengths up to 16384 a
- * direct DFT. out: device double[n_windows][2][n]. */
+ * direct DFT. Up to 4096 samples the windows are first corner-turned into
+ * per-particle signal rows (library workspace of at most 4 GiB, particles taken
+ * in batches that fit; the environment variable HM_FFT_ROW_BYTES overrides the
+ * budget). out: device double[n_windows][2][n]. */
 int hm_fft_tunes(const double *hist, size_t n, uint32_t n_samples, const double *means, const uint32_t *from,
                  const uint32_t *len, size_t n_windows, double *out, void *stream);
 

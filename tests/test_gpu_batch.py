@@ -102,3 +102,35 @@ def test_batch_empty_and_degenerate_jobs():
     for g, w in zip([g for k, g in enumerate(got) if k != 1], want):
         assert same(g, w)
     assert np.all(got[4][4][:3] == 0)
+
+
+@pytest.mark.parametrize("mode", [C.MODE_EXACT, C.MODE_FAST])
+def test_batch_megno_jobs_equal_indicator_runs(mode):
+    """Jobs that request MEGNO (about a third, mixed with tracking-only jobs)
+    give the same MEGNO, coordinates and survival turns as one
+    hm_track_indicators call per job (same renormalisation, one sample at the
+    job's last turn)."""
+    jobs = make_jobs(30, seed=15, n_range=(10, 1500), t_range=(50, 2500))
+    dev, megs = [], {}
+    for k, j in enumerate(jobs):
+        parts = C.DeviceParticles(*j["x0"])
+        tdev = C.DeviceBuffer.from_array(C.build_table(j["wx"], j["wy"]))
+        if k % 3 == 1:
+            megs[k] = C.DeviceBuffer(8 * parts.n)
+            dev.append((parts, tdev, j["T"], j["T"], j["mu"], j["barrier"], megs[k], 100 if k % 2 else 0))
+        else:
+            dev.append((parts, tdev, j["T"], j["T"], j["mu"], j["barrier"]))
+    C.track_batch(dev, mode, C.BatchCfg(0, 300, 0, 0))
+    assert C.take_status() == 0
+    for k, j in enumerate(jobs):
+        got = dev[k][0].get()
+        parts = C.DeviceParticles(*j["x0"])
+        tdev = C.DeviceBuffer.from_array(C.build_table(j["wx"], j["wy"]))
+        if k in megs:
+            r = C.track_indicators(parts, tdev, j["T"], j["T"], j["mu"], j["barrier"], mode=mode,
+                                   renorm_every=100 if k % 2 else 0, samples=[j["T"]], matrix=None)
+            m = megs[k].download(np.float64, parts.n)
+            assert np.array_equal(m, r["megno"][0], equal_nan=True), k
+        else:
+            C.track(parts, tdev, j["T"], j["T"], j["mu"], j["barrier"], False, mode)
+        assert same(got, parts.get()), k

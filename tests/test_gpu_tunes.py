@@ -93,7 +93,7 @@ def test_all_tunes_golden_on_gpu(golden):
 
 def test_single_tone_pow2_and_edges():
     # analytic tone z_m = exp(2 pi i f m): peak at f N, interpolation returns |1 - f|
-    for N, f in [(1024, 0.2371), (2048, 0.0913), (777, 0.3333)]:
+    for N, f in [(1024, 0.2371), (2048, 0.0913), (777, 0.3333), (4096, 0.4107), (8192, 0.0517)]:
         m = np.arange(N)
         z = np.exp(2j * np.pi * f * m) * 0.01
         h = np.zeros((N, 4, 1))
@@ -104,3 +104,30 @@ def test_single_tone_pow2_and_edges():
         want = [O.fft_tune(h[:, 0, 0], h[:, 1, 0]), O.fft_tune(h[:, 2, 0], h[:, 3, 0])]
         np.testing.assert_allclose(got[0, :, 0], want, atol=1e-12)
         assert got[0, 0, 0] == pytest.approx(1 - f, abs=1e-6)
+
+
+def test_fft_tunes_in_particle_batches(monkeypatch):
+    """The power-of-two path corner-turns signals for batches of particles; a
+    small staging budget forces several batches (the first and last particle of
+    each batch must land in the right output slot)."""
+    N, n = 256, 1000
+    rng = np.random.default_rng(9)
+    f = rng.uniform(0.02, 0.48, n)
+    m = np.arange(N)[:, None]
+    zx = 0.01 * np.exp(2j * np.pi * f[None, :] * m)
+    zy = 0.02 * np.exp(-2j * np.pi * f[None, :] * m)
+    h = np.zeros((N, 4, n))
+    h[:, 0], h[:, 1], h[:, 2], h[:, 3] = zx.real, zx.imag, zy.real, zy.imag
+    h[5, 2, 17] = np.nan  # one signal with a NaN sample
+    d = C.DeviceBuffer.from_array(np.ascontiguousarray(h))
+    monkeypatch.setenv("HM_FFT_ROW_BYTES", str(64 * 4 * N * 8 * 3))  # 192 particles per batch
+    got = C.history_tunes(d, n, N, [(0, N - 1)], "fft")
+    monkeypatch.delenv("HM_FFT_ROW_BYTES")
+    ref = C.history_tunes(d, n, N, [(0, N - 1)], "fft")
+    assert np.array_equal(got, ref, equal_nan=True)
+    for i in [0, 1, 191, 192, 193, 383, 384, 999, 17]:
+        want = [O.fft_tune(h[:, 0, i], h[:, 1, i]), O.fft_tune(h[:, 2, i], h[:, 3, i])]
+        if i == 17:
+            assert np.isnan(got[0, 0, i]) and np.isnan(got[0, 1, i])
+        else:
+            np.testing.assert_allclose(got[0, :, i], want, atol=1e-12)
