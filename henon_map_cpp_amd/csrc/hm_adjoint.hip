@@ -113,7 +113,7 @@ struct AdjointBwdArgs {
     const double *seed_s;     // [4][n]: dL/d(final state)
     const double *seed_b;     // [n]:    dL/dB, or NULL
     const uint8_t *alive;     // [n]
-    double *sub;              // scratch [K/kChunk + 1][4][n]
+    double *sub;              // scratch [2][ceil(K/kChunk) + 1][4][n]
     double *grad_x0;          // [4][n]
     double *grad_p;           // [4][n]: d/d eps, d/d mu, d/d omega_x0, d/d omega_y0
 };
@@ -163,14 +163,17 @@ __device__ __forceinline__ void copy4(double (&d)[4], const double (&s)[4])
     d[0] = s[0]; d[1] = s[1]; d[2] = s[2]; d[3] = s[3];
 }
 
-// Backward sweep. Per K-segment (in reverse): recompute it from its checkpoint,
-// writing the state every kChunk = 8 turns to scratch; the state after the
-// segment is the post-step point of its last turn. Per 8-turn chunk (in
-// reverse): load its table records, recompute its states S1..S7 from S0 into
-// registers (7 turns; the post-step point of its last turn, `nxt`, is known),
-// then run the adjoint back through its turns; S0 becomes the next chunk's
-// `nxt`. Table records are requested ahead of use, because with one particle
-// per lane and few wavefronts per SIMD nothing else hides their latency.
+// Backward sweep. The K-segments are processed in reverse. A segment's states
+// every kChunk = 8 turns (sub-checkpoints, in scratch) come from recomputing it
+// forward from its checkpoint; per 8-turn chunk (in reverse) the chunk's table
+// records are loaded, its states S1..S7 recomputed from S0 into registers (the
+// post-step point of its last turn, `nxt`, is known), and the adjoint run back
+// through its turns; S0 is the next chunk's `nxt`.
+// The forward recomputation of segment s-1 is interleaved with the reverse
+// sweep of segment s (one 8-turn block per chunk, into the other half of the
+// scratch): the two dependency chains are independent, so one wavefront per SIMD
+// (small particle counts) still keeps the FP64 pipeline busy. The state after
+// segment s-1 is the checkpoint of segment s.
 template <int MODE, bool OCT>
 __global__ void __launch_bounds__(256) k_adjoint_backward(AdjointBwdArgs a)
 {
@@ -186,30 +189,62 @@ __global__ void __launch_bounds__(256) k_adjoint_backward(AdjointBwdArgs a)
     const double mu = a.mu, m3mu = -3.0 * mu, mu3 = 3.0 * mu, mu6 = 6.0 * mu;
     const uint32_t T = a.n_turns, K = a.every;
     const uint32_t n_seg = (T + K - 1) / K;
+    const uint32_t seg_chunks = (K + kChunk - 1) / kChunk;  // sub-checkpoints of a full segment
     // tables and weights are read-only here and indexed by the (wave-uniform)
     // turn: scalar loads
     const ConstPtr<TabEntry> tab = as_const(a.tab);
     const ConstPtr<double> dtab = as_const(a.dtab), weights = as_const(a.weights);
+    auto sub_at = [&](uint32_t seg, uint32_t c) -> double * {
+        return a.sub + ((size_t)(seg & 1u) * (seg_chunks + 1) + c) * 4 * n;
+    };
+    auto load_ckpt = [&](uint32_t seg, double (&v)[4]) {
+        const size_t o = (size_t)seg * 4 * n;
+        v[0] = a.ckpt[o + i]; v[1] = a.ckpt[o + n + i]; v[2] = a.ckpt[o + 2 * n + i]; v[3] = a.ckpt[o + 3 * n + i];
+    };
+    auto store4 = [&](double *p, const double (&v)[4]) {
+        p[i] = v[0]; p[n + i] = v[1]; p[2 * n + i] = v[2]; p[3 * n + i] = v[3];
+    };
+    // forward recomputation of one segment, one 8-turn block per call: writes the
+    // block's sub-checkpoint, then advances unless it is the segment's last block
+    struct Recompute {
+        double v[4];
+        uint32_t seg, done, chunks, s0;
+    } rc{};
+    auto rc_start = [&](uint32_t seg) {
+        rc.seg = seg;
+        rc.done = 0;
+        rc.s0 = seg * K;
+        rc.chunks = (min(rc.s0 + K, T) - rc.s0 + kChunk - 1) / kChunk;
+        load_ckpt(seg, rc.v);
+    };
+    auto rc_block = [&]() {
+        if (rc.done >= rc.chunks) return;
+        store4(sub_at(rc.seg, rc.done), rc.v);
+        if (rc.done + 1 < rc.chunks) {
+            const uint32_t k0 = rc.s0 + rc.done * kChunk;
+#pragma unroll
+            for (int m = 0; m < kChunk; ++m) step4<MODE, OCT>(rc.v, mu, m3mu, load_record(tab + k0 + m));
+        }
+        ++rc.done;
+    };
+
+    // prologue: sub-checkpoints of the last segment
+    rc_start(n_seg - 1);
+    while (rc.done < rc.chunks) rc_block();
+    // the state after the last turn: from the last sub-checkpoint (< 8 turns)
+    double nxt[4];
+    {
+        const uint32_t q = rc.chunks - 1;
+        const double *p = sub_at(n_seg - 1, q);
+        double v[4] = {p[i], p[n + i], p[2 * n + i], p[3 * n + i]};
+        for (uint32_t k = (n_seg - 1) * K + q * kChunk; k < T; ++k)
+            step4<MODE, OCT>(v, mu, m3mu, load_record(tab + k));
+        copy4(nxt, v);
+    }
     for (int seg = (int)n_seg - 1; seg >= 0; --seg) {
         const uint32_t s0 = (uint32_t)seg * K, s1 = min(s0 + K, T);
-        double nxt[4];
-        {
-            const size_t o = (size_t)seg * 4 * n;
-            double v[4] = {a.ckpt[o + i], a.ckpt[o + n + i], a.ckpt[o + 2 * n + i], a.ckpt[o + 3 * n + i]};
-            double *sub = a.sub;
-            TabEntry nt = load_record(tab + s0);
-            for (uint32_t k = s0, until = 0; k < s1; ++k, --until) {
-                const TabEntry t = nt;
-                if (k + 1 < s1) nt = load_record(tab + k + 1);
-                if (until == 0) {
-                    sub[i] = v[0]; sub[n + i] = v[1]; sub[2 * n + i] = v[2]; sub[3 * n + i] = v[3];
-                    sub += 4 * n;
-                    until = kChunk;
-                }
-                step4<MODE, OCT>(v, mu, m3mu, t);
-            }
-            copy4(nxt, v);
-        }
+        const bool overlap = seg > 0;
+        if (overlap) rc_start((uint32_t)seg - 1);
         const uint32_t n_chunk = (s1 - s0 + kChunk - 1) / kChunk;
         for (int ch = (int)n_chunk - 1; ch >= 0; --ch) {
             const uint32_t c0 = s0 + (uint32_t)ch * kChunk;
@@ -217,15 +252,17 @@ __global__ void __launch_bounds__(256) k_adjoint_backward(AdjointBwdArgs a)
             TabEntry R[kChunk];
 #pragma unroll
             for (int m = 0; m < kChunk; ++m) R[m] = load_record(tab + min(c0 + m, s1 - 1));
-            const size_t q = (size_t)ch * 4 * n;
+            const double *q = sub_at((uint32_t)seg, (uint32_t)ch);
             double S[kChunk][4];
-            S[0][0] = a.sub[q + i]; S[0][1] = a.sub[q + n + i]; S[0][2] = a.sub[q + 2 * n + i];
-            S[0][3] = a.sub[q + 3 * n + i];
+            S[0][0] = q[i]; S[0][1] = q[n + i]; S[0][2] = q[2 * n + i]; S[0][3] = q[3 * n + i];
 #pragma unroll
             for (int m = 1; m < kChunk; ++m) {
                 copy4(S[m], S[m - 1]);
                 if ((uint32_t)m < len) step4<MODE, OCT>(S[m], mu, m3mu, R[m - 1]);
             }
+            // the previous segment's next 8-turn block: independent of the
+            // adjoint chain below, so the two interleave
+            if (overlap) rc_block();
 #pragma unroll
             for (int m = kChunk - 1; m >= 0; --m) {
                 if ((uint32_t)m >= len) continue;
@@ -237,6 +274,10 @@ __global__ void __launch_bounds__(256) k_adjoint_backward(AdjointBwdArgs a)
                 else adjoint_turn<OCT>(S[m], S[m + 1 < kChunk ? m + 1 : m], R[m], bw, dt, mu3, mu6, acc);
             }
             copy4(nxt, S[0]);
+        }
+        if (overlap) {
+            while (rc.done < rc.chunks) rc_block();  // a short last segment leaves blocks over
+            load_ckpt((uint32_t)seg, nxt);           // the state after segment seg-1
         }
     }
     a.grad_x0[i] = acc.lx; a.grad_x0[n + i] = acc.lpx; a.grad_x0[2 * n + i] = acc.ly; a.grad_x0[3 * n + i] = acc.lpy;
@@ -278,7 +319,8 @@ extern "C" int hm_adjoint_scratch_bytes(size_t n, uint32_t every, size_t *bytes)
 {
     return guarded([&] {
         if (!bytes || every == 0) throw Error(HM_ERR_ARG, "adjoint: bad arguments");
-        *bytes = ((size_t)(every + kChunk - 1) / kChunk + 1) * 4 * n * sizeof(double);
+        // two halves (the segment being swept and the one being recomputed)
+        *bytes = 2 * ((size_t)(every + kChunk - 1) / kChunk + 1) * 4 * n * sizeof(double);
     });
 }
 

@@ -211,7 +211,12 @@ __device__ __forceinline__ void megno_chunk(const BatchSeg &S, ConstPtr<JobDesc>
     const ConstPtr<double> inv_idx = as_const(S.inv_idx);
     const uint32_t renorm = J->renorm;
     uint32_t renorm_in = renorm ? renorm - S.seg_start % renorm : 0;
+    // as in k_indicators, the sums of turn k are added at the top of turn k+1
+    // (same operations, so the same values) to overlap the log10 chain
+    double q_prev = 1.0, w_prev = 0.0, inv_prev = 0.0;
     for (uint32_t k = 0; k < turns && live0; ++k) {
+        l1 = l1 + (0.5 * log10(q_prev)) * w_prev;
+        l2 = w_prev != 0.0 ? l2 + inv_prev * l1 : l2;  // nothing pending before the first turn
         TabEntry t;
         if (uniform) {
             t = load_record(tp + k);
@@ -236,8 +241,9 @@ __device__ __forceinline__ void megno_chunk(const BatchSeg &S, ConstPtr<JobDesc>
         }
         const uint32_t idx = S.seg_start + k + 1;
         const double fa2 = frob2_fused(m);
-        l1 = l1 + (0.5 * log10(fa2 / fb2)) * (double)idx;
-        l2 = l2 + inv_idx[idx - 1] * l1;
+        q_prev = fa2 / fb2;
+        w_prev = (double)idx;
+        inv_prev = inv_idx[idx - 1];
         fb2 = fa2;
         if (renorm_in && --renorm_in == 0) {
             renorm_in = renorm;
@@ -251,6 +257,8 @@ __device__ __forceinline__ void megno_chunk(const BatchSeg &S, ConstPtr<JobDesc>
         }
         if (!__any(alive)) break;
     }
+    l1 = l1 + (0.5 * log10(q_prev)) * w_prev;
+    l2 = w_prev != 0.0 ? l2 + inv_prev * l1 : l2;
     if (alive0) {
         gx[j] = x; gpx[j] = px; gy[j] = y; gpy[j] = py;
         gsteps[j] = s + done;

@@ -82,39 +82,40 @@ struct WindowSet {
 // (x - mean) + i (px - mean), wrapped differences (+2pi when negative), weights
 // birkhoff_weights(len-1); NaN when a sample of the window is NaN
 // (dynamic_indicator.cpp:228-241). out: [window][plane][n].
+// One thread per (particle, plane, window): enough wavefronts to hide the
+// latency of the per-sample atan2 chains.
 __global__ void k_birkhoff_hist(const double *__restrict__ hist, uint32_t n, const double *__restrict__ means,
                                 WindowSet ws, const double *__restrict__ w_flat, double *__restrict__ out)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = (uint32_t)(idx % n);
+    const uint64_t rest = idx / n;
+    const int p = (int)(rest % 2);
+    const uint32_t w = (uint32_t)(rest / 2);
+    if (w >= ws.count) return;
     const size_t nn = n;
-    double mu_[4] = {0.0, 0.0, 0.0, 0.0};
-    if (means)
-        for (int c = 0; c < 4; c++) mu_[c] = means[c * nn + i];
-    for (uint32_t w = 0; w < ws.count; w++) {
-        for (int p = 0; p < 2; p++) {
-            const int cx = 2 * p, cp = 2 * p + 1;
-            double acc = 0.0, th_prev = 0.0;
-            bool bad = false;
-            for (uint32_t m = 0; m < ws.len[w]; m++) {
-                const size_t o = (size_t)(ws.from[w] + m) * 4 * nn;
-                const double xv = hist[o + cx * nn + i] - mu_[cx];
-                const double pv = hist[o + cp * nn + i] - mu_[cp];
-                // any NaN among the four coordinates of the window
-                if (__builtin_isnan(hist[o + 0 * nn + i]) || __builtin_isnan(hist[o + 1 * nn + i]) ||
-                    __builtin_isnan(hist[o + 2 * nn + i]) || __builtin_isnan(hist[o + 3 * nn + i]))
-                    bad = true;
-                const double th = atan2(pv, xv);
-                if (m > 0) {
-                    double d = th - th_prev;
-                    if (d < 0.0) d += 2.0 * M_PI;
-                    acc = acc + w_flat[ws.w_off[w] + m - 1] * d;
-                }
-                th_prev = th;
-            }
-            out[((size_t)w * 2 + p) * nn + i] = bad ? __builtin_nan("") : 1 - acc / (2 * M_PI);
+    const int cx = 2 * p, cp = 2 * p + 1;
+    const double mx = means ? means[cx * nn + i] : 0.0;
+    const double mp = means ? means[cp * nn + i] : 0.0;
+    double acc = 0.0, th_prev = 0.0;
+    bool bad = false;
+    for (uint32_t m = 0; m < ws.len[w]; m++) {
+        const size_t o = (size_t)(ws.from[w] + m) * 4 * nn;
+        const double a0 = hist[o + 0 * nn + i], a1 = hist[o + 1 * nn + i];
+        const double a2 = hist[o + 2 * nn + i], a3 = hist[o + 3 * nn + i];
+        // any NaN among the four coordinates of the window
+        bad |= any_nan(a0, a1, a2, a3);
+        const double xv = (p ? a2 : a0) - mx;
+        const double pv = (p ? a3 : a1) - mp;
+        const double th = atan2(pv, xv);
+        if (m > 0) {
+            double d = th - th_prev;
+            if (d < 0.0) d += 2.0 * M_PI;
+            acc = acc + w_flat[ws.w_off[w] + m - 1] * d;
         }
+        th_prev = th;
     }
+    out[((size_t)w * 2 + p) * nn + i] = bad ? __builtin_nan("") : 1 - acc / (2 * M_PI);
 }
 
 // ---------------------------------------------------------------- FFT tune
@@ -582,8 +583,8 @@ extern "C" int hm_birkhoff_tunes_history(const double *hist, size_t n, uint32_t 
             double *dw = static_cast<double *>(workspace(std::max<size_t>(total, 1) * sizeof(double), 2));
             hip_check(hipMemcpyAsync(dw, wts.data(), total * sizeof(double), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
             LaunchTimer timer("k_birkhoff_hist", s);
-            hipLaunchKernelGGL(k_birkhoff_hist, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, hist, (uint32_t)n,
-                               means, ws, (const double *)dw, out + w0 * 2 * n);
+            hipLaunchKernelGGL(k_birkhoff_hist, dim3((uint32_t)((n * 2 * ws.count + 255) / 256)), dim3(256), 0, s, hist,
+                               (uint32_t)n, means, ws, (const double *)dw, out + w0 * 2 * n);
             launch_check("k_birkhoff_hist");
             hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
         }

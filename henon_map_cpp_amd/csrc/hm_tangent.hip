@@ -205,14 +205,31 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
     const ConstPtr<TabEntry> tp = as_const(a.tab + s0);
     const ConstPtr<double> inv_idx = as_const(a.inv_idx);
 
-    uint32_t sample_q = 0;
+    uint32_t sample_q = 0;  // FLI samples (taken at their turn)
+    uint32_t sample_m = 0;  // MEGNO samples (taken once the turn's sums are in)
     uint32_t next_sample = a.n_samples ? a.samples[0] : 0;
+    uint32_t next_megno = next_sample;
     uint32_t k = 0;
     // squared Frobenius norm of M before the turn; the MEGNO ratio
     // |M_{k+1}|/|M_k| is taken as sqrt(fa2 / fb2): one division, no square root
     double fb2 = f0;
     uint32_t renorm_in = a.renorm_every;
+    // The MEGNO sums of turn k are added at the top of turn k+1: the log10 of the
+    // ratio (a long dependent chain) then overlaps the next turn's matrix work.
+    // q_prev = 1, w_prev = 0 make the first addition a no-op.
+    double q_prev = 1.0, w_prev = 0.0, inv_prev = 0.0;
+    uint32_t idx_prev = 0;
+    auto megno_sample = [&](uint32_t idx) {
+        if (idx && idx == next_megno) {
+            if (in_range && a.megno) a.megno[(size_t)sample_m * a.n + i] = inv_prev * l2;
+            ++sample_m;
+            next_megno = sample_m < a.n_samples ? a.samples[sample_m] : 0;
+        }
+    };
     for (; k < a.n_turns; ++k) {
+        l1 = l1 + (0.5 * log10(q_prev)) * w_prev;
+        l2 = w_prev != 0.0 ? l2 + inv_prev * l1 : l2;  // nothing pending before the first turn
+        megno_sample(idx_prev);
         TabEntry t;
         if (uniform) {
             t = load_record(tp + k);
@@ -237,12 +254,13 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
         }
         const uint32_t idx = k + 1;
         const double fa2 = frob2_fused(m);
-        l1 = l1 + (0.5 * log10(fa2 / fb2)) * (double)idx;
-        l2 = l2 + inv_idx[k] * l1;
+        q_prev = fa2 / fb2;
+        w_prev = (double)idx;
+        inv_prev = inv_idx[k];
+        idx_prev = idx;
         fb2 = fa2;
         if (idx == next_sample) {
             const size_t o = (size_t)sample_q * a.n + i;
-            if (in_range && a.megno) a.megno[o] = inv_idx[k] * l2;
             if (in_range && a.fli) {
                 double v[4];
 #pragma unroll
@@ -289,13 +307,16 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
         }
         if (!__any(alive) && !a.hist_len && !a.n_birk) break;
     }
+    // the last turn's MEGNO sums
+    l1 = l1 + (0.5 * log10(q_prev)) * w_prev;
+    l2 = w_prev != 0.0 ? l2 + inv_prev * l1 : l2;
+    megno_sample(idx_prev);
     if (!in_range) return;
     // samples not reached because every particle of the wavefront was lost
-    for (; sample_q < a.n_samples; ++sample_q) {
-        const size_t o = (size_t)sample_q * a.n + i;
-        if (a.megno) a.megno[o] = __builtin_nan("");
-        if (a.fli) a.fli[o] = __builtin_nan("");
-    }
+    for (; sample_q < a.n_samples; ++sample_q)
+        if (a.fli) a.fli[(size_t)sample_q * a.n + i] = __builtin_nan("");
+    for (; sample_m < a.n_samples; ++sample_m)
+        if (a.megno) a.megno[(size_t)sample_m * a.n + i] = __builtin_nan("");
     if (alive0) {
         a.x[i] = x; a.px[i] = px; a.y[i] = y; a.py[i] = py;
         a.steps[i] = s + done;

@@ -327,8 +327,9 @@ int hm_adjoint_scratch_bytes(size_t n, uint32_t every, size_t *bytes);
  * (double[4][n]), with dtable[k] = (d omega_x(k)/d omega_x0, d omega_x(k)/d eps,
  * d omega_y(k)/d omega_y0, d omega_y(k)/d eps) (device double[n_turns][4], from
  * hm_lines_modulation_derivatives). Lost particles get zero gradients. Each
- * K-turn segment is recomputed from its checkpoint and each 8-turn chunk of it
- * from a sub-checkpoint in scratch; the map is never inverted. */
+ * K-turn segment is recomputed from its checkpoint (interleaved with the reverse
+ * sweep of the segment after it) and each 8-turn chunk of it from a
+ * sub-checkpoint in scratch; the map is never inverted. */
 int hm_adjoint_backward(const double *checkpoints, size_t n, const double *table, const double *dtable,
                         size_t table_len, uint32_t n_turns, double mu, uint32_t every, const double *weights,
                         const double *seed_state, const double *seed_birk, const uint8_t *alive, double *scratch,
