@@ -16,6 +16,7 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "henon_hip.h"
@@ -1094,14 +1095,14 @@ void track_batch(const py::list &jobs, const std::string &mode)
 {
     std::vector<hm_job> js;
     std::vector<std::pair<Particles4D *, unsigned int>> adv;
+    std::unordered_set<const Particles4D *> seen;
     for (const py::handle &h : jobs) {
         const py::tuple t = h.cast<py::tuple>();
         if (t.size() != 5) throw std::invalid_argument("track_batch: jobs are (tracker, particles, n_turns, mu, barrier)");
         HenonTracker &tr = t[0].cast<HenonTracker &>();
         Particles4D &p = t[1].cast<Particles4D &>();
         const unsigned int n_turns = t[2].cast<unsigned int>();
-        for (const auto &a : adv)
-            if (a.first == &p) throw std::invalid_argument("track_batch: a particle set appears in two jobs");
+        if (!seen.insert(&p).second) throw std::invalid_argument("track_batch: a particle set appears in two jobs");
         js.push_back(tr.batch_job(p, n_turns, t[3].cast<double>(), t[4].cast<double>()));
         adv.emplace_back(&p, n_turns);
     }

@@ -7,6 +7,9 @@
  * allocations (hm_malloc, or any hipMalloc'd / torch CUDA tensor memory); a
  * `stream` argument is a hipStream_t passed as void* (NULL = the null stream).
  * Kernel launches are asynchronous with respect to the host unless stated.
+ * The library keeps per-device scratch buffers (live lists, staging, tables);
+ * calls that overlap in time on one device, from several host threads or
+ * streams, share them and must be serialised by the caller.
  *
  * Layouts in HBM
  *   particles : structure of arrays, x, px, y, py (double[n]) and steps (uint32[n])
