@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--cpu-sample-stride", type=int, default=13)
     ap.add_argument("--single-mode", action="store_true", help="time only --mode (profiling runs)")
     ap.add_argument("--compact-every", type=int, default=0)
-    ap.add_argument("--legs", default="indicators,tunes,adjoint,batch",
+    ap.add_argument("--legs", default="indicators,reversibility,tunes,adjoint,batch",
                     help="extra measurements after the headline: indicators, adjoint, batch (comma list, or none)")
     ap.add_argument("--batch-jobs", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -187,6 +187,43 @@ def indicators_leg(steps, warmup, mode_name="exact"):
         "frac": live * fpt / ks / 1e12 / FP64_PEAK_TFLOPS,
         "kernel_ms_per_step": kms / steps,
     }
+
+
+def reversibility_leg():
+    """Reversibility error |x0 - B^t F^t x0| (henon_map_cpp/dynamic_indicators.py:
+    136-151) of the chaos-indicator particle set at t = 1e3, 1e4, 1e5: one
+    forward pass with snapshots, then the inverse map from each snapshot."""
+    from henon_map_cpp_amd import _capi as C
+    from henon_map_cpp_amd import workloads as W
+
+    c = W.CHAOS_INDICATORS
+    x0 = W.indicator_points()
+    T = c["n_turns"]
+    tdev = C.DeviceBuffer.from_array(C.build_table(_lines(c["tune_x"], c["epsilon"], T),
+                                                   _lines(c["tune_y"], c["epsilon"], T)))
+    n = x0[0].size
+    parts = C.DeviceParticles(*x0)
+    C.check(C.lib().hm_stream_synchronize(None))
+    C.profile_reset()
+    C.profile_enable(True)
+    out, done = {}, 0
+    for t in (1_000, 10_000, 100_000):
+        C.track(parts, tdev, T, t - done, c["mu"], c["barrier"], False, C.MODE_EXACT)
+        done = t
+        g = parts.get()
+        snap = C.DeviceParticles(*g[:4], steps=g[4])
+        C.track(snap, tdev, T, t, c["mu"], c["barrier"], True, C.MODE_EXACT)
+        back = snap.get()
+        err = np.sqrt(sum((b - a) ** 2 for a, b in zip(x0, back[:4])))
+        ok = np.isfinite(err)
+        out[str(t)] = {"median": float(np.median(err[ok])) if ok.any() else None,
+                       "p99": float(np.quantile(err[ok], 0.99)) if ok.any() else None,
+                       "survivors": int(ok.sum())}
+    ms, launches = C.profile_read("k_track")
+    C.profile_enable(False)
+    pt = n * (T + 1_000 + 10_000 + 100_000)
+    return {"workload": f"{n} particles (chaos-indicator set), forward 1e5 turns + inverse 1e3, 1e4, 1e5 turns",
+            "kernel_ms": ms, "particle_turns_per_s": pt / (ms / 1e3), "error": out}
 
 
 def tunes_leg():
@@ -465,6 +502,9 @@ def main():
         line["tangent_megno"]["other_mode"] = {"fast": {k: fast[k] for k in ("value", "live_value", "frac",
                                                                             "kernel_ms_per_step")}}
         print(f"[bench] tangent/MEGNO {line['tangent_megno']}", file=sys.stderr, flush=True)
+    if "reversibility" in legs:
+        line["reversibility"] = reversibility_leg()
+        print(f"[bench] reversibility {line['reversibility']}", file=sys.stderr, flush=True)
     if "tunes" in legs:
         line["tunes"] = tunes_leg()
         print(f"[bench] tunes {line['tunes']}", file=sys.stderr, flush=True)

@@ -214,6 +214,8 @@ __device__ __forceinline__ void megno_chunk(const BatchSeg &S, ConstPtr<JobDesc>
     // as in k_indicators, the sums of turn k are added at the top of turn k+1
     // (same operations, so the same values) to overlap the log10 chain
     double q_prev = 1.0, w_prev = 0.0, inv_prev = 0.0;
+    double xx = x * x, yy = y * y;  // carried between turns in exact mode
+    const double b2s = screen_bound(b2);
     for (uint32_t k = 0; k < turns && live0; ++k) {
         l1 = l1 + (0.5 * log10(q_prev)) * w_prev;
         l2 = w_prev != 0.0 ? l2 + inv_prev * l1 : l2;  // nothing pending before the first turn
@@ -231,9 +233,8 @@ __device__ __forceinline__ void megno_chunk(const BatchSeg &S, ConstPtr<JobDesc>
         }
         const Jac jc = jac_coeffs<MODE>(x, y, mu3, mu6);
         mat_mul_structured<MODE>(jc, t, m);
-        map_step<MODE, false, OCT>(x, px, y, py, mu, m3mu, t);
-        if (alive && !(radius2<MODE>(x, px, y, py) < b2)) {
-            x = px = y = py = __builtin_nan("");
+        if (forward_turn_lost<MODE, OCT>(x, px, y, py, xx, yy, mu, m3mu, t, b2, b2s) && alive) {
+            x = px = y = py = xx = yy = __builtin_nan("");
 #pragma unroll
             for (int q = 0; q < 16; q++) m[q] = __builtin_nan("");
             alive = false;

@@ -193,6 +193,24 @@ __device__ __forceinline__ double radius2(double x, double px, double y, double 
     return fma(x, x, fma(px, px, fma(y, y, py * py)));
 }
 
+// One forward turn and the loss test, with hm_track's decisions: exact mode
+// carries x*x, y*y (xx, yy) between turns and screens the radius (b2s =
+// screen_bound(b2)), confirming candidates with the reference's expression;
+// fast mode tests its fused radius. Returns true when the point leaves the
+// barrier (or is NaN).
+template <int MODE, bool OCT>
+__device__ __forceinline__ bool forward_turn_lost(double &x, double &px, double &y, double &py, double &xx, double &yy,
+                                                  double mu, double m3mu, const TabEntry &t, double b2, double b2s)
+{
+    if (MODE == kExact) {
+        const double r2 = fwd_exact_carry<OCT>(x, px, y, py, xx, yy, mu, t);
+        if (r2 < b2s) return false;
+        return !(exact_radius2(xx, px, yy, py) < b2);
+    }
+    map_step<kFast, false, OCT>(x, px, y, py, mu, m3mu, t);
+    return !(radius2<kFast>(x, px, y, py) < b2);
+}
+
 __device__ __forceinline__ bool any_nan(double x, double px, double y, double py)
 {
     return __builtin_isnan(x) || __builtin_isnan(px) || __builtin_isnan(y) || __builtin_isnan(py);

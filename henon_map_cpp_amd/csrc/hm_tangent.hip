@@ -214,6 +214,8 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
     // |M_{k+1}|/|M_k| is taken as sqrt(fa2 / fb2): one division, no square root
     double fb2 = f0;
     uint32_t renorm_in = a.renorm_every;
+    double xx = x * x, yy = y * y;  // carried between turns in exact mode
+    const double b2s = screen_bound(a.b2);
     // The MEGNO sums of turn k are added at the top of turn k+1: the log10 of the
     // ratio (a long dependent chain) then overlaps the next turn's matrix work.
     // q_prev = 1, w_prev = 0 make the first addition a no-op.
@@ -244,9 +246,8 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
         }
         const Jac jc = jac_coeffs<MODE>(x, y, mu3, mu6);
         mat_mul_structured<MODE>(jc, t, m);
-        map_step<MODE, false, OCT>(x, px, y, py, a.mu, m3mu, t);
-        if (alive && !(radius2<MODE>(x, px, y, py) < a.b2)) {
-            x = px = y = py = __builtin_nan("");
+        if (forward_turn_lost<MODE, OCT>(x, px, y, py, xx, yy, a.mu, m3mu, t, a.b2, b2s) && alive) {
+            x = px = y = py = xx = yy = __builtin_nan("");
 #pragma unroll
             for (int q = 0; q < 16; q++) m[q] = __builtin_nan("");
             alive = false;
