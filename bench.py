@@ -347,14 +347,14 @@ def batch_leg(n_jobs):
 
     megno_job = [k % 10 < 3 for k in range(n_jobs)]  # 30% of the jobs also accumulate MEGNO
 
-    def batched():
+    def batched(graph=False):
         parts = fresh()
         megs = [C.DeviceBuffer(8 * p[0]) if m else None for p, m in zip(params, megno_job)]
         jobs = [(pp, tb, p[1], p[1], p[5], c["barrier"], mg, 100)
                 for pp, tb, p, mg in zip(parts, tabs, params, megs)]
         _sync()
         t0 = time.perf_counter()
-        C.track_batch(jobs, C.MODE_EXACT)
+        C.track_batch(jobs, C.MODE_EXACT, C.BatchCfg(0, 0, 0, 1) if graph else None)
         C.check(C.lib().hm_stream_synchronize(None))
         return time.perf_counter() - t0, parts, megs
 
@@ -376,13 +376,15 @@ def batch_leg(n_jobs):
 
     batched()
     tb, pb, mb = min((batched() for _ in range(2)), key=lambda r: r[0])
+    tg = batched(graph=True)[0]
     tp, pp, mp = plain()
     identical = all(np.array_equal(u, v, equal_nan=True) for a, b in zip(pb, pp) for u, v in zip(a.get(), b.get()))
     identical = identical and all(
         np.array_equal(a.download(np.float64, len(b)), b, equal_nan=True) for a, b in zip(mb, mp) if a is not None)
     return {"workload": c["name"] + ", 30% of the jobs with MEGNO (renormalised every 100 turns)",
             "jobs": n_jobs, "megno_jobs": int(sum(megno_job)), "nominal_particle_turns": pt,
-            "batched_s": tb, "plain_s": tp, "batched_value": pt / tb, "plain_value": pt / tp,
+            "batched_s": tb, "batched_graph_s": tg, "plain_s": tp, "batched_value": pt / tb,
+            "plain_value": pt / tp,
             "speedup": tp / tb, "identical_to_plain": identical}
 
 

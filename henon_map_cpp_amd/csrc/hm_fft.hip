@@ -460,6 +460,25 @@ struct FftTables {
 
 // Signal rows staged per launch pair (bytes); particles are taken in batches that fit.
 static constexpr uint64_t kRowBudget = 4ull << 30;
+// Static LDS of the FFT kernels besides the signal (argmax scratch and flags).
+static constexpr size_t kRowsStaticLds = 256 * (sizeof(double) + sizeof(uint32_t)) + 64;
+
+// LDS a workgroup may use on the current device.
+static size_t lds_per_block()
+{
+    int dev = 0, v = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
+    return (size_t)v;
+}
+
+// Lets `fn` use `bytes` of dynamic LDS (needed above the default limit).
+static void allow_dynamic_lds(const void *fn, size_t bytes)
+{
+    hip_check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes),
+              "hipFuncSetAttribute(dynamic LDS)");
+}
+
 // HM_FFT_ROW_BYTES overrides the budget (small values force several batches).
 static uint64_t row_budget()
 {
@@ -623,7 +642,8 @@ extern "C" int hm_fft_tunes(const double *hist, size_t n, uint32_t n_samples, co
                 for (size_t q = b0; q < b1; q++) a.from[q - b0] = from[ws[q]];
                 const uint64_t signals = (uint64_t)n * 2 * a.count;
                 const uint32_t blocks = (uint32_t)(((signals + 7) / 8) * 8);
-                if (pow2 && N <= kRowsMaxN) {
+                const size_t rows_lds = 4 * (size_t)N * sizeof(double) + kRowsStaticLds;
+                if (pow2 && N <= kRowsMaxN && rows_lds <= lds_per_block()) {
                     // per window: signal rows for a batch of particles (corner
                     // turn), then one workgroup per particle reading them
                     const uint64_t row_bytes = 4ull * N * sizeof(double);
@@ -640,13 +660,15 @@ extern "C" int hm_fft_tunes(const double *hist, size_t n, uint32_t n_samples, co
                                 launch_check("k_signal_rows");
                             }
                             FftRowArgs r{rows, nb, p0, (uint32_t)n, N, t.hann, t.tw, out, ws[q]};
+                            allow_dynamic_lds((const void *)k_fft_tune_rows, 4 * N * sizeof(double));
                             LaunchTimer timer("k_fft_tune_pow2", s);
                             hipLaunchKernelGGL(k_fft_tune_rows, dim3(nb), dim3(256), 4 * N * sizeof(double), s, r);
                             launch_check("k_fft_tune_rows");
                         }
                     }
-                } else if (pow2) {
-                    // 8192 samples: one signal per workgroup (2 N doubles of LDS)
+                } else if (pow2 && 2 * (size_t)N * sizeof(double) + kRowsStaticLds <= lds_per_block()) {
+                    // longer windows: one signal per workgroup (2 N doubles of LDS)
+                    allow_dynamic_lds((const void *)k_fft_tune_pow2, 2 * N * sizeof(double));
                     LaunchTimer timer("k_fft_tune_pow2", s);
                     hipLaunchKernelGGL(k_fft_tune_pow2, dim3(blocks), dim3(256), 2 * N * sizeof(double), s, a);
                     launch_check("k_fft_tune_pow2");
