@@ -42,7 +42,10 @@ struct AdjointFwdArgs {
 
 // Every lane runs every turn (a lost particle's lane carries on with values
 // that are discarded), so the table record and the weight of a turn are
-// wave-uniform scalar loads and the loss test is one compare.
+// wave-uniform scalar loads, requested ahead of use. Per K-turn segment: the
+// checkpoint, then the turns four at a time (the tracking loop's shape: two
+// alternating pairs of prefetched records, the loss test on a rare path), with
+// the Birkhoff sum taking the carried x*x in exact mode.
 template <int MODE, bool OCT>
 __global__ void __launch_bounds__(256) k_adjoint_forward(AdjointFwdArgs a)
 {
@@ -63,37 +66,53 @@ __global__ void __launch_bounds__(256) k_adjoint_forward(AdjointFwdArgs a)
         }
         return;
     }
+    if (!alive) x = px = y = py = 0.0;  // NaN inputs and lanes past n: parked (see below)
     const ConstPtr<TabEntry> tab = as_const(a.tab);
     const ConstPtr<double> weights = as_const(a.weights);
+    const double mu = a.mu, m3mu = -3.0 * mu, b2 = a.b2, b2s = screen_bound(b2);
+    double xx = x * x, yy = y * y;
     double bsum = 0.0;
-    const double m3mu = -3.0 * a.mu;
-    const double b2s = MODE == kExact ? screen_bound(a.b2) : a.b2;
+    // a lost particle is parked at the origin, a fixed point of the map, so
+    // that its lane never again takes the loss test's rare path
+    auto turn = [&](const TabEntry &t, double w) {
+        if (forward_turn_lost<MODE, OCT>(x, px, y, py, xx, yy, mu, m3mu, t, b2, b2s)) {
+            alive = false;
+            x = px = y = py = xx = yy = 0.0;
+        }
+        bsum = fma(w, MODE == kExact ? xx : x * x, bsum);
+    };
+    const uint32_t T = a.n_turns, K = a.every;
     double *ck = a.ckpt;
-    // the next turn's record and weight are requested a turn ahead
-    TabEntry nt = load_record(tab);
-    double nw = weights[0];
-    for (uint32_t k = 0, until = 0; k < a.n_turns; ++k, --until) {
-        const TabEntry t = nt;
-        const double w = nw;
-        if (k + 1 < a.n_turns) {
-            nt = load_record(tab + k + 1);
-            nw = weights[k + 1];
+    for (uint32_t s0 = 0; s0 < T; s0 += K) {
+        if (in) {
+            ck[i] = x; ck[n + i] = px; ck[2 * n + i] = y; ck[3 * n + i] = py;
         }
-        if (until == 0) {  // checkpoint before turns 0, every, 2 every, ...
-            if (in) {
-                ck[i] = x; ck[n + i] = px; ck[2 * n + i] = y; ck[3 * n + i] = py;
+        ck += 4 * n;
+        const uint32_t s1 = min(s0 + K, T);
+        uint32_t k = s0;
+        if (s1 - s0 >= 6) {
+            ConstPtr<TabEntry> p = tab + s0;
+            ConstPtr<double> pw = weights + s0;
+            TabEntry a0 = load_record(p), a1 = load_record(p + 1);
+            double w0 = pw[0], w1 = pw[1];
+            // turns k..k+3 use (a0, a1), (b0, b1); the next quad's pair must exist
+            for (uint32_t quads = (s1 - s0 - 2) / 4; quads; --quads) {
+                const TabEntry b0 = load_record(p + 2), b1 = load_record(p + 3);
+                const double v0 = pw[2], v1 = pw[3];
+                turn(a0, w0);
+                turn(a1, w1);
+                a0 = load_record(p + 4);
+                a1 = load_record(p + 5);
+                w0 = pw[4];
+                w1 = pw[5];
+                turn(b0, v0);
+                turn(b1, v1);
+                p = p + 4;
+                pw = pw + 4;
+                k += 4;
             }
-            ck += 4 * n;
-            until = a.every;
         }
-        map_step<MODE, false, OCT>(x, px, y, py, a.mu, m3mu, t);
-        // exact mode: screened radius, confirmed with the reference's
-        // expression for the rare candidates (same decisions as hm_track)
-        const double r2 = MODE == kExact ? screen_radius2(x, px, y, py) : radius2<MODE>(x, px, y, py);
-        bool out = alive && !(r2 < b2s);
-        if (MODE == kExact && __any(out)) out = out && !(radius2<kExact>(x, px, y, py) < a.b2);
-        alive = alive && !out;
-        bsum = fma(w, x * x, bsum);
+        for (; k < s1; ++k) turn(load_record(tab + k), weights[k]);
     }
     if (!in) return;
     if (!alive) x = px = y = py = bsum = __builtin_nan("");

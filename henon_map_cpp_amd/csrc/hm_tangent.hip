@@ -246,7 +246,7 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
         }
         const Jac jc = jac_coeffs<MODE>(x, y, mu3, mu6);
         mat_mul_structured<MODE>(jc, t, m);
-        if (forward_turn_lost<MODE, OCT>(x, px, y, py, xx, yy, a.mu, m3mu, t, a.b2, b2s) && alive) {
+        if (alive && forward_turn_lost<MODE, OCT>(x, px, y, py, xx, yy, a.mu, m3mu, t, a.b2, b2s)) {
             x = px = y = py = xx = yy = __builtin_nan("");
 #pragma unroll
             for (int q = 0; q < 16; q++) m[q] = __builtin_nan("");
