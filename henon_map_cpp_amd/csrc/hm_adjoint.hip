@@ -268,29 +268,42 @@ __global__ void __launch_bounds__(256) k_adjoint_backward(AdjointBwdArgs a)
         for (int ch = (int)n_chunk - 1; ch >= 0; --ch) {
             const uint32_t c0 = s0 + (uint32_t)ch * kChunk;
             const uint32_t len = min((uint32_t)kChunk, s1 - c0);
-            TabEntry R[kChunk];
-#pragma unroll
-            for (int m = 0; m < kChunk; ++m) R[m] = load_record(tab + min(c0 + m, s1 - 1));
             const double *q = sub_at((uint32_t)seg, (uint32_t)ch);
             double S[kChunk][4];
             S[0][0] = q[i]; S[0][1] = q[n + i]; S[0][2] = q[2 * n + i]; S[0][3] = q[3 * n + i];
-#pragma unroll
-            for (int m = 1; m < kChunk; ++m) {
-                copy4(S[m], S[m - 1]);
-                if ((uint32_t)m < len) step4<MODE, OCT>(S[m], mu, m3mu, R[m - 1]);
-            }
-            // the previous segment's next 8-turn block: independent of the
-            // adjoint chain below, so the two interleave
-            if (overlap) rc_block();
-#pragma unroll
-            for (int m = kChunk - 1; m >= 0; --m) {
-                if ((uint32_t)m >= len) continue;
+            // records are loaded where they are used (scalar cache hits the
+            // second time): holding a chunk's eight records would crowd the SGPRs
+            auto adjoint_m = [&](int m, const double (&post)[4]) {
                 const uint32_t k = c0 + m;
                 const ConstPtr<double> d = dtab + 4 * (size_t)k;
                 const double4 dt{d[0], d[1], d[2], d[3]};
-                const double bw = bseed * weights[k];
-                if ((uint32_t)m + 1 == len) adjoint_turn<OCT>(S[m], nxt, R[m], bw, dt, mu3, mu6, acc);
-                else adjoint_turn<OCT>(S[m], S[m + 1 < kChunk ? m + 1 : m], R[m], bw, dt, mu3, mu6, acc);
+                adjoint_turn<OCT>(S[m], post, load_record(tab + k), bseed * weights[k], dt, mu3, mu6, acc);
+            };
+            if (len == kChunk) {  // every chunk but possibly the very last one
+#pragma unroll
+                for (int m = 1; m < kChunk; ++m) {
+                    copy4(S[m], S[m - 1]);
+                    step4<MODE, OCT>(S[m], mu, m3mu, load_record(tab + c0 + m - 1));
+                }
+                // the previous segment's next 8-turn block: independent of the
+                // adjoint chain below, so the two interleave
+                if (overlap) rc_block();
+                adjoint_m(kChunk - 1, nxt);
+#pragma unroll
+                for (int m = kChunk - 2; m >= 0; --m) adjoint_m(m, S[m + 1]);
+            } else {
+#pragma unroll
+                for (int m = 1; m < kChunk; ++m) {
+                    copy4(S[m], S[m - 1]);
+                    if ((uint32_t)m < len) step4<MODE, OCT>(S[m], mu, m3mu, load_record(tab + c0 + m - 1));
+                }
+                if (overlap) rc_block();
+#pragma unroll
+                for (int m = kChunk - 1; m >= 0; --m) {
+                    if ((uint32_t)m >= len) continue;
+                    if ((uint32_t)m + 1 == len) adjoint_m(m, nxt);
+                    else adjoint_m(m, S[m + 1 < kChunk ? m + 1 : m]);
+                }
             }
             copy4(nxt, S[0]);
         }

@@ -152,7 +152,7 @@ struct IndicatorArgs {
 //   Birkhoff windows (sample indices from < m <= to): acc += w[m-1-from] dtheta_m
 //     with dtheta wrapped into [0, 2pi) (dynamic_indicator.cpp:110-117); tune =
 //     1 - acc / 2pi (NaN once a sample of the window is NaN).
-template <int MODE, bool OCT>
+template <int MODE, bool OCT, bool EXTRA>
 __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -189,11 +189,11 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
 #pragma unroll
     for (int w = 0; w < kMaxBirk; w++) bacc[w][0] = bacc[w][1] = 0.0;
     double thx = 0.0, thy = 0.0;
-    if (a.n_birk) {
+    if (EXTRA && a.n_birk) {
         thx = atan2(px, x);
         thy = atan2(py, y);
     }
-    if (in_range && a.hist_len && a.hist_from == 0) {
+    if (EXTRA && in_range && a.hist_len && a.hist_from == 0) {
         const size_t n = a.n;
         a.hist[0 * n + i] = x; a.hist[1 * n + i] = px; a.hist[2 * n + i] = y; a.hist[3 * n + i] = py;
     }
@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
                 fb2 = frob2_fused(m);
             }
         }
-        if (a.n_birk) {
+        if (EXTRA && a.n_birk) {
             const double nx = atan2(px, x), ny = atan2(py, y);
             double dx = nx - thx, dy = ny - thy;
             if (dx < 0.0) dx += 2.0 * M_PI;
@@ -301,12 +301,12 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
                 }
             }
         }
-        if (in_range && a.hist_len && idx >= a.hist_from && idx < a.hist_from + a.hist_len) {
+        if (EXTRA && in_range && a.hist_len && idx >= a.hist_from && idx < a.hist_from + a.hist_len) {
             const size_t n = a.n, base = (size_t)(idx - a.hist_from) * 4 * n;
             a.hist[base + 0 * n + i] = x; a.hist[base + 1 * n + i] = px;
             a.hist[base + 2 * n + i] = y; a.hist[base + 3 * n + i] = py;
         }
-        if (!__any(alive) && !a.hist_len && !a.n_birk) break;
+        if (!__any(alive) && !(EXTRA && (a.hist_len || a.n_birk))) break;
     }
     // the last turn's MEGNO sums
     l1 = l1 + (0.5 * log10(q_prev)) * w_prev;
@@ -329,7 +329,7 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
     if (a.log_scale) a.log_scale[i] = lscale;
 #pragma unroll
     for (int w = 0; w < kMaxBirk; w++) {
-        if (w < (int)a.n_birk) {
+        if (EXTRA && w < (int)a.n_birk) {
             a.birk_out[((size_t)w * 2 + 0) * a.n + i] = 1.0 - bacc[w][0] / (2 * M_PI);
             a.birk_out[((size_t)w * 2 + 1) * a.n + i] = 1.0 - bacc[w][1] / (2 * M_PI);
         }
@@ -619,13 +619,19 @@ extern "C" int hm_track_indicators(double *x, double *px, double *y, double *py,
         a.status = status_word();
         const bool oct = mu != 0.0;
         LaunchTimer timer("k_indicators", s);
+        // Birkhoff windows and the trajectory window are compiled into their own
+        // variants, so the MEGNO/FLI-only kernel carries none of their code
+        const bool extra = a.n_birk || a.hist_len;
+        using K = void (*)(IndicatorArgs);
+        K k = nullptr;
         if (mode == HM_MODE_EXACT) {
-            if (oct) hipLaunchKernelGGL((k_indicators<kExact, true>), dim3(grid_of(n)), dim3(256), 0, s, a);
-            else hipLaunchKernelGGL((k_indicators<kExact, false>), dim3(grid_of(n)), dim3(256), 0, s, a);
+            if (oct) k = extra ? k_indicators<kExact, true, true> : k_indicators<kExact, true, false>;
+            else k = extra ? k_indicators<kExact, false, true> : k_indicators<kExact, false, false>;
         } else {
-            if (oct) hipLaunchKernelGGL((k_indicators<kFast, true>), dim3(grid_of(n)), dim3(256), 0, s, a);
-            else hipLaunchKernelGGL((k_indicators<kFast, false>), dim3(grid_of(n)), dim3(256), 0, s, a);
+            if (oct) k = extra ? k_indicators<kFast, true, true> : k_indicators<kFast, true, false>;
+            else k = extra ? k_indicators<kFast, false, true> : k_indicators<kFast, false, false>;
         }
+        hipLaunchKernelGGL(k, dim3(grid_of(n)), dim3(256), 0, s, a);
         launch_check("k_indicators");
     });
 }
