@@ -211,14 +211,22 @@ __device__ __forceinline__ void megno_chunk(const BatchSeg &S, ConstPtr<JobDesc>
     const ConstPtr<double> inv_idx = as_const(S.inv_idx);
     const uint32_t renorm = J->renorm;
     uint32_t renorm_in = renorm ? renorm - S.seg_start % renorm : 0;
-    // as in k_indicators, the sums of turn k are added at the top of turn k+1
-    // (same operations, so the same values) to overlap the log10 chain
-    double q_prev = 1.0, w_prev = 0.0, inv_prev = 0.0;
+    // as in k_indicators (same operations, so the same values): the MEGNO sums
+    // of turn k are formed at the top of turn k+1, or at once on a
+    // renormalisation turn
+    bool pending = false;
+    double w_prev = 0.0, inv_prev = 0.0;
+    auto megno_update = [&]() {
+        const double fa2 = frob2_fused(m);
+        l1 = l1 + (0.5 * log10(fa2 / fb2)) * w_prev;
+        l2 = l2 + inv_prev * l1;
+        fb2 = fa2;
+        pending = false;
+    };
     double xx = x * x, yy = y * y;  // carried between turns in exact mode
     const double b2s = screen_bound(b2);
     for (uint32_t k = 0; k < turns && live0; ++k) {
-        l1 = l1 + (0.5 * log10(q_prev)) * w_prev;
-        l2 = w_prev != 0.0 ? l2 + inv_prev * l1 : l2;  // nothing pending before the first turn
+        if (pending) megno_update();
         TabEntry t;
         if (uniform) {
             t = load_record(tp + k);
@@ -241,13 +249,12 @@ __device__ __forceinline__ void megno_chunk(const BatchSeg &S, ConstPtr<JobDesc>
             done = k;
         }
         const uint32_t idx = S.seg_start + k + 1;
-        const double fa2 = frob2_fused(m);
-        q_prev = fa2 / fb2;
+        pending = true;
         w_prev = (double)idx;
         inv_prev = inv_idx[idx - 1];
-        fb2 = fa2;
         if (renorm_in && --renorm_in == 0) {
             renorm_in = renorm;
+            megno_update();
             if (alive) {
                 const double fa = sqrt(frob2<MODE>(m));
                 const double inv = 1.0 / fa;
@@ -258,8 +265,7 @@ __device__ __forceinline__ void megno_chunk(const BatchSeg &S, ConstPtr<JobDesc>
         }
         if (!__any(alive)) break;
     }
-    l1 = l1 + (0.5 * log10(q_prev)) * w_prev;
-    l2 = w_prev != 0.0 ? l2 + inv_prev * l1 : l2;
+    if (pending) megno_update();
     if (alive0) {
         gx[j] = x; gpx[j] = px; gy[j] = y; gpy[j] = py;
         gsteps[j] = s + done;

@@ -216,22 +216,27 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
     uint32_t renorm_in = a.renorm_every;
     double xx = x * x, yy = y * y;  // carried between turns in exact mode
     const double b2s = screen_bound(a.b2);
-    // The MEGNO sums of turn k are added at the top of turn k+1: the log10 of the
-    // ratio (a long dependent chain) then overlaps the next turn's matrix work.
-    // q_prev = 1, w_prev = 0 make the first addition a no-op.
-    double q_prev = 1.0, w_prev = 0.0, inv_prev = 0.0;
+    // The MEGNO sums of turn k (norm of M_{k+1}, ratio, log10, L1, L2) are
+    // formed at the top of turn k+1, where they do not wait on and are not
+    // waited on by that turn's map and matrix product; a renormalisation turn
+    // forms them at once, before it rescales M.
+    bool pending = false;
+    double w_prev = 0.0, inv_prev = 0.0;
     uint32_t idx_prev = 0;
-    auto megno_sample = [&](uint32_t idx) {
-        if (idx && idx == next_megno) {
+    auto megno_update = [&]() {
+        const double fa2 = frob2_fused(m);
+        l1 = l1 + (0.5 * log10(fa2 / fb2)) * w_prev;
+        l2 = l2 + inv_prev * l1;
+        fb2 = fa2;
+        if (idx_prev == next_megno) {
             if (in_range && a.megno) a.megno[(size_t)sample_m * a.n + i] = inv_prev * l2;
             ++sample_m;
             next_megno = sample_m < a.n_samples ? a.samples[sample_m] : 0;
         }
+        pending = false;
     };
     for (; k < a.n_turns; ++k) {
-        l1 = l1 + (0.5 * log10(q_prev)) * w_prev;
-        l2 = w_prev != 0.0 ? l2 + inv_prev * l1 : l2;  // nothing pending before the first turn
-        megno_sample(idx_prev);
+        if (pending) megno_update();
         TabEntry t;
         if (uniform) {
             t = load_record(tp + k);
@@ -254,12 +259,10 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
             done = k;
         }
         const uint32_t idx = k + 1;
-        const double fa2 = frob2_fused(m);
-        q_prev = fa2 / fb2;
+        pending = true;
         w_prev = (double)idx;
         inv_prev = inv_idx[k];
         idx_prev = idx;
-        fb2 = fa2;
         if (idx == next_sample) {
             const size_t o = (size_t)sample_q * a.n + i;
             if (in_range && a.fli) {
@@ -276,6 +279,7 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
         }
         if (renorm_in && --renorm_in == 0) {
             renorm_in = a.renorm_every;
+            megno_update();
             if (alive) {
                 const double fa = sqrt(frob2<MODE>(m));
                 const double inv = 1.0 / fa;
@@ -309,9 +313,7 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
         if (!__any(alive) && !(EXTRA && (a.hist_len || a.n_birk))) break;
     }
     // the last turn's MEGNO sums
-    l1 = l1 + (0.5 * log10(q_prev)) * w_prev;
-    l2 = w_prev != 0.0 ? l2 + inv_prev * l1 : l2;
-    megno_sample(idx_prev);
+    if (pending) megno_update();
     if (!in_range) return;
     // samples not reached because every particle of the wavefront was lost
     for (; sample_q < a.n_samples; ++sample_q)
