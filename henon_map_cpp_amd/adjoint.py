@@ -36,8 +36,9 @@ def lines_derivatives(tune, epsilon, coef, freq, start, end) -> np.ndarray:
 
 
 def default_every(n_turns: int) -> int:
-    """Checkpoint interval minimising checkpoint + scratch memory
-    (ceil(T/K) + K/8 states per particle): K ~ sqrt(8 T), a multiple of 8."""
+    """Checkpoint interval K ~ sqrt(8 T), a multiple of 8: ceil(T/K) checkpoint
+    states per particle, and a window of one segment (K/8 states per particle)
+    is the least scratch the backward sweep can run in."""
     k = int(math.sqrt(8.0 * max(n_turns, 1)))
     return max(8, (k + 7) // 8 * 8)
 
@@ -104,9 +105,11 @@ class OrbitAdjoint:
         return st, birk, alive, ck
 
     def backward(self, ck: torch.Tensor, alive: torch.Tensor, seed_state: torch.Tensor,
-                 seed_birk: torch.Tensor | None = None):
+                 seed_birk: torch.Tensor | None = None, scratch_bytes: int | None = None):
         """Returns (grad_x0 [4][n], grad_params [4][n]) with grad_params rows
-        (d/d epsilon, d/d mu, d/d omega_x0, d/d omega_y0) per particle."""
+        (d/d epsilon, d/d mu, d/d omega_x0, d/d omega_y0) per particle.
+        scratch_bytes (default: hm_adjoint_scratch_bytes' size) bounds the
+        sub-checkpoint buffer; a smaller one sweeps in windows of segments."""
         n = ck.shape[2]
         seed_state = self._state(seed_state)
         if seed_state.shape[1] != n:
@@ -116,13 +119,14 @@ class OrbitAdjoint:
             if seed_birk.shape != (n,):
                 raise ValueError("seed_birk must have one entry per particle")
         nb = ctypes.c_size_t(0)
-        C.check(C.lib().hm_adjoint_scratch_bytes(n, self.every, ctypes.byref(nb)))
-        scratch = torch.empty(nb.value // 8, dtype=torch.float64, device=self.device)
+        C.check(C.lib().hm_adjoint_scratch_bytes(n, self.n_turns, self.every, ctypes.byref(nb)))
+        nbytes = nb.value if scratch_bytes is None else int(scratch_bytes)
+        scratch = torch.empty(max(nbytes // 8, 1), dtype=torch.float64, device=self.device)
         gx = torch.empty((4, n), dtype=torch.float64, device=self.device)
         gp = torch.empty((4, n), dtype=torch.float64, device=self.device)
         C.check(C.lib().hm_adjoint_backward(_p(ck), n, _p(self.table), _p(self.dtable), self.table.shape[0],
                                             self.n_turns, self.mu, self.every, _p(self.weights), _p(seed_state),
-                                            _p(seed_birk), _p(alive), _p(scratch), _p(gx), _p(gp), self.mode,
+                                            _p(seed_birk), _p(alive), _p(scratch), nbytes, _p(gx), _p(gp), self.mode,
                                             _stream()))
         return gx, gp
 

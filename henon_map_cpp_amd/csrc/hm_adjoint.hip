@@ -7,18 +7,21 @@
 // Birkhoff-weighted sum B = sum_{k=1..T} w_{k-1} x_k^2 (w = birkhoff_weights(T))
 // and whether the particle survived all T turns.
 //
-// Backward sweep (k_adjoint_backward): per particle, for each K-turn segment in
-// reverse, recompute the segment from its checkpoint writing sub-checkpoints
-// every kChunk turns to scratch, then per chunk (in reverse) recompute its
-// kChunk+1 states into registers and propagate the adjoint lambda <- J^T lambda
-// (+ the Birkhoff seed), accumulating the parameter derivatives
+// Backward sweep (k_adjoint_segments + k_adjoint_sweep): the K-turn segments
+// are recomputed from their checkpoints, writing sub-checkpoints every kChunk
+// turns to scratch; then per particle, per chunk (in reverse), the chunk's
+// kChunk+1 states are recomputed into registers and the adjoint
+// lambda <- J^T lambda (+ the Birkhoff seed) propagated, accumulating the
+// parameter derivatives
 //   d/dmu     : lambda_p~ . (x^3 - 3 x y^2, y^3 - 3 x^2 y)   (octupole kick)
 //   d/domega_k: lambda_x' p' - lambda_p' x'                   (rotation, per plane)
 // chained through d omega_k / d(omega_0, epsilon) (a per-turn table).
 // Recomputation never inverts the map, which is unstable on chaotic orbits.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 
 #include "hm_device.h"
 #include "hm_internal.h"
@@ -26,6 +29,8 @@
 namespace hm {
 
 constexpr int kChunk = 8;
+
+__device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
 struct AdjointFwdArgs {
     const double *x0, *px0, *y0, *py0;  // initial points [n]
@@ -40,14 +45,25 @@ struct AdjointFwdArgs {
     uint8_t *alive;           // [n]
 };
 
+// LDS ring of the aligned forward sweep, one per wavefront: three 8-turn
+// chunks, per turn sx cx sy cy | weight, pad.
+constexpr int kFwdTurn = 6;
+constexpr int kFwdChunk = 8 * kFwdTurn;
+constexpr int kFwdWave = 3 * kFwdChunk;
+constexpr int kFwdBlock = 256;
+
 // Every lane runs every turn (a lost particle's lane carries on with values
 // that are discarded), so the table record and the weight of a turn are
-// wave-uniform scalar loads, requested ahead of use. Per K-turn segment: the
-// checkpoint, then the turns four at a time (the tracking loop's shape: two
-// alternating pairs of prefetched records, the loss test on a rare path), with
-// the Birkhoff sum taking the carried x*x in exact mode.
-template <int MODE, bool OCT>
-__global__ void __launch_bounds__(256) k_adjoint_forward(AdjointFwdArgs a)
+// wave-uniform. A lost particle is parked at the origin, a fixed point of the
+// map, so that its lane never again takes the loss test's rare path.
+// With every % 8 == 0 (ALIGNED) the turns run in 8-turn chunks whose table
+// records and weights the wavefront stages into its LDS ring two chunks ahead
+// (one vector load and one LDS store per lane and chunk); a turn reads its
+// entries from LDS one turn ahead, so neither scalar-cache misses nor LDS
+// latency sit on the dependency chain. Otherwise the turns run four at a time
+// with scalar loads of two alternating prefetched pairs of records.
+template <int MODE, bool OCT, bool ALIGNED>
+__global__ void __launch_bounds__(kFwdBlock) k_adjoint_forward(AdjointFwdArgs a)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = i < a.n;
@@ -66,14 +82,12 @@ __global__ void __launch_bounds__(256) k_adjoint_forward(AdjointFwdArgs a)
         }
         return;
     }
-    if (!alive) x = px = y = py = 0.0;  // NaN inputs and lanes past n: parked (see below)
+    if (!alive) x = px = y = py = 0.0;  // NaN inputs and lanes past n: parked
     const ConstPtr<TabEntry> tab = as_const(a.tab);
     const ConstPtr<double> weights = as_const(a.weights);
     const double mu = a.mu, m3mu = -3.0 * mu, b2 = a.b2, b2s = screen_bound(b2);
     double xx = x * x, yy = y * y;
     double bsum = 0.0;
-    // a lost particle is parked at the origin, a fixed point of the map, so
-    // that its lane never again takes the loss test's rare path
     auto turn = [&](const TabEntry &t, double w) {
         if (forward_turn_lost<MODE, OCT>(x, px, y, py, xx, yy, mu, m3mu, t, b2, b2s)) {
             alive = false;
@@ -81,38 +95,105 @@ __global__ void __launch_bounds__(256) k_adjoint_forward(AdjointFwdArgs a)
         }
         bsum = fma(w, MODE == kExact ? xx : x * x, bsum);
     };
-    const uint32_t T = a.n_turns, K = a.every;
-    double *ck = a.ckpt;
-    for (uint32_t s0 = 0; s0 < T; s0 += K) {
+    auto checkpoint = [&](double *ck) {
         if (in) {
             ck[i] = x; ck[n + i] = px; ck[2 * n + i] = y; ck[3 * n + i] = py;
         }
-        ck += 4 * n;
-        const uint32_t s1 = min(s0 + K, T);
-        uint32_t k = s0;
-        if (s1 - s0 >= 6) {
-            ConstPtr<TabEntry> p = tab + s0;
-            ConstPtr<double> pw = weights + s0;
-            TabEntry a0 = load_record(p), a1 = load_record(p + 1);
-            double w0 = pw[0], w1 = pw[1];
-            // turns k..k+3 use (a0, a1), (b0, b1); the next quad's pair must exist
-            for (uint32_t quads = (s1 - s0 - 2) / 4; quads; --quads) {
-                const TabEntry b0 = load_record(p + 2), b1 = load_record(p + 3);
-                const double v0 = pw[2], v1 = pw[3];
-                turn(a0, w0);
-                turn(a1, w1);
-                a0 = load_record(p + 4);
-                a1 = load_record(p + 5);
-                w0 = pw[4];
-                w1 = pw[5];
-                turn(b0, v0);
-                turn(b1, v1);
-                p = p + 4;
-                pw = pw + 4;
-                k += 4;
+    };
+    const uint32_t T = a.n_turns, K = a.every;
+    if constexpr (ALIGNED) {
+        __shared__ __align__(16) double ring_all[(kFwdBlock / 64) * kFwdWave];
+        double *ring = ring_all + (threadIdx.x >> 6) * kFwdWave;
+        const uint32_t lane = threadIdx.x & 63;
+        const int G = (int)((T + 7) / 8);
+        const double *tabd = reinterpret_cast<const double *>(a.tab);
+        // chunk g: lane l < 32 carries table double l (turn l/4), the others the
+        // weight of turn l % 8 (identical writes); turns past T read turn T-1
+        auto stage_load = [&](int g) -> double {
+            const uint32_t t0 = (uint32_t)g * 8;
+            return lane < 32 ? tabd[4 * (size_t)umin(t0 + (lane >> 2), T - 1) + (lane & 3)]
+                             : a.weights[umin(t0 + (lane & 7), T - 1)];
+        };
+        auto stage_store = [&](int g, double e) {
+            double *slot = ring + ((uint32_t)g % 3) * kFwdChunk;
+            slot[lane < 32 ? (lane >> 2) * kFwdTurn + (lane & 3) : (lane & 7) * kFwdTurn + 4] = e;
+        };
+        auto ring_at = [&](int g, int m, TabEntry &t, double &w) {
+            const double *q = ring + ((uint32_t)g % 3) * kFwdChunk + m * kFwdTurn;
+            t = TabEntry{q[0], q[1], q[2], q[3]};
+            w = q[4];
+        };
+        auto clampg = [&](int g) { return g < G ? g : G - 1; };
+        stage_store(0, stage_load(0));
+        stage_store(clampg(1), stage_load(clampg(1)));
+        double e = stage_load(clampg(2));
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t K8 = K / 8;
+        double *ck = a.ckpt;
+        uint32_t to_ck = 0;  // chunks until the next checkpoint
+        TabEntry r;
+        double w;
+        ring_at(0, 0, r, w);
+        for (int g = 0; g < G; ++g) {
+            if (to_ck == 0) {
+                checkpoint(ck);
+                ck += 4 * n;
+                to_ck = K8;
+            }
+            --to_ck;
+            stage_store(clampg(g + 2), e);
+            e = stage_load(clampg(g + 3));
+            __builtin_amdgcn_wave_barrier();  // the stores precede later chunks' reads
+            const uint32_t len = umin(8, T - (uint32_t)g * 8);
+            if (len == 8) {
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    TabEntry rn;
+                    double wn;
+                    if (m + 1 < 8) ring_at(g, m + 1, rn, wn);
+                    else ring_at(g + 1, 0, rn, wn);  // the next chunk's (its slot is staged)
+                    turn(r, w);
+                    r = rn;
+                    w = wn;
+                }
+            } else {
+                for (uint32_t m = 0; m < len; ++m) {
+                    ring_at(g, (int)m, r, w);
+                    turn(r, w);
+                }
             }
         }
-        for (; k < s1; ++k) turn(load_record(tab + k), weights[k]);
+    } else {
+        double *ck = a.ckpt;
+        for (uint32_t s0 = 0; s0 < T; s0 += K) {
+            checkpoint(ck);
+            ck += 4 * n;
+            const uint32_t s1 = min(s0 + K, T);
+            uint32_t k = s0;
+            if (s1 - s0 >= 6) {
+                ConstPtr<TabEntry> p = tab + s0;
+                ConstPtr<double> pw = weights + s0;
+                TabEntry a0 = load_record(p), a1 = load_record(p + 1);
+                double w0 = pw[0], w1 = pw[1];
+                // turns k..k+3 use (a0, a1), (b0, b1); the next quad's pair must exist
+                for (uint32_t quads = (s1 - s0 - 2) / 4; quads; --quads) {
+                    const TabEntry b0 = load_record(p + 2), b1 = load_record(p + 3);
+                    const double v0 = pw[2], v1 = pw[3];
+                    turn(a0, w0);
+                    turn(a1, w1);
+                    a0 = load_record(p + 4);
+                    a1 = load_record(p + 5);
+                    w0 = pw[4];
+                    w1 = pw[5];
+                    turn(b0, v0);
+                    turn(b1, v1);
+                    p = p + 4;
+                    pw = pw + 4;
+                    k += 4;
+                }
+            }
+            for (; k < s1; ++k) turn(load_record(tab + k), weights[k]);
+        }
     }
     if (!in) return;
     if (!alive) x = px = y = py = bsum = __builtin_nan("");
@@ -120,22 +201,6 @@ __global__ void __launch_bounds__(256) k_adjoint_forward(AdjointFwdArgs a)
     a.birk[i] = bsum;
     a.alive[i] = alive ? 1 : 0;
 }
-
-struct AdjointBwdArgs {
-    const double *ckpt;
-    uint32_t n;
-    const TabEntry *tab;
-    const double *dtab;       // [T][4]: d wx/d wx0, d wx/d eps, d wy/d wy0, d wy/d eps
-    uint32_t n_turns, every;
-    double mu;
-    const double *weights;    // [T]
-    const double *seed_s;     // [4][n]: dL/d(final state)
-    const double *seed_b;     // [n]:    dL/dB, or NULL
-    const uint8_t *alive;     // [n]
-    double *sub;              // scratch [2][ceil(K/kChunk) + 1][4][n]
-    double *grad_x0;          // [4][n]
-    double *grad_p;           // [4][n]: d/d eps, d/d mu, d/d omega_x0, d/d omega_y0
-};
 
 // One turn of the adjoint, turn k: pre-step point `pre`, post-step point
 // `post`, post-step adjoint (lx, lpx, ly, lpy) -> pre-step adjoint, with the
@@ -182,138 +247,371 @@ __device__ __forceinline__ void copy4(double (&d)[4], const double (&s)[4])
     d[0] = s[0]; d[1] = s[1]; d[2] = s[2]; d[3] = s[3];
 }
 
-// Backward sweep. The K-segments are processed in reverse. A segment's states
-// every kChunk = 8 turns (sub-checkpoints, in scratch) come from recomputing it
-// forward from its checkpoint; per 8-turn chunk (in reverse) the chunk's table
-// records are loaded, its states S1..S7 recomputed from S0 into registers (the
-// post-step point of its last turn, `nxt`, is known), and the adjoint run back
-// through its turns; S0 is the next chunk's `nxt`.
-// The forward recomputation of segment s-1 is interleaved with the reverse
-// sweep of segment s (one 8-turn block per chunk, into the other half of the
-// scratch): the two dependency chains are independent, so one wavefront per SIMD
-// (small particle counts) still keeps the FP64 pipeline busy. The state after
-// segment s-1 is the checkpoint of segment s.
+// What the adjoint of one turn reads from the tables: the rotation record, the
+// angle derivatives and the Birkhoff weight.
+struct AdjRec {
+    TabEntry t;
+    double4 dt;
+    double w;
+};
+
+// The backward sweep runs over windows of consecutive K-turn segments, the
+// last window first. Per window two kernels run:
+//  - k_adjoint_segments recomputes every segment of the window from its
+//    checkpoint, one thread per (particle, segment), and writes the state at the
+//    start of each 8-turn chunk after the first (sub-checkpoints). The segments
+//    are independent, so this part runs at full occupancy even for few particles.
+//  - k_adjoint_sweep, one thread per particle, walks the window's chunks in
+//    reverse: the chunk's nine states S0..S8 (S8 = the next chunk's S0) come from
+//    its sub-checkpoint (the segment's checkpoint for a first chunk), and the
+//    adjoint runs back through its turns. The states of the chunk before it are
+//    recomputed in the same straight-line block: the two dependency chains are
+//    independent and interleave, which matters at one wavefront per SIMD, where
+//    nothing else hides FP64 latency.
+// The adjoint and the parameter sums carry from window to window in the output
+// arrays.
+struct AdjointSegArgs {
+    const double *ckpt;       // [n_seg][4][n]
+    uint32_t n;
+    const TabEntry *tab;
+    uint32_t n_turns, every;
+    double mu;
+    const uint8_t *alive;     // [n]
+    double *sub;              // [window segments][seg_chunks - 1][4][n]
+    uint32_t seg_first, seg_chunks;
+};
+
 template <int MODE, bool OCT>
-__global__ void __launch_bounds__(256) k_adjoint_backward(AdjointBwdArgs a)
+__global__ void __launch_bounds__(256) k_adjoint_segments(AdjointSegArgs a)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
+    if (i >= a.n || !a.alive[i]) return;
     const size_t n = a.n;
-    if (!a.alive[i]) {
-        for (int c = 0; c < 4; c++) { a.grad_x0[c * n + i] = 0.0; a.grad_p[c * n + i] = 0.0; }
-        return;
+    const uint32_t w = blockIdx.y, seg = a.seg_first + w;
+    const uint32_t s0 = seg * a.every, s1 = umin(s0 + a.every, a.n_turns);
+    const uint32_t chunks = (s1 - s0 + kChunk - 1) / kChunk;
+    const double mu = a.mu, m3mu = -3.0 * mu;
+    const ConstPtr<TabEntry> tab = as_const(a.tab);
+    const double *ck = a.ckpt + (size_t)seg * 4 * n;
+    double v[4] = {ck[i], ck[n + i], ck[2 * n + i], ck[3 * n + i]};
+    double *out = a.sub + (size_t)w * (a.seg_chunks - 1) * 4 * n;
+    for (uint32_t j = 1; j < chunks; ++j) {
+        const ConstPtr<TabEntry> p = tab + (s0 + (j - 1) * kChunk);
+#pragma unroll
+        for (int m = 0; m < kChunk; ++m) step4<MODE, OCT>(v, mu, m3mu, load_record(p + m));
+        out[i] = v[0]; out[n + i] = v[1]; out[2 * n + i] = v[2]; out[3 * n + i] = v[3];
+        out += 4 * n;
     }
-    AdjAcc acc{a.seed_s[i], a.seed_s[n + i], a.seed_s[2 * n + i], a.seed_s[3 * n + i], 0.0, 0.0, 0.0, 0.0};
-    const double bseed = 2.0 * (a.seed_b ? a.seed_b[i] : 0.0);
+}
+
+struct AdjointSweepArgs {
+    const double *ckpt;       // [n_seg][4][n]
+    uint32_t n;
+    const TabEntry *tab;
+    const double *dtab;       // [T][4]: d wx/d wx0, d wx/d eps, d wy/d wy0, d wy/d eps
+    uint32_t n_turns, every;
+    double mu;
+    const double *weights;    // [T]
+    const double *seed_s;     // [4][n]: dL/d(final state)
+    const double *seed_b;     // [n]:    dL/dB, or NULL
+    const uint8_t *alive;     // [n]
+    const double *sub;        // k_adjoint_segments' output for this window
+    uint32_t seg_first, seg_end, seg_chunks;
+    int last_window;          // the window holding turn T-1: start from the seeds
+    double *grad_x0;          // [4][n]: the adjoint, carried between windows
+    double *grad_p;           // [4][n]: d/d eps, d/d mu, d/d omega_x0, d/d omega_y0
+};
+
+// A chunk: turns c0 .. c0+len-1 (len <= kChunk) of segment seg, its j-th.
+struct Chunk {
+    uint32_t seg, j, c0, len;
+};
+
+// LDS ring of the aligned sweep, one per wavefront: the table entries of three
+// 8-turn chunks, per turn sx cx sy cy | the four angle derivatives | weight, pad.
+constexpr int kRingTurn = 10;                                // doubles per turn
+constexpr int kRingSlots = 3;                                // chunks
+constexpr int kRingChunk = kChunk * kRingTurn;               // doubles per chunk
+constexpr int kRingWave = kRingSlots * kRingChunk;           // doubles per wavefront
+constexpr int kSweepBlock = 256;
+
+template <int MODE, bool OCT, bool ALIGNED>
+__global__ void __launch_bounds__(kSweepBlock) k_adjoint_sweep(AdjointSweepArgs a)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = a.n;
+    // The aligned sweep stages table entries cooperatively per wavefront, so
+    // every lane of a wavefront with live lanes runs to the end (lanes past n
+    // read particle n-1; lost particles compute on their parked states); only
+    // live lanes store.
+    const uint32_t ic = i < a.n ? i : a.n - 1;
+    const bool live = i < a.n && a.alive[ic];
+    if (!live && i < a.n && a.last_window)
+        for (int c = 0; c < 4; c++) { a.grad_x0[c * n + i] = 0.0; a.grad_p[c * n + i] = 0.0; }
+    if (ALIGNED ? !__any(live) : !live) return;
+    AdjAcc acc;
+    if (a.last_window) {
+        acc = AdjAcc{a.seed_s[ic], a.seed_s[n + ic], a.seed_s[2 * n + ic], a.seed_s[3 * n + ic], 0.0, 0.0, 0.0, 0.0};
+    } else {
+        acc = AdjAcc{a.grad_x0[ic], a.grad_x0[n + ic], a.grad_x0[2 * n + ic], a.grad_x0[3 * n + ic],
+                     a.grad_p[ic],  a.grad_p[n + ic],  a.grad_p[2 * n + ic],  a.grad_p[3 * n + ic]};
+    }
+    const double bseed = 2.0 * (a.seed_b ? a.seed_b[ic] : 0.0);
     const double mu = a.mu, m3mu = -3.0 * mu, mu3 = 3.0 * mu, mu6 = 6.0 * mu;
     const uint32_t T = a.n_turns, K = a.every;
     const uint32_t n_seg = (T + K - 1) / K;
-    const uint32_t seg_chunks = (K + kChunk - 1) / kChunk;  // sub-checkpoints of a full segment
     // tables and weights are read-only here and indexed by the (wave-uniform)
     // turn: scalar loads
     const ConstPtr<TabEntry> tab = as_const(a.tab);
     const ConstPtr<double> dtab = as_const(a.dtab), weights = as_const(a.weights);
-    auto sub_at = [&](uint32_t seg, uint32_t c) -> double * {
-        return a.sub + ((size_t)(seg & 1u) * (seg_chunks + 1) + c) * 4 * n;
+
+    auto load4 = [&](const double *p, double (&v)[4]) {
+        v[0] = p[ic]; v[1] = p[n + ic]; v[2] = p[2 * n + ic]; v[3] = p[3 * n + ic];
     };
-    auto load_ckpt = [&](uint32_t seg, double (&v)[4]) {
-        const size_t o = (size_t)seg * 4 * n;
-        v[0] = a.ckpt[o + i]; v[1] = a.ckpt[o + n + i]; v[2] = a.ckpt[o + 2 * n + i]; v[3] = a.ckpt[o + 3 * n + i];
+    auto adj_rec = [&](uint32_t k) -> AdjRec {
+        const ConstPtr<double> d = dtab + 4 * (size_t)k;
+        return AdjRec{load_record(tab + k), double4{d[0], d[1], d[2], d[3]}, weights[k]};
     };
-    auto store4 = [&](double *p, const double (&v)[4]) {
-        p[i] = v[0]; p[n + i] = v[1]; p[2 * n + i] = v[2]; p[3 * n + i] = v[3];
+    auto adjoint_at = [&](uint32_t k, const double (&pre)[4], const double (&post)[4]) {
+        const AdjRec r = adj_rec(k);
+        adjoint_turn<OCT>(pre, post, r.t, bseed * r.w, r.dt, mu3, mu6, acc);
     };
-    // forward recomputation of one segment, one 8-turn block per call: writes the
-    // block's sub-checkpoint, then advances unless it is the segment's last block
-    struct Recompute {
-        double v[4];
-        uint32_t seg, done, chunks, s0;
-    } rc{};
-    auto rc_start = [&](uint32_t seg) {
-        rc.seg = seg;
-        rc.done = 0;
-        rc.s0 = seg * K;
-        rc.chunks = (min(rc.s0 + K, T) - rc.s0 + kChunk - 1) / kChunk;
-        load_ckpt(seg, rc.v);
-    };
-    auto rc_block = [&]() {
-        if (rc.done >= rc.chunks) return;
-        store4(sub_at(rc.seg, rc.done), rc.v);
-        if (rc.done + 1 < rc.chunks) {
-            const uint32_t k0 = rc.s0 + rc.done * kChunk;
+    // states S[0..len-1] of the chunk starting at turn c0 from S[0]
+    auto fill = [&](double (&S)[kChunk][4], uint32_t c0, uint32_t len) {
 #pragma unroll
-            for (int m = 0; m < kChunk; ++m) step4<MODE, OCT>(rc.v, mu, m3mu, load_record(tab + k0 + m));
+        for (int m = 1; m < kChunk; ++m) {
+            copy4(S[m], S[m - 1]);
+            if ((uint32_t)m < len) step4<MODE, OCT>(S[m], mu, m3mu, load_record(tab + c0 + m - 1));
         }
-        ++rc.done;
+    };
+    // adjoint through turns c0+len-1 .. c0 given their states and the state
+    // after the last one
+    auto sweep = [&](const double (&S)[kChunk][4], uint32_t c0, uint32_t len, const double (&after)[4]) {
+#pragma unroll
+        for (int m = kChunk - 1; m >= 0; --m) {
+            if ((uint32_t)m >= len) continue;
+            if ((uint32_t)m + 1 == len) adjoint_at(c0 + m, S[m], after);
+            else adjoint_at(c0 + m, S[m], S[m + 1 < kChunk ? m + 1 : m]);
+        }
     };
 
-    // prologue: sub-checkpoints of the last segment
-    rc_start(n_seg - 1);
-    while (rc.done < rc.chunks) rc_block();
-    // the state after the last turn: from the last sub-checkpoint (< 8 turns)
-    double nxt[4];
-    {
-        const uint32_t q = rc.chunks - 1;
-        const double *p = sub_at(n_seg - 1, q);
-        double v[4] = {p[i], p[n + i], p[2 * n + i], p[3 * n + i]};
-        for (uint32_t k = (n_seg - 1) * K + q * kChunk; k < T; ++k)
-            step4<MODE, OCT>(v, mu, m3mu, load_record(tab + k));
-        copy4(nxt, v);
-    }
-    for (int seg = (int)n_seg - 1; seg >= 0; --seg) {
-        const uint32_t s0 = (uint32_t)seg * K, s1 = min(s0 + K, T);
-        const bool overlap = seg > 0;
-        if (overlap) rc_start((uint32_t)seg - 1);
-        const uint32_t n_chunk = (s1 - s0 + kChunk - 1) / kChunk;
-        for (int ch = (int)n_chunk - 1; ch >= 0; --ch) {
-            const uint32_t c0 = s0 + (uint32_t)ch * kChunk;
-            const uint32_t len = min((uint32_t)kChunk, s1 - c0);
-            const double *q = sub_at((uint32_t)seg, (uint32_t)ch);
-            double S[kChunk][4];
-            S[0][0] = q[i]; S[0][1] = q[n + i]; S[0][2] = q[2 * n + i]; S[0][3] = q[3 * n + i];
-            // records are loaded where they are used (scalar cache hits the
-            // second time): holding a chunk's eight records would crowd the SGPRs
-            auto adjoint_m = [&](int m, const double (&post)[4]) {
-                const uint32_t k = c0 + m;
-                const ConstPtr<double> d = dtab + 4 * (size_t)k;
-                const double4 dt{d[0], d[1], d[2], d[3]};
-                adjoint_turn<OCT>(S[m], post, load_record(tab + k), bseed * weights[k], dt, mu3, mu6, acc);
+    double SA[kChunk][4], SB[kChunk][4], nxt[4], P[4];
+
+    if constexpr (ALIGNED) {
+        // every % 8 == 0: chunk g is turns 8g .. min(8g+8, T)-1; only the very
+        // last chunk can be short
+        __shared__ __align__(16) double ring_all[(kSweepBlock / 64) * kRingWave];
+        const uint32_t K8 = K / kChunk;
+        const uint32_t G = (T + kChunk - 1) / kChunk;
+        const int g_lo = (int)(a.seg_first * K8);
+        const int gt = (int)umin(a.seg_end * K8, G) - 1;
+        const uint32_t lane = threadIdx.x & 63;
+        double *ring = ring_all + (threadIdx.x >> 6) * kRingWave;
+        auto start_g = [&](int g) -> const double * {
+            const uint32_t seg = (uint32_t)g / K8, j = (uint32_t)g - seg * K8;
+            if (j == 0) return a.ckpt + (size_t)seg * 4 * n;
+            return a.sub + ((size_t)(seg - a.seg_first) * (K8 - 1) + j - 1) * 4 * n;
+        };
+        auto clampg = [&](int g) { return g < g_lo ? g_lo : g; };
+        auto slot_of = [&](int g) { return (uint32_t)g % kRingSlots; };
+        // staging of chunk g (a full chunk): lane l < 32 carries table double l
+        // (turn l/4), lane l >= 32 angle-derivative double l-32, every lane also
+        // the weight of turn l % 8 (eight lanes per value, identical writes)
+        const double *tabd = reinterpret_cast<const double *>(a.tab);
+        auto stage_load = [&](int g, double &e0, double &e1) {
+            const size_t b = (size_t)g * kChunk;
+            const double *src = lane < 32 ? tabd + 4 * b + lane : a.dtab + 4 * b + (lane - 32);
+            e0 = *src;
+            e1 = a.weights[b + (lane & 7)];
+        };
+        auto stage_store = [&](int g, double e0, double e1) {
+            double *slot = ring + slot_of(g) * kRingChunk;
+            slot[((lane & 31) >> 2) * kRingTurn + (lane & 3) + (lane < 32 ? 0 : 4)] = e0;
+            slot[(lane & 7) * kRingTurn + 8] = e1;
+        };
+        auto ring_rec = [&](int g, int m) -> TabEntry {
+            const double *p = ring + slot_of(g) * kRingChunk + m * kRingTurn;
+            return TabEntry{p[0], p[1], p[2], p[3]};
+        };
+        auto ring_adj = [&](int g, int m) -> AdjRec {
+            const double *p = ring + slot_of(g) * kRingChunk + m * kRingTurn;
+            return AdjRec{TabEntry{p[0], p[1], p[2], p[3]}, double4{p[4], p[5], p[6], p[7]}, p[8]};
+        };
+
+        // the top chunk: states from its start, the state after it
+        const uint32_t ct = (uint32_t)gt * kChunk, lent = umin(kChunk, T - ct);
+        load4(start_g(gt), SA[0]);
+        fill(SA, ct, lent);
+        if (a.seg_end < n_seg) {
+            load4(a.ckpt + (size_t)a.seg_end * 4 * n, nxt);
+        } else {
+            double v[4];
+#pragma unroll
+            for (int m = 0; m < kChunk; ++m)
+                if ((uint32_t)m + 1 == lent) copy4(v, SA[m]);
+            step4<MODE, OCT>(v, mu, m3mu, load_record(tab + T - 1));
+            copy4(nxt, v);
+        }
+        // the ring (used below gt only, full chunks): chunks gt-1 and gt-2 now,
+        // chunk gt-3 requested
+        double e0 = 0.0, e1 = 0.0;
+        if (gt > g_lo) {
+            stage_load(gt - 1, e0, e1);
+            stage_store(gt - 1, e0, e1);
+            stage_load(clampg(gt - 2), e0, e1);
+            stage_store(clampg(gt - 2), e0, e1);
+            stage_load(clampg(gt - 3), e0, e1);
+            __builtin_amdgcn_wave_barrier();
+        }
+        // top chunk: adjoint with scalar table loads, states of gt-1 into SB
+        if (gt > g_lo) {
+            load4(start_g(gt - 1), SB[0]);
+            fill(SB, ct - kChunk, kChunk);
+        }
+        sweep(SA, ct, lent, nxt);
+        if (gt > g_lo) {
+            copy4(nxt, SA[0]);
+            load4(start_g(clampg(gt - 2)), P);
+            // full chunk g: adjoint from the ring slot of g, states of chunk g-1
+            // into R from the slot of g-1, source-interleaved; the ring takes
+            // chunk g-2 (requested one chunk ago) and chunk g-3 is requested
+            auto fast = [&](int g, const double (&S)[kChunk][4], double (&R)[kChunk][4]) __attribute__((always_inline)) {
+                stage_store(clampg(g - 2), e0, e1);
+                stage_load(clampg(g - 3), e0, e1);
+                __builtin_amdgcn_wave_barrier();  // the stores precede later bodies' reads
+                double Q[4];
+                load4(start_g(clampg(g - 2)), Q);
+                copy4(R[0], P);
+#pragma unroll
+                for (int q = 0; q < kChunk; ++q) {
+                    const int m = kChunk - 1 - q;
+                    const AdjRec r = ring_adj(g, m);
+                    if (m == kChunk - 1) adjoint_turn<OCT>(S[m], nxt, r.t, bseed * r.w, r.dt, mu3, mu6, acc);
+                    else adjoint_turn<OCT>(S[m], S[m + 1 < kChunk ? m + 1 : m], r.t, bseed * r.w, r.dt, mu3, mu6, acc);
+                    if (q + 1 < kChunk) {
+                        copy4(R[q + 1 < kChunk ? q + 1 : q], R[q]);
+                        step4<MODE, OCT>(R[q + 1 < kChunk ? q + 1 : q], mu, m3mu, ring_rec(g - 1, q));
+                    }
+                }
+                copy4(nxt, S[0]);
+                copy4(P, Q);
             };
-            if (len == kChunk) {  // every chunk but possibly the very last one
-#pragma unroll
-                for (int m = 1; m < kChunk; ++m) {
-                    copy4(S[m], S[m - 1]);
-                    step4<MODE, OCT>(S[m], mu, m3mu, load_record(tab + c0 + m - 1));
-                }
-                // the previous segment's next 8-turn block: independent of the
-                // adjoint chain below, so the two interleave
-                if (overlap) rc_block();
-                adjoint_m(kChunk - 1, nxt);
-#pragma unroll
-                for (int m = kChunk - 2; m >= 0; --m) adjoint_m(m, S[m + 1]);
-            } else {
-#pragma unroll
-                for (int m = 1; m < kChunk; ++m) {
-                    copy4(S[m], S[m - 1]);
-                    if ((uint32_t)m < len) step4<MODE, OCT>(S[m], mu, m3mu, load_record(tab + c0 + m - 1));
-                }
-                if (overlap) rc_block();
+            // the window's first chunk: adjoint only
+            auto last = [&](int g, const double (&S)[kChunk][4]) __attribute__((always_inline)) {
 #pragma unroll
                 for (int m = kChunk - 1; m >= 0; --m) {
-                    if ((uint32_t)m >= len) continue;
-                    if ((uint32_t)m + 1 == len) adjoint_m(m, nxt);
-                    else adjoint_m(m, S[m + 1 < kChunk ? m + 1 : m]);
+                    const AdjRec r = ring_adj(g, m);
+                    if (m == kChunk - 1) adjoint_turn<OCT>(S[m], nxt, r.t, bseed * r.w, r.dt, mu3, mu6, acc);
+                    else adjoint_turn<OCT>(S[m], S[m + 1 < kChunk ? m + 1 : m], r.t, bseed * r.w, r.dt, mu3, mu6, acc);
                 }
+            };
+            int g = gt - 1;  // its states are in SB
+            for (;;) {
+                if (g == g_lo) { last(g, SB); break; }
+                fast(g, SB, SA);
+                --g;
+                if (g == g_lo) { last(g, SA); break; }
+                fast(g, SA, SB);
+                --g;
             }
-            copy4(nxt, S[0]);
         }
-        if (overlap) {
-            while (rc.done < rc.chunks) rc_block();  // a short last segment leaves blocks over
-            load_ckpt((uint32_t)seg, nxt);           // the state after segment seg-1
+    } else {
+        auto seg_chunks = [&](uint32_t seg) -> uint32_t {
+            const uint32_t s0 = seg * K;
+            return (umin(s0 + K, T) - s0 + kChunk - 1) / kChunk;
+        };
+        auto chunk = [&](uint32_t seg, uint32_t j) -> Chunk {
+            const uint32_t s1 = umin(seg * K + K, T), c0 = seg * K + j * kChunk;
+            return Chunk{seg, j, c0, umin(kChunk, s1 - c0)};
+        };
+        // the chunk before c (in turn order) inside this window
+        auto pred = [&](const Chunk &c, Chunk &p) -> bool {
+            if (c.j > 0) { p = chunk(c.seg, c.j - 1); return true; }
+            if (c.seg > a.seg_first) { p = chunk(c.seg - 1, seg_chunks(c.seg - 1) - 1); return true; }
+            return false;
+        };
+        auto start_of = [&](const Chunk &c) -> const double * {
+            if (c.j == 0) return a.ckpt + (size_t)c.seg * 4 * n;
+            return a.sub + ((size_t)(c.seg - a.seg_first) * (a.seg_chunks - 1) + c.j - 1) * 4 * n;
+        };
+        Chunk cur = chunk(a.seg_end - 1, seg_chunks(a.seg_end - 1) - 1), prv{};
+        load4(start_of(cur), SA[0]);
+        fill(SA, cur.c0, cur.len);
+        if (a.seg_end < n_seg) {
+            load4(a.ckpt + (size_t)a.seg_end * 4 * n, nxt);  // the state after the window
+        } else {
+            double v[4];
+#pragma unroll
+            for (int m = 0; m < kChunk; ++m)
+                if ((uint32_t)m + 1 == cur.len) copy4(v, SA[m]);
+            step4<MODE, OCT>(v, mu, m3mu, load_record(tab + T - 1));
+            copy4(nxt, v);
+        }
+        bool has = pred(cur, prv);
+        load4(has ? start_of(prv) : a.ckpt, P);
+        // One chunk: S holds cur's states, R receives prv's (chunks of any
+        // length, scalar table loads). Returns whether a chunk is left.
+        auto body = [&](double (&S)[kChunk][4], double (&R)[kChunk][4]) __attribute__((always_inline)) -> bool {
+            Chunk pp{};
+            const bool has2 = has && pred(prv, pp);
+            double Q[4];
+            load4(has2 ? start_of(pp) : a.ckpt, Q);  // consumed by the next chunk
+            if (has) {
+                copy4(R[0], P);
+                fill(R, prv.c0, prv.len);
+            }
+            sweep(S, cur.c0, cur.len, nxt);
+            if (!has) return false;
+            copy4(nxt, S[0]);
+            cur = prv;
+            prv = pp;
+            has = has2;
+            copy4(P, Q);
+            return true;
+        };
+        while (body(SA, SB) && body(SB, SA)) {
         }
     }
+    if (!live) return;
     a.grad_x0[i] = acc.lx; a.grad_x0[n + i] = acc.lpx; a.grad_x0[2 * n + i] = acc.ly; a.grad_x0[3 * n + i] = acc.lpy;
     a.grad_p[i] = acc.g_eps; a.grad_p[n + i] = acc.g_mu; a.grad_p[2 * n + i] = acc.g_wx; a.grad_p[3 * n + i] = acc.g_wy;
+}
+
+template <int MODE, bool OCT>
+static void launch_forward(bool aligned, uint32_t grid, hipStream_t s, const AdjointFwdArgs &a)
+{
+    if (aligned) hipLaunchKernelGGL((k_adjoint_forward<MODE, OCT, true>), dim3(grid), dim3(kFwdBlock), 0, s, a);
+    else hipLaunchKernelGGL((k_adjoint_forward<MODE, OCT, false>), dim3(grid), dim3(kFwdBlock), 0, s, a);
+}
+
+template <int MODE, bool OCT>
+static void launch_sweep(bool aligned, uint32_t grid, hipStream_t s, const AdjointSweepArgs &w)
+{
+    if (aligned) hipLaunchKernelGGL((k_adjoint_sweep<MODE, OCT, true>), dim3(grid), dim3(kSweepBlock), 0, s, w);
+    else hipLaunchKernelGGL((k_adjoint_sweep<MODE, OCT, false>), dim3(grid), dim3(kSweepBlock), 0, s, w);
+}
+
+// Scratch layout: sub-checkpoints of every 8-turn chunk but the first, per
+// segment. A window is as many segments as the scratch holds.
+static size_t segment_bytes(size_t n, uint32_t every)
+{
+    return (size_t)((every + kChunk - 1) / kChunk - 1) * 4 * n * sizeof(double);
+}
+
+// Default scratch: every segment at once, up to 4 GiB (HM_ADJOINT_SCRATCH_BYTES
+// overrides the cap; at least one segment).
+static constexpr uint64_t kAdjointScratchCap = 4ull << 30;
+// Turn indices, segment ends (s0 + every <= 2 n_turns) stay within 32 bits.
+static constexpr uint32_t kAdjointMaxTurns = 1u << 31;
+
+static uint64_t adjoint_scratch_cap()
+{
+    const char *e = std::getenv("HM_ADJOINT_SCRATCH_BYTES");
+    if (e && *e) {
+        const unsigned long long v = std::strtoull(e, nullptr, 10);
+        if (v > 0) return v;
+    }
+    return kAdjointScratchCap;
 }
 
 }  // namespace hm
@@ -328,62 +626,103 @@ extern "C" int hm_adjoint_forward(const double *x0, const double *px0, const dou
     return guarded([&] {
         if (every == 0) throw Error(HM_ERR_ARG, "adjoint: checkpoint interval must be positive");
         if (n_turns == 0 || n_turns > table_len) throw Error(HM_ERR_RANGE, "adjoint: n_turns must lie in 1..table_len");
+        if (n_turns > kAdjointMaxTurns) throw Error(HM_ERR_RANGE, "adjoint: more than 2^31 turns");
         if (mode != HM_MODE_EXACT && mode != HM_MODE_FAST) throw Error(HM_ERR_ARG, "adjoint: unknown mode");
         if (n == 0) return;
+        if (n > UINT32_MAX) throw Error(HM_ERR_RANGE, "adjoint: too many particles");
+        every = std::min(every, n_turns);  // one segment either way
         AdjointFwdArgs a{x0, px0, y0, py0, (uint32_t)n, reinterpret_cast<const TabEntry *>(table), n_turns, every,
                          mu, barrier * barrier, weights, checkpoints, final_state, birk, alive};
         hipStream_t s = as_stream(stream);
-        const uint32_t grid = (uint32_t)((n + 255) / 256);
+        const uint32_t grid = (uint32_t)((n + kFwdBlock - 1) / kFwdBlock);
         const bool oct = mu != 0.0;
+        const bool aligned = every % 8 == 0;
         LaunchTimer timer("k_adjoint_forward", s);
         if (mode == HM_MODE_EXACT) {
-            if (oct) hipLaunchKernelGGL((k_adjoint_forward<kExact, true>), dim3(grid), dim3(256), 0, s, a);
-            else hipLaunchKernelGGL((k_adjoint_forward<kExact, false>), dim3(grid), dim3(256), 0, s, a);
+            if (oct) launch_forward<kExact, true>(aligned, grid, s, a);
+            else launch_forward<kExact, false>(aligned, grid, s, a);
         } else {
-            if (oct) hipLaunchKernelGGL((k_adjoint_forward<kFast, true>), dim3(grid), dim3(256), 0, s, a);
-            else hipLaunchKernelGGL((k_adjoint_forward<kFast, false>), dim3(grid), dim3(256), 0, s, a);
+            if (oct) launch_forward<kFast, true>(aligned, grid, s, a);
+            else launch_forward<kFast, false>(aligned, grid, s, a);
         }
         launch_check("k_adjoint_forward");
     });
 }
 
-extern "C" int hm_adjoint_scratch_bytes(size_t n, uint32_t every, size_t *bytes)
+extern "C" int hm_adjoint_scratch_bytes(size_t n, uint32_t n_turns, uint32_t every, size_t *bytes)
 {
     return guarded([&] {
         if (!bytes || every == 0) throw Error(HM_ERR_ARG, "adjoint: bad arguments");
-        // two halves (the segment being swept and the one being recomputed)
-        *bytes = 2 * ((size_t)(every + kChunk - 1) / kChunk + 1) * 4 * n * sizeof(double);
+        if (n_turns) every = std::min(every, n_turns);
+        const uint64_t n_seg = n_turns ? (n_turns + (uint64_t)every - 1) / every : 0;
+        const uint64_t seg = segment_bytes(n, every);
+        const uint64_t fit = seg ? std::max<uint64_t>(1, adjoint_scratch_cap() / seg) : 0;
+        *bytes = (size_t)(std::min<uint64_t>(n_seg, fit) * seg);
     });
 }
 
 extern "C" int hm_adjoint_backward(const double *checkpoints, size_t n, const double *table, const double *dtable,
                                    size_t table_len, uint32_t n_turns, double mu, uint32_t every,
                                    const double *weights, const double *seed_state, const double *seed_birk,
-                                   const uint8_t *alive, double *scratch, double *grad_x0, double *grad_params,
-                                   int mode, void *stream)
+                                   const uint8_t *alive, double *scratch, size_t scratch_bytes, double *grad_x0,
+                                   double *grad_params, int mode, void *stream)
 {
     return guarded([&] {
         if (every == 0) throw Error(HM_ERR_ARG, "adjoint: checkpoint interval must be positive");
         if (n_turns == 0 || n_turns > table_len) throw Error(HM_ERR_RANGE, "adjoint: n_turns must lie in 1..table_len");
+        if (n_turns > kAdjointMaxTurns) throw Error(HM_ERR_RANGE, "adjoint: more than 2^31 turns");
         if (mode != HM_MODE_EXACT && mode != HM_MODE_FAST) throw Error(HM_ERR_ARG, "adjoint: unknown mode");
         if (n == 0) return;
-        if (!checkpoints || !table || !dtable || !weights || !seed_state || !alive || !scratch || !grad_x0 ||
-            !grad_params)
+        if (n > UINT32_MAX) throw Error(HM_ERR_RANGE, "adjoint: too many particles");
+        every = std::min(every, n_turns);  // one segment either way
+        if (!checkpoints || !table || !dtable || !weights || !seed_state || !alive || !grad_x0 || !grad_params)
             throw Error(HM_ERR_ARG, "adjoint: NULL pointer");
-        AdjointBwdArgs a{checkpoints, (uint32_t)n, reinterpret_cast<const TabEntry *>(table), dtable, n_turns,
-                         every, mu, weights, seed_state, seed_birk, alive, scratch, grad_x0, grad_params};
+        const uint32_t n_seg = (n_turns + every - 1) / every;
+        const uint32_t seg_chunks = (every + kChunk - 1) / kChunk;
+        const size_t seg = segment_bytes(n, every);
+        // segments per window: all of them when no sub-checkpoints are needed
+        // (every <= 8); else what the scratch holds (grid.y bounds one launch)
+        uint32_t win = n_seg;
+        if (seg) {
+            if (!scratch) throw Error(HM_ERR_ARG, "adjoint: NULL scratch");
+            if (scratch_bytes < seg)
+                throw Error(HM_ERR_ARG, "adjoint: scratch smaller than one segment (see hm_adjoint_scratch_bytes)");
+            win = (uint32_t)std::min<uint64_t>({(uint64_t)n_seg, scratch_bytes / seg, 65535});
+        }
         hipStream_t s = as_stream(stream);
         const uint32_t grid = (uint32_t)((n + 255) / 256);
         const bool oct = mu != 0.0;
+        const bool exact = mode == HM_MODE_EXACT;
         LaunchTimer timer("k_adjoint_backward", s);
-        if (mode == HM_MODE_EXACT) {
-            if (oct) hipLaunchKernelGGL((k_adjoint_backward<kExact, true>), dim3(grid), dim3(256), 0, s, a);
-            else hipLaunchKernelGGL((k_adjoint_backward<kExact, false>), dim3(grid), dim3(256), 0, s, a);
-        } else {
-            if (oct) hipLaunchKernelGGL((k_adjoint_backward<kFast, true>), dim3(grid), dim3(256), 0, s, a);
-            else hipLaunchKernelGGL((k_adjoint_backward<kFast, false>), dim3(grid), dim3(256), 0, s, a);
+        for (uint32_t end = n_seg; end > 0;) {
+            const uint32_t first = end > win ? end - win : 0;
+            if (seg) {
+                AdjointSegArgs g{checkpoints, (uint32_t)n, reinterpret_cast<const TabEntry *>(table), n_turns, every,
+                                 mu, alive, scratch, first, seg_chunks};
+                const dim3 gd(grid, end - first);
+                if (exact) {
+                    if (oct) hipLaunchKernelGGL((k_adjoint_segments<kExact, true>), gd, dim3(256), 0, s, g);
+                    else hipLaunchKernelGGL((k_adjoint_segments<kExact, false>), gd, dim3(256), 0, s, g);
+                } else {
+                    if (oct) hipLaunchKernelGGL((k_adjoint_segments<kFast, true>), gd, dim3(256), 0, s, g);
+                    else hipLaunchKernelGGL((k_adjoint_segments<kFast, false>), gd, dim3(256), 0, s, g);
+                }
+                launch_check("k_adjoint_segments");
+            }
+            AdjointSweepArgs w{checkpoints, (uint32_t)n, reinterpret_cast<const TabEntry *>(table), dtable, n_turns,
+                               every, mu, weights, seed_state, seed_birk, alive, scratch, first, end, seg_chunks,
+                               end == n_seg ? 1 : 0, grad_x0, grad_params};
+            const bool aligned = every % kChunk == 0;
+            if (exact) {
+                if (oct) launch_sweep<kExact, true>(aligned, grid, s, w);
+                else launch_sweep<kExact, false>(aligned, grid, s, w);
+            } else {
+                if (oct) launch_sweep<kFast, true>(aligned, grid, s, w);
+                else launch_sweep<kFast, false>(aligned, grid, s, w);
+            }
+            launch_check("k_adjoint_sweep");
+            end = first;
         }
-        launch_check("k_adjoint_backward");
     });
 }
 

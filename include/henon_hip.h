@@ -321,22 +321,27 @@ int hm_adjoint_forward(const double *x0, const double *px0, const double *y0, co
                        const double *table, size_t table_len, uint32_t n_turns, double mu, double barrier,
                        uint32_t every, const double *weights, double *checkpoints, double *final_state, double *birk,
                        uint8_t *alive, int mode, void *stream);
-/* Bytes of the scratch buffer hm_adjoint_backward needs for n particles and
- * checkpoint interval every. */
-int hm_adjoint_scratch_bytes(size_t n, uint32_t every, size_t *bytes);
+/* Bytes of scratch for hm_adjoint_backward with n particles, n_turns turns and
+ * checkpoint interval every: the 8-turn sub-checkpoints of every K-turn segment
+ * (4 doubles per particle per chunk after a segment's first), capped at 4 GiB
+ * (environment variable HM_ADJOINT_SCRATCH_BYTES sets another cap) but never
+ * below one segment. 0 when every <= 8. */
+int hm_adjoint_scratch_bytes(size_t n, uint32_t n_turns, uint32_t every, size_t *bytes);
 /* Backward sweep: given seed_state = dL/ds_T (double[4][n]) and seed_birk =
  * dL/dB (double[n], or NULL for 0), writes grad_x0 = dL/dx0 (double[4][n]) and
  * grad_params = (dL/d epsilon, dL/d mu, dL/d omega_x0, dL/d omega_y0)
  * (double[4][n]), with dtable[k] = (d omega_x(k)/d omega_x0, d omega_x(k)/d eps,
  * d omega_y(k)/d omega_y0, d omega_y(k)/d eps) (device double[n_turns][4], from
- * hm_lines_modulation_derivatives). Lost particles get zero gradients. Each
- * K-turn segment is recomputed from its checkpoint (interleaved with the reverse
- * sweep of the segment after it) and each 8-turn chunk of it from a
- * sub-checkpoint in scratch; the map is never inverted. */
+ * hm_lines_modulation_derivatives). Lost particles get zero gradients. The
+ * K-turn segments are recomputed from their checkpoints, all at once, into
+ * sub-checkpoints every 8 turns in scratch (scratch_bytes of device memory,
+ * at least one segment's worth: fewer bytes than all segments need make the
+ * sweep run in windows of segments); then each particle's 8-turn chunks are
+ * recomputed and swept in reverse. The map is never inverted. */
 int hm_adjoint_backward(const double *checkpoints, size_t n, const double *table, const double *dtable,
                         size_t table_len, uint32_t n_turns, double mu, uint32_t every, const double *weights,
                         const double *seed_state, const double *seed_birk, const uint8_t *alive, double *scratch,
-                        double *grad_x0, double *grad_params, int mode, void *stream);
+                        size_t scratch_bytes, double *grad_x0, double *grad_params, int mode, void *stream);
 /* Host: for omega(k) = 2 pi tune (1 + eps sum_j coef_j cos(freq_j (k+start)))
  * (hm_lines_modulation, hm_sps_modulation with hm_sps_lines' set), out[2k] =
  * d omega(k)/d omega_0 with omega_0 = 2 pi tune and out[2k+1] = d omega(k)/d eps. */

@@ -123,3 +123,20 @@ def test_track_orbit_autograd(case):
     np.testing.assert_array_equal(xt.grad.cpu().numpy(), gx)
     want = gp.sum(axis=1) * np.array([1, 1, 2 * np.pi, 2 * np.pi])
     np.testing.assert_allclose(par.grad.cpu().numpy(), want, rtol=1e-12)
+
+
+@pytest.mark.parametrize("every,segments", [(40, 1), (40, 3), (300, 1), (17, 2)])
+def test_adjoint_windows_match_one_pass(case, every, segments):
+    """A scratch buffer of a few segments sweeps in windows of segments, carrying
+    the adjoint between them: the gradients are the one-pass gradients bit for bit."""
+    x0, a, c = case
+    adj = A.OrbitAdjoint(T, QX, QY, EPS, MU, barrier=10.0, every=every)
+    st, b, alive, ck = adj.forward(torch.as_tensor(x0, device="cuda"))
+    at, ct = torch.as_tensor(a, device="cuda"), torch.as_tensor(c, device="cuda")
+    gx, gp = adj.backward(ck, alive, at, ct)
+    seg = ((every + 7) // 8 - 1) * 4 * x0.shape[1] * 8
+    gxw, gpw = adj.backward(ck, alive, at, ct, scratch_bytes=segments * seg)
+    torch.cuda.synchronize()
+    assert torch.equal(gx, gxw) and torch.equal(gp, gpw)
+    with pytest.raises(Exception, match="scratch"):
+        adj.backward(ck, alive, at, ct, scratch_bytes=seg - 8)
