@@ -327,56 +327,41 @@ struct Chunk {
     uint32_t seg, j, c0, len;
 };
 
-// LDS ring of the aligned sweep, one per wavefront: the table entries of three
-// 8-turn chunks, per turn sx cx sy cy | the four angle derivatives | weight, pad.
-constexpr int kRingTurn = 10;                                // doubles per turn
-constexpr int kRingSlots = 3;                                // chunks
-constexpr int kRingChunk = kChunk * kRingTurn;               // doubles per chunk
-constexpr int kRingWave = kRingSlots * kRingChunk;           // doubles per wavefront
-constexpr int kSweepBlock = 256;
-
-template <int MODE, bool OCT, bool ALIGNED>
-__global__ void __launch_bounds__(kSweepBlock) k_adjoint_sweep(AdjointSweepArgs a)
+// Reverse sweep for any checkpoint interval: one thread per particle walks the
+// window's chunks in reverse with scalar table loads, recomputing the states of
+// the chunk before the current one while it runs the adjoint over the current.
+template <int MODE, bool OCT>
+__global__ void __launch_bounds__(256) k_adjoint_sweep(AdjointSweepArgs a)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const size_t n = a.n;
-    // The aligned sweep stages table entries cooperatively per wavefront, so
-    // every lane of a wavefront with live lanes runs to the end (lanes past n
-    // read particle n-1; lost particles compute on their parked states); only
-    // live lanes store.
-    const uint32_t ic = i < a.n ? i : a.n - 1;
-    const bool live = i < a.n && a.alive[ic];
-    if (!live && i < a.n && a.last_window)
-        for (int c = 0; c < 4; c++) { a.grad_x0[c * n + i] = 0.0; a.grad_p[c * n + i] = 0.0; }
-    if (ALIGNED ? !__any(live) : !live) return;
+    if (i >= a.n) return;
+    if (!a.alive[i]) {
+        if (a.last_window)
+            for (int c = 0; c < 4; c++) { a.grad_x0[c * n + i] = 0.0; a.grad_p[c * n + i] = 0.0; }
+        return;
+    }
     AdjAcc acc;
     if (a.last_window) {
-        acc = AdjAcc{a.seed_s[ic], a.seed_s[n + ic], a.seed_s[2 * n + ic], a.seed_s[3 * n + ic], 0.0, 0.0, 0.0, 0.0};
+        acc = AdjAcc{a.seed_s[i], a.seed_s[n + i], a.seed_s[2 * n + i], a.seed_s[3 * n + i], 0.0, 0.0, 0.0, 0.0};
     } else {
-        acc = AdjAcc{a.grad_x0[ic], a.grad_x0[n + ic], a.grad_x0[2 * n + ic], a.grad_x0[3 * n + ic],
-                     a.grad_p[ic],  a.grad_p[n + ic],  a.grad_p[2 * n + ic],  a.grad_p[3 * n + ic]};
+        acc = AdjAcc{a.grad_x0[i], a.grad_x0[n + i], a.grad_x0[2 * n + i], a.grad_x0[3 * n + i],
+                     a.grad_p[i],  a.grad_p[n + i],  a.grad_p[2 * n + i],  a.grad_p[3 * n + i]};
     }
-    const double bseed = 2.0 * (a.seed_b ? a.seed_b[ic] : 0.0);
+    const double bseed = 2.0 * (a.seed_b ? a.seed_b[i] : 0.0);
     const double mu = a.mu, m3mu = -3.0 * mu, mu3 = 3.0 * mu, mu6 = 6.0 * mu;
     const uint32_t T = a.n_turns, K = a.every;
     const uint32_t n_seg = (T + K - 1) / K;
-    // tables and weights are read-only here and indexed by the (wave-uniform)
-    // turn: scalar loads
     const ConstPtr<TabEntry> tab = as_const(a.tab);
     const ConstPtr<double> dtab = as_const(a.dtab), weights = as_const(a.weights);
-
     auto load4 = [&](const double *p, double (&v)[4]) {
-        v[0] = p[ic]; v[1] = p[n + ic]; v[2] = p[2 * n + ic]; v[3] = p[3 * n + ic];
-    };
-    auto adj_rec = [&](uint32_t k) -> AdjRec {
-        const ConstPtr<double> d = dtab + 4 * (size_t)k;
-        return AdjRec{load_record(tab + k), double4{d[0], d[1], d[2], d[3]}, weights[k]};
+        v[0] = p[i]; v[1] = p[n + i]; v[2] = p[2 * n + i]; v[3] = p[3 * n + i];
     };
     auto adjoint_at = [&](uint32_t k, const double (&pre)[4], const double (&post)[4]) {
-        const AdjRec r = adj_rec(k);
-        adjoint_turn<OCT>(pre, post, r.t, bseed * r.w, r.dt, mu3, mu6, acc);
+        const ConstPtr<double> d = dtab + 4 * (size_t)k;
+        const double4 dt{d[0], d[1], d[2], d[3]};
+        adjoint_turn<OCT>(pre, post, load_record(tab + k), bseed * weights[k], dt, mu3, mu6, acc);
     };
-    // states S[0..len-1] of the chunk starting at turn c0 from S[0]
     auto fill = [&](double (&S)[kChunk][4], uint32_t c0, uint32_t len) {
 #pragma unroll
         for (int m = 1; m < kChunk; ++m) {
@@ -384,8 +369,6 @@ __global__ void __launch_bounds__(kSweepBlock) k_adjoint_sweep(AdjointSweepArgs 
             if ((uint32_t)m < len) step4<MODE, OCT>(S[m], mu, m3mu, load_record(tab + c0 + m - 1));
         }
     };
-    // adjoint through turns c0+len-1 .. c0 given their states and the state
-    // after the last one
     auto sweep = [&](const double (&S)[kChunk][4], uint32_t c0, uint32_t len, const double (&after)[4]) {
 #pragma unroll
         for (int m = kChunk - 1; m >= 0; --m) {
@@ -394,183 +377,286 @@ __global__ void __launch_bounds__(kSweepBlock) k_adjoint_sweep(AdjointSweepArgs 
             else adjoint_at(c0 + m, S[m], S[m + 1 < kChunk ? m + 1 : m]);
         }
     };
-
+    auto seg_chunks = [&](uint32_t seg) -> uint32_t {
+        const uint32_t s0 = seg * K;
+        return (umin(s0 + K, T) - s0 + kChunk - 1) / kChunk;
+    };
+    auto chunk = [&](uint32_t seg, uint32_t j) -> Chunk {
+        const uint32_t s1 = umin(seg * K + K, T), c0 = seg * K + j * kChunk;
+        return Chunk{seg, j, c0, umin(kChunk, s1 - c0)};
+    };
+    // the chunk before c (in turn order) inside this window
+    auto pred = [&](const Chunk &c, Chunk &p) -> bool {
+        if (c.j > 0) { p = chunk(c.seg, c.j - 1); return true; }
+        if (c.seg > a.seg_first) { p = chunk(c.seg - 1, seg_chunks(c.seg - 1) - 1); return true; }
+        return false;
+    };
+    auto start_of = [&](const Chunk &c) -> const double * {
+        if (c.j == 0) return a.ckpt + (size_t)c.seg * 4 * n;
+        return a.sub + ((size_t)(c.seg - a.seg_first) * (a.seg_chunks - 1) + c.j - 1) * 4 * n;
+    };
     double SA[kChunk][4], SB[kChunk][4], nxt[4], P[4];
-
-    if constexpr (ALIGNED) {
-        // every % 8 == 0: chunk g is turns 8g .. min(8g+8, T)-1; only the very
-        // last chunk can be short
-        __shared__ __align__(16) double ring_all[(kSweepBlock / 64) * kRingWave];
-        const uint32_t K8 = K / kChunk;
-        const uint32_t G = (T + kChunk - 1) / kChunk;
-        const int g_lo = (int)(a.seg_first * K8);
-        const int gt = (int)umin(a.seg_end * K8, G) - 1;
-        const uint32_t lane = threadIdx.x & 63;
-        double *ring = ring_all + (threadIdx.x >> 6) * kRingWave;
-        auto start_g = [&](int g) -> const double * {
-            const uint32_t seg = (uint32_t)g / K8, j = (uint32_t)g - seg * K8;
-            if (j == 0) return a.ckpt + (size_t)seg * 4 * n;
-            return a.sub + ((size_t)(seg - a.seg_first) * (K8 - 1) + j - 1) * 4 * n;
-        };
-        auto clampg = [&](int g) { return g < g_lo ? g_lo : g; };
-        auto slot_of = [&](int g) { return (uint32_t)g % kRingSlots; };
-        // staging of chunk g (a full chunk): lane l < 32 carries table double l
-        // (turn l/4), lane l >= 32 angle-derivative double l-32, every lane also
-        // the weight of turn l % 8 (eight lanes per value, identical writes)
-        const double *tabd = reinterpret_cast<const double *>(a.tab);
-        auto stage_load = [&](int g, double &e0, double &e1) {
-            const size_t b = (size_t)g * kChunk;
-            const double *src = lane < 32 ? tabd + 4 * b + lane : a.dtab + 4 * b + (lane - 32);
-            e0 = *src;
-            e1 = a.weights[b + (lane & 7)];
-        };
-        auto stage_store = [&](int g, double e0, double e1) {
-            double *slot = ring + slot_of(g) * kRingChunk;
-            slot[((lane & 31) >> 2) * kRingTurn + (lane & 3) + (lane < 32 ? 0 : 4)] = e0;
-            slot[(lane & 7) * kRingTurn + 8] = e1;
-        };
-        auto ring_rec = [&](int g, int m) -> TabEntry {
-            const double *p = ring + slot_of(g) * kRingChunk + m * kRingTurn;
-            return TabEntry{p[0], p[1], p[2], p[3]};
-        };
-        auto ring_adj = [&](int g, int m) -> AdjRec {
-            const double *p = ring + slot_of(g) * kRingChunk + m * kRingTurn;
-            return AdjRec{TabEntry{p[0], p[1], p[2], p[3]}, double4{p[4], p[5], p[6], p[7]}, p[8]};
-        };
-
-        // the top chunk: states from its start, the state after it
-        const uint32_t ct = (uint32_t)gt * kChunk, lent = umin(kChunk, T - ct);
-        load4(start_g(gt), SA[0]);
-        fill(SA, ct, lent);
-        if (a.seg_end < n_seg) {
-            load4(a.ckpt + (size_t)a.seg_end * 4 * n, nxt);
-        } else {
-            double v[4];
-#pragma unroll
-            for (int m = 0; m < kChunk; ++m)
-                if ((uint32_t)m + 1 == lent) copy4(v, SA[m]);
-            step4<MODE, OCT>(v, mu, m3mu, load_record(tab + T - 1));
-            copy4(nxt, v);
-        }
-        // the ring (used below gt only, full chunks): chunks gt-1 and gt-2 now,
-        // chunk gt-3 requested
-        double e0 = 0.0, e1 = 0.0;
-        if (gt > g_lo) {
-            stage_load(gt - 1, e0, e1);
-            stage_store(gt - 1, e0, e1);
-            stage_load(clampg(gt - 2), e0, e1);
-            stage_store(clampg(gt - 2), e0, e1);
-            stage_load(clampg(gt - 3), e0, e1);
-            __builtin_amdgcn_wave_barrier();
-        }
-        // top chunk: adjoint with scalar table loads, states of gt-1 into SB
-        if (gt > g_lo) {
-            load4(start_g(gt - 1), SB[0]);
-            fill(SB, ct - kChunk, kChunk);
-        }
-        sweep(SA, ct, lent, nxt);
-        if (gt > g_lo) {
-            copy4(nxt, SA[0]);
-            load4(start_g(clampg(gt - 2)), P);
-            // full chunk g: adjoint from the ring slot of g, states of chunk g-1
-            // into R from the slot of g-1, source-interleaved; the ring takes
-            // chunk g-2 (requested one chunk ago) and chunk g-3 is requested
-            auto fast = [&](int g, const double (&S)[kChunk][4], double (&R)[kChunk][4]) __attribute__((always_inline)) {
-                stage_store(clampg(g - 2), e0, e1);
-                stage_load(clampg(g - 3), e0, e1);
-                __builtin_amdgcn_wave_barrier();  // the stores precede later bodies' reads
-                double Q[4];
-                load4(start_g(clampg(g - 2)), Q);
-                copy4(R[0], P);
-#pragma unroll
-                for (int q = 0; q < kChunk; ++q) {
-                    const int m = kChunk - 1 - q;
-                    const AdjRec r = ring_adj(g, m);
-                    if (m == kChunk - 1) adjoint_turn<OCT>(S[m], nxt, r.t, bseed * r.w, r.dt, mu3, mu6, acc);
-                    else adjoint_turn<OCT>(S[m], S[m + 1 < kChunk ? m + 1 : m], r.t, bseed * r.w, r.dt, mu3, mu6, acc);
-                    if (q + 1 < kChunk) {
-                        copy4(R[q + 1 < kChunk ? q + 1 : q], R[q]);
-                        step4<MODE, OCT>(R[q + 1 < kChunk ? q + 1 : q], mu, m3mu, ring_rec(g - 1, q));
-                    }
-                }
-                copy4(nxt, S[0]);
-                copy4(P, Q);
-            };
-            // the window's first chunk: adjoint only
-            auto last = [&](int g, const double (&S)[kChunk][4]) __attribute__((always_inline)) {
-#pragma unroll
-                for (int m = kChunk - 1; m >= 0; --m) {
-                    const AdjRec r = ring_adj(g, m);
-                    if (m == kChunk - 1) adjoint_turn<OCT>(S[m], nxt, r.t, bseed * r.w, r.dt, mu3, mu6, acc);
-                    else adjoint_turn<OCT>(S[m], S[m + 1 < kChunk ? m + 1 : m], r.t, bseed * r.w, r.dt, mu3, mu6, acc);
-                }
-            };
-            int g = gt - 1;  // its states are in SB
-            for (;;) {
-                if (g == g_lo) { last(g, SB); break; }
-                fast(g, SB, SA);
-                --g;
-                if (g == g_lo) { last(g, SA); break; }
-                fast(g, SA, SB);
-                --g;
-            }
-        }
+    Chunk cur = chunk(a.seg_end - 1, seg_chunks(a.seg_end - 1) - 1), prv{};
+    load4(start_of(cur), SA[0]);
+    fill(SA, cur.c0, cur.len);
+    if (a.seg_end < n_seg) {
+        load4(a.ckpt + (size_t)a.seg_end * 4 * n, nxt);  // the state after the window
     } else {
-        auto seg_chunks = [&](uint32_t seg) -> uint32_t {
-            const uint32_t s0 = seg * K;
-            return (umin(s0 + K, T) - s0 + kChunk - 1) / kChunk;
+        double v[4];
+#pragma unroll
+        for (int m = 0; m < kChunk; ++m)
+            if ((uint32_t)m + 1 == cur.len) copy4(v, SA[m]);
+        step4<MODE, OCT>(v, mu, m3mu, load_record(tab + T - 1));
+        copy4(nxt, v);
+    }
+    bool has = pred(cur, prv);
+    load4(has ? start_of(prv) : a.ckpt, P);
+    // One chunk: S holds cur's states, R receives prv's. Returns whether a
+    // chunk is left.
+    auto body = [&](double (&S)[kChunk][4], double (&R)[kChunk][4]) __attribute__((always_inline)) -> bool {
+        Chunk pp{};
+        const bool has2 = has && pred(prv, pp);
+        double Q[4];
+        load4(has2 ? start_of(pp) : a.ckpt, Q);  // consumed by the next chunk
+        if (has) {
+            copy4(R[0], P);
+            fill(R, prv.c0, prv.len);
+        }
+        sweep(S, cur.c0, cur.len, nxt);
+        if (!has) return false;
+        copy4(nxt, S[0]);
+        cur = prv;
+        prv = pp;
+        has = has2;
+        copy4(P, Q);
+        return true;
+    };
+    while (body(SA, SB) && body(SB, SA)) {
+    }
+    a.grad_x0[i] = acc.lx; a.grad_x0[n + i] = acc.lpx; a.grad_x0[2 * n + i] = acc.ly; a.grad_x0[3 * n + i] = acc.lpy;
+    a.grad_p[i] = acc.g_eps; a.grad_p[n + i] = acc.g_mu; a.grad_p[2 * n + i] = acc.g_wx; a.grad_p[3 * n + i] = acc.g_wy;
+}
+
+// Reverse sweep for every % 8 == 0 (chunk g is turns 8g .. min(8g+8, T)-1; only
+// the very last chunk can be short), two wavefronts per 64 particles. Wavefront
+// 0 (producer) recomputes the states of the chunk after the one being swept
+// (in reverse order: the chunk before it in turn order) into one half of a
+// double-buffered LDS array; wavefront 1 (consumer) runs the adjoint over the
+// current chunk from the other half; a workgroup barrier ends each step. The
+// two dependency chains thus sit on separate wavefronts and the SIMD
+// interleaves them. Each wavefront stages the table entries it reads into its
+// own LDS ring one chunk ahead (one vector load and LDS store per lane and
+// chunk) and reads them one turn ahead, so no scalar-cache miss or LDS latency
+// sits on a dependency chain. The window's top chunk (possibly short) is swept
+// by the consumer alone with scalar table loads while the producer computes the
+// first full chunk.
+constexpr int kRingSlots = 3;                 // chunks per ring
+constexpr int kProdTurn = 4;                  // producer ring: sx cx sy cy
+constexpr int kConsTurn = 10;                 // consumer ring: sx cx sy cy | 4 angle derivatives | weight, pad
+constexpr int kPairBlock = 128;
+
+template <int MODE, bool OCT>
+__global__ void __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(2)))
+k_adjoint_sweep_pair(AdjointSweepArgs a)
+{
+    __shared__ double2 st[2][kChunk][2][64];  // [buffer][state][(x,px) | (y,py)][lane]
+    __shared__ __align__(16) double ring_p[kRingSlots * kChunk * kProdTurn];
+    __shared__ __align__(16) double ring_c[kRingSlots * kChunk * kConsTurn];
+    const uint32_t lane = threadIdx.x & 63;
+    const bool producer = threadIdx.x < 64;
+    const uint32_t i = blockIdx.x * 64 + lane;
+    const size_t n = a.n;
+    // every lane runs to the end (lanes past n read particle n-1, lost
+    // particles compute on parked or stale states); only live lanes store
+    const uint32_t ic = i < a.n ? i : a.n - 1;
+    const bool live = i < a.n && a.alive[ic];
+    if (!producer && !live && i < a.n && a.last_window)
+        for (int c = 0; c < 4; c++) { a.grad_x0[c * n + i] = 0.0; a.grad_p[c * n + i] = 0.0; }
+    if (!__syncthreads_or(live)) return;  // both wavefronts hold the same 64 particles
+
+    const double mu = a.mu, m3mu = -3.0 * mu, mu3 = 3.0 * mu, mu6 = 6.0 * mu;
+    const uint32_t T = a.n_turns, K = a.every;
+    const uint32_t n_seg = (T + K - 1) / K;
+    const uint32_t K8 = K / kChunk;
+    const uint32_t G = (T + kChunk - 1) / kChunk;
+    const int g_lo = (int)(a.seg_first * K8);
+    const int gt = (int)umin(a.seg_end * K8, G) - 1;
+    const double *tabd = reinterpret_cast<const double *>(a.tab);
+    const double *sub_base = a.sub ? a.sub : a.ckpt;
+    auto clampg = [&](int g) { return g < g_lo ? g_lo : g; };
+    auto slot_of = [&](int g) { return (uint32_t)g % kRingSlots; };
+    auto load4 = [&](const double *p, double (&v)[4]) {
+        v[0] = p[ic]; v[1] = p[n + ic]; v[2] = p[2 * n + ic]; v[3] = p[3 * n + ic];
+    };
+    auto start_at = [&](uint32_t seg, uint32_t j) -> const double * {
+        return j == 0 ? a.ckpt + (size_t)seg * 4 * n
+                      : sub_base + ((size_t)(seg - a.seg_first) * (K8 - 1) + j - 1) * 4 * n;
+    };
+    const int steps = gt - g_lo;  // full chunks below the top one
+
+    if (producer) {
+        // ring of table records: lane l carries double l % 32 of the chunk's
+        // 32 (turn (l % 32) / 4); lanes 32..63 repeat lanes 0..31
+        auto stage_load = [&](int g) { return tabd[32 * (size_t)g + (lane & 31)]; };
+        auto stage_store = [&](int g, double e) { ring_p[slot_of(g) * (kChunk * kProdTurn) + (lane & 31)] = e; };
+        auto rec = [&](int g, int m) -> TabEntry {
+            const double *q = ring_p + slot_of(g) * (kChunk * kProdTurn) + m * kProdTurn;
+            return TabEntry{q[0], q[1], q[2], q[3]};
         };
-        auto chunk = [&](uint32_t seg, uint32_t j) -> Chunk {
-            const uint32_t s1 = umin(seg * K + K, T), c0 = seg * K + j * kChunk;
-            return Chunk{seg, j, c0, umin(kChunk, s1 - c0)};
+        double e = 0.0, P[4] = {0.0, 0.0, 0.0, 0.0};
+        int h = 0;
+        uint32_t hs = 0, hj = 0;
+        if (steps > 0) {
+            stage_store(gt - 1, stage_load(gt - 1));
+            e = stage_load(clampg(gt - 2));
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t g1 = (uint32_t)(gt - 1), s1 = g1 / K8;
+            load4(start_at(s1, g1 - s1 * K8), P);
+            h = clampg(gt - 2);
+            hs = (uint32_t)h / K8;
+            hj = (uint32_t)h - hs * K8;
+        }
+        // states of chunk gp (full) into buffer b; the start of the chunk
+        // before it is requested first
+        auto produce = [&](int gp, int b) {
+            stage_store(clampg(gp - 1), e);
+            e = stage_load(clampg(gp - 2));
+            double Q[4];
+            load4(start_at(hs, hj), Q);
+            {
+                const bool down = h > g_lo, wrap = hj == 0;
+                h = down ? h - 1 : h;
+                hs = down && wrap ? hs - 1 : hs;
+                hj = down ? (wrap ? K8 - 1 : hj - 1) : hj;
+            }
+            double v[4];
+            copy4(v, P);
+            st[b][0][0][lane] = double2{v[0], v[1]};
+            st[b][0][1][lane] = double2{v[2], v[3]};
+            TabEntry r = rec(gp, 0);
+#pragma unroll
+            for (int m = 1; m < kChunk; ++m) {
+                TabEntry rn{};
+                if (m + 1 < kChunk) rn = rec(gp, m);
+                step4<MODE, OCT>(v, mu, m3mu, r);
+                st[b][m][0][lane] = double2{v[0], v[1]};
+                st[b][m][1][lane] = double2{v[2], v[3]};
+                r = rn;
+                // one turn per scheduling region: the next turn's entries
+                // are requested one turn ahead, no further
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            copy4(P, Q);
         };
-        // the chunk before c (in turn order) inside this window
-        auto pred = [&](const Chunk &c, Chunk &p) -> bool {
-            if (c.j > 0) { p = chunk(c.seg, c.j - 1); return true; }
-            if (c.seg > a.seg_first) { p = chunk(c.seg - 1, seg_chunks(c.seg - 1) - 1); return true; }
-            return false;
-        };
-        auto start_of = [&](const Chunk &c) -> const double * {
-            if (c.j == 0) return a.ckpt + (size_t)c.seg * 4 * n;
-            return a.sub + ((size_t)(c.seg - a.seg_first) * (a.seg_chunks - 1) + c.j - 1) * 4 * n;
-        };
-        Chunk cur = chunk(a.seg_end - 1, seg_chunks(a.seg_end - 1) - 1), prv{};
-        load4(start_of(cur), SA[0]);
-        fill(SA, cur.c0, cur.len);
+        if (steps > 0) produce(gt - 1, 1);
+        __syncthreads();
+        for (int c = 1; c <= steps; ++c) {
+            if (c < steps) produce(gt - 1 - c, (c + 1) & 1);
+            __syncthreads();
+        }
+        return;
+    }
+
+    // consumer
+    AdjAcc acc;
+    if (a.last_window) {
+        acc = AdjAcc{a.seed_s[ic], a.seed_s[n + ic], a.seed_s[2 * n + ic], a.seed_s[3 * n + ic], 0.0, 0.0, 0.0, 0.0};
+    } else {
+        acc = AdjAcc{a.grad_x0[ic], a.grad_x0[n + ic], a.grad_x0[2 * n + ic], a.grad_x0[3 * n + ic],
+                     a.grad_p[ic],  a.grad_p[n + ic],  a.grad_p[2 * n + ic],  a.grad_p[3 * n + ic]};
+    }
+    const double bseed = 2.0 * (a.seed_b ? a.seed_b[ic] : 0.0);
+    const ConstPtr<TabEntry> tab = as_const(a.tab);
+    const ConstPtr<double> dtab = as_const(a.dtab), weights = as_const(a.weights);
+    // ring: lane l < 32 carries table double l (turn l/4), lane l >= 32 angle-
+    // derivative double l-32, every lane the weight of turn l % 8 (eight lanes
+    // per value, identical writes)
+    auto stage_load = [&](int g, double &e0, double &e1) {
+        const size_t b = (size_t)g * kChunk;
+        const double *src = lane < 32 ? tabd + 4 * b + lane : a.dtab + 4 * b + (lane - 32);
+        e0 = *src;
+        e1 = a.weights[b + (lane & 7)];
+    };
+    auto stage_store = [&](int g, double e0, double e1) {
+        double *slot = ring_c + slot_of(g) * (kChunk * kConsTurn);
+        slot[((lane & 31) >> 2) * kConsTurn + (lane & 3) + (lane < 32 ? 0 : 4)] = e0;
+        slot[(lane & 7) * kConsTurn + 8] = e1;
+    };
+    auto ring_adj = [&](int g, int m) -> AdjRec {
+        const double *p = ring_c + slot_of(g) * (kChunk * kConsTurn) + m * kConsTurn;
+        return AdjRec{TabEntry{p[0], p[1], p[2], p[3]}, double4{p[4], p[5], p[6], p[7]}, p[8]};
+    };
+    double e0 = 0.0, e1 = 0.0;
+    if (steps > 0) {
+        stage_load(gt - 1, e0, e1);
+        stage_store(gt - 1, e0, e1);
+        stage_load(clampg(gt - 2), e0, e1);
+    }
+    double nxt[4];
+    {
+        // the top chunk, scalar table loads
+        const uint32_t ct = (uint32_t)gt * kChunk, lent = umin(kChunk, T - ct);
+        const uint32_t st_seg = (uint32_t)gt / K8;
+        double S[kChunk][4];
+        load4(start_at(st_seg, (uint32_t)gt - st_seg * K8), S[0]);
+#pragma unroll
+        for (int m = 1; m < kChunk; ++m) {
+            copy4(S[m], S[m - 1]);
+            if ((uint32_t)m < lent) step4<MODE, OCT>(S[m], mu, m3mu, load_record(tab + ct + m - 1));
+        }
         if (a.seg_end < n_seg) {
             load4(a.ckpt + (size_t)a.seg_end * 4 * n, nxt);  // the state after the window
         } else {
             double v[4];
 #pragma unroll
             for (int m = 0; m < kChunk; ++m)
-                if ((uint32_t)m + 1 == cur.len) copy4(v, SA[m]);
+                if ((uint32_t)m + 1 == lent) copy4(v, S[m]);
             step4<MODE, OCT>(v, mu, m3mu, load_record(tab + T - 1));
             copy4(nxt, v);
         }
-        bool has = pred(cur, prv);
-        load4(has ? start_of(prv) : a.ckpt, P);
-        // One chunk: S holds cur's states, R receives prv's (chunks of any
-        // length, scalar table loads). Returns whether a chunk is left.
-        auto body = [&](double (&S)[kChunk][4], double (&R)[kChunk][4]) __attribute__((always_inline)) -> bool {
-            Chunk pp{};
-            const bool has2 = has && pred(prv, pp);
-            double Q[4];
-            load4(has2 ? start_of(pp) : a.ckpt, Q);  // consumed by the next chunk
-            if (has) {
-                copy4(R[0], P);
-                fill(R, prv.c0, prv.len);
-            }
-            sweep(S, cur.c0, cur.len, nxt);
-            if (!has) return false;
-            copy4(nxt, S[0]);
-            cur = prv;
-            prv = pp;
-            has = has2;
-            copy4(P, Q);
-            return true;
-        };
-        while (body(SA, SB) && body(SB, SA)) {
+#pragma unroll
+        for (int m = kChunk - 1; m >= 0; --m) {
+            if ((uint32_t)m >= lent) continue;
+            const uint32_t k = ct + m;
+            const ConstPtr<double> d = dtab + 4 * (size_t)k;
+            const double4 dt{d[0], d[1], d[2], d[3]};
+            if ((uint32_t)m + 1 == lent)
+                adjoint_turn<OCT>(S[m], nxt, load_record(tab + k), bseed * weights[k], dt, mu3, mu6, acc);
+            else
+                adjoint_turn<OCT>(S[m], S[m + 1 < kChunk ? m + 1 : m], load_record(tab + k), bseed * weights[k], dt,
+                                  mu3, mu6, acc);
         }
+        copy4(nxt, S[0]);
+    }
+    __syncthreads();
+    for (int c = 1; c <= steps; ++c) {
+        const int g = gt - c, b = c & 1;
+        stage_store(clampg(g - 1), e0, e1);
+        stage_load(clampg(g - 2), e0, e1);
+        double post[4], pre[4];
+        copy4(post, nxt);
+        {
+            const double2 u = st[b][kChunk - 1][0][lane], v = st[b][kChunk - 1][1][lane];
+            pre[0] = u.x; pre[1] = u.y; pre[2] = v.x; pre[3] = v.y;
+        }
+        AdjRec r = ring_adj(g, kChunk - 1);
+#pragma unroll
+        for (int m = kChunk - 1; m >= 0; --m) {
+            double prn[4] = {0.0, 0.0, 0.0, 0.0};
+            AdjRec rn{};
+            if (m > 0) {
+                const double2 u = st[b][m - 1][0][lane], v = st[b][m - 1][1][lane];
+                prn[0] = u.x; prn[1] = u.y; prn[2] = v.x; prn[3] = v.y;
+                rn = ring_adj(g, m - 1);
+            }
+            adjoint_turn<OCT>(pre, post, r.t, bseed * r.w, r.dt, mu3, mu6, acc);
+            copy4(post, pre);
+            copy4(pre, prn);
+            r = rn;
+            __builtin_amdgcn_sched_barrier(0);  // as in the producer
+        }
+        copy4(nxt, post);
+        __syncthreads();
     }
     if (!live) return;
     a.grad_x0[i] = acc.lx; a.grad_x0[n + i] = acc.lpx; a.grad_x0[2 * n + i] = acc.ly; a.grad_x0[3 * n + i] = acc.lpy;
@@ -585,10 +671,12 @@ static void launch_forward(bool aligned, uint32_t grid, hipStream_t s, const Adj
 }
 
 template <int MODE, bool OCT>
-static void launch_sweep(bool aligned, uint32_t grid, hipStream_t s, const AdjointSweepArgs &w)
+static void launch_sweep(bool aligned, size_t n, hipStream_t s, const AdjointSweepArgs &w)
 {
-    if (aligned) hipLaunchKernelGGL((k_adjoint_sweep<MODE, OCT, true>), dim3(grid), dim3(kSweepBlock), 0, s, w);
-    else hipLaunchKernelGGL((k_adjoint_sweep<MODE, OCT, false>), dim3(grid), dim3(kSweepBlock), 0, s, w);
+    if (aligned)
+        hipLaunchKernelGGL((k_adjoint_sweep_pair<MODE, OCT>), dim3((uint32_t)((n + 63) / 64)), dim3(kPairBlock), 0, s,
+                           w);
+    else hipLaunchKernelGGL((k_adjoint_sweep<MODE, OCT>), dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, w);
 }
 
 // Scratch layout: sub-checkpoints of every 8-turn chunk but the first, per
@@ -693,34 +781,39 @@ extern "C" int hm_adjoint_backward(const double *checkpoints, size_t n, const do
         const uint32_t grid = (uint32_t)((n + 255) / 256);
         const bool oct = mu != 0.0;
         const bool exact = mode == HM_MODE_EXACT;
+        const bool aligned = every % kChunk == 0;
+        // segments [first, end): sub-checkpoints at sub
+        auto segments = [&](uint32_t first, uint32_t end, double *sub) {
+            AdjointSegArgs g{checkpoints, (uint32_t)n, reinterpret_cast<const TabEntry *>(table), n_turns, every, mu,
+                             alive, sub, first, seg_chunks};
+            const dim3 gd(grid, end - first);
+            if (exact) {
+                if (oct) hipLaunchKernelGGL((k_adjoint_segments<kExact, true>), gd, dim3(256), 0, s, g);
+                else hipLaunchKernelGGL((k_adjoint_segments<kExact, false>), gd, dim3(256), 0, s, g);
+            } else {
+                if (oct) hipLaunchKernelGGL((k_adjoint_segments<kFast, true>), gd, dim3(256), 0, s, g);
+                else hipLaunchKernelGGL((k_adjoint_segments<kFast, false>), gd, dim3(256), 0, s, g);
+            }
+            launch_check("k_adjoint_segments");
+        };
+        auto sweep = [&](uint32_t first, uint32_t end, const double *sub) {
+            AdjointSweepArgs w{checkpoints, (uint32_t)n, reinterpret_cast<const TabEntry *>(table), dtable, n_turns,
+                               every, mu, weights, seed_state, seed_birk, alive, sub, first, end, seg_chunks,
+                               end == n_seg ? 1 : 0, grad_x0, grad_params};
+            if (exact) {
+                if (oct) launch_sweep<kExact, true>(aligned, n, s, w);
+                else launch_sweep<kExact, false>(aligned, n, s, w);
+            } else {
+                if (oct) launch_sweep<kFast, true>(aligned, n, s, w);
+                else launch_sweep<kFast, false>(aligned, n, s, w);
+            }
+            launch_check("k_adjoint_sweep");
+        };
         LaunchTimer timer("k_adjoint_backward", s);
         for (uint32_t end = n_seg; end > 0;) {
             const uint32_t first = end > win ? end - win : 0;
-            if (seg) {
-                AdjointSegArgs g{checkpoints, (uint32_t)n, reinterpret_cast<const TabEntry *>(table), n_turns, every,
-                                 mu, alive, scratch, first, seg_chunks};
-                const dim3 gd(grid, end - first);
-                if (exact) {
-                    if (oct) hipLaunchKernelGGL((k_adjoint_segments<kExact, true>), gd, dim3(256), 0, s, g);
-                    else hipLaunchKernelGGL((k_adjoint_segments<kExact, false>), gd, dim3(256), 0, s, g);
-                } else {
-                    if (oct) hipLaunchKernelGGL((k_adjoint_segments<kFast, true>), gd, dim3(256), 0, s, g);
-                    else hipLaunchKernelGGL((k_adjoint_segments<kFast, false>), gd, dim3(256), 0, s, g);
-                }
-                launch_check("k_adjoint_segments");
-            }
-            AdjointSweepArgs w{checkpoints, (uint32_t)n, reinterpret_cast<const TabEntry *>(table), dtable, n_turns,
-                               every, mu, weights, seed_state, seed_birk, alive, scratch, first, end, seg_chunks,
-                               end == n_seg ? 1 : 0, grad_x0, grad_params};
-            const bool aligned = every % kChunk == 0;
-            if (exact) {
-                if (oct) launch_sweep<kExact, true>(aligned, grid, s, w);
-                else launch_sweep<kExact, false>(aligned, grid, s, w);
-            } else {
-                if (oct) launch_sweep<kFast, true>(aligned, grid, s, w);
-                else launch_sweep<kFast, false>(aligned, grid, s, w);
-            }
-            launch_check("k_adjoint_sweep");
+            if (seg) segments(first, end, scratch);
+            sweep(first, end, scratch);
             end = first;
         }
     });
