@@ -152,6 +152,23 @@ struct IndicatorArgs {
 //   Birkhoff windows (sample indices from < m <= to): acc += w[m-1-from] dtheta_m
 //     with dtheta wrapped into [0, 2pi) (dynamic_indicator.cpp:110-117); tune =
 //     1 - acc / 2pi (NaN once a sample of the window is NaN).
+// forward_turn_lost without branches: both radius tests are evaluated (exact
+// mode: the screen, then the reference's expression), so a plain turn stays one
+// basic block. Same decisions as forward_turn_lost.
+template <int MODE, bool OCT>
+__device__ __forceinline__ bool forward_turn_out(double &x, double &px, double &y, double &py, double &xx, double &yy,
+                                                 double mu, double m3mu, const TabEntry &t, double b2, double b2s)
+{
+    if (MODE == kExact) {
+        const double r2 = fwd_exact_carry<OCT>(x, px, y, py, xx, yy, mu, t);
+        const bool screen = !(r2 < b2s);
+        const bool out = !(exact_radius2(xx, px, yy, py) < b2);
+        return screen & out;
+    }
+    map_step<kFast, false, OCT>(x, px, y, py, mu, m3mu, t);
+    return !(radius2<kFast>(x, px, y, py) < b2);
+}
+
 template <int MODE, bool OCT, bool EXTRA>
 __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
 {
@@ -236,6 +253,57 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
         pending = false;
     };
     for (; k < a.n_turns; ++k) {
+        if (!EXTRA && uniform && pending) {
+            // Plain turns up to the next event (an FLI sample at its turn, a
+            // MEGNO sample at the top of the turn after it, a renormalisation
+            // turn): no per-turn checks, so the turn is one straight block in
+            // which the previous turn's MEGNO sums, the matrix product and the
+            // map interleave. The next turn's table record and reciprocal are
+            // requested once this turn's have arrived (scalar loads complete
+            // together), a turn ahead of their use.
+            uint32_t stop = a.n_turns;
+            if (sample_q < a.n_samples) stop = min(stop, next_sample - 1);
+            if (sample_m < a.n_samples) stop = min(stop, next_megno);
+            if (renorm_in) stop = min(stop, k + renorm_in - 1);
+            if (stop > k) {
+                // a particle lost in the run is marked (its loss turn kept) and
+                // set to NaN when the run ends: nothing reads it before that
+                const bool alive_start = alive;
+                TabEntry t = load_record(tp + k);
+                double inv_k = inv_idx[k];
+                for (uint32_t kk = k; kk < stop; ++kk) {
+                    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this turn's entries
+                    __builtin_amdgcn_sched_barrier(0);
+                    const uint32_t kn = min(kk + 1, a.n_turns - 1);
+                    const TabEntry tn = load_record(tp + kn);
+                    const double inv_n = inv_idx[kn];
+                    {
+                        const double fa2 = frob2_fused(m);
+                        l1 = l1 + (0.5 * log10(fa2 / fb2)) * w_prev;
+                        l2 = l2 + inv_prev * l1;
+                        fb2 = fa2;
+                    }
+                    const Jac jc = jac_coeffs<MODE>(x, y, mu3, mu6);
+                    mat_mul_structured<MODE>(jc, t, m);
+                    const bool out = forward_turn_out<MODE, OCT>(x, px, y, py, xx, yy, a.mu, m3mu, t, a.b2, b2s);
+                    done = (alive & out) ? kk : done;
+                    alive = alive & !out;
+                    w_prev = (double)(kk + 1);
+                    inv_prev = inv_k;
+                    idx_prev = kk + 1;
+                    t = tn;
+                    inv_k = inv_n;
+                }
+                if (alive_start && !alive) {
+                    x = px = y = py = xx = yy = __builtin_nan("");
+#pragma unroll
+                    for (int q = 0; q < 16; q++) m[q] = __builtin_nan("");
+                }
+                if (renorm_in) renorm_in -= stop - k;
+                k = stop;
+                if (k == a.n_turns || !__any(alive)) break;
+            }
+        }
         if (pending) megno_update();
         TabEntry t;
         if (uniform) {
