@@ -32,7 +32,8 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
-HIP_SOURCES = ["hm_runtime.hip", "hm_track.hip", "hm_tangent.hip", "hm_fft.hip", "hm_adjoint.hip", "hm_batch.hip"]
+HIP_SOURCES = ["hm_runtime.hip", "hm_track.hip", "hm_tangent.hip", "hm_fft.hip", "hm_adjoint.hip", "hm_batch.hip",
+               "hm_ghost.hip"]
 
 LIB_NAME = "libhenon_hip.so"
 ENGINE_SOURCES = ["engine.cpp"]

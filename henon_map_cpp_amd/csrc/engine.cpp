@@ -128,7 +128,27 @@ double nan_std(const std::vector<double> &v)
 }
 
 // ---------------------------------------------------------------- particles
+// henon.cu:362-384: a kick of length module along a direction drawn from four
+// standard normals (a fresh distribution per kick, as there); the reference's
+// particle generator is a default-seeded std::mt19937_64, so the kicks repeat
+// run to run.
+void random_4d_kick(double &x, double &px, double &y, double &py, double module, std::mt19937_64 &rng)
+{
+    std::normal_distribution<double> nd(0.0, 1.0);
+    double k[4];
+    for (double &c : k) c = nd(rng);
+    const double mod = std::sqrt(((k[0] * k[0] + k[1] * k[1]) + k[2] * k[2]) + k[3] * k[3]);
+    x += k[0] * module / mod;
+    px += k[1] * module / mod;
+    y += k[2] * module / mod;
+    py += k[3] * module / mod;
+}
+
 // particles_4d (src/henon.h:41-118, src/henon.cu:699-1151) backed by device SoA.
+// With shadow particles the arrays hold the n_base base particles, then one set
+// of n_base ghosts per add_ghost call (src/henon.cu:737-817); every accessor
+// that returns per-particle arrays returns all entries, as the reference's GPU
+// class does (src/henon.cu:1444-1690), and tracking advances all of them.
 class Particles4D {
   public:
     Particles4D(const std::vector<double> &x, const std::vector<double> &px, const std::vector<double> &y,
@@ -137,7 +157,7 @@ class Particles4D {
         if (px.size() != x.size() || y.size() != x.size() || py.size() != x.size())
             throw std::invalid_argument("particles_4d: x, px, y, py must have the same length");
         require_device();
-        n_particles = x.size();
+        n_base = n_particles = x.size();
         x0 = x; px0 = px; y0 = y; py0 = py;
         upload_initial();
     }
@@ -145,22 +165,79 @@ class Particles4D {
 
     virtual void reset()
     {
-        // src/henon.cu:727-735: global_steps is not reset
+        // src/henon.cu:727-735: global_steps is not reset; ghosts return to
+        // the positions they were created at
         upload_initial();
     }
-    virtual void add_ghost(const double &, const std::string &)
+    virtual void add_ghost(const double &module, const std::string &direction)
     {
-        throw NotImplemented("add_ghost: shadow particles are not available in this build (FLI and MEGNO come "
-                             "from the tangent map: henon_tracker.tangent_indicators)");
+        // src/henon.cu:737-817: one ghost per base particle, displaced from the
+        // particle's current state, with its current steps and valid flag.
+        // (The reference's GPU class builds them from its host copy, which is
+        // the initial state, and re-uploads every particle; here the device
+        // state of the existing particles is kept.)
+        if (direction != "x" && direction != "y" && direction != "px" && direction != "py" && direction != "random")
+            throw std::runtime_error("Invalid direction.");
+        const size_t nb = n_base, total = n_particles;
+        std::vector<double> X = get_x(), PX = get_px(), Y = get_y(), PY = get_py();
+        std::vector<unsigned int> S = get_steps();
+        std::vector<uint8_t> V = get_valid();
+        X.resize(total + nb); PX.resize(total + nb); Y.resize(total + nb); PY.resize(total + nb);
+        S.resize(total + nb); V.resize(total + nb);
+        x0.resize(total + nb); px0.resize(total + nb); y0.resize(total + nb); py0.resize(total + nb);
+        for (size_t b = 0; b < nb; b++) {
+            double gx = X[b], gpx = PX[b], gy = Y[b], gpy = PY[b];
+            if (direction == "x") gx = gx + module;
+            else if (direction == "y") gy = gy + module;
+            else if (direction == "px") gpx = gpx + module;
+            else if (direction == "py") gpy = gpy + module;
+            else random_4d_kick(gx, gpx, gy, gpy, module, rng);
+            const size_t g = total + b;
+            X[g] = x0[g] = gx; PX[g] = px0[g] = gpx; Y[g] = y0[g] = gy; PY[g] = py0[g] = gpy;
+            S[g] = S[b];
+            V[g] = V[b];
+        }
+        dx.upload(X); dpx.upload(PX); dy.upload(Y); dpy.upload(PY);
+        dsteps.upload(S); dvalid.upload(V);
+        n_ghosts += 1;
+        n_particles = total + nb;
     }
-    virtual void renormalize(const double &) { throw NotImplemented("renormalize: shadow particles are not available"); }
+    virtual void renormalize(const double &module_target)
+    {
+        // src/henon.cu:1257-1282
+        check(hm_ghost_renormalize(dx.as<double>(), dpx.as<double>(), dy.as<double>(), dpy.as<double>(), n_base,
+                                   n_ghosts, module_target, nullptr));
+        sync();
+    }
     virtual std::vector<std::vector<double>> get_displacement_module() const
     {
-        throw NotImplemented("get_displacement_module: shadow particles are not available");
+        // src/henon.cu:1315-1335: [n_particles][n_ghosts_per_particle]
+        std::vector<std::vector<double>> out(n_base, std::vector<double>(n_ghosts));
+        if (n_ghosts == 0 || n_base == 0) return out;
+        DevBuf d(n_base * n_ghosts * sizeof(double));
+        check(hm_ghost_displacement(dx.as<double>(), dpx.as<double>(), dy.as<double>(), dpy.as<double>(), n_base,
+                                    n_ghosts, d.as<double>(), nullptr, nullptr));
+        const std::vector<double> h = d.download<double>(n_base * n_ghosts);
+        for (size_t b = 0; b < n_base; b++)
+            std::copy(h.begin() + b * n_ghosts, h.begin() + (b + 1) * n_ghosts, out[b].begin());
+        return out;
     }
     virtual std::vector<std::vector<std::vector<double>>> get_displacement_direction() const
     {
-        throw NotImplemented("get_displacement_direction: shadow particles are not available");
+        // src/henon.cu:1391-1442: [4][n_particles][n_ghosts_per_particle]
+        std::vector<std::vector<std::vector<double>>> out(
+            4, std::vector<std::vector<double>>(n_base, std::vector<double>(n_ghosts)));
+        if (n_ghosts == 0 || n_base == 0) return out;
+        const size_t plane = n_base * n_ghosts;
+        DevBuf d(4 * plane * sizeof(double));
+        check(hm_ghost_displacement(dx.as<double>(), dpx.as<double>(), dy.as<double>(), dpy.as<double>(), n_base,
+                                    n_ghosts, nullptr, d.as<double>(), nullptr));
+        const std::vector<double> h = d.download<double>(4 * plane);
+        for (size_t c = 0; c < 4; c++)
+            for (size_t b = 0; b < n_base; b++)
+                std::copy(h.begin() + c * plane + b * n_ghosts, h.begin() + c * plane + (b + 1) * n_ghosts,
+                          out[c][b].begin());
+        return out;
     }
 
     virtual std::vector<double> get_x() const { return dx.download<double>(n_particles); }
@@ -196,18 +273,31 @@ class Particles4D {
     virtual double get_angles_y_std() const { return nan_std(get_angles_y()); }
 
     virtual std::vector<uint8_t> get_valid() const { return dvalid.download<uint8_t>(n_particles); }
-    virtual std::vector<uint8_t> get_ghost() const { return std::vector<uint8_t>(n_particles, 0); }
-    virtual std::vector<size_t> get_idx() const
+    virtual std::vector<uint8_t> get_ghost() const
     {
-        std::vector<size_t> v(n_particles);
-        for (size_t i = 0; i < n_particles; i++) v[i] = i;
+        std::vector<uint8_t> v(n_particles, 1);
+        std::fill(v.begin(), v.begin() + n_base, 0);
         return v;
     }
-    virtual std::vector<size_t> get_idx_base() const { return get_idx(); }
-    virtual size_t get_n_particles() const { return n_particles; }
-    virtual size_t get_n_ghosts_per_particle() const { return 0; }
+    virtual std::vector<size_t> get_idx() const
+    {
+        // base particles: their index; ghosts: their set (src/henon.cu:755)
+        std::vector<size_t> v(n_particles);
+        for (size_t i = 0; i < n_particles; i++) v[i] = i < n_base ? i : i / n_base - 1;
+        return v;
+    }
+    virtual std::vector<size_t> get_idx_base() const
+    {
+        std::vector<size_t> v(n_particles);
+        for (size_t i = 0; i < n_particles; i++) v[i] = n_base ? i % n_base : i;
+        return v;
+    }
+    virtual size_t get_n_particles() const { return n_base; }
+    virtual size_t get_n_ghosts_per_particle() const { return n_ghosts; }
 
-    size_t n_particles = 0;
+    size_t n_base = 0;       // the reference's n_particles: particles without their ghosts
+    size_t n_ghosts = 0;     // ghost sets
+    size_t n_particles = 0;  // entries of the device arrays: n_base * (1 + n_ghosts)
     unsigned int global_steps = 0;
     std::vector<double> x0, px0, y0, py0;
     DevBuf dx, dpx, dy, dpy, dsteps, dvalid;
@@ -222,6 +312,7 @@ class Particles4D {
         dsteps.upload(std::vector<unsigned int>(n_particles, 0u));
         dvalid.upload(std::vector<uint8_t>(n_particles, 1));
     }
+    std::mt19937_64 rng;  // src/henon.h:63, default-seeded
 };
 
 class Particles4DGpu : public Particles4D {

@@ -189,6 +189,28 @@ enum {
 int hm_observable(int kind, const double *x, const double *px, const double *y, const double *py, size_t n,
                   double *out, void *stream);
 
+/* ---- shadow (ghost) particles --------------------------------------------
+ * Layout of the particle arrays with ghosts (src/henon.cu:737-817): n_base base
+ * particles, then `sets` ghost sets of n_base entries each; entry
+ * n_base * (1 + g) + b is ghost g of base particle b. The ghosts themselves are
+ * built on the host (henon_map_engine's particles_4d.add_ghost). */
+
+/* Rescales every ghost's displacement from its base to length target:
+ * g <- b + (g - b) * (target / |g - b|). Bases are untouched.
+ * Replaces gpu_particle_renormalize / particles_4d_gpu::renormalize
+ * (src/henon.cu:1257-1282; CPU: 826-858). */
+int hm_ghost_renormalize(double *x, double *px, double *y, double *py, size_t n_base, size_t sets, double target,
+                         void *stream);
+/* Displacement of every ghost from its base: module_out (device double
+ * [n_base][sets], or NULL) = |g - b|; direction_out (device double
+ * [4][n_base][sets], or NULL) = (b + (g - b) * (1 / |g - b|)) - b per
+ * coordinate (x, px, y, py). NaN where the ghost or its base holds a NaN.
+ * Replaces gpu_particle_displacements / gpu_particle_directions and the
+ * get_displacement_module / get_displacement_direction accessors
+ * (src/henon.cu:1284-1442; CPU: 860-952). */
+int hm_ghost_displacement(const double *x, const double *px, const double *y, const double *py, size_t n_base,
+                          size_t sets, double *module_out, double *direction_out, void *stream);
+
 /* Tangent matrix of one turn at the current points, table index table_index:
  * J (reverse=0) or the analytic inverse J^-1 (reverse=1); particles with
  * valid[i]==0 get a NaN matrix. out: device double[n][16].

@@ -79,6 +79,9 @@ def lib():
             "or_normalize": (None, [P, SZ]),
             "or_nan_mean": (D, [P, SZ]),
             "or_nan_std": (D, [P, SZ]),
+            "or_random_kicks": (None, [ctypes.c_uint64, SZ, D, P, P, P, P]),
+            "or_ghost_renormalize": (None, [P, P, P, P, SZ, SZ, D]),
+            "or_ghost_displacement": (None, [P, P, P, P, SZ, SZ, P, P]),
             "or_indicators": (I, [P, P, P, P, P, SZ, P, P, P, P, SZ, U, D, D, U, P, SZ, P, P, P, P, P, P, P, SZ,
                                   P, I]),
         }
@@ -190,7 +193,9 @@ class Tables:
 
 # ---------------------------------------------------------------- particles + tracker
 class Particles:
-    """particles_4d state (henon.cu:699-735), host arrays."""
+    """particles_4d state (henon.cu:699-735), host arrays. With shadow particles
+    (add_ghost, henon.cu:737-817) the arrays hold the base particles, then one
+    set of ghosts per add_ghost call."""
 
     def __init__(self, x, px, y, py):
         self.x, self.px, self.y, self.py = (_f64(a).copy() for a in (x, px, y, py))
@@ -198,6 +203,9 @@ class Particles:
         self.steps = np.zeros(self.x.size, dtype=np.uint32)
         self.valid = np.ones(self.x.size, dtype=np.uint8)
         self.global_steps = 0
+        self.n_base = self.x.size
+        self.n_ghosts = 0
+        self._kicks = 0  # random kicks drawn from the particle generator so far
 
     @property
     def n(self):
@@ -211,6 +219,44 @@ class Particles:
 
     def state(self):
         return _p(self.x), _p(self.px), _p(self.y), _p(self.py), _p(self.steps)
+
+    def add_ghost(self, module, direction):
+        """henon.cu:737-817: one ghost per base particle, from its current state."""
+        nb = self.n_base
+        g = [a[:nb].copy() for a in (self.x, self.px, self.y, self.py)]
+        axis = {"x": 0, "px": 1, "y": 2, "py": 3}
+        if direction in axis:
+            g[axis[direction]] = g[axis[direction]] + module
+        elif direction == "random":
+            lib().or_random_kicks(self._kicks, nb, float(module), *(_p(a) for a in g))
+            self._kicks += nb
+        else:
+            raise RuntimeError("Invalid direction.")
+        self.x, self.px, self.y, self.py = (np.concatenate((a, b)) for a, b in zip((self.x, self.px, self.y, self.py), g))
+        self.x0, self.px0, self.y0, self.py0 = (np.concatenate((a, b)) for a, b in
+                                                zip((self.x0, self.px0, self.y0, self.py0), g))
+        self.steps = np.concatenate((self.steps, self.steps[:nb]))
+        self.valid = np.concatenate((self.valid, self.valid[:nb]))
+        self.n_ghosts += 1
+
+    def renormalize(self, target):
+        """henon.cu:826-858."""
+        lib().or_ghost_renormalize(_p(self.x), _p(self.px), _p(self.y), _p(self.py), self.n_base, self.n_ghosts,
+                                   float(target))
+
+    def displacement_module(self):
+        """henon.cu:860-897: [n_base][n_ghosts]."""
+        out = np.empty((self.n_base, self.n_ghosts))
+        lib().or_ghost_displacement(_p(self.x), _p(self.px), _p(self.y), _p(self.py), self.n_base, self.n_ghosts,
+                                    _p(out), None)
+        return out
+
+    def displacement_direction(self):
+        """henon.cu:899-952: [4][n_base][n_ghosts]."""
+        out = np.empty((4, self.n_base, self.n_ghosts))
+        lib().or_ghost_displacement(_p(self.x), _p(self.px), _p(self.y), _p(self.py), self.n_base, self.n_ghosts,
+                                    None, _p(out))
+        return out
 
 
 class Tracker:

@@ -28,6 +28,9 @@
 //                      the DFT, analytic single tones
 //   megno (trace)    : closed-form recursions on linear maps
 //   constructs       : golden lyapunov multi values (test_construct.ipynb:328-345)
+//   shadow particles : golden ghost coordinates and random directions
+//                      (tests/test_new_standard.ipynb cells 47-54), unit length
+//                      after renormalize(1), closed-form axis directions
 //   fused indicators : (new algorithms) closed forms on rotations, finite
 //   and adjoint        differences of the oracle map, tangent-map products
 
@@ -41,6 +44,8 @@
 #include <numeric>
 #include <complex>
 #include <limits>
+#include <random>
+#include <array>
 
 namespace {
 
@@ -703,6 +708,78 @@ double or_nan_std(const double *v, size_t n)
     for (size_t i = 0; i < n; i++) if (!std::isnan(v[i])) { sum += (v[i] - mean) * (v[i] - mean); count++; }
     if (count == 0) return std::numeric_limits<double>::quiet_NaN();
     return std::sqrt(sum / count);
+}
+
+// ---- shadow particles: henon.cu:362-418, 737-952 --------------------------------
+
+static double displacement(double x1, double px1, double y1, double py1, double x2, double px2, double y2, double py2)
+{
+    // henon.cu:407-410
+    return std::sqrt((x1 - x2) * (x1 - x2) + (px1 - px2) * (px1 - px2) + (y1 - y2) * (y1 - y2) +
+                     (py1 - py2) * (py1 - py2));
+}
+
+static void realign(double &x, double &px, double &y, double &py, double xc, double pxc, double yc, double pyc,
+                    double initial, double final_)
+{
+    // henon.cu:412-418
+    x = xc + (x - xc) * (final_ / initial);
+    px = pxc + (px - pxc) * (final_ / initial);
+    y = yc + (y - yc) * (final_ / initial);
+    py = pyc + (py - pyc) * (final_ / initial);
+}
+
+// henon.cu:362-384 (random_4d_kick) on the particle generator of henon.h:63 (a
+// default-seeded std::mt19937_64 that add_ghost keeps using): the generator is
+// replayed past the `skip` kicks drawn before, then kicks the n points in order.
+void or_random_kicks(uint64_t skip, size_t n, double module, double *x, double *px, double *y, double *py)
+{
+    std::mt19937_64 rng;
+    auto kick = [&](double &a, double &b, double &c, double &d) {
+        std::normal_distribution<double> nd(0.0, 1.0);
+        std::array<double, 4> k;
+        for (auto &v : k) v = nd(rng);
+        const double mod = std::sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2] + k[3] * k[3]);
+        a += k[0] * module / mod;
+        b += k[1] * module / mod;
+        c += k[2] * module / mod;
+        d += k[3] * module / mod;
+    };
+    double dummy[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint64_t q = 0; q < skip; q++) kick(dummy[0], dummy[1], dummy[2], dummy[3]);
+    for (size_t i = 0; i < n; i++) kick(x[i], px[i], y[i], py[i]);
+}
+
+// henon.cu:826-858: ghost i = n_base * (1 + g) + b realigned to distance target
+// from base b.
+void or_ghost_renormalize(double *x, double *px, double *y, double *py, size_t n_base, size_t sets, double target)
+{
+    for (size_t t = 0; t < n_base * sets; t++) {
+        const size_t b = t % n_base, i = n_base + t;
+        const double module = displacement(x[i], px[i], y[i], py[i], x[b], px[b], y[b], py[b]);
+        realign(x[i], px[i], y[i], py[i], x[b], px[b], y[b], py[b], module, target);
+    }
+}
+
+// henon.cu:860-952 (CPU accessors): module [n_base][sets], direction
+// [4][n_base][sets] = the ghost realigned to distance 1, minus the base.
+void or_ghost_displacement(const double *x, const double *px, const double *y, const double *py, size_t n_base,
+                           size_t sets, double *module_out, double *dir_out)
+{
+    const size_t plane = n_base * sets;
+    for (size_t t = 0; t < plane; t++) {
+        const size_t g = t / n_base, b = t % n_base, i = n_base * (1 + g) + b, o = b * sets + g;
+        const double module = displacement(x[i], px[i], y[i], py[i], x[b], px[b], y[b], py[b]);
+        if (module_out) module_out[o] = module;
+        if (dir_out) {
+            double d[4] = {x[i], px[i], y[i], py[i]};
+            realign(d[0], d[1], d[2], d[3], x[b], px[b], y[b], py[b], module, 1.0);
+            dir_out[o] = d[0] - x[b];
+            dir_out[plane + o] = d[1] - px[b];
+            dir_out[2 * plane + o] = d[2] - y[b];
+            dir_out[3 * plane + o] = d[3] - py[b];
+        }
+    }
 }
 
 }  // extern "C"

@@ -242,3 +242,41 @@ def test_cpu_megno_thread_carry_quirk():
     assert np.array_equal(a[0], b[0], equal_nan=True)
     assert np.array_equal(b[1], c[1], equal_nan=True)
     assert not np.array_equal(a[1], c[1], equal_nan=True)
+
+
+def test_ghost_golden(golden):
+    """Shadow particles (henon.cu:737-952) against test_new_standard.ipynb: the
+    displaced coordinates, the random directions of the default-seeded particle
+    generator, unit modules after renormalize(1)."""
+    g = golden["ghost_3x5"]
+    p = O.Particles(np.ones(3), np.ones(3), np.ones(3), np.ones(3))
+    for d in ("x", "px", "y", "py", "random"):
+        p.add_ghost(1e-4, d)
+    assert p.n == 18 and p.n_ghosts == 5
+    np.testing.assert_allclose(p.x, g["x"], rtol=0, atol=0.6 * 10.0 ** -g["x_digits"])
+    direction = p.displacement_direction()
+    np.testing.assert_allclose(direction[:, :, 4], g["random_direction"], rtol=0,
+                               atol=0.6 * 10.0 ** -g["direction_digits"])
+    # axis ghosts point along their axis; every ghost sits 1e-4 from its base
+    np.testing.assert_allclose(direction[:, :, :4], np.broadcast_to(np.eye(4)[:, None, :], (4, 3, 4)), atol=1e-11)
+    np.testing.assert_allclose(p.displacement_module(), g["module"], rtol=1e-11)
+    np.testing.assert_allclose(np.sum(direction ** 2, axis=0), 1.0, rtol=1e-12)
+    p.renormalize(1.0)
+    np.testing.assert_allclose(p.displacement_module(), g["module_after_renormalize_1"], rtol=1e-14)
+    np.testing.assert_allclose(p.displacement_direction(), direction, atol=1e-11)
+    # the base particles never move
+    np.testing.assert_array_equal(p.x[:3], np.ones(3))
+
+
+def test_ghost_random_kicks_continue_the_generator():
+    """A second random set continues the generator: it differs from the first,
+    and a fresh particle set replays the same sequence."""
+    a = O.Particles(np.zeros(4), np.zeros(4), np.zeros(4), np.zeros(4))
+    a.add_ghost(1.0, "random")
+    a.add_ghost(1.0, "random")
+    b = O.Particles(np.zeros(8), np.zeros(8), np.zeros(8), np.zeros(8))
+    b.add_ghost(1.0, "random")
+    np.testing.assert_array_equal(a.x[4:], b.x[8:])
+    assert not np.array_equal(a.x[4:8], a.x[8:12])
+    with pytest.raises(RuntimeError):
+        a.add_ghost(1.0, "z")
