@@ -104,6 +104,7 @@ SIGNATURES = {
     "hm_adjoint_forward": (I, [P, P, P, P, SZ, P, SZ, U32, D, D, U32, P, P, P, P, P, I, P]),
     "hm_ghost_renormalize": (I, [P, P, P, P, SZ, SZ, D, P]),
     "hm_ghost_displacement": (I, [P, P, P, P, SZ, SZ, P, P, P]),
+    "hm_ghost_alignment": (I, [P, P, P, P, SZ, SZ, I, P, P]),
     "hm_adjoint_scratch_bytes": (I, [SZ, U32, U32, ctypes.POINTER(SZ)]),
     "hm_adjoint_backward": (I, [P, SZ, P, P, SZ, U32, D, U32, P, P, P, P, P, SZ, P, P, I, P]),
     "hm_lines_modulation_derivatives": (I, [D, D, P, P, I, I, I, P]),

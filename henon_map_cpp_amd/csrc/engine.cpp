@@ -240,6 +240,16 @@ class Particles4D {
         return out;
     }
 
+    // Alignment indices of the first ghost sets at the current state
+    // (henon_map_cpp/dynamic_indicators.py:154-366): no reference method.
+    std::vector<double> alignment_index(int kind) const
+    {
+        DevBuf d(n_base * sizeof(double));
+        check(hm_ghost_alignment(dx.as<double>(), dpx.as<double>(), dy.as<double>(), dpy.as<double>(), n_base,
+                                 n_ghosts, kind, d.as<double>(), nullptr));
+        return d.download<double>(n_base);
+    }
+
     virtual std::vector<double> get_x() const { return dx.download<double>(n_particles); }
     virtual std::vector<double> get_px() const { return dpx.download<double>(n_particles); }
     virtual std::vector<double> get_y() const { return dy.download<double>(n_particles); }
@@ -1284,6 +1294,8 @@ PYBIND11_MODULE(henon_map_engine, m)
         .def("renormalize", &Particles4D::renormalize, py::arg("module_target"))
         .def("get_displacement_module", &Particles4D::get_displacement_module)
         .def("get_displacement_direction", &Particles4D::get_displacement_direction)
+        .def("get_sali", [](Particles4D &s) { return as_array(s.alignment_index(HM_ALIGN_SALI)); })
+        .def("get_gali", [](Particles4D &s) { return as_array(s.alignment_index(HM_ALIGN_GALI)); })
         .def("get_x", [](Particles4D &s) { return as_array(s.get_x()); })
         .def("get_px", [](Particles4D &s) { return as_array(s.get_px()); })
         .def("get_y", [](Particles4D &s) { return as_array(s.get_y()); })

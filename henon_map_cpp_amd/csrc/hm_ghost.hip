@@ -77,6 +77,86 @@ __global__ void k_ghost_displacement(const double *__restrict__ x, const double 
     }
 }
 
+// henon_map_cpp/dynamic_indicators.py:154-366 (SALI, GALI): the deviation of
+// ghost set k from base b normalised as there, d_k = (g - b) / |g - b|, with
+// |.| = sqrt(((x^2 + px^2) + y^2) + py^2).
+//   SALI (kind 0, sets 0-1): min(|d_0 + d_1|, |d_0 - d_1|)
+//   GALI (kind 1, sets 0-3): |det [d_0 d_1 d_2 d_3]|, the product of the
+//                            singular values the reference takes from an SVD
+// NaN where any deviation holds a NaN.
+__device__ __forceinline__ void deviation(const double *__restrict__ x, const double *__restrict__ px,
+                                          const double *__restrict__ y, const double *__restrict__ py, uint64_t i,
+                                          const double (&b)[4], double (&d)[4])
+{
+    d[0] = x[i] - b[0];
+    d[1] = px[i] - b[1];
+    d[2] = y[i] - b[2];
+    d[3] = py[i] - b[3];
+    const double norm = sqrt(((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]) + d[3] * d[3]);
+    for (int c = 0; c < 4; c++) d[c] = d[c] / norm;
+}
+
+__global__ void k_ghost_alignment(const double *__restrict__ x, const double *__restrict__ px,
+                                  const double *__restrict__ y, const double *__restrict__ py, uint64_t n_base,
+                                  int kind, double *__restrict__ out)
+{
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n_base) return;
+    const double base[4] = {x[b], px[b], y[b], py[b]};
+    const double q = __builtin_nan("");
+    if (kind == 0) {
+        double d0[4], d1[4];
+        deviation(x, px, y, py, n_base + b, base, d0);
+        deviation(x, px, y, py, 2 * n_base + b, base, d1);
+        const double plus = sqrt((((d0[0] + d1[0]) * (d0[0] + d1[0]) + (d0[1] + d1[1]) * (d0[1] + d1[1])) +
+                                  (d0[2] + d1[2]) * (d0[2] + d1[2])) +
+                                 (d0[3] + d1[3]) * (d0[3] + d1[3]));
+        const double minus = sqrt((((d0[0] - d1[0]) * (d0[0] - d1[0]) + (d0[1] - d1[1]) * (d0[1] - d1[1])) +
+                                   (d0[2] - d1[2]) * (d0[2] - d1[2])) +
+                                  (d0[3] - d1[3]) * (d0[3] - d1[3]));
+        out[b] = (isnan(plus) || isnan(minus)) ? q : (plus < minus ? plus : minus);
+        return;
+    }
+    // a[r][c]: component r of deviation c; |det| by elimination with partial
+    // pivoting
+    double a[4][4];
+    bool nan_in = false;
+    for (int c = 0; c < 4; c++) {
+        double d[4];
+        deviation(x, px, y, py, (uint64_t)(c + 1) * n_base + b, base, d);
+        for (int r = 0; r < 4; r++) {
+            a[r][c] = d[r];
+            nan_in = nan_in || isnan(d[r]);
+        }
+    }
+    double det = 1.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int p = k;
+#pragma unroll
+        for (int r = k + 1; r < 4; r++)
+            if (fabs(a[r][k]) > fabs(a[p][k])) p = r;
+        if (p != k) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const double t = a[k][c];
+                a[k][c] = a[p][c];
+                a[p][c] = t;
+            }
+        }
+        det = det * a[k][k];
+        if (a[k][k] != 0.0) {
+#pragma unroll
+            for (int r = k + 1; r < 4; r++) {
+                const double f = a[r][k] / a[k][k];
+#pragma unroll
+                for (int c = k + 1; c < 4; c++) a[r][c] = a[r][c] - f * a[k][c];
+            }
+        }
+    }
+    out[b] = nan_in ? q : fabs(det);
+}
+
 }  // namespace hm
 
 using namespace hm;
@@ -113,5 +193,24 @@ extern "C" int hm_ghost_displacement(const double *x, const double *px, const do
         hipLaunchKernelGGL(k_ghost_displacement, dim3((uint32_t)grid), dim3(256), 0, s, x, px, y, py,
                            (uint64_t)n_base, (uint64_t)sets, module_out, direction_out);
         launch_check("k_ghost_displacement");
+    });
+}
+
+extern "C" int hm_ghost_alignment(const double *x, const double *px, const double *y, const double *py, size_t n_base,
+                                  size_t sets, int kind, double *out, void *stream)
+{
+    return guarded([&] {
+        if (kind != HM_ALIGN_SALI && kind != HM_ALIGN_GALI) throw Error(HM_ERR_ARG, "alignment: unknown kind");
+        if (sets < (kind == HM_ALIGN_SALI ? 2u : 4u))
+            throw Error(HM_ERR_ARG, "alignment: SALI needs 2 ghost sets, GALI 4");
+        if (n_base == 0) return;
+        if (!x || !px || !y || !py || !out) throw Error(HM_ERR_ARG, "alignment: NULL pointer");
+        hipStream_t s = as_stream(stream);
+        const uint64_t grid = ((uint64_t)n_base + 255) / 256;
+        if (grid > 0x7fffffffull) throw Error(HM_ERR_RANGE, "alignment: too many particles");
+        LaunchTimer timer("k_ghost_alignment", s);
+        hipLaunchKernelGGL(k_ghost_alignment, dim3((uint32_t)grid), dim3(256), 0, s, x, px, y, py, (uint64_t)n_base,
+                           kind, out);
+        launch_check("k_ghost_alignment");
     });
 }

@@ -211,6 +211,18 @@ int hm_ghost_renormalize(double *x, double *px, double *y, double *py, size_t n_
 int hm_ghost_displacement(const double *x, const double *px, const double *y, const double *py, size_t n_base,
                           size_t sets, double *module_out, double *direction_out, void *stream);
 
+/* Alignment indices of the deviation vectors d_k = (g_k - b) / |g_k - b| of
+ * the first ghost sets (henon_map_cpp/dynamic_indicators.py:154-366, which
+ * computes them in numpy from shadow particles it tracks alongside):
+ * HM_ALIGN_SALI (needs sets >= 2): min(|d_0 + d_1|, |d_0 - d_1|);
+ * HM_ALIGN_GALI (needs sets >= 4): |det [d_0 d_1 d_2 d_3]| (the product of the
+ * singular values). out: device double[n_base], the index at the current
+ * state (the running minimum over time is the caller's); NaN where a
+ * deviation holds a NaN. */
+enum { HM_ALIGN_SALI = 0, HM_ALIGN_GALI = 1 };
+int hm_ghost_alignment(const double *x, const double *px, const double *y, const double *py, size_t n_base,
+                       size_t sets, int kind, double *out, void *stream);
+
 /* Tangent matrix of one turn at the current points, table index table_index:
  * J (reverse=0) or the analytic inverse J^-1 (reverse=1); particles with
  * valid[i]==0 get a NaN matrix. out: device double[n][16].

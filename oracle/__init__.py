@@ -259,6 +259,37 @@ class Particles:
         return out
 
 
+def _deviations(p: "Particles", k: int):
+    # henon_map_cpp/dynamic_indicators.py:198-222: (ghost - base) / norm
+    n = p.n_base
+    d = [a[(k + 1) * n:(k + 2) * n] - a[:n] for a in (p.x, p.px, p.y, p.py)]
+    norm = np.sqrt(d[0] ** 2 + d[1] ** 2 + d[2] ** 2 + d[3] ** 2)
+    return [c / norm for c in d]
+
+
+def sali(p: "Particles"):
+    """henon_map_cpp/dynamic_indicators.py:198-224 at the current state, ghost
+    sets 0 and 1 as the two displaced copies: min(|d1 + d2|, |d1 - d2|)."""
+    d1, d2 = _deviations(p, 0), _deviations(p, 1)
+    plus = np.sqrt((d1[0] + d2[0]) ** 2 + (d1[1] + d2[1]) ** 2 + (d1[2] + d2[2]) ** 2 + (d1[3] + d2[3]) ** 2)
+    minus = np.sqrt((d1[0] - d2[0]) ** 2 + (d1[1] - d2[1]) ** 2 + (d1[2] - d2[2]) ** 2 + (d1[3] - d2[3]) ** 2)
+    return np.min([plus, minus], axis=0)
+
+
+def gali(p: "Particles"):
+    """henon_map_cpp/dynamic_indicators.py:300-356 at the current state, ghost
+    sets 0-3: the product of the singular values of the matrix of normalised
+    deviations (rows x, px, y, py), NaN where it holds a NaN."""
+    d = [_deviations(p, k) for k in range(4)]
+    m = np.array([[d[c][r] for c in range(4)] for r in range(4)])  # [r][c][particle]
+    m = np.moveaxis(m, 2, 0)
+    ok = np.all(~np.isnan(m), axis=(1, 2))
+    out = np.full(p.n_base, np.nan)
+    if ok.any():
+        out[ok] = np.prod(np.linalg.svd(m[ok], compute_uv=False), axis=-1)
+    return out
+
+
 class Tracker:
     """henon_tracker (CPU): tables + track/megno/birkhoff_tunes/all_tunes."""
 

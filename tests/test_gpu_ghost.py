@@ -111,3 +111,77 @@ def test_ghost_added_after_tracking_uses_the_current_state(hm):
     ot.track(op, 50, 0.0, barrier=10.0)
     assert np.array_equal(p.get_x(), op.x, equal_nan=True)
     assert np.array_equal(p.get_displacement_direction(), op.displacement_direction(), equal_nan=True)
+
+
+def _grid(samples, lo, hi):
+    v = np.linspace(lo, hi, samples)
+    xx, yy = (a.flatten() for a in np.meshgrid(v, v))
+    return xx, np.zeros_like(xx), yy, np.zeros_like(xx)
+
+
+def test_alignment_indices_match_oracle(hm):
+    """hm_ghost_alignment at a tracked state: SALI bit for bit with the
+    reference harness's numpy formula, GALI (|det| here, SVD product there) to
+    rounding; NaN for lost orbits."""
+    x, px, y, py = _grid(12, 0.05, 0.5)
+    t = hm.henon_tracker(3000, 0.168, 0.201, "sps", epsilon=32.0)
+    p = hm.particles(x, px, y, py)
+    ot = O.Tracker(3000, 0.168, 0.201, "sps", epsilon=32.0)
+    op = O.Particles(x, px, y, py)
+    for d in ("x", "y", "px", "py"):
+        p.add_ghost(1e-8, d)
+        op.add_ghost(1e-8, d)
+    t.track(p, 1500, 0.5, barrier=10.0)
+    ot.track(op, 1500, 0.5, barrier=10.0)
+    s, g = p.particles.get_sali(), p.particles.get_gali()
+    assert np.array_equal(s, O.sali(op), equal_nan=True)
+    want = O.gali(op)
+    assert np.array_equal(np.isnan(g), np.isnan(want))
+    ok = ~np.isnan(want)
+    np.testing.assert_allclose(g[ok], want[ok], rtol=1e-9, atol=1e-14)
+    assert np.isnan(s).any() and (~np.isnan(s)).any()  # lost and surviving orbits both covered
+
+
+def test_sali_gali_fli_functions(hm):
+    from henon_map_cpp_amd import dynamic_indicators as DI
+
+    x, px, y, py = _grid(8, 0.05, 0.45)
+    t = hm.henon_tracker(5000, 0.168, 0.201, "sps", epsilon=32.0)
+    ts, tau = [100, 1000, 2000], 10
+    sali = DI.smallest_alignment_index(t, x, px, y, py, ts, tau, 1e-8, mu=0.5)
+    gali = DI.global_alignment_index(t, x, px, y, py, ts, tau, 1e-8, mu=0.5)
+    fli = DI.fast_lyapunov_indicator(t, x, px, y, py, ts, 1e-8, mu=0.5)
+    assert list(sali.index) == ts and sali.shape == (3, x.size)
+    # the running minima never increase
+    for frame in (sali, gali):
+        a = frame.to_numpy()
+        fin = np.isfinite(a).all(axis=0)
+        assert np.all(np.diff(a[:, fin], axis=0) <= 0)
+    # the SALI loop, restated with the oracle: bit for bit
+    ot = O.Tracker(5000, 0.168, 0.201, "sps", epsilon=32.0)
+    op = O.Particles(x, px, y, py)
+    op.add_ghost(1e-8, "x")
+    op.add_ghost(1e-8, "y")
+    acc, rows, j = np.full(x.size, np.sqrt(2.0)), [], 0
+    for i in range(tau, ts[-1] + tau - 1, tau):
+        ot.track(op, tau, 0.5, barrier=10.0)
+        acc = np.minimum(acc, O.sali(op))
+        while j < len(ts) and i >= ts[j]:
+            rows.append(acc.copy())
+            j += 1
+    assert np.array_equal(sali.to_numpy(), np.array(rows), equal_nan=True)
+    # FLI from the shadow particle: bit for bit with the oracle's ghost
+    op = O.Particles(x, px, y, py)
+    op.add_ghost(1e-8, "random")
+    done, want = 0, []
+    for tt in ts:
+        ot.track(op, tt - done, 0.5, barrier=10.0)
+        done = tt
+        want.append(np.log10(op.displacement_module()[:, 0] / 1e-8) / tt)
+    assert np.array_equal(fli.to_numpy(), np.array(want), equal_nan=True)
+    # chaotic orbits: SALI and GALI collapse where the FLI grows
+    last = np.isfinite(fli.to_numpy()[-1]) & np.isfinite(sali.to_numpy()[-1])
+    chaotic = last & (fli.to_numpy()[-1] > 0.01)
+    if chaotic.any():
+        assert np.median(sali.to_numpy()[-1][chaotic]) < 1e-3
+        assert np.median(gali.to_numpy()[-1][chaotic]) < 1e-6

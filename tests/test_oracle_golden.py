@@ -280,3 +280,27 @@ def test_ghost_random_kicks_continue_the_generator():
     assert not np.array_equal(a.x[4:8], a.x[8:12])
     with pytest.raises(RuntimeError):
         a.add_ghost(1.0, "z")
+
+
+def test_alignment_indices_closed_forms():
+    """SALI and GALI as the reference's harness computes them
+    (dynamic_indicators.py:154-366) on an unmodulated, nearly linear orbit: the
+    x- and y-plane deviations stay orthogonal (SALI = sqrt 2) and the normalised
+    deviation matrix stays a rotation (GALI = 1); on a chaotic orbit both
+    collapse. GALI is cross-checked against |det|."""
+    t = O.Tracker(4000, 0.168, 0.201, "sps", epsilon=0.0)
+    p = O.Particles([1e-6], [0.0], [1e-6], [0.0])
+    for d in ("x", "y", "px", "py"):
+        p.add_ghost(1e-9, d)
+    t.track(p, 1000, 0.0, barrier=10.0)
+    assert abs(O.sali(p)[0] - np.sqrt(2.0)) < 1e-5
+    assert abs(O.gali(p)[0] - 1.0) < 1e-5
+    c = O.Particles([0.3], [0.0], [0.2], [0.0])
+    for d in ("x", "y", "px", "py"):
+        c.add_ghost(1e-9, d)
+    O.Tracker(4000, 0.168, 0.201, "sps", epsilon=32.0).track(c, 2000, 0.5, barrier=10.0)
+    assert np.all(c.steps == 2000)
+    dev = [O._deviations(c, k) for k in range(4)]
+    m = np.array([[dev[k][r][0] for k in range(4)] for r in range(4)])
+    assert O.gali(c)[0] == pytest.approx(abs(np.linalg.det(m)), abs=1e-14)
+    assert O.gali(c)[0] < 1e-9 and O.sali(c)[0] < 0.1
