@@ -1,0 +1,42 @@
+"""The parameter-scan driver (htcondor_jobs/compose_config.py, run_sim.py):
+job-list format and output names on the CPU; on the GPU, a batched scan equal
+to the oracle's tracking of each configuration."""
+import os
+
+import numpy as np
+import pytest
+
+from henon_map_cpp_amd import scan as S
+
+
+def test_job_list_and_names(tmp_path):
+    path = tmp_path / "configs" / "all_jobs.txt"
+    configs = S.compose_config(str(path))
+    lines = path.read_text().splitlines()
+    assert len(lines) == len(configs) == 2 * 10 * 5
+    assert lines[0] == "0.168 0.201 0.0 0.0 0.75"
+    assert lines[-1] == "0.31 0.32 256.0 1.0 0.25"
+    assert S.read_config(str(path)) == configs
+    assert S.ScanConfig(0.168, 0.201, 32.0, 0.5, 0.75).file_stem() == "stability_henon_0d168_0d201_32d0_0d5_0d75"
+    x, px, y, py = S.initial_conditions(0.25, 4)
+    assert x.size == 16 and np.all(px == 0) and np.all(py == 0)
+    np.testing.assert_array_equal(x[:4], np.linspace(0, 0.25, 4))
+    np.testing.assert_array_equal(y[::4], np.linspace(0, 0.25, 4))
+
+
+@pytest.mark.gpu
+def test_batched_scan_matches_oracle(tmp_path):
+    import oracle as O
+
+    configs = [S.ScanConfig(ox, oy, e, m, r) for (ox, oy, r) in S.OMEGA_LIST for e in (0.0, 64.0) for m in (0.0, 0.75)]
+    n_turns, pts = 3000, 15
+    # a budget of three tables forces two launch sequences
+    res = S.run_scan(configs, pts, n_turns, out_dir=str(tmp_path), table_budget=3 * 32 * n_turns)
+    for c in configs:
+        t = O.Tracker(n_turns, c.omega_x, c.omega_y, "sps", epsilon=c.epsilon)
+        p = O.Particles(*S.initial_conditions(c.max_radius, pts))
+        t.track(p, n_turns, c.mu, barrier=10.0)
+        assert np.array_equal(res[c], p.steps), c
+        saved = np.load(os.path.join(tmp_path, c.file_stem() + ".npy"))
+        assert saved.dtype == np.uint32 and np.array_equal(saved, p.steps)
+    assert len(set(tuple(v) for v in res.values())) > 1
