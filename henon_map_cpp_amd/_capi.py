@@ -102,6 +102,7 @@ SIGNATURES = {
     "hm_birkhoff_tunes_history": (I, [P, SZ, U32, P, P, P, SZ, P, P]),
     "hm_fft_tunes": (I, [P, SZ, U32, P, P, P, SZ, P, P]),
     "hm_adjoint_forward": (I, [P, P, P, P, SZ, P, SZ, U32, D, D, U32, P, P, P, P, P, I, P]),
+    "hm_track_kicked": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, D, D, I, I, ctypes.c_uint64, ctypes.c_uint64, P]),
     "hm_ghost_renormalize": (I, [P, P, P, P, SZ, SZ, D, P]),
     "hm_ghost_displacement": (I, [P, P, P, P, SZ, SZ, P, P, P]),
     "hm_ghost_alignment": (I, [P, P, P, P, SZ, SZ, I, P, P]),

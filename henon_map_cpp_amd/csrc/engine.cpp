@@ -128,6 +128,8 @@ double nan_std(const std::vector<double> &v)
 }
 
 // ---------------------------------------------------------------- particles
+uint64_t fresh_seed() { return ((uint64_t)std::random_device{}() << 32) ^ std::random_device{}(); }
+
 // henon.cu:362-384: a kick of length module along a direction drawn from four
 // standard normals (a fresh distribution per kick, as there); the reference's
 // particle generator is a default-seeded std::mt19937_64, so the kicks repeat
@@ -305,6 +307,16 @@ class Particles4D {
     virtual size_t get_n_particles() const { return n_base; }
     virtual size_t get_n_ghosts_per_particle() const { return n_ghosts; }
 
+    // Kick generator (hm_track_kicked): a fresh seed per object, as the
+    // reference seeds its states from the clock; set_kick_seed fixes it.
+    void set_kick_seed(uint64_t seed)
+    {
+        kick_seed = seed;
+        kick_draws = 0;
+    }
+    uint64_t kick_seed = fresh_seed();
+    uint64_t kick_draws = 0;
+
     size_t n_base = 0;       // the reference's n_particles: particles without their ghosts
     size_t n_ghosts = 0;     // ghost sets
     size_t n_particles = 0;  // entries of the device arrays: n_base * (1 + n_ghosts)
@@ -357,7 +369,6 @@ std::vector<double> lines_modulation(double tune, double epsilon, const std::vec
     return v;
 }
 
-uint64_t fresh_seed() { return ((uint64_t)std::random_device{}() << 32) ^ std::random_device{}(); }
 
 std::vector<double> gaussian_modulation(double tune, double sigma, int start, int end, py::object seed)
 {
@@ -427,9 +438,19 @@ class HenonTracker {
         const int m = mode_of(mode);
         {
             py::gil_scoped_release nogil;
-            check(hm_track(p.dx.as<double>(), p.dpx.as<double>(), p.dy.as<double>(), p.dpy.as<double>(),
-                           p.dsteps.as<uint32_t>(), p.n_particles, dtable.as<double>(), allowed_steps, n_turns, mu,
-                           barrier, kick_module, inverse ? 1 : 0, m, nullptr, nullptr));
+            if (std::isnan(kick_module)) {
+                check(hm_track(p.dx.as<double>(), p.dpx.as<double>(), p.dy.as<double>(), p.dpy.as<double>(),
+                               p.dsteps.as<uint32_t>(), p.n_particles, dtable.as<double>(), allowed_steps, n_turns,
+                               mu, barrier, kick_module, inverse ? 1 : 0, m, nullptr, nullptr));
+            } else {
+                // src/henon.cu:3155-3185 with the kick: the particle object's
+                // generator stream (seed, draws so far) continues call to call
+                check(hm_track_kicked(p.dx.as<double>(), p.dpx.as<double>(), p.dy.as<double>(), p.dpy.as<double>(),
+                                      p.dsteps.as<uint32_t>(), p.n_particles, dtable.as<double>(), allowed_steps,
+                                      n_turns, mu, barrier, kick_module, inverse ? 1 : 0, m, p.kick_seed,
+                                      p.kick_draws, nullptr));
+                p.kick_draws += n_turns;
+            }
             uint32_t flags = 0;
             check(hm_take_status(&flags, nullptr));
             if (flags & 1u)
@@ -1294,6 +1315,7 @@ PYBIND11_MODULE(henon_map_engine, m)
         .def("renormalize", &Particles4D::renormalize, py::arg("module_target"))
         .def("get_displacement_module", &Particles4D::get_displacement_module)
         .def("get_displacement_direction", &Particles4D::get_displacement_direction)
+        .def("set_kick_seed", &Particles4D::set_kick_seed, py::arg("seed"))
         .def("get_sali", [](Particles4D &s) { return as_array(s.alignment_index(HM_ALIGN_SALI)); })
         .def("get_gali", [](Particles4D &s) { return as_array(s.alignment_index(HM_ALIGN_GALI)); })
         .def("get_x", [](Particles4D &s) { return as_array(s.get_x()); })

@@ -209,7 +209,7 @@ extern "C" int hm_track(double *x, double *px, double *y, double *py, uint32_t *
 {
     return guarded([&] {
         if (!std::isnan(kick_module))
-            throw Error(HM_ERR_UNSUPPORTED, "track: the random kick (kick_module) is not available in this build");
+            throw Error(HM_ERR_ARG, "track: a finite kick_module needs hm_track_kicked (seed and draw index)");
         if (mode != HM_MODE_EXACT && mode != HM_MODE_FAST) throw Error(HM_ERR_ARG, "track: unknown mode");
         if (n == 0 || n_turns == 0) return;
         if (!x || !px || !y || !py || !steps || !table) throw Error(HM_ERR_ARG, "track: NULL pointer");

@@ -110,7 +110,7 @@ typedef struct hm_launch_cfg {
  *   mu                  : octupole strength (the octupole kick is skipped when 0)
  *   barrier             : loss radius; lost when x^2+px^2+y^2+py^2 >= barrier^2,
  *                         then all four coordinates become NaN
- *   kick_module         : must be NaN (the random kick is not available)
+ *   kick_module         : must be NaN (for the random kick, hm_track_kicked)
  *   inverse             : 0 forward (table index steps), 1 inverse (steps-1)
  *   mode                : HM_MODE_EXACT (bit-exact with the reference CPU path)
  *                         or HM_MODE_FAST (fused multiply-adds)
@@ -121,6 +121,20 @@ typedef struct hm_launch_cfg {
 int hm_track(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n,
              const double *table, size_t table_len, uint32_t n_turns, double mu, double barrier,
              double kick_module, int inverse, int mode, const hm_launch_cfg *cfg, void *stream);
+
+/* hm_track with the random 4D kick of src/henon.cu:362-405, 420-542: every
+ * turn, before the forward map (after the inverse map), each particle moves by
+ * kick_module (finite) along a direction drawn from four standard normals. The
+ * reference draws them from clock-seeded cuRAND states (not reproducible); here
+ * particle i's k-th turn of the call uses Philox-4x32-10 keyed by `seed` at draw
+ * index draw + k (Box-Muller normals), so a seed and draw index fix the kicks,
+ * and a caller continues the stream by advancing `draw` by n_turns per call.
+ * Same loss rule, NaN semantics and status bit 0 (turn index outside the table)
+ * as hm_track. Replaces the kick branch of henon_step / gpu_henon_track
+ * (src/henon.cu:420-542, 636-691). */
+int hm_track_kicked(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n, const double *table,
+                    size_t table_len, uint32_t n_turns, double mu, double barrier, double kick_module, int inverse,
+                    int mode, uint64_t seed, uint64_t draw, void *stream);
 
 /* Autotuner (no reference counterpart). Times hm_track launch configurations
  * (block 64 / 256, compact_every 500 / 1000 / 2000, one or two particles per

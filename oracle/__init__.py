@@ -80,6 +80,10 @@ def lib():
             "or_nan_mean": (D, [P, SZ]),
             "or_nan_std": (D, [P, SZ]),
             "or_random_kicks": (None, [ctypes.c_uint64, SZ, D, P, P, P, P]),
+            "or_philox4x32_10": (None, [P, U, U]),
+            "or_kick_normals": (None, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, P]),
+            "or_track_kicked": (I, [P, P, P, P, P, SZ, P, P, P, P, SZ, U, D, D, D, I, ctypes.c_uint64,
+                                    ctypes.c_uint64]),
             "or_ghost_renormalize": (None, [P, P, P, P, SZ, SZ, D]),
             "or_ghost_displacement": (None, [P, P, P, P, SZ, SZ, P, P]),
             "or_indicators": (I, [P, P, P, P, P, SZ, P, P, P, P, SZ, U, D, D, U, P, SZ, P, P, P, P, P, P, P, SZ,
@@ -267,6 +271,20 @@ def _deviations(p: "Particles", k: int):
     return [c / norm for c in d]
 
 
+def philox4x32_10(counter, key):
+    """Philox-4x32-10 block (Salmon et al., SC'11): four 32-bit words."""
+    c = np.ascontiguousarray(counter, dtype=np.uint32).copy()
+    lib().or_philox4x32_10(_p(c), int(key[0]), int(key[1]))
+    return c
+
+
+def kick_normals(seed, i, d):
+    """The four Box-Muller normals of particle i's draw d."""
+    out = np.empty(4)
+    lib().or_kick_normals(seed, i, d, _p(out))
+    return out
+
+
 def sali(p: "Particles"):
     """henon_map_cpp/dynamic_indicators.py:198-224 at the current state, ghost
     sets 0 and 1 as the two displaced copies: min(|d1 + d2|, |d1 - d2|)."""
@@ -314,6 +332,17 @@ class Tracker:
 
     def _advance(self, p, n_turns, inverse):
         p.global_steps += -n_turns if inverse else n_turns
+
+    def track_kicked(self, p, n_turns, mu, barrier, kick_module, seed, draw, inverse=False):
+        """henon_step with the random kick (henon.cu:420-480), the kick's normals
+        from Philox-4x32-10 keyed by seed at draw index draw + k (the product's
+        generator); per-particle table indices, one thread."""
+        self._check(p, n_turns, inverse)
+        bad = lib().or_track_kicked(*p.state(), p.n, *self.t.ptrs(), self.t.allowed_steps, n_turns, mu, barrier,
+                                    kick_module, int(inverse), seed, draw)
+        if bad:
+            raise RuntimeError("a turn index left the table")
+        self._advance(p, n_turns, inverse)
 
     def track(self, p, n_turns, mu, barrier=100.0, kick_module=np.nan, inverse=False, n_threads=-1):
         self._check(p, n_turns, inverse)

@@ -782,6 +782,86 @@ void or_ghost_displacement(const double *x, const double *px, const double *y, c
     }
 }
 
+// ---- random kick: henon.cu:362-405, 420-480 ---------------------------------------
+// The reference draws the normals from std::mt19937_64 (CPU) or clock-seeded
+// cuRAND (GPU); the product draws them from Philox-4x32-10 (Salmon et al.,
+// SC'11) keyed by a seed with counter (particle, draw), Box-Muller normals. The
+// generator is restated here from the paper and pinned by its published
+// known-answer vectors; the kicked step is henon_step's.
+
+void or_philox4x32_10(uint32_t *c, uint32_t k0, uint32_t k1)
+{
+    for (int r = 0; r < 10; r++) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0, n1 = (uint32_t)p1;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1, n3 = (uint32_t)p0;
+        c[0] = n0; c[1] = n1; c[2] = n2; c[3] = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+static double uniform53(uint32_t a, uint32_t b)
+{
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+void or_kick_normals(uint64_t seed, uint64_t i, uint64_t d, double *n)
+{
+    for (uint32_t half = 0; half < 2; half++) {
+        uint32_t c[4] = {(uint32_t)i, (uint32_t)(i >> 32), (uint32_t)d, ((uint32_t)(d >> 32) << 1) | half};
+        or_philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const double u1 = 1.0 - uniform53(c[0], c[1]), u2 = uniform53(c[2], c[3]);
+        const double rad = std::sqrt(-2.0 * std::log(u1));
+        n[2 * half] = rad * std::cos(2.0 * M_PI * u2);
+        n[2 * half + 1] = rad * std::sin(2.0 * M_PI * u2);
+    }
+}
+
+// henon.cu:420-480 (henon_step) with the kick above; per particle, particle
+// index i, draw index draw + k at the call's k-th turn.
+int or_track_kicked(double *x, double *px, double *y, double *py, unsigned *steps, size_t n, const double *sx,
+                    const double *cx, const double *sy, const double *cy, size_t tab_len, unsigned n_turns, double mu,
+                    double barrier, double module, int reverse, uint64_t seed, uint64_t draw)
+{
+    const double b2 = barrier * barrier;
+    int bad = 0;
+    for (size_t i = 0; i < n; i++) {
+        for (unsigned k = 0; k < n_turns; k++) {
+            if (check_nan(x[i], px[i], y[i], py[i])) break;
+            const unsigned ti = reverse ? steps[i] - 1 : steps[i];
+            if ((reverse && steps[i] == 0) || ti >= tab_len) { bad = 1; break; }
+            double kk[4];
+            if (!reverse) {
+                or_kick_normals(seed, i, draw + k, kk);
+                const double mod = std::sqrt(kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2] + kk[3] * kk[3]);
+                x[i] += kk[0] * module / mod; px[i] += kk[1] * module / mod;
+                y[i] += kk[2] * module / mod; py[i] += kk[3] * module / mod;
+                sextupole(x[i], px[i], y[i], py[i], false);
+                if (mu != 0.0) octupole(x[i], px[i], y[i], py[i], mu, false);
+                rotation(x[i], px[i], sx[ti], cx[ti], false);
+                rotation(y[i], py[i], sy[ti], cy[ti], false);
+            } else {
+                rotation(x[i], px[i], sx[ti], cx[ti], true);
+                rotation(y[i], py[i], sy[ti], cy[ti], true);
+                if (mu != 0.0) octupole(x[i], px[i], y[i], py[i], mu, true);
+                sextupole(x[i], px[i], y[i], py[i], true);
+                or_kick_normals(seed, i, draw + k, kk);
+                const double mod = std::sqrt(kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2] + kk[3] * kk[3]);
+                x[i] += kk[0] * module / mod; px[i] += kk[1] * module / mod;
+                y[i] += kk[2] * module / mod; py[i] += kk[3] * module / mod;
+            }
+            if (check_barrier(x[i], px[i], y[i], py[i], b2)) {
+                steps[i] += reverse ? -1 : 1;
+            } else {
+                x[i] = px[i] = y[i] = py[i] = std::numeric_limits<double>::quiet_NaN();
+                break;
+            }
+        }
+    }
+    return bad;
+}
+
 }  // extern "C"
 
 // ---- fused indicators (new algorithm: no reference counterpart) ---------------

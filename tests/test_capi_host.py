@@ -38,9 +38,13 @@ def test_version_and_errors():
     assert "start" in C.lib().hm_last_error().decode()
     with pytest.raises(ValueError):
         C.check(rc)
-    # the random kick is refused before any device work
-    with pytest.raises(NotImplementedError):
+    # hm_track refuses a finite kick before any device work (hm_track_kicked
+    # takes it, with the generator's seed and draw index)
+    with pytest.raises(ValueError, match="hm_track_kicked"):
         C.check(C.lib().hm_track(None, None, None, None, None, 4, None, 4, 10, 0.0, 1.0, 0.5, 0, 0, None, None))
+    with pytest.raises(ValueError, match="finite"):
+        C.check(C.lib().hm_track_kicked(None, None, None, None, None, 4, None, 4, 10, 0.0, 1.0, float("nan"), 0, 0,
+                                        1, 0, None))
 
 
 @pytest.mark.parametrize("tune,eps,start,end", [(0.168, 0.0, 0, 777), (0.31, 32.0, 0, 70000), (0.2, 16.0, 99, 1500)])

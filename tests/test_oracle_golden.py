@@ -304,3 +304,31 @@ def test_alignment_indices_closed_forms():
     m = np.array([[dev[k][r][0] for k in range(4)] for r in range(4)])
     assert O.gali(c)[0] == pytest.approx(abs(np.linalg.det(m)), abs=1e-14)
     assert O.gali(c)[0] < 1e-9 and O.sali(c)[0] < 0.1
+
+
+def test_philox_known_answers():
+    """The kick generator: Philox-4x32-10's published known-answer vectors
+    (Random123 distribution, kat_vectors: philox4x32_10)."""
+    cases = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+             ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+             ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+              (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+    for ctr, key, want in cases:
+        assert tuple(int(v) for v in O.philox4x32_10(ctr, key)) == want
+
+
+def test_kick_normals_and_kicked_step():
+    """Box-Muller normals from the generator: standard normal moments; the kick
+    moves a point by exactly kick_module before the map (inverse map undoes the
+    map, leaving the kick)."""
+    z = np.array([O.kick_normals(7, i, d) for i in range(500) for d in range(40)]).ravel()
+    assert abs(z.mean()) < 0.02 and abs(z.var() - 1.0) < 0.03 and abs((z ** 4).mean() - 3.0) < 0.15
+    t = O.Tracker(100, 0.168, 0.201, "sps", epsilon=8.0)
+    x0 = np.array([0.1, 0.2, 0.05])
+    p = O.Particles(x0, x0 / 2, x0, -x0)
+    t.track_kicked(p, 1, 0.3, 10.0, 1e-3, seed=11, draw=0)
+    t.track(p, 1, 0.3, barrier=10.0, inverse=True)
+    d = np.stack([p.x - x0, p.px - x0 / 2, p.y - x0, p.py + x0])
+    np.testing.assert_allclose(np.sqrt((d ** 2).sum(axis=0)), 1e-3, rtol=1e-10)
+    n = np.array([O.kick_normals(11, i, 0) for i in range(3)]).T
+    np.testing.assert_allclose(d / 1e-3, n / np.sqrt((n ** 2).sum(axis=0)), atol=1e-10)
