@@ -69,8 +69,8 @@ def test_turn_limits_and_errors(hm):
     np.testing.assert_allclose(p.get_x(), [0.1], atol=1e-14)
     with pytest.raises(RuntimeError, match="Unknown modulation kind"):
         hm.henon_tracker(100, 0.1, 0.2, "triangle")
-    with pytest.raises(NotImplementedError):
-        t.track(p, 1, 0.0, kick_module=1e-3)
+    with pytest.raises(NotImplementedError):  # the kick is taken by track only
+        t.megno(p, 1, 0.0, kick_module=1e-3)
     with pytest.raises(TypeError):
         hm.particles([0.1], [0.0], [0.1], [0.0])
 
