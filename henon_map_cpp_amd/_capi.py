@@ -101,6 +101,10 @@ SIGNATURES = {
     "hm_history_means": (I, [P, SZ, U32, P, P]),
     "hm_birkhoff_tunes_history": (I, [P, SZ, U32, P, P, P, SZ, P, P]),
     "hm_fft_tunes": (I, [P, SZ, U32, P, P, P, SZ, P, P]),
+    "hm_track_rows": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, D, I, P, SZ, P]),
+    "hm_rows_means": (I, [P, SZ, SZ, U32, P, P]),
+    "hm_birkhoff_tunes_rows": (I, [P, SZ, SZ, U32, P, P, P, SZ, P, P]),
+    "hm_fft_tunes_rows": (I, [P, SZ, SZ, U32, P, P, P, SZ, P, P]),
     "hm_adjoint_forward": (I, [P, P, P, P, SZ, P, SZ, U32, D, D, U32, P, P, P, P, P, I, P]),
     "hm_track_kicked": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, D, D, I, I, ctypes.c_uint64, ctypes.c_uint64, P]),
     "hm_ghost_renormalize": (I, [P, P, P, P, SZ, SZ, D, P]),
@@ -393,22 +397,50 @@ def track_history(parts: DeviceParticles, table_dev: DeviceBuffer, table_len: in
     return hist
 
 
+def row_length(n_turns: int) -> int:
+    """Samples per particle row of hm_track_rows: n_turns + 1 rounded up to 16."""
+    return (n_turns + 1 + 15) // 16 * 16
+
+
+def track_rows(parts: DeviceParticles, table_dev: DeviceBuffer, table_len: int, n_turns: int, mu: float,
+               barrier: float, inverse=False, stream=None) -> DeviceBuffer:
+    """The trajectory as particle rows [n][4][row_length(n_turns)]."""
+    L = row_length(n_turns)
+    rows = DeviceBuffer(8 * parts.n * 4 * L)
+    check(lib().hm_track_rows(*parts.ptrs(), parts.n, table_dev.ptr, table_len, n_turns, mu, barrier, int(inverse),
+                              rows.ptr, L, stream))
+    return rows
+
+
 def history_tunes(hist: DeviceBuffer, n: int, n_samples: int, windows, kind="fft", subtract_mean=False,
-                  stream=None):
-    """Tunes of windows [(from, to)] (inclusive) of a device history; returns
-    numpy [n_windows][2][n]."""
+                  stream=None, row_len=None):
+    """Tunes of windows [(from, to)] (inclusive) of a device history
+    (sample-major, or particle rows of length row_len); returns numpy
+    [n_windows][2][n]."""
+    L = lib()
     means = None
     if subtract_mean:
         means = DeviceBuffer(8 * 4 * n)
-        check(lib().hm_history_means(hist.ptr, n, n_samples, means.ptr, stream))
+        if row_len is None:
+            check(L.hm_history_means(hist.ptr, n, n_samples, means.ptr, stream))
+        else:
+            check(L.hm_rows_means(hist.ptr, n, row_len, n_samples, means.ptr, stream))
     f = np.ascontiguousarray([w[0] for w in windows], dtype=np.uint32)
     t = np.ascontiguousarray([w[1] for w in windows], dtype=np.uint32)
     out = DeviceBuffer(8 * n * 2 * len(windows))
     mp = means.ptr if means is not None else None
     if kind == "fft":
         ln = np.ascontiguousarray(t - f + 1, dtype=np.uint32)
-        check(lib().hm_fft_tunes(hist.ptr, n, n_samples, mp, ptr(f), ptr(ln), len(windows), out.ptr, stream))
+        if row_len is None:
+            check(L.hm_fft_tunes(hist.ptr, n, n_samples, mp, ptr(f), ptr(ln), len(windows), out.ptr, stream))
+        else:
+            check(L.hm_fft_tunes_rows(hist.ptr, n, row_len, n_samples, mp, ptr(f), ptr(ln), len(windows), out.ptr,
+                                      stream))
     else:
-        check(lib().hm_birkhoff_tunes_history(hist.ptr, n, n_samples, mp, ptr(f), ptr(t), len(windows), out.ptr,
+        if row_len is None:
+            check(L.hm_birkhoff_tunes_history(hist.ptr, n, n_samples, mp, ptr(f), ptr(t), len(windows), out.ptr,
                                               stream))
+        else:
+            check(L.hm_birkhoff_tunes_rows(hist.ptr, n, row_len, n_samples, mp, ptr(f), ptr(t), len(windows),
+                                           out.ptr, stream))
     return out.download(np.float64, n * 2 * len(windows)).reshape(len(windows), 2, n)

@@ -585,7 +585,10 @@ class HenonTracker {
         ff.back() = 1;
         fl.push_back(L - 1);
         const size_t NW = bf.size();
-        const size_t per = (size_t)L * 32;
+        // the trajectory as particle rows (row_len a multiple of 16 samples):
+        // the FFT reads each particle's samples in place
+        const size_t row_len = ((size_t)L + 15) / 16 * 16;
+        const size_t per = row_len * 32;
         const size_t budget = (size_t)8 << 30;
         const size_t chunk = std::max<size_t>(1, std::min<size_t>(p.n_particles, budget / per));
         std::vector<double> birk(p.n_particles * 2 * NW), fft(all ? p.n_particles * 2 * NW : 0);
@@ -594,18 +597,20 @@ class HenonTracker {
             py::gil_scoped_release nogil;
             for (size_t c0 = 0; c0 < p.n_particles; c0 += chunk) {
                 const size_t nc = std::min(chunk, p.n_particles - c0);
-                check(hm_track_history(p.dx.as<double>() + c0, p.dpx.as<double>() + c0, p.dy.as<double>() + c0,
-                                       p.dpy.as<double>() + c0, p.dsteps.as<uint32_t>() + c0, nc, dtable.as<double>(),
-                                       allowed_steps, n_turns, mu, barrier, inverse ? 1 : 0, hist.as<double>(), nullptr));
+                check(hm_track_rows(p.dx.as<double>() + c0, p.dpx.as<double>() + c0, p.dy.as<double>() + c0,
+                                    p.dpy.as<double>() + c0, p.dsteps.as<uint32_t>() + c0, nc, dtable.as<double>(),
+                                    allowed_steps, n_turns, mu, barrier, inverse ? 1 : 0, hist.as<double>(), row_len,
+                                    nullptr));
                 const double *mp = nullptr;
                 if (all) {
-                    check(hm_history_means(hist.as<double>(), nc, L, means.as<double>(), nullptr));
+                    check(hm_rows_means(hist.as<double>(), nc, row_len, L, means.as<double>(), nullptr));
                     mp = means.as<double>();
                 }
-                check(hm_birkhoff_tunes_history(hist.as<double>(), nc, L, mp, bf.data(), bt.data(), NW,
-                                                ob.as<double>(), nullptr));
+                check(hm_birkhoff_tunes_rows(hist.as<double>(), nc, row_len, L, mp, bf.data(), bt.data(), NW,
+                                             ob.as<double>(), nullptr));
                 if (all)
-                    check(hm_fft_tunes(hist.as<double>(), nc, L, mp, ff.data(), fl.data(), NW, of.as<double>(), nullptr));
+                    check(hm_fft_tunes_rows(hist.as<double>(), nc, row_len, L, mp, ff.data(), fl.data(), NW,
+                                            of.as<double>(), nullptr));
                 auto vb = ob.download<double>(nc * 2 * NW);
                 for (size_t w = 0; w < NW; w++)
                     for (int pl = 0; pl < 2; pl++)

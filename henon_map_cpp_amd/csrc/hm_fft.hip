@@ -5,8 +5,12 @@
 //  * k_birkhoff_hist  : Birkhoff tunes of windows of the stored samples
 //  * k_fft_tune_pow2  : Hanning-windowed FFT tune, power-of-two length, radix-2 in LDS
 //  * k_fft_tune_dft   : same for any length, direct DFT with exactly reduced twiddles
-// History layout: hist[sample][coord][particle] (coord = x, px, y, py), so a
-// tracking wavefront writes one contiguous row per turn.
+// Two history layouts, addressed through strides (HistLayout):
+//  * sample-major hist[sample][coord][particle] (coord = x, px, y, py): a
+//    tracking wavefront writes one contiguous row per turn (k_history);
+//  * particle rows rows[particle][coord][L]: each particle's samples contiguous
+//    (k_history_rows buffers 16 turns in registers so that every store
+//    completes 128-byte lines), which the FFT reads without a corner turn.
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdint>
@@ -21,6 +25,17 @@
 namespace hm {
 
 constexpr int kMaxWindows = 64;
+
+// Element (sample m, coordinate c, particle p) of a history at m*ms + c*cs + p*ps.
+struct HistLayout {
+    uint64_t ms, cs, ps;
+};
+__host__ __device__ inline HistLayout sample_major(uint64_t n) { return HistLayout{4 * n, n, 1}; }
+__host__ __device__ inline HistLayout particle_rows(uint64_t L) { return HistLayout{1, L, 4 * L}; }
+__device__ __forceinline__ size_t hat(const HistLayout &h, uint64_t m, uint64_t c, uint64_t p)
+{
+    return (size_t)(m * h.ms + c * h.cs + p * h.ps);
+}
 
 // ---------------------------------------------------------------- history
 template <bool INV, bool OCT>
@@ -60,15 +75,67 @@ k_history(double *__restrict__ gx, double *__restrict__ gpx, double *__restrict_
     gx[i] = x; gpx[i] = px; gy[i] = y; gpy[i] = py; gsteps[i] = s;
 }
 
+// Same tracking and samples as k_history, written as particle rows
+// rows[particle][coord][L] (L a multiple of 16, samples 0..n_turns, the tail of
+// the last block padded with the final state): 16 samples per coordinate are
+// kept in registers and stored as whole 128-byte lines.
+template <bool INV, bool OCT>
+__global__ void __launch_bounds__(256)
+k_history_rows(double *__restrict__ gx, double *__restrict__ gpx, double *__restrict__ gy, double *__restrict__ gpy,
+               uint32_t *__restrict__ gsteps, uint32_t n, const TabEntry *__restrict__ tab, uint32_t tab_len,
+               uint32_t n_turns, double mu, double b2, double *__restrict__ rows, uint64_t L,
+               uint32_t *__restrict__ status)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x = gx[i], px = gpx[i], y = gy[i], py = gpy[i];
+    uint32_t s = gsteps[i];
+    bool alive = !any_nan(x, px, y, py);
+    const double m3mu = -3.0 * mu;
+    double *row = rows + (size_t)i * 4 * L;
+    const uint32_t samples = n_turns + 1;
+    for (uint32_t m0 = 0; m0 < samples; m0 += 16) {
+        double buf[4][16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const uint32_t m = m0 + q;
+            if (m > 0 && m < samples && alive) {
+                const int64_t ti = INV ? (int64_t)s - 1 : (int64_t)s;
+                if (ti < 0 || ti >= (int64_t)tab_len) {
+                    atomicOr(status, 1u);
+                    alive = false;
+                } else {
+                    map_step<kExact, INV, OCT>(x, px, y, py, mu, m3mu, tab[ti]);
+                    if (radius2<kExact>(x, px, y, py) < b2) {
+                        s = INV ? s - 1 : s + 1;
+                    } else {
+                        x = px = y = py = __builtin_nan("");
+                        alive = false;
+                    }
+                }
+            }
+            buf[0][q] = x; buf[1][q] = px; buf[2][q] = y; buf[3][q] = py;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            double2 *dst = reinterpret_cast<double2 *>(row + c * L + m0);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) dst[q] = double2{buf[c][2 * q], buf[c][2 * q + 1]};
+        }
+    }
+    gx[i] = x; gpx[i] = px; gy[i] = y; gpy[i] = py; gsteps[i] = s;
+}
+
 // std::accumulate order (henon.cu:3041-3051): sum from 0.0 in sample order, / L
-__global__ void k_means(const double *__restrict__ hist, uint32_t n, uint32_t L, double *__restrict__ means)
+__global__ void k_means(const double *__restrict__ hist, HistLayout h, uint32_t n, uint32_t L,
+                        double *__restrict__ means)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const size_t nn = n;
     for (int c = 0; c < 4; c++) {
         double s = 0.0;
-        for (uint32_t m = 0; m < L; m++) s = s + hist[(size_t)m * 4 * nn + c * nn + i];
+        for (uint32_t m = 0; m < L; m++) s = s + hist[hat(h, m, c, i)];
         means[c * nn + i] = s / (double)L;
     }
 }
@@ -84,8 +151,9 @@ struct WindowSet {
 // (dynamic_indicator.cpp:228-241). out: [window][plane][n].
 // One thread per (particle, plane, window): enough wavefronts to hide the
 // latency of the per-sample atan2 chains.
-__global__ void k_birkhoff_hist(const double *__restrict__ hist, uint32_t n, const double *__restrict__ means,
-                                WindowSet ws, const double *__restrict__ w_flat, double *__restrict__ out)
+__global__ void k_birkhoff_hist(const double *__restrict__ hist, HistLayout h, uint32_t n,
+                                const double *__restrict__ means, WindowSet ws, const double *__restrict__ w_flat,
+                                double *__restrict__ out)
 {
     const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t i = (uint32_t)(idx % n);
@@ -100,9 +168,9 @@ __global__ void k_birkhoff_hist(const double *__restrict__ hist, uint32_t n, con
     double acc = 0.0, th_prev = 0.0;
     bool bad = false;
     for (uint32_t m = 0; m < ws.len[w]; m++) {
-        const size_t o = (size_t)(ws.from[w] + m) * 4 * nn;
-        const double a0 = hist[o + 0 * nn + i], a1 = hist[o + 1 * nn + i];
-        const double a2 = hist[o + 2 * nn + i], a3 = hist[o + 3 * nn + i];
+        const uint64_t sm = ws.from[w] + m;
+        const double a0 = hist[hat(h, sm, 0, i)], a1 = hist[hat(h, sm, 1, i)];
+        const double a2 = hist[hat(h, sm, 2, i)], a3 = hist[hat(h, sm, 3, i)];
         // any NaN among the four coordinates of the window
         bad |= any_nan(a0, a1, a2, a3);
         const double xv = (p ? a2 : a0) - mx;
@@ -121,6 +189,7 @@ __global__ void k_birkhoff_hist(const double *__restrict__ hist, uint32_t n, con
 // ---------------------------------------------------------------- FFT tune
 struct FftArgs {
     const double *hist;
+    HistLayout lay;
     uint32_t n;            // particles
     const double *means;   // [4][n] or NULL
     uint32_t from[kMaxWindows];
@@ -218,12 +287,12 @@ __global__ void __launch_bounds__(256) k_fft_tune_pow2(FftArgs a)
     __syncthreads();
     bool nan_seen = false;
     for (uint32_t m = threadIdx.x; m < N; m += blockDim.x) {
-        const size_t o = (size_t)(a.from[window] + m) * 4 * nn;
-        for (int c = 0; c < 4; c++) nan_seen |= __builtin_isnan(a.hist[o + c * nn + particle]);
+        const uint64_t sm = a.from[window] + m;
+        for (int c = 0; c < 4; c++) nan_seen |= __builtin_isnan(a.hist[hat(a.lay, sm, c, particle)]);
         const double h = a.hann[m];
         const uint32_t r = __brev(m) >> (32 - logn);
-        re[r] = (a.hist[o + cx * nn + particle] - mx) * h;
-        im[r] = (a.hist[o + cp * nn + particle] - mp) * h;
+        re[r] = (a.hist[hat(a.lay, sm, cx, particle)] - mx) * h;
+        im[r] = (a.hist[hat(a.lay, sm, cp, particle)] - mp) * h;
     }
     if (nan_seen) atomicOr(&any_nan_flag, 1);
     __syncthreads();
@@ -290,11 +359,14 @@ __global__ void __launch_bounds__(256) k_signal_rows(const double *__restrict__ 
 }
 
 struct FftRowArgs {
-    const double *rows;  // [nb][4][N], means already removed
+    const double *rows;    // signal of block b, coordinate c, sample m at
+                           // rows[(row0 + b) * pstride + c * cstride + from + m]
+    uint64_t pstride, cstride, row0, from;
+    const double *means;   // [4][n] subtracted here, or NULL (already removed)
     uint32_t nb, p0, n, N;
-    const double *hann;  // [N]
-    const double *tw;    // [N] complex (cos, -sin)
-    double *out;         // [window][plane][n]
+    const double *hann;    // [N]
+    const double *tw;      // [N] complex (cos, -sin)
+    double *out;           // [window][plane][n]
     uint32_t window;
 };
 
@@ -312,13 +384,17 @@ __global__ void __launch_bounds__(256) k_fft_tune_rows(FftRowArgs a)
     const uint32_t b = blockIdx.x;
     if (b >= a.nb) return;
     const uint32_t logn = 31 - __builtin_clz(N);
-    const double *rx = a.rows + (size_t)b * 4 * N, *rpx = rx + N, *ry = rx + 2 * N, *rpy = rx + 3 * N;
+    const double *rx = a.rows + (size_t)((a.row0 + b) * a.pstride + a.from), *rpx = rx + a.cstride,
+                 *ry = rx + 2 * a.cstride, *rpy = rx + 3 * a.cstride;
+    const size_t nn0 = a.n, pp = a.p0 + b;
+    const double m0 = a.means ? a.means[pp] : 0.0, m1 = a.means ? a.means[nn0 + pp] : 0.0;
+    const double m2 = a.means ? a.means[2 * nn0 + pp] : 0.0, m3 = a.means ? a.means[3 * nn0 + pp] : 0.0;
     if (threadIdx.x == 0) any_nan_flag = 0;
     __syncthreads();
     bool nan_seen = false;
     for (uint32_t m = threadIdx.x; m < N; m += blockDim.x) {
         const double h = a.hann[m];
-        const double vx = rx[m], vpx = rpx[m], vy = ry[m], vpy = rpy[m];
+        const double vx = rx[m] - m0, vpx = rpx[m] - m1, vy = ry[m] - m2, vpy = rpy[m] - m3;
         nan_seen |= any_nan(vx, vpx, vy, vpy);
         const uint32_t r = __brev(m) >> (32 - logn);
         re0[r] = vx * h; im0[r] = vpx * h;
@@ -405,11 +481,11 @@ __global__ void __launch_bounds__(256) k_fft_tune_dft(FftArgs a, double *__restr
         __syncthreads();
         bool nan_seen = false;
         for (uint32_t m = threadIdx.x; m < cn; m += blockDim.x) {
-            const size_t o = (size_t)(a.from[window] + c0 + m) * 4 * nn;
-            for (int c = 0; c < 4; c++) nan_seen |= __builtin_isnan(a.hist[o + c * nn + particle]);
+            const uint64_t sm = a.from[window] + c0 + m;
+            for (int c = 0; c < 4; c++) nan_seen |= __builtin_isnan(a.hist[hat(a.lay, sm, c, particle)]);
             const double h = a.hann[c0 + m];
-            zr[m] = (a.hist[o + cx * nn + particle] - mx) * h;
-            zi[m] = (a.hist[o + cp * nn + particle] - mp) * h;
+            zr[m] = (a.hist[hat(a.lay, sm, cx, particle)] - mx) * h;
+            zi[m] = (a.hist[hat(a.lay, sm, cp, particle)] - mp) * h;
         }
         if (nan_seen) atomicOr(&any_nan_flag, 1);
         __syncthreads();
@@ -564,16 +640,93 @@ extern "C" int hm_track_history(double *x, double *px, double *y, double *py, ui
     });
 }
 
-extern "C" int hm_history_means(const double *hist, size_t n, uint32_t n_samples, double *means, void *stream)
+extern "C" int hm_track_rows(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n,
+                             const double *table, size_t table_len, uint32_t n_turns, double mu, double barrier,
+                             int inverse, double *rows, size_t row_len, void *stream)
 {
     return guarded([&] {
         if (n == 0) return;
-        if (!hist || !means || n_samples == 0) throw Error(HM_ERR_ARG, "means: bad arguments");
+        if (!x || !px || !y || !py || !steps || !table || !rows) throw Error(HM_ERR_ARG, "history: NULL pointer");
+        if (row_len % 16 != 0 || row_len < (size_t)n_turns + 1)
+            throw Error(HM_ERR_ARG, "history rows: row_len must be a multiple of 16 of at least n_turns + 1");
+        if (reinterpret_cast<uintptr_t>(rows) % 16 != 0) throw Error(HM_ERR_ARG, "history rows: rows not 16-byte aligned");
+        if (n > UINT32_MAX) throw Error(HM_ERR_RANGE, "history rows: too many particles");
         hipStream_t s = as_stream(stream);
-        hipLaunchKernelGGL(k_means, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, hist, (uint32_t)n, n_samples,
-                           means);
-        launch_check("k_means");
+        const uint32_t grid = (uint32_t)((n + 255) / 256);
+        const double b2 = barrier * barrier;
+        const TabEntry *tab = reinterpret_cast<const TabEntry *>(table);
+        LaunchTimer timer("k_history_rows", s);
+        const bool oct = mu != 0.0;
+        const uint64_t L = row_len;
+        if (inverse) {
+            if (oct) hipLaunchKernelGGL((k_history_rows<true, true>), dim3(grid), dim3(256), 0, s, x, px, y, py, steps,
+                                        (uint32_t)n, tab, (uint32_t)table_len, n_turns, mu, b2, rows, L,
+                                        status_word());
+            else hipLaunchKernelGGL((k_history_rows<true, false>), dim3(grid), dim3(256), 0, s, x, px, y, py, steps,
+                                    (uint32_t)n, tab, (uint32_t)table_len, n_turns, mu, b2, rows, L, status_word());
+        } else {
+            if (oct) hipLaunchKernelGGL((k_history_rows<false, true>), dim3(grid), dim3(256), 0, s, x, px, y, py,
+                                        steps, (uint32_t)n, tab, (uint32_t)table_len, n_turns, mu, b2, rows, L,
+                                        status_word());
+            else hipLaunchKernelGGL((k_history_rows<false, false>), dim3(grid), dim3(256), 0, s, x, px, y, py, steps,
+                                    (uint32_t)n, tab, (uint32_t)table_len, n_turns, mu, b2, rows, L, status_word());
+        }
+        launch_check("k_history_rows");
     });
+}
+
+static void means_impl(const double *hist, HistLayout lay, size_t n, uint32_t n_samples, double *means,
+                       hipStream_t s)
+{
+    if (n == 0) return;
+    if (!hist || !means || n_samples == 0) throw Error(HM_ERR_ARG, "means: bad arguments");
+    hipLaunchKernelGGL(k_means, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, hist, lay, (uint32_t)n, n_samples,
+                       means);
+    launch_check("k_means");
+}
+
+extern "C" int hm_history_means(const double *hist, size_t n, uint32_t n_samples, double *means, void *stream)
+{
+    return guarded([&] { means_impl(hist, sample_major(n), n, n_samples, means, as_stream(stream)); });
+}
+
+extern "C" int hm_rows_means(const double *rows, size_t n, size_t row_len, uint32_t n_samples, double *means,
+                             void *stream)
+{
+    return guarded([&] {
+        if (n_samples > row_len) throw Error(HM_ERR_RANGE, "means: more samples than row_len");
+        means_impl(rows, particle_rows(row_len), n, n_samples, means, as_stream(stream));
+    });
+}
+
+static void birkhoff_impl(const double *hist, HistLayout lay, size_t n, uint32_t n_samples, const double *means,
+                          const uint32_t *from, const uint32_t *to, size_t n_windows, double *out, hipStream_t s)
+{
+    if (n == 0 || n_windows == 0) return;
+    if (!hist || !out || !from || !to) throw Error(HM_ERR_ARG, "birkhoff: NULL pointer");
+    for (size_t w0 = 0; w0 < n_windows; w0 += kMaxWindows) {
+        WindowSet ws{};
+        ws.count = (uint32_t)std::min<size_t>(kMaxWindows, n_windows - w0);
+        size_t total = 0;
+        for (uint32_t w = 0; w < ws.count; w++) {
+            const uint32_t f = from[w0 + w], t = to[w0 + w];
+            if (f >= n_samples || t >= n_samples) throw Error(HM_ERR_RANGE, "compute_tune: from and to must be valid indices");
+            if (t < f + 1) throw Error(HM_ERR_ARG, "birkhoff: a window needs at least two samples");
+            ws.from[w] = f;
+            ws.len[w] = t - f + 1;
+            ws.w_off[w] = (uint32_t)total;
+            total += ws.len[w] - 1;
+        }
+        std::vector<double> wts(total);
+        for (uint32_t w = 0; w < ws.count; w++) host_birkhoff_weights(ws.len[w] - 1, wts.data() + ws.w_off[w]);
+        double *dw = static_cast<double *>(workspace(std::max<size_t>(total, 1) * sizeof(double), 2));
+        hip_check(hipMemcpyAsync(dw, wts.data(), total * sizeof(double), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+        LaunchTimer timer("k_birkhoff_hist", s);
+        hipLaunchKernelGGL(k_birkhoff_hist, dim3((uint32_t)((n * 2 * ws.count + 255) / 256)), dim3(256), 0, s, hist, lay,
+                           (uint32_t)n, means, ws, (const double *)dw, out + w0 * 2 * n);
+        launch_check("k_birkhoff_hist");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    }
 }
 
 extern "C" int hm_birkhoff_tunes_history(const double *hist, size_t n, uint32_t n_samples, const double *means,
@@ -581,69 +734,65 @@ extern "C" int hm_birkhoff_tunes_history(const double *hist, size_t n, uint32_t 
                                          void *stream)
 {
     return guarded([&] {
-        if (n == 0 || n_windows == 0) return;
-        if (!hist || !out || !from || !to) throw Error(HM_ERR_ARG, "birkhoff: NULL pointer");
-        hipStream_t s = as_stream(stream);
-        for (size_t w0 = 0; w0 < n_windows; w0 += kMaxWindows) {
-            WindowSet ws{};
-            ws.count = (uint32_t)std::min<size_t>(kMaxWindows, n_windows - w0);
-            size_t total = 0;
-            for (uint32_t w = 0; w < ws.count; w++) {
-                const uint32_t f = from[w0 + w], t = to[w0 + w];
-                if (f >= n_samples || t >= n_samples) throw Error(HM_ERR_RANGE, "compute_tune: from and to must be valid indices");
-                if (t < f + 1) throw Error(HM_ERR_ARG, "birkhoff: a window needs at least two samples");
-                ws.from[w] = f;
-                ws.len[w] = t - f + 1;
-                ws.w_off[w] = (uint32_t)total;
-                total += ws.len[w] - 1;
-            }
-            std::vector<double> wts(total);
-            for (uint32_t w = 0; w < ws.count; w++) host_birkhoff_weights(ws.len[w] - 1, wts.data() + ws.w_off[w]);
-            double *dw = static_cast<double *>(workspace(std::max<size_t>(total, 1) * sizeof(double), 2));
-            hip_check(hipMemcpyAsync(dw, wts.data(), total * sizeof(double), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
-            LaunchTimer timer("k_birkhoff_hist", s);
-            hipLaunchKernelGGL(k_birkhoff_hist, dim3((uint32_t)((n * 2 * ws.count + 255) / 256)), dim3(256), 0, s, hist,
-                               (uint32_t)n, means, ws, (const double *)dw, out + w0 * 2 * n);
-            launch_check("k_birkhoff_hist");
-            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-        }
+        birkhoff_impl(hist, sample_major(n), n, n_samples, means, from, to, n_windows, out, as_stream(stream));
     });
 }
 
-extern "C" int hm_fft_tunes(const double *hist, size_t n, uint32_t n_samples, const double *means,
-                            const uint32_t *from, const uint32_t *len, size_t n_windows, double *out, void *stream)
+extern "C" int hm_birkhoff_tunes_rows(const double *rows, size_t n, size_t row_len, uint32_t n_samples,
+                                      const double *means, const uint32_t *from, const uint32_t *to, size_t n_windows,
+                                      double *out, void *stream)
 {
     return guarded([&] {
-        if (n == 0 || n_windows == 0) return;
-        if (!hist || !out || !from || !len) throw Error(HM_ERR_ARG, "fft_tunes: NULL pointer");
-        if (n > 0x3fffffffull) throw Error(HM_ERR_ARG, "fft_tunes: too many particles");
-        hipStream_t s = as_stream(stream);
-        // group windows by length: one launch per (length, up to 64 windows)
-        std::map<uint32_t, std::vector<uint32_t>> by_len;
-        for (size_t w = 0; w < n_windows; w++) {
-            if (len[w] < 2) throw Error(HM_ERR_ARG, "fft_tunes: a window needs at least two samples");
-            if ((uint64_t)from[w] + len[w] > n_samples)
-                throw Error(HM_ERR_RANGE, "compute_tune: from and to must be valid indices");
-            by_len[len[w]].push_back((uint32_t)w);
-        }
-        for (auto &kv : by_len) {
-            const uint32_t N = kv.first;
-            const FftTables t = fft_tables(N, s);
-            const bool pow2 = (N & (N - 1)) == 0 && N <= 8192;
-            if (!pow2 && N > 64 * 256) throw Error(HM_ERR_UNSUPPORTED, "fft_tunes: window longer than 16384 samples");
-            const auto &ws = kv.second;
-            for (size_t b0 = 0; b0 < ws.size();) {
-                // consecutive output windows only (one launch writes a contiguous window range)
-                size_t b1 = b0 + 1;
-                while (b1 < ws.size() && b1 - b0 < (size_t)kMaxWindows && ws[b1] == ws[b1 - 1] + 1) ++b1;
-                FftArgs a{};
-                a.hist = hist; a.n = (uint32_t)n; a.means = means; a.count = (uint32_t)(b1 - b0); a.N = N;
-                a.hann = t.hann; a.tw = t.tw; a.out = out; a.out_window0 = ws[b0];
-                for (size_t q = b0; q < b1; q++) a.from[q - b0] = from[ws[q]];
-                const uint64_t signals = (uint64_t)n * 2 * a.count;
-                const uint32_t blocks = (uint32_t)(((signals + 7) / 8) * 8);
-                const size_t rows_lds = 4 * (size_t)N * sizeof(double) + kRowsStaticLds;
-                if (pow2 && N <= kRowsMaxN && rows_lds <= lds_per_block()) {
+        if (n_samples > row_len) throw Error(HM_ERR_RANGE, "birkhoff: more samples than row_len");
+        birkhoff_impl(rows, particle_rows(row_len), n, n_samples, means, from, to, n_windows, out, as_stream(stream));
+    });
+}
+
+// rows_input: hist is particle rows (row_len = lay.cs); else sample-major.
+static void fft_impl(const double *hist, HistLayout lay, bool rows_input, size_t n, uint32_t n_samples,
+                     const double *means, const uint32_t *from, const uint32_t *len, size_t n_windows, double *out,
+                     hipStream_t s)
+{
+    if (n == 0 || n_windows == 0) return;
+    if (!hist || !out || !from || !len) throw Error(HM_ERR_ARG, "fft_tunes: NULL pointer");
+    if (n > 0x3fffffffull) throw Error(HM_ERR_ARG, "fft_tunes: too many particles");
+    // group windows by length: one launch per (length, up to 64 windows)
+    std::map<uint32_t, std::vector<uint32_t>> by_len;
+    for (size_t w = 0; w < n_windows; w++) {
+        if (len[w] < 2) throw Error(HM_ERR_ARG, "fft_tunes: a window needs at least two samples");
+        if ((uint64_t)from[w] + len[w] > n_samples)
+            throw Error(HM_ERR_RANGE, "compute_tune: from and to must be valid indices");
+        by_len[len[w]].push_back((uint32_t)w);
+    }
+    for (auto &kv : by_len) {
+        const uint32_t N = kv.first;
+        const FftTables t = fft_tables(N, s);
+        const bool pow2 = (N & (N - 1)) == 0 && N <= 8192;
+        if (!pow2 && N > 64 * 256) throw Error(HM_ERR_UNSUPPORTED, "fft_tunes: window longer than 16384 samples");
+        const auto &ws = kv.second;
+        for (size_t b0 = 0; b0 < ws.size();) {
+            // consecutive output windows only (one launch writes a contiguous window range)
+            size_t b1 = b0 + 1;
+            while (b1 < ws.size() && b1 - b0 < (size_t)kMaxWindows && ws[b1] == ws[b1 - 1] + 1) ++b1;
+            FftArgs a{};
+            a.hist = hist; a.lay = lay; a.n = (uint32_t)n; a.means = means; a.count = (uint32_t)(b1 - b0); a.N = N;
+            a.hann = t.hann; a.tw = t.tw; a.out = out; a.out_window0 = ws[b0];
+            for (size_t q = b0; q < b1; q++) a.from[q - b0] = from[ws[q]];
+            const uint64_t signals = (uint64_t)n * 2 * a.count;
+            const uint32_t blocks = (uint32_t)(((signals + 7) / 8) * 8);
+            const size_t rows_lds = 4 * (size_t)N * sizeof(double) + kRowsStaticLds;
+            if (pow2 && N <= kRowsMaxN && rows_lds <= lds_per_block()) {
+                allow_dynamic_lds((const void *)k_fft_tune_rows, 4 * N * sizeof(double));
+                if (rows_input) {
+                    // one workgroup per particle, reading its rows in place
+                    for (size_t q = b0; q < b1; q++) {
+                        FftRowArgs r{hist, lay.ps, lay.cs, 0, from[ws[q]], means, (uint32_t)n, 0, (uint32_t)n, N,
+                                     t.hann, t.tw, out, ws[q]};
+                        LaunchTimer timer("k_fft_tune_rows", s);
+                        hipLaunchKernelGGL(k_fft_tune_rows, dim3((uint32_t)n), dim3(256), 4 * N * sizeof(double), s, r);
+                        launch_check("k_fft_tune_rows");
+                    }
+                } else {
                     // per window: signal rows for a batch of particles (corner
                     // turn), then one workgroup per particle reading them
                     const uint64_t row_bytes = 4ull * N * sizeof(double);
@@ -659,27 +808,46 @@ extern "C" int hm_fft_tunes(const double *hist, size_t n, uint32_t n_samples, co
                                                    s, hist, (uint32_t)n, means, from[ws[q]], N, p0, nb, rows);
                                 launch_check("k_signal_rows");
                             }
-                            FftRowArgs r{rows, nb, p0, (uint32_t)n, N, t.hann, t.tw, out, ws[q]};
-                            allow_dynamic_lds((const void *)k_fft_tune_rows, 4 * N * sizeof(double));
-                            LaunchTimer timer("k_fft_tune_pow2", s);
+                            FftRowArgs r{rows, 4ull * N, N, 0, 0, nullptr, nb, p0, (uint32_t)n, N, t.hann, t.tw, out,
+                                         ws[q]};
+                            LaunchTimer timer("k_fft_tune_rows", s);
                             hipLaunchKernelGGL(k_fft_tune_rows, dim3(nb), dim3(256), 4 * N * sizeof(double), s, r);
                             launch_check("k_fft_tune_rows");
                         }
                     }
-                } else if (pow2 && 2 * (size_t)N * sizeof(double) + kRowsStaticLds <= lds_per_block()) {
-                    // longer windows: one signal per workgroup (2 N doubles of LDS)
-                    allow_dynamic_lds((const void *)k_fft_tune_pow2, 2 * N * sizeof(double));
-                    LaunchTimer timer("k_fft_tune_pow2", s);
-                    hipLaunchKernelGGL(k_fft_tune_pow2, dim3(blocks), dim3(256), 2 * N * sizeof(double), s, a);
-                    launch_check("k_fft_tune_pow2");
-                } else {
-                    double *mag = static_cast<double *>(workspace((size_t)blocks * N * sizeof(double), 3));
-                    LaunchTimer timer("k_fft_tune_dft", s);
-                    hipLaunchKernelGGL(k_fft_tune_dft, dim3(blocks), dim3(256), 0, s, a, mag);
-                    launch_check("k_fft_tune_dft");
                 }
-                b0 = b1;
+            } else if (pow2 && 2 * (size_t)N * sizeof(double) + kRowsStaticLds <= lds_per_block()) {
+                // longer windows: one signal per workgroup (2 N doubles of LDS)
+                allow_dynamic_lds((const void *)k_fft_tune_pow2, 2 * N * sizeof(double));
+                LaunchTimer timer("k_fft_tune_pow2", s);
+                hipLaunchKernelGGL(k_fft_tune_pow2, dim3(blocks), dim3(256), 2 * N * sizeof(double), s, a);
+                launch_check("k_fft_tune_pow2");
+            } else {
+                double *mag = static_cast<double *>(workspace((size_t)blocks * N * sizeof(double), 3));
+                LaunchTimer timer("k_fft_tune_dft", s);
+                hipLaunchKernelGGL(k_fft_tune_dft, dim3(blocks), dim3(256), 0, s, a, mag);
+                launch_check("k_fft_tune_dft");
             }
+            b0 = b1;
         }
+    }
+}
+
+extern "C" int hm_fft_tunes(const double *hist, size_t n, uint32_t n_samples, const double *means,
+                            const uint32_t *from, const uint32_t *len, size_t n_windows, double *out, void *stream)
+{
+    return guarded([&] {
+        fft_impl(hist, sample_major(n), false, n, n_samples, means, from, len, n_windows, out, as_stream(stream));
+    });
+}
+
+extern "C" int hm_fft_tunes_rows(const double *rows, size_t n, size_t row_len, uint32_t n_samples,
+                                 const double *means, const uint32_t *from, const uint32_t *len, size_t n_windows,
+                                 double *out, void *stream)
+{
+    return guarded([&] {
+        if (n_samples > row_len) throw Error(HM_ERR_RANGE, "fft_tunes: more samples than row_len");
+        fft_impl(rows, particle_rows(row_len), true, n, n_samples, means, from, len, n_windows, out,
+                 as_stream(stream));
     });
 }

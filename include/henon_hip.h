@@ -353,6 +353,23 @@ int hm_birkhoff_tunes_history(const double *hist, size_t n, uint32_t n_samples, 
 int hm_fft_tunes(const double *hist, size_t n, uint32_t n_samples, const double *means, const uint32_t *from,
                  const uint32_t *len, size_t n_windows, double *out, void *stream);
 
+/* The same trajectory store as particle rows: rows[particle][4][row_len]
+ * (x, px, y, py), samples 0..n_turns at row positions 0..n_turns; row_len a
+ * multiple of 16 of at least n_turns+1 (positions past n_turns hold the final
+ * state), rows 16-byte aligned. Each particle's samples are contiguous, so the
+ * tune kernels below read them in place (no corner turn); every store
+ * completes 128-byte lines. */
+int hm_track_rows(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n, const double *table,
+                  size_t table_len, uint32_t n_turns, double mu, double barrier, int inverse, double *rows,
+                  size_t row_len, void *stream);
+/* hm_history_means, hm_birkhoff_tunes_history and hm_fft_tunes on particle rows
+ * (same results bit for bit); n_samples <= row_len. */
+int hm_rows_means(const double *rows, size_t n, size_t row_len, uint32_t n_samples, double *means, void *stream);
+int hm_birkhoff_tunes_rows(const double *rows, size_t n, size_t row_len, uint32_t n_samples, const double *means,
+                           const uint32_t *from, const uint32_t *to, size_t n_windows, double *out, void *stream);
+int hm_fft_tunes_rows(const double *rows, size_t n, size_t row_len, uint32_t n_samples, const double *means,
+                      const uint32_t *from, const uint32_t *len, size_t n_windows, double *out, void *stream);
+
 /* ---- reverse-mode derivatives ---------------------------------------------- */
 /* No counterpart in the reference, whose only derivative is the one-turn tangent
  * matrix (src/henon.cu:175-265); these extend it to whole-orbit gradients.

@@ -131,3 +131,32 @@ def test_fft_tunes_in_particle_batches(monkeypatch):
             assert np.isnan(got[0, 0, i]) and np.isnan(got[0, 1, i])
         else:
             np.testing.assert_allclose(got[0, :, i], want, atol=1e-12)
+
+
+def test_rows_history_and_tunes_equal_sample_major():
+    """The particle-row trajectory store (hm_track_rows) holds the sample-major
+    history transposed, bit for bit (padding = the final state), and every tune
+    kernel gives the same result from either layout: the in-place row FFT
+    (2048), the one-signal FFT (8192), the DFT (501), Birkhoff windows, with and
+    without the mean removal."""
+    n_turns, n = 8300, 257
+    x, px, y, py = W.ball(n, 0.35, seed=34)
+    wx, wy = C.sps_modulation(0.168, 8.0, 0, n_turns), C.sps_modulation(0.201, 8.0, 0, n_turns)
+    tdev = C.DeviceBuffer.from_array(C.build_table(wx, wy))
+    d1, d2 = C.DeviceParticles(x, px, y, py), C.DeviceParticles(x, px, y, py)
+    hist = C.track_history(d1, tdev, n_turns, n_turns, 0.3, 10.0)
+    rows = C.track_rows(d2, tdev, n_turns, n_turns, 0.3, 10.0)
+    L = C.row_length(n_turns)
+    assert L % 16 == 0 and L >= n_turns + 1
+    h = hist.download(np.float64, (n_turns + 1) * 4 * n).reshape(n_turns + 1, 4, n)
+    r = rows.download(np.float64, n * 4 * L).reshape(n, 4, L)
+    assert np.array_equal(r[:, :, :n_turns + 1], np.transpose(h, (2, 1, 0)), equal_nan=True)
+    assert np.array_equal(r[:, :, n_turns + 1:], np.repeat(r[:, :, n_turns:n_turns + 1], L - n_turns - 1, axis=2),
+                          equal_nan=True)
+    assert np.array_equal(d1.get()[4], d2.get()[4])
+    windows = [(0, 2047), (100, 2147), (0, 8191), (3, 503), (0, n_turns)]
+    for mean in (False, True):
+        for kind in ("fft", "birkhoff"):
+            a = C.history_tunes(hist, n, n_turns + 1, windows, kind, subtract_mean=mean)
+            b = C.history_tunes(rows, n, n_turns + 1, windows, kind, subtract_mean=mean, row_len=L)
+            assert np.array_equal(a, b, equal_nan=True), (kind, mean)
