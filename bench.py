@@ -229,8 +229,9 @@ def reversibility_leg():
 def tunes_leg():
     """Trajectory store + 2048-sample FFT tunes and Birkhoff tunes of the
     chaos-indicator particle set (256x256 grid + 65536 ball points): the
-    HBM-bound stages. Bytes: 32 B per particle-sample written by k_history and
-    read by each tune kernel."""
+    HBM-bound stages. The trajectory is stored as particle rows (hm_track_rows),
+    which the FFT reads in place. Bytes: 32 B per particle-sample written by the
+    store and read by each tune stage; every kernel of a stage is timed."""
     from henon_map_cpp_amd import _capi as C
     from henon_map_cpp_amd import workloads as W
 
@@ -240,20 +241,22 @@ def tunes_leg():
     tdev = C.DeviceBuffer.from_array(C.build_table(_lines(c["tune_x"], c["epsilon"], T),
                                                    _lines(c["tune_y"], c["epsilon"], T)))
     n = x0[0].size
+    L = C.row_length(T - 1)
     best = None
     for it in range(2):
         parts = C.DeviceParticles(*x0)
         C.check(C.lib().hm_stream_synchronize(None))
         C.profile_reset()
         C.profile_enable(True)
-        hist = C.track_history(parts, tdev, T, T - 1, c["mu"], c["barrier"])   # samples 0..2047
-        fft = C.history_tunes(hist, n, T, [(0, T - 1)], "fft")
-        birk = C.history_tunes(hist, n, T, [(0, T - 1)], "birkhoff")
-        h_ms = C.profile_read("k_history")[0]
-        f_ms = C.profile_read("k_fft_tune_pow2")[0]
+        rows = C.track_rows(parts, tdev, T, T - 1, c["mu"], c["barrier"])   # samples 0..2047
+        fft = C.history_tunes(rows, n, T, [(0, T - 1)], "fft", row_len=L)
+        birk = C.history_tunes(rows, n, T, [(0, T - 1)], "birkhoff", row_len=L)
+        h_ms = C.profile_read("k_history_rows")[0]
+        f_ms = sum(C.profile_read(k)[0] for k in ("k_signal_rows", "k_fft_tune_rows", "k_fft_tune_pow2",
+                                                  "k_fft_tune_dft"))
         b_ms = C.profile_read("k_birkhoff_hist")[0]
         C.profile_enable(False)
-        hist.free()
+        rows.free()
         if best is None or h_ms + f_ms + b_ms < sum(best):
             best = (h_ms, f_ms, b_ms)
     h_ms, f_ms, b_ms = best

@@ -186,6 +186,56 @@ __global__ void k_birkhoff_hist(const double *__restrict__ hist, HistLayout h, u
     out[((size_t)w * 2 + p) * nn + i] = bad ? __builtin_nan("") : 1 - acc / (2 * M_PI);
 }
 
+// Birkhoff tunes of particle rows: one wavefront per (particle, plane,
+// window); lane l takes samples m0 + l of each 64-sample step (coalesced along
+// the row), the angle of the sample before comes from the neighbouring lane
+// (from the previous step's lane 63 for lane 0), and the weighted wrapped
+// differences are summed per lane and then across the wavefront in a fixed
+// order. Same terms as k_birkhoff_hist, summed in that order instead of
+// sample order (within rounding of it).
+__global__ void __launch_bounds__(256) k_birkhoff_rows(const double *__restrict__ rows, uint64_t L, uint32_t n,
+                                                       const double *__restrict__ means, WindowSet ws,
+                                                       const double *__restrict__ w_flat, double *__restrict__ out)
+{
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t i = (uint32_t)(wave % n);
+    const uint64_t rest = wave / n;
+    const int p = (int)(rest % 2);
+    const uint32_t w = (uint32_t)(rest / 2);
+    if (w >= ws.count) return;  // whole wavefronts
+    const size_t nn = n;
+    const int cx = 2 * p, cp = 2 * p + 1;
+    const double mx = means ? means[cx * nn + i] : 0.0;
+    const double mp = means ? means[cp * nn + i] : 0.0;
+    const double *r = rows + (size_t)i * 4 * L + ws.from[w];
+    const uint32_t len = ws.len[w];
+    double acc = 0.0, carry = 0.0;
+    bool bad = false;
+    for (uint32_t m0 = 0; m0 < len; m0 += 64) {
+        const uint32_t m = m0 + lane;
+        const bool valid = m < len;
+        double th = 0.0;
+        if (valid) {
+            const double a0 = r[m], a1 = r[L + m], a2 = r[2 * L + m], a3 = r[3 * L + m];
+            bad |= any_nan(a0, a1, a2, a3);
+            th = atan2((p ? a3 : a1) - mp, (p ? a2 : a0) - mx);
+        }
+        double prev = __shfl_up(th, 1);
+        if (lane == 0) prev = carry;
+        carry = __shfl(th, 63);
+        if (valid && m > 0) {
+            double d = th - prev;
+            if (d < 0.0) d += 2.0 * M_PI;
+            acc = acc + w_flat[ws.w_off[w] + m - 1] * d;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc = acc + __shfl_xor(acc, off);
+    bad = __any(bad);
+    if (lane == 0) out[((size_t)w * 2 + p) * nn + i] = bad ? __builtin_nan("") : 1 - acc / (2 * M_PI);
+}
+
 // ---------------------------------------------------------------- FFT tune
 struct FftArgs {
     const double *hist;
@@ -449,6 +499,206 @@ __global__ void __launch_bounds__(256) k_fft_tune_rows(FftRowArgs a)
     block_argmax(best1, bi1, sv, si, bv, i1);
     if (threadIdx.x == 0) *o0 = interp_tune(re0, N, i0);
     if (threadIdx.x == 1) *o1 = interp_tune(re1, N, i1);
+}
+
+// ---------------------------------------------------------------- Stockham FFT
+// Self-sorting (Stockham) FFT of one signal per workgroup, radix-8 passes with
+// a final radix-2 or radix-4 pass (N = 2048: 8 8 8 4), butterflies in
+// registers, the signal in LDS between passes. Pass with radix R and span Ns
+// (the product of the earlier radices): butterfly j reads x[j + r N/R],
+// multiplies by w^(k r N/(Ns R)) (k = j mod Ns, w = exp(-2 pi i / N), from the
+// twiddle table), takes the R-point DFT and writes element r to
+// (j / Ns) Ns R + k + r Ns; after the last pass the spectrum is in natural
+// order.
+struct Cplx {
+    double re, im;
+};
+__device__ __forceinline__ Cplx cadd(Cplx a, Cplx b) { return Cplx{a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ Cplx csub(Cplx a, Cplx b) { return Cplx{a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ Cplx cmul(Cplx a, Cplx w) { return Cplx{a.re * w.re - a.im * w.im, a.re * w.im + a.im * w.re}; }
+__device__ __forceinline__ Cplx mul_mi(Cplx a) { return Cplx{a.im, -a.re}; }  // times -i
+
+// forward DFT of sizes 2, 4, 8 (exp(-2 pi i m k / R))
+__device__ __forceinline__ void dft2(Cplx *v)
+{
+    const Cplx a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+__device__ __forceinline__ void dft4(Cplx &a, Cplx &b, Cplx &c, Cplx &d)
+{
+    const Cplx s0 = cadd(a, c), s1 = csub(a, c), s2 = cadd(b, d), s3 = mul_mi(csub(b, d));
+    a = cadd(s0, s2);
+    c = csub(s0, s2);
+    b = cadd(s1, s3);
+    d = csub(s1, s3);
+}
+__device__ __forceinline__ void dft8(Cplx *v)
+{
+    // even and odd halves, then X[k] = E[k] + w8^k O[k], X[k+4] = E[k] - w8^k O[k]
+    Cplx e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    Cplx o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4(e0, e1, e2, e3);
+    dft4(o0, o1, o2, o3);
+    constexpr double c = 0.70710678118654752440;  // sqrt(2) / 2
+    const Cplx t1 = Cplx{c * (o1.re + o1.im), c * (o1.im - o1.re)};   // (1 - i)/sqrt2
+    const Cplx t2 = mul_mi(o2);                                        // -i
+    const Cplx t3 = Cplx{c * (o3.im - o3.re), -c * (o3.re + o3.im)};  // (-1 - i)/sqrt2
+    v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+    v[1] = cadd(e1, t1); v[5] = csub(e1, t1);
+    v[2] = cadd(e2, t2); v[6] = csub(e2, t2);
+    v[3] = cadd(e3, t3); v[7] = csub(e3, t3);
+}
+template <int R>
+__device__ __forceinline__ void dft_r(Cplx *v)
+{
+    if constexpr (R == 8) dft8(v);
+    else if constexpr (R == 4) dft4(v[0], v[1], v[2], v[3]);
+    else dft2(v);
+}
+
+constexpr int kStockThreads = 256;
+
+// One pass: butterflies j = t, t + 256, ... of N / R; inputs from LDS (or, for
+// the first pass, from `load`), outputs to LDS, a barrier between the two.
+template <int LOGN, int R, bool FIRST, class Load>
+__device__ __forceinline__ void stockham_pass(Cplx *buf, uint32_t Ns, const double *__restrict__ tw, Load load)
+{
+    constexpr uint32_t N = 1u << LOGN, B = N / R;
+    constexpr uint32_t PER = (B + kStockThreads - 1) / kStockThreads;
+    Cplx v[PER][R];
+#pragma unroll
+    for (uint32_t q = 0; q < PER; ++q) {
+        const uint32_t j = threadIdx.x + q * kStockThreads;
+        if (j < B) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = FIRST ? load(j + r * B) : buf[j + r * B];
+            const uint32_t k = j % Ns;
+            if (!FIRST && k != 0) {
+                const uint32_t step = k * (N / (Ns * R));
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    const uint32_t qi = step * r;
+                    v[q][r] = cmul(v[q][r], Cplx{tw[2 * qi], tw[2 * qi + 1]});
+                }
+            }
+            dft_r<R>(v[q]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t q = 0; q < PER; ++q) {
+        const uint32_t j = threadIdx.x + q * kStockThreads;
+        if (j < B) {
+            const uint32_t k = j % Ns, base = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[base + r * Ns] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+// First maximum (lowest index among equal values) of (value, index) over the
+// workgroup: shuffles within each wavefront, then across the four.
+__device__ __forceinline__ void argmax_first(double v, uint32_t i, double *sv, uint32_t *si, uint32_t &best_i)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(v, off);
+        const uint32_t oi = __shfl_xor(i, off);
+        if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+    }
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { sv[w] = v; si[w] = i; }
+    __syncthreads();
+    double bv = sv[0];
+    uint32_t bi = si[0];
+    for (uint32_t q = 1; q < kStockThreads / 64; ++q)
+        if (sv[q] > bv || (sv[q] == bv && si[q] < bi)) { bv = sv[q]; bi = si[q]; }
+    best_i = bi;
+}
+
+// One workgroup per (particle, plane) of a batch of signal rows (FftRowArgs):
+// window, mean removal and Hanning at the load, the FFT, magnitudes, first
+// maximum, interpolated tune. N = 2^LOGN, 8 <= N <= 4096.
+template <int LOGN>
+__global__ void __launch_bounds__(kStockThreads) k_fft_tune_stockham(FftRowArgs a)
+{
+    constexpr uint32_t N = 1u << LOGN;
+    __shared__ Cplx buf[N];
+    __shared__ double sv[kStockThreads / 64];
+    __shared__ uint32_t si[kStockThreads / 64];
+    __shared__ int any_nan_flag;
+    const uint32_t b = blockIdx.x >> 1, plane = blockIdx.x & 1;
+    if (b >= a.nb) return;
+    const double *base = a.rows + (size_t)((a.row0 + b) * a.pstride + a.from);
+    const double *rx = base + (2 * plane) * a.cstride, *rp = base + (2 * plane + 1) * a.cstride;
+    const double *oth0 = base + (2 * (1 - plane)) * a.cstride, *oth1 = base + (2 * (1 - plane) + 1) * a.cstride;
+    const size_t nn = a.n, pp = a.p0 + b;
+    const double mx = a.means ? a.means[(2 * plane) * nn + pp] : 0.0;
+    const double mp = a.means ? a.means[(2 * plane + 1) * nn + pp] : 0.0;
+    const double mo0 = a.means ? a.means[(2 * (1 - plane)) * nn + pp] : 0.0;
+    const double mo1 = a.means ? a.means[(2 * (1 - plane) + 1) * nn + pp] : 0.0;
+    if (threadIdx.x == 0) any_nan_flag = 0;
+    __syncthreads();
+    // a NaN in any of the four coordinates of the window marks both planes
+    bool nan_seen = false;
+    auto load = [&](uint32_t m) -> Cplx {
+        const double vx = rx[m] - mx, vp = rp[m] - mp;
+        nan_seen |= any_nan(vx, vp, oth0[m] - mo0, oth1[m] - mo1);
+        const double h = a.hann[m];
+        return Cplx{vx * h, vp * h};
+    };
+    constexpr int R8 = LOGN / 3, REM = LOGN % 3;
+    // first pass from memory
+    uint32_t Ns = 1;
+    if constexpr (R8 > 0) {
+        stockham_pass<LOGN, 8, true>(buf, Ns, a.tw, load);
+        Ns *= 8;
+    } else if constexpr (REM == 2) {
+        stockham_pass<LOGN, 4, true>(buf, Ns, a.tw, load);
+        Ns *= 4;
+    } else {
+        stockham_pass<LOGN, 2, true>(buf, Ns, a.tw, load);
+        Ns *= 2;
+    }
+    if (nan_seen) atomicOr(&any_nan_flag, 1);
+#pragma unroll
+    for (int p = 1; p < R8; ++p) {
+        stockham_pass<LOGN, 8, false>(buf, Ns, a.tw, load);
+        Ns *= 8;
+    }
+    if constexpr (R8 > 0 && REM == 2) stockham_pass<LOGN, 4, false>(buf, Ns, a.tw, load);
+    if constexpr (R8 > 0 && REM == 1) stockham_pass<LOGN, 2, false>(buf, Ns, a.tw, load);
+    double *o = a.out + ((size_t)a.window * 2 + plane) * nn + pp;
+    if (any_nan_flag) {
+        if (threadIdx.x == 0) *o = __builtin_nan("");
+        return;
+    }
+    // magnitudes (into the first half of the buffer) and their first maximum
+    double best = -1.0;
+    uint32_t best_i = 0xffffffffu;
+    double mg[(N + kStockThreads - 1) / kStockThreads];
+#pragma unroll
+    for (uint32_t q = 0; q < (N + kStockThreads - 1) / kStockThreads; ++q) {
+        const uint32_t k = threadIdx.x + q * kStockThreads;
+        mg[q] = 0.0;
+        if (k < N) {
+            const Cplx z = buf[k];
+            mg[q] = sqrt(z.re * z.re + z.im * z.im);
+            if (mg[q] > best || best_i == 0xffffffffu) { best = mg[q]; best_i = k; }
+        }
+    }
+    __syncthreads();
+    double *mag = reinterpret_cast<double *>(buf);
+#pragma unroll
+    for (uint32_t q = 0; q < (N + kStockThreads - 1) / kStockThreads; ++q) {
+        const uint32_t k = threadIdx.x + q * kStockThreads;
+        if (k < N) mag[k] = mg[q];
+    }
+    uint32_t bi;
+    argmax_first(best, best_i, sv, si, bi);
+    if (threadIdx.x == 0) *o = interp_tune(mag, N, bi);
 }
 
 // Direct DFT for any N: X_k = sum_m z_m exp(-2 pi i k m / N), twiddle index
@@ -722,8 +972,15 @@ static void birkhoff_impl(const double *hist, HistLayout lay, size_t n, uint32_t
         double *dw = static_cast<double *>(workspace(std::max<size_t>(total, 1) * sizeof(double), 2));
         hip_check(hipMemcpyAsync(dw, wts.data(), total * sizeof(double), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
         LaunchTimer timer("k_birkhoff_hist", s);
-        hipLaunchKernelGGL(k_birkhoff_hist, dim3((uint32_t)((n * 2 * ws.count + 255) / 256)), dim3(256), 0, s, hist, lay,
-                           (uint32_t)n, means, ws, (const double *)dw, out + w0 * 2 * n);
+        if (lay.ms == 1) {
+            // particle rows: a wavefront per (particle, plane, window)
+            const uint64_t waves = (uint64_t)n * 2 * ws.count;
+            hipLaunchKernelGGL(k_birkhoff_rows, dim3((uint32_t)((waves + 3) / 4)), dim3(256), 0, s, hist, lay.cs,
+                               (uint32_t)n, means, ws, (const double *)dw, out + w0 * 2 * n);
+        } else {
+            hipLaunchKernelGGL(k_birkhoff_hist, dim3((uint32_t)((n * 2 * ws.count + 255) / 256)), dim3(256), 0, s,
+                               hist, lay, (uint32_t)n, means, ws, (const double *)dw, out + w0 * 2 * n);
+        }
         launch_check("k_birkhoff_hist");
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     }
@@ -746,6 +1003,30 @@ extern "C" int hm_birkhoff_tunes_rows(const double *rows, size_t n, size_t row_l
         if (n_samples > row_len) throw Error(HM_ERR_RANGE, "birkhoff: more samples than row_len");
         birkhoff_impl(rows, particle_rows(row_len), n, n_samples, means, from, to, n_windows, out, as_stream(stream));
     });
+}
+
+// Launches the FFT of a batch of signal rows: the Stockham kernel for
+// 8 <= N <= 2048 (its signal fits the static LDS), else the radix-2 row kernel.
+static void launch_row_fft(const FftRowArgs &r, uint32_t N, hipStream_t s)
+{
+    const uint32_t logn = 31 - __builtin_clz(N);
+    const dim3 g2(2 * r.nb);
+    LaunchTimer timer("k_fft_tune_rows", s);
+    switch (logn) {
+    case 3: hipLaunchKernelGGL(k_fft_tune_stockham<3>, g2, dim3(kStockThreads), 0, s, r); break;
+    case 4: hipLaunchKernelGGL(k_fft_tune_stockham<4>, g2, dim3(kStockThreads), 0, s, r); break;
+    case 5: hipLaunchKernelGGL(k_fft_tune_stockham<5>, g2, dim3(kStockThreads), 0, s, r); break;
+    case 6: hipLaunchKernelGGL(k_fft_tune_stockham<6>, g2, dim3(kStockThreads), 0, s, r); break;
+    case 7: hipLaunchKernelGGL(k_fft_tune_stockham<7>, g2, dim3(kStockThreads), 0, s, r); break;
+    case 8: hipLaunchKernelGGL(k_fft_tune_stockham<8>, g2, dim3(kStockThreads), 0, s, r); break;
+    case 9: hipLaunchKernelGGL(k_fft_tune_stockham<9>, g2, dim3(kStockThreads), 0, s, r); break;
+    case 10: hipLaunchKernelGGL(k_fft_tune_stockham<10>, g2, dim3(kStockThreads), 0, s, r); break;
+    case 11: hipLaunchKernelGGL(k_fft_tune_stockham<11>, g2, dim3(kStockThreads), 0, s, r); break;
+    default:
+        allow_dynamic_lds((const void *)k_fft_tune_rows, 4 * N * sizeof(double));
+        hipLaunchKernelGGL(k_fft_tune_rows, dim3(r.nb), dim3(256), 4 * N * sizeof(double), s, r);
+    }
+    launch_check("k_fft_tune_rows");
 }
 
 // rows_input: hist is particle rows (row_len = lay.cs); else sample-major.
@@ -782,15 +1063,12 @@ static void fft_impl(const double *hist, HistLayout lay, bool rows_input, size_t
             const uint32_t blocks = (uint32_t)(((signals + 7) / 8) * 8);
             const size_t rows_lds = 4 * (size_t)N * sizeof(double) + kRowsStaticLds;
             if (pow2 && N <= kRowsMaxN && rows_lds <= lds_per_block()) {
-                allow_dynamic_lds((const void *)k_fft_tune_rows, 4 * N * sizeof(double));
                 if (rows_input) {
                     // one workgroup per particle, reading its rows in place
                     for (size_t q = b0; q < b1; q++) {
                         FftRowArgs r{hist, lay.ps, lay.cs, 0, from[ws[q]], means, (uint32_t)n, 0, (uint32_t)n, N,
                                      t.hann, t.tw, out, ws[q]};
-                        LaunchTimer timer("k_fft_tune_rows", s);
-                        hipLaunchKernelGGL(k_fft_tune_rows, dim3((uint32_t)n), dim3(256), 4 * N * sizeof(double), s, r);
-                        launch_check("k_fft_tune_rows");
+                        launch_row_fft(r, N, s);
                     }
                 } else {
                     // per window: signal rows for a batch of particles (corner
@@ -810,9 +1088,7 @@ static void fft_impl(const double *hist, HistLayout lay, bool rows_input, size_t
                             }
                             FftRowArgs r{rows, 4ull * N, N, 0, 0, nullptr, nb, p0, (uint32_t)n, N, t.hann, t.tw, out,
                                          ws[q]};
-                            LaunchTimer timer("k_fft_tune_rows", s);
-                            hipLaunchKernelGGL(k_fft_tune_rows, dim3(nb), dim3(256), 4 * N * sizeof(double), s, r);
-                            launch_check("k_fft_tune_rows");
+                            launch_row_fft(r, N, s);
                         }
                     }
                 }

@@ -156,7 +156,12 @@ def test_rows_history_and_tunes_equal_sample_major():
     assert np.array_equal(d1.get()[4], d2.get()[4])
     windows = [(0, 2047), (100, 2147), (0, 8191), (3, 503), (0, n_turns)]
     for mean in (False, True):
-        for kind in ("fft", "birkhoff"):
-            a = C.history_tunes(hist, n, n_turns + 1, windows, kind, subtract_mean=mean)
-            b = C.history_tunes(rows, n, n_turns + 1, windows, kind, subtract_mean=mean, row_len=L)
-            assert np.array_equal(a, b, equal_nan=True), (kind, mean)
+        a = C.history_tunes(hist, n, n_turns + 1, windows, "fft", subtract_mean=mean)
+        b = C.history_tunes(rows, n, n_turns + 1, windows, "fft", subtract_mean=mean, row_len=L)
+        assert np.array_equal(a, b, equal_nan=True), mean
+        # Birkhoff from rows sums the same terms in a wavefront-parallel order:
+        # sums of up to 8300 terms of size ~1e-3 agree to rounding (1e-14)
+        a = C.history_tunes(hist, n, n_turns + 1, windows, "birkhoff", subtract_mean=mean)
+        b = C.history_tunes(rows, n, n_turns + 1, windows, "birkhoff", subtract_mean=mean, row_len=L)
+        assert np.array_equal(np.isnan(a), np.isnan(b))
+        np.testing.assert_allclose(b, a, rtol=0, atol=2e-14, equal_nan=True)
