@@ -345,11 +345,12 @@ int hm_birkhoff_tunes_history(const double *hist, size_t n, uint32_t n_samples, 
  * maximum, three-point interpolation, |1 - (k/N + asin(A sin(2pi/N))/2pi)|; 1.0 /
  * 0.0 at the spectrum's edges; NaN when a sample of the window is NaN. Windows
  * start at sample from[w] and have len[w] samples (host arrays). Power-of-two
- * lengths up to 8192 use a radix-2 FFT in LDS, other lengths up to 16384 a
- * direct DFT. Up to 4096 samples the windows are first corner-turned into
- * per-particle signal rows (library workspace of at most 4 GiB, particles taken
- * in batches that fit; the environment variable HM_FFT_ROW_BYTES overrides the
- * budget). out: device double[n_windows][2][n]. */
+ * lengths up to 2048 use a Stockham radix-8 FFT in LDS, 4096 and 8192 radix-2,
+ * other lengths up to 16384 a direct DFT. Up to 4096 samples the windows are
+ * first corner-turned into per-particle signal rows (library workspace of at most
+ * 4 GiB, particles taken in batches that fit; the environment variable
+ * HM_FFT_ROW_BYTES overrides the budget); hm_fft_tunes_rows reads particle rows
+ * in place. out: device double[n_windows][2][n]. */
 int hm_fft_tunes(const double *hist, size_t n, uint32_t n_samples, const double *means, const uint32_t *from,
                  const uint32_t *len, size_t n_windows, double *out, void *stream);
 
@@ -362,8 +363,10 @@ int hm_fft_tunes(const double *hist, size_t n, uint32_t n_samples, const double 
 int hm_track_rows(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n, const double *table,
                   size_t table_len, uint32_t n_turns, double mu, double barrier, int inverse, double *rows,
                   size_t row_len, void *stream);
-/* hm_history_means, hm_birkhoff_tunes_history and hm_fft_tunes on particle rows
- * (same results bit for bit); n_samples <= row_len. */
+/* hm_history_means, hm_birkhoff_tunes_history and hm_fft_tunes on particle rows;
+ * n_samples <= row_len. Means and FFT tunes equal the sample-major results bit
+ * for bit; Birkhoff tunes sum the same terms in a wavefront-parallel order
+ * (equal to rounding). */
 int hm_rows_means(const double *rows, size_t n, size_t row_len, uint32_t n_samples, double *means, void *stream);
 int hm_birkhoff_tunes_rows(const double *rows, size_t n, size_t row_len, uint32_t n_samples, const double *means,
                            const uint32_t *from, const uint32_t *to, size_t n_windows, double *out, void *stream);
