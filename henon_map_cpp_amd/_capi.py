@@ -28,7 +28,7 @@ U64 = ctypes.c_uint64
 
 class LaunchCfg(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int32), ("compact_every", ctypes.c_int32),
-                ("particles_per_lane", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("particles_per_lane", ctypes.c_int32), ("table_staging", ctypes.c_int32)]
 
 
 class Job(ctypes.Structure):

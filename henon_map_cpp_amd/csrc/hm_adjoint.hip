@@ -88,10 +88,24 @@ __global__ void __launch_bounds__(kFwdBlock) k_adjoint_forward(AdjointFwdArgs a)
     const double mu = a.mu, m3mu = -3.0 * mu, b2 = a.b2, b2s = screen_bound(b2);
     double xx = x * x, yy = y * y;
     double bsum = 0.0;
+    // The common turn is the map, the screened radius and one wave-wide test;
+    // the exact decision and the parking of a lost point (at the origin, a
+    // fixed point of the map, so it never reaches the screen again) run only
+    // when some lane of the wavefront is at or past the screen.
     auto turn = [&](const TabEntry &t, double w) {
-        if (forward_turn_lost<MODE, OCT>(x, px, y, py, xx, yy, mu, m3mu, t, b2, b2s)) {
-            alive = false;
-            x = px = y = py = xx = yy = 0.0;
+        bool cand;
+        if (MODE == kExact) {
+            cand = !(fwd_exact_carry<OCT>(x, px, y, py, xx, yy, mu, t) < b2s);
+        } else {
+            map_step<kFast, false, OCT>(x, px, y, py, mu, m3mu, t);
+            cand = !(radius2<kFast>(x, px, y, py) < b2);
+        }
+        if (__builtin_expect(__any(cand), 0)) {
+            const bool lost = cand && (MODE != kExact || !(exact_radius2(xx, px, yy, py) < b2));
+            if (lost) {
+                alive = false;
+                x = px = y = py = xx = yy = 0.0;
+            }
         }
         bsum = fma(w, MODE == kExact ? xx : x * x, bsum);
     };
@@ -261,13 +275,14 @@ struct AdjRec {
 //    checkpoint, one thread per (particle, segment), and writes the state at the
 //    start of each 8-turn chunk after the first (sub-checkpoints). The segments
 //    are independent, so this part runs at full occupancy even for few particles.
-//  - k_adjoint_sweep, one thread per particle, walks the window's chunks in
-//    reverse: the chunk's nine states S0..S8 (S8 = the next chunk's S0) come from
-//    its sub-checkpoint (the segment's checkpoint for a first chunk), and the
-//    adjoint runs back through its turns. The states of the chunk before it are
-//    recomputed in the same straight-line block: the two dependency chains are
-//    independent and interleave, which matters at one wavefront per SIMD, where
-//    nothing else hides FP64 latency.
+//  - the sweep (k_adjoint_sweep_fused for every % 8 == 0, else k_adjoint_sweep),
+//    one thread per particle, walks the window's chunks in reverse: the chunk's
+//    nine states S0..S8 (S8 = the next chunk's S0) come from its sub-checkpoint
+//    (the segment's checkpoint for a first chunk), and the adjoint runs back
+//    through its turns. The states of the chunk before it are recomputed in the
+//    same straight-line block: the two dependency chains are independent and
+//    interleave, which matters at one wavefront per SIMD, where nothing else
+//    hides FP64 latency.
 // The adjoint and the parameter sums carry from window to window in the output
 // arrays.
 struct AdjointSegArgs {
@@ -437,42 +452,37 @@ __global__ void __launch_bounds__(256) k_adjoint_sweep(AdjointSweepArgs a)
     a.grad_p[i] = acc.g_eps; a.grad_p[n + i] = acc.g_mu; a.grad_p[2 * n + i] = acc.g_wx; a.grad_p[3 * n + i] = acc.g_wy;
 }
 
-// Reverse sweep for every % 8 == 0 (chunk g is turns 8g .. min(8g+8, T)-1; only
-// the very last chunk can be short), two wavefronts per 64 particles. Wavefront
-// 0 (producer) recomputes the states of the chunk after the one being swept
-// (in reverse order: the chunk before it in turn order) into one half of a
-// double-buffered LDS array; wavefront 1 (consumer) runs the adjoint over the
-// current chunk from the other half; a workgroup barrier ends each step. The
-// two dependency chains thus sit on separate wavefronts and the SIMD
-// interleaves them. Each wavefront stages the table entries it reads into its
-// own LDS ring one chunk ahead (one vector load and LDS store per lane and
-// chunk) and reads them one turn ahead, so no scalar-cache miss or LDS latency
-// sits on a dependency chain. The window's top chunk (possibly short) is swept
-// by the consumer alone with scalar table loads while the producer computes the
-// first full chunk.
 constexpr int kRingSlots = 3;                 // chunks per ring
-constexpr int kProdTurn = 4;                  // producer ring: sx cx sy cy
-constexpr int kConsTurn = 10;                 // consumer ring: sx cx sy cy | 4 angle derivatives | weight, pad
-constexpr int kPairBlock = 128;
+constexpr int kRingTurn = 10;                 // ring record: sx cx sy cy | 4 angle derivatives | weight, pad
+
+// Reverse sweep for every % 8 == 0, one wavefront per 64 particles holding
+// both dependency chains in registers: per step it runs the adjoint back
+// through the current chunk (states X) while it recomputes the states of the
+// chunk before it (Y) from that chunk's sub-checkpoint, one turn of each per
+// scheduling region, so the two chains interleave inside the wavefront. The
+// wavefront stages table records, angle derivatives and weights into its own
+// LDS ring of 3 chunks (one vector load and one LDS store per lane and chunk,
+// loaded a step before the step that first reads them) and reads a turn's
+// entries one region ahead. Nothing crosses wavefronts: no workgroup barrier,
+// no state traffic through LDS. The window's top chunk (possibly short) runs
+// first with scalar table loads.
+constexpr int kFusedBlock = 256;
 
 template <int MODE, bool OCT>
-__global__ void __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(2)))
-k_adjoint_sweep_pair(AdjointSweepArgs a)
+__global__ void __launch_bounds__(kFusedBlock) k_adjoint_sweep_fused(AdjointSweepArgs a)
 {
-    __shared__ double2 st[2][kChunk][2][64];  // [buffer][state][(x,px) | (y,py)][lane]
-    __shared__ __align__(16) double ring_p[kRingSlots * kChunk * kProdTurn];
-    __shared__ __align__(16) double ring_c[kRingSlots * kChunk * kConsTurn];
+    __shared__ __align__(16) double ring_all[(kFusedBlock / 64) * kRingSlots * kChunk * kRingTurn];
     const uint32_t lane = threadIdx.x & 63;
-    const bool producer = threadIdx.x < 64;
-    const uint32_t i = blockIdx.x * 64 + lane;
+    double *const ring = ring_all + (threadIdx.x >> 6) * (kRingSlots * kChunk * kRingTurn);
+    const uint32_t i = blockIdx.x * kFusedBlock + threadIdx.x;
     const size_t n = a.n;
-    // every lane runs to the end (lanes past n read particle n-1, lost
-    // particles compute on parked or stale states); only live lanes store
+    // lanes past n read particle n-1; lost particles compute on parked or stale
+    // states; only live lanes store
     const uint32_t ic = i < a.n ? i : a.n - 1;
     const bool live = i < a.n && a.alive[ic];
-    if (!producer && !live && i < a.n && a.last_window)
+    if (!live && i < a.n && a.last_window)
         for (int c = 0; c < 4; c++) { a.grad_x0[c * n + i] = 0.0; a.grad_p[c * n + i] = 0.0; }
-    if (!__syncthreads_or(live)) return;  // both wavefronts hold the same 64 particles
+    if (!__any(live)) return;  // the whole wavefront
 
     const double mu = a.mu, m3mu = -3.0 * mu, mu3 = 3.0 * mu, mu6 = 6.0 * mu;
     const uint32_t T = a.n_turns, K = a.every;
@@ -481,10 +491,11 @@ k_adjoint_sweep_pair(AdjointSweepArgs a)
     const uint32_t G = (T + kChunk - 1) / kChunk;
     const int g_lo = (int)(a.seg_first * K8);
     const int gt = (int)umin(a.seg_end * K8, G) - 1;
+    const int steps = gt - g_lo;  // full chunks below the top one
     const double *tabd = reinterpret_cast<const double *>(a.tab);
     const double *sub_base = a.sub ? a.sub : a.ckpt;
     auto clampg = [&](int g) { return g < g_lo ? g_lo : g; };
-    auto slot_of = [&](int g) { return (uint32_t)g % kRingSlots; };
+    auto slot = [&](int g) { return ring + ((uint32_t)g % kRingSlots) * (kChunk * kRingTurn); };
     auto load4 = [&](const double *p, double (&v)[4]) {
         v[0] = p[ic]; v[1] = p[n + ic]; v[2] = p[2 * n + ic]; v[3] = p[3 * n + ic];
     };
@@ -492,72 +503,29 @@ k_adjoint_sweep_pair(AdjointSweepArgs a)
         return j == 0 ? a.ckpt + (size_t)seg * 4 * n
                       : sub_base + ((size_t)(seg - a.seg_first) * (K8 - 1) + j - 1) * 4 * n;
     };
-    const int steps = gt - g_lo;  // full chunks below the top one
+    // ring record of a turn: sx cx sy cy | 4 angle derivatives | weight, pad.
+    // Lane l < 32 stages table double l (turn l/4), lane l >= 32 derivative
+    // double l-32, every lane the weight of turn l % 8 (eight identical writes).
+    auto stage_load = [&](int g, double &e0, double &e1) {
+        const size_t b = (size_t)g * kChunk;
+        const double *src = lane < 32 ? tabd + 4 * b + lane : a.dtab + 4 * b + (lane - 32);
+        e0 = *src;
+        e1 = a.weights[b + (lane & 7)];
+    };
+    auto stage_store = [&](int g, double e0, double e1) {
+        double *s = slot(g);
+        s[((lane & 31) >> 2) * kRingTurn + (lane & 3) + (lane < 32 ? 0 : 4)] = e0;
+        s[(lane & 7) * kRingTurn + 8] = e1;
+    };
+    auto rec_p = [&](int g, int m) -> TabEntry {
+        const double *q = slot(g) + m * kRingTurn;
+        return TabEntry{q[0], q[1], q[2], q[3]};
+    };
+    auto rec_c = [&](int g, int m) -> AdjRec {
+        const double *q = slot(g) + m * kRingTurn;
+        return AdjRec{TabEntry{q[0], q[1], q[2], q[3]}, double4{q[4], q[5], q[6], q[7]}, q[8]};
+    };
 
-    if (producer) {
-        // ring of table records: lane l carries double l % 32 of the chunk's
-        // 32 (turn (l % 32) / 4); lanes 32..63 repeat lanes 0..31
-        auto stage_load = [&](int g) { return tabd[32 * (size_t)g + (lane & 31)]; };
-        auto stage_store = [&](int g, double e) { ring_p[slot_of(g) * (kChunk * kProdTurn) + (lane & 31)] = e; };
-        auto rec = [&](int g, int m) -> TabEntry {
-            const double *q = ring_p + slot_of(g) * (kChunk * kProdTurn) + m * kProdTurn;
-            return TabEntry{q[0], q[1], q[2], q[3]};
-        };
-        double e = 0.0, P[4] = {0.0, 0.0, 0.0, 0.0};
-        int h = 0;
-        uint32_t hs = 0, hj = 0;
-        if (steps > 0) {
-            stage_store(gt - 1, stage_load(gt - 1));
-            e = stage_load(clampg(gt - 2));
-            __builtin_amdgcn_wave_barrier();
-            const uint32_t g1 = (uint32_t)(gt - 1), s1 = g1 / K8;
-            load4(start_at(s1, g1 - s1 * K8), P);
-            h = clampg(gt - 2);
-            hs = (uint32_t)h / K8;
-            hj = (uint32_t)h - hs * K8;
-        }
-        // states of chunk gp (full) into buffer b; the start of the chunk
-        // before it is requested first
-        auto produce = [&](int gp, int b) {
-            stage_store(clampg(gp - 1), e);
-            e = stage_load(clampg(gp - 2));
-            double Q[4];
-            load4(start_at(hs, hj), Q);
-            {
-                const bool down = h > g_lo, wrap = hj == 0;
-                h = down ? h - 1 : h;
-                hs = down && wrap ? hs - 1 : hs;
-                hj = down ? (wrap ? K8 - 1 : hj - 1) : hj;
-            }
-            double v[4];
-            copy4(v, P);
-            st[b][0][0][lane] = double2{v[0], v[1]};
-            st[b][0][1][lane] = double2{v[2], v[3]};
-            TabEntry r = rec(gp, 0);
-#pragma unroll
-            for (int m = 1; m < kChunk; ++m) {
-                TabEntry rn{};
-                if (m + 1 < kChunk) rn = rec(gp, m);
-                step4<MODE, OCT>(v, mu, m3mu, r);
-                st[b][m][0][lane] = double2{v[0], v[1]};
-                st[b][m][1][lane] = double2{v[2], v[3]};
-                r = rn;
-                // one turn per scheduling region: the next turn's entries
-                // are requested one turn ahead, no further
-                __builtin_amdgcn_sched_barrier(0);
-            }
-            copy4(P, Q);
-        };
-        if (steps > 0) produce(gt - 1, 1);
-        __syncthreads();
-        for (int c = 1; c <= steps; ++c) {
-            if (c < steps) produce(gt - 1 - c, (c + 1) & 1);
-            __syncthreads();
-        }
-        return;
-    }
-
-    // consumer
     AdjAcc acc;
     if (a.last_window) {
         acc = AdjAcc{a.seed_s[ic], a.seed_s[n + ic], a.seed_s[2 * n + ic], a.seed_s[3 * n + ic], 0.0, 0.0, 0.0, 0.0};
@@ -568,33 +536,41 @@ k_adjoint_sweep_pair(AdjointSweepArgs a)
     const double bseed = 2.0 * (a.seed_b ? a.seed_b[ic] : 0.0);
     const ConstPtr<TabEntry> tab = as_const(a.tab);
     const ConstPtr<double> dtab = as_const(a.dtab), weights = as_const(a.weights);
-    // ring: lane l < 32 carries table double l (turn l/4), lane l >= 32 angle-
-    // derivative double l-32, every lane the weight of turn l % 8 (eight lanes
-    // per value, identical writes)
-    auto stage_load = [&](int g, double &e0, double &e1) {
-        const size_t b = (size_t)g * kChunk;
-        const double *src = lane < 32 ? tabd + 4 * b + lane : a.dtab + 4 * b + (lane - 32);
-        e0 = *src;
-        e1 = a.weights[b + (lane & 7)];
-    };
-    auto stage_store = [&](int g, double e0, double e1) {
-        double *slot = ring_c + slot_of(g) * (kChunk * kConsTurn);
-        slot[((lane & 31) >> 2) * kConsTurn + (lane & 3) + (lane < 32 ? 0 : 4)] = e0;
-        slot[(lane & 7) * kConsTurn + 8] = e1;
-    };
-    auto ring_adj = [&](int g, int m) -> AdjRec {
-        const double *p = ring_c + slot_of(g) * (kChunk * kConsTurn) + m * kConsTurn;
-        return AdjRec{TabEntry{p[0], p[1], p[2], p[3]}, double4{p[4], p[5], p[6], p[7]}, p[8]};
-    };
+
+    // Ring schedule: the step sweeping chunk g first stores chunk g-1 (loaded
+    // one step earlier), which its producer half reads, into the slot chunk g+2
+    // held; its consumer half reads chunk g.
     double e0 = 0.0, e1 = 0.0;
     if (steps > 0) {
         stage_load(gt - 1, e0, e1);
         stage_store(gt - 1, e0, e1);
         stage_load(clampg(gt - 2), e0, e1);
+        __builtin_amdgcn_wave_barrier();
     }
-    double nxt[4];
+    // Sub-checkpoint walk of the producer (chunk h, segment hs, index hj):
+    // P is the start of the chunk produced next.
+    double P[4];
+    int h = g_lo;
+    uint32_t hs = 0, hj = 0;
     {
-        // the top chunk, scalar table loads
+        const int g1 = clampg(gt - 1);
+        const uint32_t s1 = (uint32_t)g1 / K8;
+        load4(start_at(s1, (uint32_t)g1 - s1 * K8), P);
+        h = clampg(gt - 2);
+        hs = (uint32_t)h / K8;
+        hj = (uint32_t)h - hs * K8;
+    }
+    auto next_start = [&](double (&Q)[4]) {
+        load4(start_at(hs, hj), Q);
+        const bool down = h > g_lo, wrap = hj == 0;
+        h = down ? h - 1 : h;
+        hs = down && wrap ? hs - 1 : hs;
+        hj = down ? (wrap ? K8 - 1 : hj - 1) : hj;
+    };
+
+    double SA[kChunk][4], SB[kChunk][4], nxt[4];
+    {
+        // the top chunk (scalar table loads) while chunk gt-1 is produced into SA
         const uint32_t ct = (uint32_t)gt * kChunk, lent = umin(kChunk, T - ct);
         const uint32_t st_seg = (uint32_t)gt / K8;
         double S[kChunk][4];
@@ -614,6 +590,16 @@ k_adjoint_sweep_pair(AdjointSweepArgs a)
             step4<MODE, OCT>(v, mu, m3mu, load_record(tab + T - 1));
             copy4(nxt, v);
         }
+        double Q[4];
+        next_start(Q);
+        copy4(SA[0], P);
+        const int gp = clampg(gt - 1);
+#pragma unroll
+        for (int m = 1; m < kChunk; ++m) {
+            copy4(SA[m], SA[m - 1]);
+            step4<MODE, OCT>(SA[m], mu, m3mu, rec_p(gp, m - 1));
+        }
+        copy4(P, Q);
 #pragma unroll
         for (int m = kChunk - 1; m >= 0; --m) {
             if ((uint32_t)m >= lent) continue;
@@ -628,35 +614,48 @@ k_adjoint_sweep_pair(AdjointSweepArgs a)
         }
         copy4(nxt, S[0]);
     }
-    __syncthreads();
-    for (int c = 1; c <= steps; ++c) {
-        const int g = gt - c, b = c & 1;
-        stage_store(clampg(g - 1), e0, e1);
+
+    // One step: the adjoint through chunk g (states X) and the states of chunk
+    // g-1 into Y. In the last step the producer runs on clamped inputs and its
+    // states are not used.
+    auto step = [&](double (&X)[kChunk][4], double (&Y)[kChunk][4], int c) __attribute__((always_inline)) {
+        const int g = gt - c, gp = clampg(g - 1);
+        stage_store(gp, e0, e1);
         stage_load(clampg(g - 2), e0, e1);
-        double post[4], pre[4];
+        __builtin_amdgcn_wave_barrier();  // the ring stores precede later reads
+        double Q[4];
+        next_start(Q);
+        copy4(Y[0], P);
+        double post[4];
         copy4(post, nxt);
-        {
-            const double2 u = st[b][kChunk - 1][0][lane], v = st[b][kChunk - 1][1][lane];
-            pre[0] = u.x; pre[1] = u.y; pre[2] = v.x; pre[3] = v.y;
-        }
-        AdjRec r = ring_adj(g, kChunk - 1);
+        AdjRec rc = rec_c(g, kChunk - 1);
+        TabEntry rp = rec_p(gp, 0);
 #pragma unroll
-        for (int m = kChunk - 1; m >= 0; --m) {
-            double prn[4] = {0.0, 0.0, 0.0, 0.0};
-            AdjRec rn{};
-            if (m > 0) {
-                const double2 u = st[b][m - 1][0][lane], v = st[b][m - 1][1][lane];
-                prn[0] = u.x; prn[1] = u.y; prn[2] = v.x; prn[3] = v.y;
-                rn = ring_adj(g, m - 1);
+        for (int j = 0; j < kChunk; ++j) {
+            const int m = kChunk - 1 - j;
+            AdjRec rcn{};
+            TabEntry rpn{};
+            if (m > 0) rcn = rec_c(g, m - 1);
+            if (j + 2 < kChunk) rpn = rec_p(gp, j + 1);
+            adjoint_turn<OCT>(X[m], post, rc.t, bseed * rc.w, rc.dt, mu3, mu6, acc);
+            copy4(post, X[m]);
+            if (j + 1 < kChunk) {
+                copy4(Y[j + 1], Y[j]);
+                step4<MODE, OCT>(Y[j + 1], mu, m3mu, rp);
             }
-            adjoint_turn<OCT>(pre, post, r.t, bseed * r.w, r.dt, mu3, mu6, acc);
-            copy4(post, pre);
-            copy4(pre, prn);
-            r = rn;
-            __builtin_amdgcn_sched_barrier(0);  // as in the producer
+            rc = rcn;
+            rp = rpn;
+            // one turn of each chain per scheduling region; the next region's
+            // ring entries are requested one region ahead
+            __builtin_amdgcn_sched_barrier(0);
         }
-        copy4(nxt, post);
-        __syncthreads();
+        copy4(nxt, X[0]);
+        copy4(P, Q);
+    };
+    for (int c = 1; c <= steps; c += 2) {
+        step(SA, SB, c);
+        if (c + 1 > steps) break;
+        step(SB, SA, c + 1);
     }
     if (!live) return;
     a.grad_x0[i] = acc.lx; a.grad_x0[n + i] = acc.lpx; a.grad_x0[2 * n + i] = acc.ly; a.grad_x0[3 * n + i] = acc.lpy;
@@ -674,8 +673,8 @@ template <int MODE, bool OCT>
 static void launch_sweep(bool aligned, size_t n, hipStream_t s, const AdjointSweepArgs &w)
 {
     if (aligned)
-        hipLaunchKernelGGL((k_adjoint_sweep_pair<MODE, OCT>), dim3((uint32_t)((n + 63) / 64)), dim3(kPairBlock), 0, s,
-                           w);
+        hipLaunchKernelGGL((k_adjoint_sweep_fused<MODE, OCT>), dim3((uint32_t)((n + kFusedBlock - 1) / kFusedBlock)),
+                           dim3(kFusedBlock), 0, s, w);
     else hipLaunchKernelGGL((k_adjoint_sweep<MODE, OCT>), dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, w);
 }
 

@@ -20,18 +20,20 @@ namespace hm {
 // exactly `slots` wavefronts launched) the first w2 wavefronts take two particles
 // per lane so that n particles fit in one resident round of wavefronts:
 // w2 = ceil((n - 64 slots) / 64) when n exceeds 64 slots.
-template <int MODE, bool INV, bool OCT>
-__global__ void __launch_bounds__(256, 8) k_track(TrackArgs A)
+// RING: the table records come through a per-wavefront LDS ring instead of
+// scalar loads (the launch picks it when there are few wavefronts per SIMD).
+template <int MODE, bool INV, bool OCT, bool RING>
+__global__ void __launch_bounds__(256, RING ? 2 : 8) k_track(TrackArgs A)
 {
     const uint32_t n = A.count_in ? *A.count_in : A.n_arg;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     uint32_t w2 = 0;
     if (A.slots && n > 64u * A.slots) w2 = min((n - 64u * A.slots + 63u) / 64u, A.slots);
     if (wave < w2) {
-        track_wave<MODE, INV, OCT, 2>(A, wave * 128u, n);
+        track_wave<MODE, INV, OCT, 2, RING>(A, wave * 128u, n);
     } else {
         const uint32_t base = w2 * 128u + (wave - w2) * 64u;
-        if (base < n) track_wave<MODE, INV, OCT, 1>(A, base, n);
+        if (base < n) track_wave<MODE, INV, OCT, 1, RING>(A, base, n);
     }
 }
 
@@ -46,7 +48,12 @@ struct TrackLaunch {
     uint32_t *idx_out, *count_out;
     int block;
     uint32_t slots;  // resident wavefront slots of the device (0 = plain one-per-lane launch)
+    int staging;     // HM_STAGING_*: table records by scalar loads or through the LDS ring
+    uint32_t simds;  // SIMDs of the device (CUs x 4)
 };
+
+// Automatic table staging: the LDS ring up to this many wavefronts per SIMD.
+static constexpr uint64_t kRingMaxWavesPerSimd = 2;
 
 template <int MODE, bool INV, bool OCT>
 static void launch_track(const TrackLaunch &a, hipStream_t s)
@@ -62,8 +69,11 @@ static void launch_track(const TrackLaunch &a, hipStream_t s)
     }
     const uint64_t grid = (waves * 64 + a.block - 1) / a.block;
     if (grid == 0) return;
+    const bool ring = a.staging == HM_STAGING_RING ||
+                      (a.staging == HM_STAGING_AUTO && waves <= (uint64_t)kRingMaxWavesPerSimd * a.simds);
     LaunchTimer timer("k_track", s);
-    hipLaunchKernelGGL((k_track<MODE, INV, OCT>), dim3((uint32_t)grid), dim3(a.block), 0, s, A);
+    if (ring) hipLaunchKernelGGL((k_track<MODE, INV, OCT, true>), dim3((uint32_t)grid), dim3(a.block), 0, s, A);
+    else hipLaunchKernelGGL((k_track<MODE, INV, OCT, false>), dim3((uint32_t)grid), dim3(a.block), 0, s, A);
     launch_check("k_track");
 }
 
@@ -235,6 +245,10 @@ extern "C" int hm_track(double *x, double *px, double *y, double *py, uint32_t *
         a.x = x; a.px = px; a.y = y; a.py = py; a.steps = steps;
         a.tab = tab; a.tab_len = (uint32_t)table_len; a.mu = mu; a.b2 = b2; a.block = block;
         a.slots = pair_mode == 1 ? 0u : resident_slots();
+        a.staging = cfg ? cfg->table_staging : HM_STAGING_AUTO;
+        if (a.staging < HM_STAGING_AUTO || a.staging > HM_STAGING_RING)
+            throw Error(HM_ERR_ARG, "track: table_staging must be 0 (auto), 1 (scalar) or 2 (ring)");
+        a.simds = resident_slots() / 8u;
 
         if (every < 0 || (uint32_t)every >= n_turns) {
             a.n_arg = a.grid_bound = N;

@@ -69,7 +69,11 @@ struct TrackArgs {
     uint32_t *idx_out, *count_out, *status;
 };
 
-template <int MODE, bool INV, bool OCT, int P>
+// LDS ring of the staged turn loop, per wavefront: 3 chunks of 8 records.
+constexpr int kTrackRingWave = 3 * 8 * 4;
+constexpr int kTrackRingWaves = 4;  // wavefronts per workgroup (at most 256 threads)
+
+template <int MODE, bool INV, bool OCT, int P, bool RING = false>
 __device__ __forceinline__ void track_wave(const TrackArgs &A, uint32_t base, uint32_t n)
 {
     double *__restrict__ gx = A.gx, *__restrict__ gpx = A.gpx, *__restrict__ gy = A.gy, *__restrict__ gpy = A.gpy;
@@ -199,7 +203,56 @@ __device__ __forceinline__ void track_wave(const TrackArgs &A, uint32_t base, ui
         // has no exit test of its own.
         uint32_t k = 0;
         bool any_live = true;
-        if (n_turns >= 6) {
+        if constexpr (RING) {
+            // With few wavefronts per SIMD nothing hides a scalar load's latency
+            // (an L2 miss the first time a table is read): the wavefront stages
+            // the records of 8-turn chunks into its own LDS ring two chunks
+            // ahead (one vector load and one LDS store per lane and chunk) and
+            // reads each turn's record one turn ahead. Lanes 32..63 repeat
+            // lanes 0..31 (identical stores).
+            __shared__ __align__(16) double ring_all[kTrackRingWaves * kTrackRingWave];
+            double *const ring = ring_all + (threadIdx.x >> 6) * kTrackRingWave;
+            const double *tabd = reinterpret_cast<const double *>(tab);
+            const uint32_t G = (n_turns + 7) / 8;
+            auto clampg = [&](uint32_t g) { return g < G ? g : G - 1; };
+            auto stage_load = [&](uint32_t g) -> double {
+                const uint32_t kk = min(g * 8 + ((lane & 31) >> 2), n_turns - 1);
+                const uint32_t ti = INV ? s0 - 1 - kk : s0 + kk;
+                return tabd[4 * (size_t)ti + (lane & 3)];
+            };
+            auto stage_store = [&](uint32_t g, double e) { ring[(g % 3) * 32 + (lane & 31)] = e; };
+            auto ring_at = [&](uint32_t g, uint32_t m) -> TabEntry {
+                const double *q = ring + (g % 3) * 32 + m * 4;
+                return TabEntry{q[0], q[1], q[2], q[3]};
+            };
+            stage_store(0, stage_load(0));
+            stage_store(clampg(1), stage_load(clampg(1)));
+            double e = stage_load(clampg(2));
+            __builtin_amdgcn_wave_barrier();
+            TabEntry r = ring_at(0, 0);
+            for (uint32_t g = 0; g < G; ++g) {
+                stage_store(clampg(g + 2), e);
+                e = stage_load(clampg(g + 3));
+                __builtin_amdgcn_wave_barrier();  // the stores precede later chunks' reads
+                const uint32_t k0 = g * 8, len = min(8u, n_turns - k0);
+                if (len == 8) {
+#pragma unroll
+                    for (uint32_t m = 0; m < 8; ++m) {
+                        const TabEntry rn = m + 1 < 8 ? ring_at(g, m + 1) : ring_at(g + 1, 0);
+                        turn(r);
+                        if (lost_mask() && !settle(k0 + m)) any_live = false;
+                        r = rn;
+                    }
+                } else {
+                    for (uint32_t m = 0; m < len; ++m) {
+                        turn(ring_at(g, m));
+                        if (lost_mask() && !settle(k0 + m)) any_live = false;
+                    }
+                }
+                if (!any_live) break;
+            }
+            k = n_turns;
+        } else if (n_turns >= 6) {
             ConstPtr<TabEntry> p = tp;  // record of turn k (forward) / k-th back (inverse)
             auto at = [&](int off) -> TabEntry {
                 const ConstPtr<TabEntry> r = INV ? p - off : p + off;

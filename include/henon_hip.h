@@ -98,8 +98,14 @@ typedef struct hm_launch_cfg {
     int32_t particles_per_lane; /* 1 = one per lane; 2 = one, or two for the leading wavefronts
                                    when the live particles exceed one round of resident
                                    wavefronts; 0 = 1 in exact mode, 2 in fast mode */
-    int32_t reserved;
+    int32_t table_staging;      /* how a wavefront reads the turn's table record: HM_STAGING_SCALAR
+                                   (scalar loads one half-quad ahead), HM_STAGING_RING (8-turn
+                                   chunks staged through a per-wavefront LDS ring), or
+                                   HM_STAGING_AUTO (0): the ring when a launch has at most 2
+                                   wavefronts per SIMD, where nothing else hides a scalar load's
+                                   latency */
 } hm_launch_cfg;
+enum { HM_STAGING_AUTO = 0, HM_STAGING_SCALAR = 1, HM_STAGING_RING = 2 };
 
 /* Track n particles for n_turns turns of the modulated map.
  * Replaces henon_tracker_gpu::track (src/henon.cu:3155-3185) + gpu_henon_track

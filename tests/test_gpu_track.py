@@ -159,6 +159,43 @@ def test_one_and_two_particles_per_lane_agree(mode):
         assert_bit_exact((p.x, p.px, p.y, p.py, p.steps), outs[0])
 
 
+@pytest.mark.parametrize("inverse", [False, True])
+@pytest.mark.parametrize("mode", [C.MODE_EXACT, C.MODE_FAST])
+def test_table_staging_ring_and_scalar_agree(inverse, mode):
+    """The two ways a wavefront reads its table records (scalar loads, or 8-turn
+    chunks through an LDS ring) give identical results: losses mid-chunk, a
+    turn count that is not a multiple of 8, compaction segments, the mixed
+    two-per-lane launch, and exact mode equal to the oracle."""
+    c = W.MODULATED_DA
+    x, px, y, py = W.da_grid(48)
+    n_turns = 1203
+    tables, tab = make_tables(2 * n_turns, c["tune_x"], c["tune_y"], "custom", c["epsilon"])
+    tdev = C.DeviceBuffer.from_array(tab)
+    outs = []
+    for staging in (1, 2):
+        for pair, every in ((1, 0), (2, 250), (1, -1)):
+            dev = C.DeviceParticles(x, px, y, py)
+            cfg = C.LaunchCfg(0, every, pair, staging)
+            C.track(dev, tdev, len(tab), n_turns, c["mu"], c["barrier"], False, mode, cfg)
+            if inverse:
+                C.track(dev, tdev, len(tab), 611, c["mu"], c["barrier"], True, mode, cfg)
+            outs.append(dev.get())
+    assert C.take_status() == 0
+    for o in outs[1:]:
+        assert_bit_exact(outs[0], o)
+    lost = np.isnan(outs[0][0])
+    assert 0.05 < lost.mean() < 0.95
+    if mode == C.MODE_EXACT:
+        p = O.Particles(x, px, y, py)
+        O.lib().or_track(*p.state(), p.n, *tables.ptrs(), n_turns, c["mu"], c["barrier"], np.nan, 0, NT)
+        if inverse:
+            O.lib().or_track(*p.state(), p.n, *tables.ptrs(), 611, c["mu"], c["barrier"], np.nan, 1, NT)
+        assert_bit_exact((p.x, p.px, p.y, p.py, p.steps), outs[0])
+    with pytest.raises(ValueError, match="table_staging"):
+        C.track(C.DeviceParticles(x, px, y, py), tdev, len(tab), 10, c["mu"], c["barrier"], False, mode,
+                C.LaunchCfg(0, 0, 0, 3))
+
+
 @pytest.mark.parametrize("block", [64, 128, 256])
 def test_block_sizes(block):
     x, px, y, py = W.da_grid(40)
