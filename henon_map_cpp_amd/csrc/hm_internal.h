@@ -74,6 +74,8 @@ inline void launch_check(const char *name)
 constexpr int kWorkspaceSlots = 12;
 uint32_t *status_word();
 void *workspace(size_t bytes, int slot);
+// SIMDs of the current device (CUs x 4), cached per device.
+uint32_t device_simds();
 
 constexpr size_t kTableRecord = 4;  // doubles per turn in the interleaved table
 

@@ -97,6 +97,20 @@ uint32_t *status_word()
     return d.status;
 }
 
+uint32_t device_simds()
+{
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    static std::mutex mu;
+    static std::map<int, uint32_t> simds;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = simds.find(dev);
+    if (it != simds.end()) return it->second;
+    int cus = 0;
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    return simds[dev] = (uint32_t)cus * 4u;
+}
+
 void *workspace(size_t bytes, int slot)
 {
     if (slot < 0 || slot >= kWorkspaceSlots) throw Error(HM_ERR_ARG, "workspace slot out of range");

@@ -248,7 +248,7 @@ extern "C" int hm_track(double *x, double *px, double *y, double *py, uint32_t *
         a.staging = cfg ? cfg->table_staging : HM_STAGING_AUTO;
         if (a.staging < HM_STAGING_AUTO || a.staging > HM_STAGING_RING)
             throw Error(HM_ERR_ARG, "track: table_staging must be 0 (auto), 1 (scalar) or 2 (ring)");
-        a.simds = resident_slots() / 8u;
+        a.simds = device_simds();
 
         if (every < 0 || (uint32_t)every >= n_turns) {
             a.n_arg = a.grid_bound = N;
