@@ -228,10 +228,12 @@ def reversibility_leg():
 
 def tunes_leg():
     """Trajectory store + 2048-sample FFT tunes and Birkhoff tunes of the
-    chaos-indicator particle set (256x256 grid + 65536 ball points): the
-    HBM-bound stages. The trajectory is stored as particle rows (hm_track_rows),
-    which the FFT reads in place. Bytes: 32 B per particle-sample written by the
-    store and read by each tune stage; every kernel of a stage is timed."""
+    chaos-indicator particle set (256x256 grid + 65536 ball points), as
+    all_tunes runs them: the HBM-bound stages. The trajectory is stored as
+    particle rows (hm_track_rows), which also sums the per-coordinate means that
+    both tune stages remove; the FFT reads the rows in place. Bytes: 32 B per
+    particle-sample written by the store and read by each tune stage; every
+    kernel of a stage is timed."""
     from henon_map_cpp_amd import _capi as C
     from henon_map_cpp_amd import workloads as W
 
@@ -248,15 +250,16 @@ def tunes_leg():
         C.check(C.lib().hm_stream_synchronize(None))
         C.profile_reset()
         C.profile_enable(True)
-        rows = C.track_rows(parts, tdev, T, T - 1, c["mu"], c["barrier"])   # samples 0..2047
-        fft = C.history_tunes(rows, n, T, [(0, T - 1)], "fft", row_len=L)
-        birk = C.history_tunes(rows, n, T, [(0, T - 1)], "birkhoff", row_len=L)
+        rows, means = C.track_rows(parts, tdev, T, T - 1, c["mu"], c["barrier"], means=True)  # samples 0..2047
+        fft = C.history_tunes(rows, n, T, [(0, T - 1)], "fft", row_len=L, means=means)
+        birk = C.history_tunes(rows, n, T, [(0, T - 1)], "birkhoff", row_len=L, means=means)
         h_ms = C.profile_read("k_history_rows")[0]
         f_ms = sum(C.profile_read(k)[0] for k in ("k_signal_rows", "k_fft_tune_rows", "k_fft_tune_pow2",
                                                   "k_fft_tune_dft"))
         b_ms = C.profile_read("k_birkhoff_hist")[0]
         C.profile_enable(False)
         rows.free()
+        means.free()
         if best is None or h_ms + f_ms + b_ms < sum(best):
             best = (h_ms, f_ms, b_ms)
     h_ms, f_ms, b_ms = best

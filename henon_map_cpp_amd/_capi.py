@@ -101,7 +101,7 @@ SIGNATURES = {
     "hm_history_means": (I, [P, SZ, U32, P, P]),
     "hm_birkhoff_tunes_history": (I, [P, SZ, U32, P, P, P, SZ, P, P]),
     "hm_fft_tunes": (I, [P, SZ, U32, P, P, P, SZ, P, P]),
-    "hm_track_rows": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, D, I, P, SZ, P]),
+    "hm_track_rows": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, D, I, P, SZ, P, P]),
     "hm_rows_means": (I, [P, SZ, SZ, U32, P, P]),
     "hm_birkhoff_tunes_rows": (I, [P, SZ, SZ, U32, P, P, P, SZ, P, P]),
     "hm_fft_tunes_rows": (I, [P, SZ, SZ, U32, P, P, P, SZ, P, P]),
@@ -403,23 +403,26 @@ def row_length(n_turns: int) -> int:
 
 
 def track_rows(parts: DeviceParticles, table_dev: DeviceBuffer, table_len: int, n_turns: int, mu: float,
-               barrier: float, inverse=False, stream=None) -> DeviceBuffer:
-    """The trajectory as particle rows [n][4][row_length(n_turns)]."""
+               barrier: float, inverse=False, stream=None, means=False):
+    """The trajectory as particle rows [n][4][row_length(n_turns)]; with
+    means=True also the per-coordinate means [4][n] over samples 0..n_turns,
+    returned as (rows, means)."""
     L = row_length(n_turns)
     rows = DeviceBuffer(8 * parts.n * 4 * L)
+    mb = DeviceBuffer(8 * 4 * parts.n) if means else None
     check(lib().hm_track_rows(*parts.ptrs(), parts.n, table_dev.ptr, table_len, n_turns, mu, barrier, int(inverse),
-                              rows.ptr, L, stream))
-    return rows
+                              rows.ptr, L, mb.ptr if mb is not None else None, stream))
+    return (rows, mb) if means else rows
 
 
 def history_tunes(hist: DeviceBuffer, n: int, n_samples: int, windows, kind="fft", subtract_mean=False,
-                  stream=None, row_len=None):
+                  stream=None, row_len=None, means: DeviceBuffer | None = None):
     """Tunes of windows [(from, to)] (inclusive) of a device history
     (sample-major, or particle rows of length row_len); returns numpy
-    [n_windows][2][n]."""
+    [n_windows][2][n]. subtract_mean computes the per-coordinate means of the
+    n_samples samples first; `means` passes ones already computed (track_rows)."""
     L = lib()
-    means = None
-    if subtract_mean:
+    if means is None and subtract_mean:
         means = DeviceBuffer(8 * 4 * n)
         if row_len is None:
             check(L.hm_history_means(hist.ptr, n, n_samples, means.ptr, stream))

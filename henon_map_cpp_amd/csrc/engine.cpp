@@ -597,15 +597,12 @@ class HenonTracker {
             py::gil_scoped_release nogil;
             for (size_t c0 = 0; c0 < p.n_particles; c0 += chunk) {
                 const size_t nc = std::min(chunk, p.n_particles - c0);
+                // all_tunes: the per-coordinate means come out of the same pass
                 check(hm_track_rows(p.dx.as<double>() + c0, p.dpx.as<double>() + c0, p.dy.as<double>() + c0,
                                     p.dpy.as<double>() + c0, p.dsteps.as<uint32_t>() + c0, nc, dtable.as<double>(),
                                     allowed_steps, n_turns, mu, barrier, inverse ? 1 : 0, hist.as<double>(), row_len,
-                                    nullptr));
-                const double *mp = nullptr;
-                if (all) {
-                    check(hm_rows_means(hist.as<double>(), nc, row_len, L, means.as<double>(), nullptr));
-                    mp = means.as<double>();
-                }
+                                    all ? means.as<double>() : nullptr, nullptr));
+                const double *mp = all ? means.as<double>() : nullptr;
                 check(hm_birkhoff_tunes_rows(hist.as<double>(), nc, row_len, L, mp, bf.data(), bt.data(), NW,
                                              ob.as<double>(), nullptr));
                 if (all)

@@ -78,13 +78,15 @@ k_history(double *__restrict__ gx, double *__restrict__ gpx, double *__restrict_
 // Same tracking and samples as k_history, written as particle rows
 // rows[particle][coord][L] (L a multiple of 16, samples 0..n_turns, the tail of
 // the last block padded with the final state): 16 samples per coordinate are
-// kept in registers and stored as whole 128-byte lines.
-template <bool INV, bool OCT>
+// kept in registers and stored as whole 128-byte lines. With MEANS each
+// coordinate's samples are also summed in sample order from 0.0 and divided by
+// n_turns + 1: k_means' result on the same rows, without reading them again.
+template <bool INV, bool OCT, bool MEANS>
 __global__ void __launch_bounds__(256)
 k_history_rows(double *__restrict__ gx, double *__restrict__ gpx, double *__restrict__ gy, double *__restrict__ gpy,
                uint32_t *__restrict__ gsteps, uint32_t n, const TabEntry *__restrict__ tab, uint32_t tab_len,
                uint32_t n_turns, double mu, double b2, double *__restrict__ rows, uint64_t L,
-               uint32_t *__restrict__ status)
+               double *__restrict__ means, uint32_t *__restrict__ status)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -94,6 +96,7 @@ k_history_rows(double *__restrict__ gx, double *__restrict__ gpx, double *__rest
     const double m3mu = -3.0 * mu;
     double *row = rows + (size_t)i * 4 * L;
     const uint32_t samples = n_turns + 1;
+    double sum[4] = {0.0, 0.0, 0.0, 0.0};
     for (uint32_t m0 = 0; m0 < samples; m0 += 16) {
         double buf[4][16];
 #pragma unroll
@@ -115,6 +118,9 @@ k_history_rows(double *__restrict__ gx, double *__restrict__ gpx, double *__rest
                 }
             }
             buf[0][q] = x; buf[1][q] = px; buf[2][q] = y; buf[3][q] = py;
+            if (MEANS && m < samples) {
+                sum[0] = sum[0] + x; sum[1] = sum[1] + px; sum[2] = sum[2] + y; sum[3] = sum[3] + py;
+            }
         }
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -124,6 +130,11 @@ k_history_rows(double *__restrict__ gx, double *__restrict__ gpx, double *__rest
         }
     }
     gx[i] = x; gpx[i] = px; gy[i] = y; gpy[i] = py; gsteps[i] = s;
+    if (MEANS) {
+        const size_t nn = n;
+#pragma unroll
+        for (int c = 0; c < 4; c++) means[c * nn + i] = sum[c] / (double)samples;
+    }
 }
 
 // std::accumulate order (henon.cu:3041-3051): sum from 0.0 in sample order, / L
@@ -210,24 +221,45 @@ __global__ void __launch_bounds__(256) k_birkhoff_rows(const double *__restrict_
     const double mp = means ? means[cp * nn + i] : 0.0;
     const double *r = rows + (size_t)i * 4 * L + ws.from[w];
     const uint32_t len = ws.len[w];
+    // means of the rows that are all finite prove that no sample of the
+    // particle is NaN: then only this plane's two rows are read
+    const double mo0 = means ? means[(2 - cx) * nn + i] : 0.0, mo1 = means ? means[(3 - cx) * nn + i] : 0.0;
+    const bool scan_other = !means || !(isfinite(mx) && isfinite(mp) && isfinite(mo0) && isfinite(mo1));
+    const double *rx = r + cx * L, *rp = r + cp * L, *ro0 = r + (2 - cx) * L, *ro1 = r + (3 - cx) * L;
     double acc = 0.0, carry = 0.0;
     bool bad = false;
-    for (uint32_t m0 = 0; m0 < len; m0 += 64) {
-        const uint32_t m = m0 + lane;
-        const bool valid = m < len;
-        double th = 0.0;
-        if (valid) {
-            const double a0 = r[m], a1 = r[L + m], a2 = r[2 * L + m], a3 = r[3 * L + m];
-            bad |= any_nan(a0, a1, a2, a3);
-            th = atan2((p ? a3 : a1) - mp, (p ? a2 : a0) - mx);
+    // four 64-sample steps per iteration: their loads are issued before the
+    // first angle, so several are in flight per wavefront
+    constexpr uint32_t U = 4;
+    for (uint32_t m00 = 0; m00 < len; m00 += 64 * U) {
+        double a0[U], a1[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t m = m00 + 64 * u + lane;
+            a0[u] = a1[u] = 0.0;
+            if (m < len) {
+                a0[u] = rx[m];
+                a1[u] = rp[m];
+                if (scan_other) bad |= any_nan(ro0[m], ro1[m], 0.0, 0.0);
+            }
         }
-        double prev = __shfl_up(th, 1);
-        if (lane == 0) prev = carry;
-        carry = __shfl(th, 63);
-        if (valid && m > 0) {
-            double d = th - prev;
-            if (d < 0.0) d += 2.0 * M_PI;
-            acc = acc + w_flat[ws.w_off[w] + m - 1] * d;
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t m = m00 + 64 * u + lane;
+            const bool valid = m < len;
+            double th = 0.0;
+            if (valid) {
+                bad |= any_nan(a0[u], a1[u], 0.0, 0.0);
+                th = atan2(a1[u] - mp, a0[u] - mx);
+            }
+            double prev = __shfl_up(th, 1);
+            if (lane == 0) prev = carry;
+            carry = __shfl(th, 63);
+            if (valid && m > 0) {
+                double d = th - prev;
+                if (d < 0.0) d += 2.0 * M_PI;
+                acc = acc + w_flat[ws.w_off[w] + m - 1] * d;
+            }
         }
     }
 #pragma unroll
@@ -641,27 +673,31 @@ __global__ void __launch_bounds__(kStockThreads) k_fft_tune_stockham(FftRowArgs 
     const double mo1 = a.means ? a.means[(2 * (1 - plane) + 1) * nn + pp] : 0.0;
     if (threadIdx.x == 0) any_nan_flag = 0;
     __syncthreads();
-    // a NaN in any of the four coordinates of the window marks both planes
+    // A NaN in any of the four coordinates of the window marks both planes.
+    // With means given (over every sample of the rows), four finite means
+    // prove that no sample of the particle is NaN or infinite, and the other
+    // plane's rows need not be read.
+    const bool scan_other = !a.means || !(isfinite(mx) && isfinite(mp) && isfinite(mo0) && isfinite(mo1));
     bool nan_seen = false;
-    auto load = [&](uint32_t m) -> Cplx {
+    auto load4 = [&](uint32_t m) -> Cplx {
         const double vx = rx[m] - mx, vp = rp[m] - mp;
         nan_seen |= any_nan(vx, vp, oth0[m] - mo0, oth1[m] - mo1);
         const double h = a.hann[m];
         return Cplx{vx * h, vp * h};
     };
+    auto load2 = [&](uint32_t m) -> Cplx {
+        const double vx = rx[m] - mx, vp = rp[m] - mp;
+        nan_seen |= any_nan(vx, vp, 0.0, 0.0);
+        const double h = a.hann[m];
+        return Cplx{vx * h, vp * h};
+    };
     constexpr int R8 = LOGN / 3, REM = LOGN % 3;
-    // first pass from memory
-    uint32_t Ns = 1;
-    if constexpr (R8 > 0) {
-        stockham_pass<LOGN, 8, true>(buf, Ns, a.tw, load);
-        Ns *= 8;
-    } else if constexpr (REM == 2) {
-        stockham_pass<LOGN, 4, true>(buf, Ns, a.tw, load);
-        Ns *= 4;
-    } else {
-        stockham_pass<LOGN, 2, true>(buf, Ns, a.tw, load);
-        Ns *= 2;
-    }
+    constexpr int R1 = R8 > 0 ? 8 : (REM == 2 ? 4 : 2);
+    // first pass from memory (the same pass twice: a uniform branch)
+    if (scan_other) stockham_pass<LOGN, R1, true>(buf, 1, a.tw, load4);
+    else stockham_pass<LOGN, R1, true>(buf, 1, a.tw, load2);
+    uint32_t Ns = R1;
+    auto &load = load2;  // unused by the later passes
     if (nan_seen) atomicOr(&any_nan_flag, 1);
 #pragma unroll
     for (int p = 1; p < R8; ++p) {
@@ -892,7 +928,7 @@ extern "C" int hm_track_history(double *x, double *px, double *y, double *py, ui
 
 extern "C" int hm_track_rows(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n,
                              const double *table, size_t table_len, uint32_t n_turns, double mu, double barrier,
-                             int inverse, double *rows, size_t row_len, void *stream)
+                             int inverse, double *rows, size_t row_len, double *means, void *stream)
 {
     return guarded([&] {
         if (n == 0) return;
@@ -906,21 +942,19 @@ extern "C" int hm_track_rows(double *x, double *px, double *y, double *py, uint3
         const double b2 = barrier * barrier;
         const TabEntry *tab = reinterpret_cast<const TabEntry *>(table);
         LaunchTimer timer("k_history_rows", s);
-        const bool oct = mu != 0.0;
-        const uint64_t L = row_len;
-        if (inverse) {
-            if (oct) hipLaunchKernelGGL((k_history_rows<true, true>), dim3(grid), dim3(256), 0, s, x, px, y, py, steps,
-                                        (uint32_t)n, tab, (uint32_t)table_len, n_turns, mu, b2, rows, L,
-                                        status_word());
-            else hipLaunchKernelGGL((k_history_rows<true, false>), dim3(grid), dim3(256), 0, s, x, px, y, py, steps,
-                                    (uint32_t)n, tab, (uint32_t)table_len, n_turns, mu, b2, rows, L, status_word());
+        const bool oct = mu != 0.0, inv = inverse != 0, with_means = means != nullptr;
+        using K = void (*)(double *, double *, double *, double *, uint32_t *, uint32_t, const TabEntry *, uint32_t,
+                           uint32_t, double, double, double *, uint64_t, double *, uint32_t *);
+        K k;
+        if (inv) {
+            if (oct) k = with_means ? k_history_rows<true, true, true> : k_history_rows<true, true, false>;
+            else k = with_means ? k_history_rows<true, false, true> : k_history_rows<true, false, false>;
         } else {
-            if (oct) hipLaunchKernelGGL((k_history_rows<false, true>), dim3(grid), dim3(256), 0, s, x, px, y, py,
-                                        steps, (uint32_t)n, tab, (uint32_t)table_len, n_turns, mu, b2, rows, L,
-                                        status_word());
-            else hipLaunchKernelGGL((k_history_rows<false, false>), dim3(grid), dim3(256), 0, s, x, px, y, py, steps,
-                                    (uint32_t)n, tab, (uint32_t)table_len, n_turns, mu, b2, rows, L, status_word());
+            if (oct) k = with_means ? k_history_rows<false, true, true> : k_history_rows<false, true, false>;
+            else k = with_means ? k_history_rows<false, false, true> : k_history_rows<false, false, false>;
         }
+        hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, s, x, px, y, py, steps, (uint32_t)n, tab, (uint32_t)table_len,
+                           n_turns, mu, b2, rows, (uint64_t)row_len, means, status_word());
         launch_check("k_history_rows");
     });
 }

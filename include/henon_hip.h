@@ -365,14 +365,19 @@ int hm_fft_tunes(const double *hist, size_t n, uint32_t n_samples, const double 
  * multiple of 16 of at least n_turns+1 (positions past n_turns hold the final
  * state), rows 16-byte aligned. Each particle's samples are contiguous, so the
  * tune kernels below read them in place (no corner turn); every store
- * completes 128-byte lines. */
+ * completes 128-byte lines. means: NULL, or device double[4][n] that receives
+ * each coordinate's mean over samples 0..n_turns, equal bit for bit to
+ * hm_rows_means(rows, n, row_len, n_turns + 1) (the mean removal of all_tunes,
+ * src/henon.cu:3041-3051) without a second pass over the rows. */
 int hm_track_rows(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n, const double *table,
                   size_t table_len, uint32_t n_turns, double mu, double barrier, int inverse, double *rows,
-                  size_t row_len, void *stream);
+                  size_t row_len, double *means, void *stream);
 /* hm_history_means, hm_birkhoff_tunes_history and hm_fft_tunes on particle rows;
  * n_samples <= row_len. Means and FFT tunes equal the sample-major results bit
  * for bit; Birkhoff tunes sum the same terms in a wavefront-parallel order
- * (equal to rounding). */
+ * (equal to rounding). `means` is NULL or hm_rows_means of the same rows and
+ * n_samples: four finite means of a particle then prove that none of its samples
+ * is NaN, and hm_fft_tunes_rows reads only the rows of the plane it transforms. */
 int hm_rows_means(const double *rows, size_t n, size_t row_len, uint32_t n_samples, double *means, void *stream);
 int hm_birkhoff_tunes_rows(const double *rows, size_t n, size_t row_len, uint32_t n_samples, const double *means,
                            const uint32_t *from, const uint32_t *to, size_t n_windows, double *out, void *stream);

@@ -140,13 +140,19 @@ def test_rows_history_and_tunes_equal_sample_major():
     (2048), the one-signal FFT (8192), the DFT (501), Birkhoff windows, with and
     without the mean removal."""
     n_turns, n = 8300, 257
-    x, px, y, py = W.ball(n, 0.35, seed=34)
+    x, px, y, py = W.ball(n, 0.5, seed=34)  # 3 particles lost before sample 2047, 6 after 2147
     wx, wy = C.sps_modulation(0.168, 8.0, 0, n_turns), C.sps_modulation(0.201, 8.0, 0, n_turns)
     tdev = C.DeviceBuffer.from_array(C.build_table(wx, wy))
     d1, d2 = C.DeviceParticles(x, px, y, py), C.DeviceParticles(x, px, y, py)
     hist = C.track_history(d1, tdev, n_turns, n_turns, 0.3, 10.0)
-    rows = C.track_rows(d2, tdev, n_turns, n_turns, 0.3, 10.0)
+    rows, fused_means = C.track_rows(d2, tdev, n_turns, n_turns, 0.3, 10.0, means=True)
     L = C.row_length(n_turns)
+    # the means summed while the rows are written equal a separate pass bit for bit
+    sep = C.DeviceBuffer(8 * 4 * n)
+    C.check(C.lib().hm_rows_means(rows.ptr, n, L, n_turns + 1, sep.ptr, None))
+    fm = fused_means.download(np.float64, 4 * n)
+    assert np.array_equal(fm, sep.download(np.float64, 4 * n), equal_nan=True)
+    assert np.isnan(fm).any() and np.isfinite(fm).any()
     assert L % 16 == 0 and L >= n_turns + 1
     h = hist.download(np.float64, (n_turns + 1) * 4 * n).reshape(n_turns + 1, 4, n)
     r = rows.download(np.float64, n * 4 * L).reshape(n, 4, L)
@@ -159,9 +165,44 @@ def test_rows_history_and_tunes_equal_sample_major():
         a = C.history_tunes(hist, n, n_turns + 1, windows, "fft", subtract_mean=mean)
         b = C.history_tunes(rows, n, n_turns + 1, windows, "fft", subtract_mean=mean, row_len=L)
         assert np.array_equal(a, b, equal_nan=True), mean
+        if mean:
+            c = C.history_tunes(rows, n, n_turns + 1, windows, "fft", row_len=L, means=fused_means)
+            assert np.array_equal(a, c, equal_nan=True)
+        # NaN exactly where a sample of the window (any coordinate) is NaN, and
+        # with the mean removal also where the mean over all samples is NaN:
+        # particles lost inside a window and particles lost only after it occur
+        for w, (f, t) in enumerate(windows[:2]):
+            bad = np.isnan(r[:, :, f:t + 1]).any(axis=(1, 2))
+            later = ~bad & np.isnan(r[:, :, :n_turns + 1]).any(axis=(1, 2))
+            assert bad.any() and later.any()
+            for plane in (0, 1):
+                assert np.array_equal(np.isnan(b[w, plane]), bad | later if mean else bad), (mean, w, plane)
         # Birkhoff from rows sums the same terms in a wavefront-parallel order:
         # sums of up to 8300 terms of size ~1e-3 agree to rounding (1e-14)
         a = C.history_tunes(hist, n, n_turns + 1, windows, "birkhoff", subtract_mean=mean)
         b = C.history_tunes(rows, n, n_turns + 1, windows, "birkhoff", subtract_mean=mean, row_len=L)
         assert np.array_equal(np.isnan(a), np.isnan(b))
         np.testing.assert_allclose(b, a, rtol=0, atol=2e-14, equal_nan=True)
+
+
+def test_fft_rows_nan_rule_on_given_rows():
+    """A NaN in any of the four coordinates inside a window makes both planes'
+    FFT tunes NaN, also when only the other plane holds it (rows written
+    directly, not by the tracker, which sets all four together); with the mean
+    removal, a NaN anywhere in a coordinate makes its mean, and so the tunes,
+    NaN. Finite rows give finite tunes either way."""
+    L, n_samples = 2064, 2049
+    t = np.arange(L)
+    base = np.stack([np.cos(2 * np.pi * 0.21 * t), -np.sin(2 * np.pi * 0.21 * t),
+                     np.cos(2 * np.pi * 0.17 * t), -np.sin(2 * np.pi * 0.17 * t)])
+    rows = np.repeat(base[None], 4, axis=0)       # [particle][coord][L]
+    rows[1, 3, 10] = np.nan                        # py inside the window
+    rows[2, 2, 2048:] = np.nan                     # y after the window only (sample 2048 counts in the mean)
+    rows[3, 0, 2047] = np.nan                      # x at the window's last sample
+    dev = C.DeviceBuffer.from_array(np.ascontiguousarray(rows))
+    for mean in (False, True):
+        out = C.history_tunes(dev, 4, n_samples, [(0, 2047)], "fft", subtract_mean=mean, row_len=L)[0]
+        want_nan = np.array([False, True, mean, True])
+        for plane in (0, 1):
+            assert np.array_equal(np.isnan(out[plane]), want_nan), (mean, plane)
+        assert out[0, 0] == pytest.approx(0.21, abs=1e-5) and out[1, 0] == pytest.approx(0.17, abs=1e-5)
