@@ -78,7 +78,9 @@ k_history(double *__restrict__ gx, double *__restrict__ gpx, double *__restrict_
 // Same tracking and samples as k_history, written as particle rows
 // rows[particle][coord][L] (L a multiple of 16, samples 0..n_turns, the tail of
 // the last block padded with the final state): 16 samples per coordinate are
-// kept in registers and stored as whole 128-byte lines. With MEANS each
+// kept in registers; per coordinate the wavefront passes them through LDS so
+// that each store instruction writes 8 whole 128-byte lines (8 particles x 16
+// samples) instead of touching 64 lines (one per particle). With MEANS each
 // coordinate's samples are also summed in sample order from 0.0 and divided by
 // n_turns + 1: k_means' result on the same rows, without reading them again.
 template <bool INV, bool OCT, bool MEANS>
@@ -88,13 +90,23 @@ k_history_rows(double *__restrict__ gx, double *__restrict__ gpx, double *__rest
                uint32_t n_turns, double mu, double b2, double *__restrict__ rows, uint64_t L,
                double *__restrict__ means, uint32_t *__restrict__ status)
 {
+    // per wavefront: 64 particles x 16 samples, rows padded to 18 doubles
+    // (16-byte aligned, spread over the LDS banks)
+    constexpr int kPad = 18;
+    __shared__ __align__(16) double tr_all[4 * 64 * kPad];
+    double *const tr = tr_all + (threadIdx.x >> 6) * 64 * kPad;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double x = gx[i], px = gpx[i], y = gy[i], py = gpy[i];
-    uint32_t s = gsteps[i];
-    bool alive = !any_nan(x, px, y, py);
+    const uint32_t lane = threadIdx.x & 63, wave0 = i - lane;
+    if (wave0 >= n) return;  // whole wavefronts; lanes past n run along and store nothing
+    const bool in = i < n;
+    double x = __builtin_nan(""), px = x, y = x, py = x;
+    uint32_t s = 0;
+    if (in) {
+        x = gx[i]; px = gpx[i]; y = gy[i]; py = gpy[i];
+        s = gsteps[i];
+    }
+    bool alive = in && !any_nan(x, px, y, py);
     const double m3mu = -3.0 * mu;
-    double *row = rows + (size_t)i * 4 * L;
     const uint32_t samples = n_turns + 1;
     double sum[4] = {0.0, 0.0, 0.0, 0.0};
     for (uint32_t m0 = 0; m0 < samples; m0 += 16) {
@@ -124,11 +136,23 @@ k_history_rows(double *__restrict__ gx, double *__restrict__ gpx, double *__rest
         }
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            double2 *dst = reinterpret_cast<double2 *>(row + c * L + m0);
+            double2 *mine = reinterpret_cast<double2 *>(tr + lane * kPad);
 #pragma unroll
-            for (int q = 0; q < 8; ++q) dst[q] = double2{buf[c][2 * q], buf[c][2 * q + 1]};
+            for (int q = 0; q < 8; ++q) mine[q] = double2{buf[c][2 * q], buf[c][2 * q + 1]};
+            __builtin_amdgcn_wave_barrier();  // the LDS writes precede the reads (in order per wavefront)
+            // store t: particles 8t..8t+7 of the wavefront, lane l the 16 bytes
+            // (l % 8) of particle 8t + l / 8
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const uint32_t pw = 8 * t + (lane >> 3), part = lane & 7;
+                const double2 v = reinterpret_cast<const double2 *>(tr + pw * kPad)[part];
+                if (wave0 + pw < n)
+                    reinterpret_cast<double2 *>(rows + (size_t)(wave0 + pw) * 4 * L + c * L + m0)[part] = v;
+            }
+            __builtin_amdgcn_wave_barrier();  // the reads precede the next coordinate's writes
         }
     }
+    if (!in) return;
     gx[i] = x; gpx[i] = px; gy[i] = y; gpy[i] = py; gsteps[i] = s;
     if (MEANS) {
         const size_t nn = n;
