@@ -252,9 +252,9 @@ __global__ void __launch_bounds__(256) k_birkhoff_rows(const double *__restrict_
     const double *rx = r + cx * L, *rp = r + cp * L, *ro0 = r + (2 - cx) * L, *ro1 = r + (3 - cx) * L;
     double acc = 0.0, carry = 0.0;
     bool bad = false;
-    // four 64-sample steps per iteration: their loads are issued before the
+    // eight 64-sample steps per iteration: their loads are issued before the
     // first angle, so several are in flight per wavefront
-    constexpr uint32_t U = 4;
+    constexpr uint32_t U = 8;
     for (uint32_t m00 = 0; m00 < len; m00 += 64 * U) {
         double a0[U], a1[U];
 #pragma unroll
