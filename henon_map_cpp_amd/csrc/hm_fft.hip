@@ -315,21 +315,27 @@ __device__ __forceinline__ double interp_A(double a, double b, double c)
 }
 
 // dynamic_indicator.cpp:139-176 on magnitudes mag[0..N): first maximum wins
-__device__ double interp_tune(const double *mag, uint32_t N, uint32_t index)
+// `mag(k)` gives the magnitude of spectrum entry k.
+template <class Mag>
+__device__ double interp_tune_at(Mag mag, uint32_t N, uint32_t index)
 {
     const double Nd = (double)N;
     if (index == 0) return 1.0;
     if (index == N - 1) return 0.0;
     uint32_t i1, i2;
-    if (mag[index - 1] > mag[index + 1]) {
+    if (mag(index - 1) > mag(index + 1)) {
         i1 = index - 1; i2 = index; index = index - 1;
     } else {
         i1 = index; i2 = index + 1;
     }
     const double value = ((double)index / Nd) +
-                         (1.0 / (2.0 * M_PI)) * asin(interp_A(mag[i1], mag[i2], cos(2.0 * M_PI / Nd)) *
+                         (1.0 / (2.0 * M_PI)) * asin(interp_A(mag(i1), mag(i2), cos(2.0 * M_PI / Nd)) *
                                                      sin(2.0 * M_PI / Nd));
     return fabs(1.0 - value);
+}
+__device__ double interp_tune(const double *mag, uint32_t N, uint32_t index)
+{
+    return interp_tune_at([&](uint32_t k) { return mag[k]; }, N, index);
 }
 
 // Signal of block b: XCD-aware order, so that workgroups dispatched to the same
@@ -735,30 +741,25 @@ __global__ void __launch_bounds__(kStockThreads) k_fft_tune_stockham(FftRowArgs 
         if (threadIdx.x == 0) *o = __builtin_nan("");
         return;
     }
-    // magnitudes (into the first half of the buffer) and their first maximum
+    // magnitudes and their first maximum; the interpolation recomputes the
+    // three magnitudes it needs from the spectrum, which stays in LDS
+    auto magnitude = [&](uint32_t k) {
+        const Cplx z = buf[k];
+        return sqrt(z.re * z.re + z.im * z.im);
+    };
     double best = -1.0;
     uint32_t best_i = 0xffffffffu;
-    double mg[(N + kStockThreads - 1) / kStockThreads];
 #pragma unroll
     for (uint32_t q = 0; q < (N + kStockThreads - 1) / kStockThreads; ++q) {
         const uint32_t k = threadIdx.x + q * kStockThreads;
-        mg[q] = 0.0;
         if (k < N) {
-            const Cplx z = buf[k];
-            mg[q] = sqrt(z.re * z.re + z.im * z.im);
-            if (mg[q] > best || best_i == 0xffffffffu) { best = mg[q]; best_i = k; }
+            const double mg = magnitude(k);
+            if (mg > best || best_i == 0xffffffffu) { best = mg; best_i = k; }
         }
-    }
-    __syncthreads();
-    double *mag = reinterpret_cast<double *>(buf);
-#pragma unroll
-    for (uint32_t q = 0; q < (N + kStockThreads - 1) / kStockThreads; ++q) {
-        const uint32_t k = threadIdx.x + q * kStockThreads;
-        if (k < N) mag[k] = mg[q];
     }
     uint32_t bi;
     argmax_first(best, best_i, sv, si, bi);
-    if (threadIdx.x == 0) *o = interp_tune(mag, N, bi);
+    if (threadIdx.x == 0) *o = interp_tune_at(magnitude, N, bi);
 }
 
 // Direct DFT for any N: X_k = sum_m z_m exp(-2 pi i k m / N), twiddle index
