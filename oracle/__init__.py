@@ -519,8 +519,10 @@ def indicators(p: Particles, tables: Tables, n_turns, mu, barrier=10.0, renorm_e
     s = _u32(samples)
     megno = np.empty((s.size, n))
     fli = np.empty((s.size, n))
-    m = (np.tile(np.eye(4), (n, 1, 1)) if matrix is None else np.array(matrix, dtype=np.float64)).copy()
-    ls = np.zeros(n) if log_scale is None else np.array(log_scale, dtype=np.float64).copy()
+    m = np.ascontiguousarray(np.tile(np.eye(4), (n, 1, 1)) if matrix is None else np.array(matrix, dtype=np.float64))
+    ls = np.ascontiguousarray(np.zeros(n) if log_scale is None else np.array(log_scale, dtype=np.float64))
+    if m.shape != (n, 4, 4) or ls.shape != (n,):
+        raise ValueError("matrix must be [n][4][4] and log_scale [n]")
     bf = _u32([w[0] for w in birkhoff])
     bt = _u32([w[1] for w in birkhoff])
     bout = np.empty((bf.size, 2, n))

@@ -8,6 +8,7 @@ import pytest
 import torch
 
 import oracle as O
+from henon_map_cpp_amd import _capi as C
 from henon_map_cpp_amd import adjoint as A
 from henon_map_cpp_amd import workloads as W
 
@@ -140,3 +141,69 @@ def test_adjoint_windows_match_one_pass(case, every, segments):
     assert torch.equal(gx, gxw) and torch.equal(gp, gpw)
     with pytest.raises(Exception, match="scratch"):
         adj.backward(ck, alive, at, ct, scratch_bytes=seg - 8)
+
+
+@pytest.fixture(scope="module")
+def config_subset():
+    """The full adjoint workload (65 536 points, r ~ U[0.05, 0.4], 1e4 turns,
+    eps 16, mu 0.5, seeded 7-line modulation): the 64 most regular of its first
+    512 particles by the oracle's FLI, their final tangent matrices M_T and
+    random seeds."""
+    c = W.ADJOINT_GRADIENTS
+    T = c["n_turns"]
+    x0 = np.stack(W.shell_uniform_radius(c["n"], c["r_min"], c["r_max"], c["seed"]))
+    coef, freq = W.seeded_line_coefficients(), W.line_frequencies()
+
+    def tables(tx, ty, eps):
+        return O.Tables(C.lines_modulation(tx, eps, coef, freq, 0, T), C.lines_modulation(ty, eps, coef, freq, 0, T))
+
+    tb = tables(c["tune_x"], c["tune_y"], c["epsilon"])
+    p = O.Particles(*[np.ascontiguousarray(v[:512]) for v in x0])
+    fli = O.indicators(p, tb, T, c["mu"], c["barrier"], 100, [T], n_threads=16)["fli"][0]
+    sub = np.sort(np.argsort(np.where(np.isfinite(p.x), fli, np.inf))[:64])
+    p = O.Particles(*[np.ascontiguousarray(v[sub]) for v in x0])
+    M = O.indicators(p, tb, T, c["mu"], c["barrier"], 0, [T], n_threads=16)["matrix"]
+    a = np.random.default_rng(7).normal(size=(4, sub.size))
+    return dict(c=c, T=T, x0=x0, coef=coef, freq=freq, tables=tables, sub=sub, M=M, final=p, a=a)
+
+
+@pytest.mark.parametrize("every", [64, 256, 1024])
+def test_adjoint_full_workload_vs_jacobian_and_differences(config_subset, every):
+    """At the workload's full size and checkpoint intervals: the forward equals
+    the oracle's tracking, dL/dx0 equals M_T^T a within 1e-8 of its largest
+    component, and the parameter gradients equal central differences of the
+    oracle's tracking (regular orbits; steps mu 1e-6, eps 1e-3, tunes 1e-8,
+    whose own truncation and rounding, measured by changing the step threefold,
+    stay below 2e-6 relative for mu and eps and 2e-4 for the tunes)."""
+    s = config_subset
+    c, T, sub, a = s["c"], s["T"], s["sub"], s["a"]
+    adj = A.OrbitAdjoint(T, c["tune_x"], c["tune_y"], c["epsilon"], c["mu"], c["barrier"], coef=s["coef"],
+                         freq=s["freq"], every=every)
+    st, birk, alive, ck = adj.forward(torch.as_tensor(s["x0"], device="cuda"))
+    seed = np.zeros((4, c["n"]))
+    seed[:, sub] = a
+    gx, gp = adj.backward(ck, alive, torch.as_tensor(seed, device="cuda"))
+    st, alive, gx, gp = st.cpu().numpy(), alive.cpu().numpy(), gx.cpu().numpy(), gp.cpu().numpy()
+    f = s["final"]
+    assert np.all(alive[sub] == 1)
+    assert np.array_equal(st[:, sub], np.stack([f.x, f.px, f.y, f.py]))
+    want = np.einsum("nij,in->jn", s["M"], a)
+    scale = np.abs(want).max(axis=0, keepdims=True)
+    assert np.all(np.abs(gx[:, sub] - want) <= 1e-8 * scale)
+    others = np.setdiff1d(np.arange(c["n"]), sub)
+    assert np.all(gx[:, others] == 0) and np.all(gp[:, others] == 0)
+    g = gp[:, sub].sum(axis=1)  # d/d eps, d/d mu, d/d omega_x0, d/d omega_y0
+    base = dict(mu=c["mu"], eps=c["epsilon"], tx=c["tune_x"], ty=c["tune_y"])
+
+    def loss(**kw):
+        k = dict(base)
+        k.update(kw)
+        q = O.Particles(*[np.ascontiguousarray(v[sub]) for v in s["x0"]])
+        O.Tracker.from_tables(s["tables"](k["tx"], k["ty"], k["eps"])).track(q, T, k["mu"], barrier=c["barrier"],
+                                                                            n_threads=16)
+        return float(np.sum(a * np.stack([q.x, q.px, q.y, q.py])))
+
+    for name, h, got, rel in (("eps", 1e-3, g[0], 2e-5), ("mu", 1e-6, g[1], 2e-5),
+                              ("tx", 1e-8, 2 * np.pi * g[2], 1e-3), ("ty", 1e-8, 2 * np.pi * g[3], 1e-3)):
+        fd = (loss(**{name: base[name] + h}) - loss(**{name: base[name] - h})) / (2 * h)
+        assert got == pytest.approx(fd, rel=rel), name
