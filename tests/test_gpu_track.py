@@ -70,6 +70,26 @@ def test_modulated_da_slice_bit_exact(kind):
     assert_bit_exact(ref, got)
 
 
+def test_full_modulated_scan_sampled_bit_exact():
+    """The full modulated DA scan (1000x1000 grid, 1e5 turns, mu=0.5, seeded
+    7-line modulation at eps=32: the benchmark's workload) with the default
+    launch shape and loss compaction; every 499th particle (2005, across the
+    whole grid) equals the oracle's 1e5-turn tracking bit for bit."""
+    c = W.MODULATED_DA
+    x, px, y, py = W.da_grid(c["n_side"])
+    tables, tab = make_tables(c["n_turns"], c["tune_x"], c["tune_y"], "custom", c["epsilon"])
+    dev = C.DeviceParticles(x, px, y, py)
+    C.track(dev, C.DeviceBuffer.from_array(tab), len(tab), c["n_turns"], c["mu"], c["barrier"])
+    got = dev.get()
+    assert C.take_status() == 0
+    pick = np.arange(0, x.size, 499)
+    p = O.Particles(x[pick], px[pick], y[pick], py[pick])
+    O.lib().or_track(*p.state(), p.n, *tables.ptrs(), c["n_turns"], c["mu"], c["barrier"], np.nan, 0, NT)
+    assert_bit_exact((p.x, p.px, p.y, p.py, p.steps), tuple(g[pick] for g in got))
+    survived = got[4] == c["n_turns"]
+    assert 0.2 < survived.mean() < 0.8 and (got[4] < 1000).any()
+
+
 @pytest.mark.parametrize("mu", [0.0, 0.5])
 def test_inverse_tracking_bit_exact(mu):
     x, px, y, py = W.ball(4096, 0.45, seed=11)
