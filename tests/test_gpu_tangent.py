@@ -119,3 +119,31 @@ def test_construct_kernels_match_reference_formulas():
     r = [np.zeros(n) for _ in range(3)]
     O.megno_construct_add(r[0], r[1], r[2], want, np.tile(np.eye(4), (n, 1, 1)).reshape(n, 16), 1)
     np.testing.assert_allclose(l[2].download(np.float64, n), r[2], rtol=1e-14)
+
+
+def test_full_indicator_workload_sampled():
+    """The full chaos-indicator workload (256x256 grid + 65536 ball points, 1e5
+    turns, mu=0.5, seeded 7-line modulation at eps=32, renormalisation every 100
+    turns, samples 1e3, 1e4, 1e5) in exact mode; every 257th particle (511,
+    grid and ball) against the oracle: state, survival turns and the tangent
+    matrix bit for bit, the log10 scale to 1e-13, MEGNO and FLI to 1e-11 (sums
+    of 1e5 logarithms, whose last bits differ between the device and glibc)."""
+    c = W.CHAOS_INDICATORS
+    T = c["n_turns"]
+    x, px, y, py = W.indicator_points()
+    ot, tab, tdev = tables(T, c["tune_x"], c["tune_y"], c["epsilon"], "custom")
+    dev = C.DeviceParticles(x, px, y, py)
+    got = C.track_indicators(dev, tdev, T, T, c["mu"], c["barrier"], C.MODE_EXACT, c["renorm_every"], c["samples"],
+                             matrix=True)
+    gx, gpx, gy, gpy, gs = dev.get()
+    pick = np.arange(0, x.size, 257)
+    p = O.Particles(x[pick], px[pick], y[pick], py[pick])
+    want = O.indicators(p, ot, T, c["mu"], c["barrier"], c["renorm_every"], c["samples"], n_threads=NT)
+    assert np.array_equal(gx[pick], p.x, equal_nan=True) and np.array_equal(gpy[pick], p.py, equal_nan=True)
+    assert np.array_equal(gs[pick], p.steps)
+    assert np.array_equal(got["matrix"][pick], want["matrix"], equal_nan=True)
+    np.testing.assert_allclose(got["log_scale"][pick], want["log_scale"], rtol=1e-13, equal_nan=True)
+    for key in ("megno", "fli"):
+        np.testing.assert_allclose(got[key][:, pick], want[key], rtol=1e-11, atol=1e-12, equal_nan=True)
+    lost = np.isnan(p.x)
+    assert lost.any() and (~lost).any()
