@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--legs", default="indicators,reversibility,tunes,adjoint,batch",
                     help="extra measurements after the headline: indicators, adjoint, batch (comma list, or none)")
     ap.add_argument("--batch-jobs", type=int, default=512)
+    ap.add_argument("--batch-max-turns", type=int, default=100_000,
+                    help="upper end of the batch leg's log-uniform turn counts (the full job mix: 1000000)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run `--steps` device-to-device copies of MIB MiB (the known-size copy that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters), then exit")
@@ -347,14 +349,15 @@ def adjoint_leg():
     return out
 
 
-def batch_leg(n_jobs):
+def batch_leg(n_jobs, max_turns=100_000):
     """Uneven independent jobs in one batched launch sequence against one plain
     hm_track call per job (same GPU, same exact arithmetic), wall time each."""
     from henon_map_cpp_amd import _capi as C
     from henon_map_cpp_amd import workloads as W
 
-    c = W.BATCHED_JOBS
-    params = W.batch_job_params(n_jobs)
+    c = dict(W.BATCHED_JOBS, t_range=(W.BATCHED_JOBS["t_range"][0], max_turns))
+    c["name"] = c["name"].replace("logU[1e3, 1e5]", f"logU[1e3, {max_turns:.0e}]")
+    params = W.batch_job_params(n_jobs, c)
     coef, freq = W.seeded_line_coefficients(), W.line_frequencies()
     ics, tabs = [], []
     for (n, T, qx, qy, eps, mu, seed) in params:
@@ -385,7 +388,11 @@ def batch_leg(n_jobs):
         megs = []
         _sync()
         t0 = time.perf_counter()
-        for pp, tb, p, m in zip(parts, tabs, params, megno_job):
+        for k, (pp, tb, p, m) in enumerate(zip(parts, tabs, params, megno_job)):
+            if k and k % 512 == 0:
+                C.check(C.lib().hm_stream_synchronize(None))
+                print(f"[bench] batch: {k} of {n_jobs} plain calls, {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                      flush=True)
             if m:
                 r = C.track_indicators(pp, tb, p[1], p[1], p[5], c["barrier"], mode=C.MODE_EXACT, renorm_every=100,
                                        samples=[p[1]])
@@ -396,10 +403,19 @@ def batch_leg(n_jobs):
         C.check(C.lib().hm_stream_synchronize(None))
         return time.perf_counter() - t0, parts, megs
 
-    batched()
-    tb, pb, mb = min((batched() for _ in range(2)), key=lambda r: r[0])
+    def note(what, t):
+        print(f"[bench] batch: {what} {t:.2f} s", file=sys.stderr, flush=True)
+
+    note("warm-up", batched()[0])
+    runs = []
+    for _ in range(2):
+        runs.append(batched())
+        note("batched", runs[-1][0])
+    tb, pb, mb = min(runs, key=lambda r: r[0])
     tg = batched(graph=True)[0]
+    note("batched, graph replay", tg)
     tp, pp, mp = plain()
+    note("one call per job", tp)
     identical = all(np.array_equal(u, v, equal_nan=True) for a, b in zip(pb, pp) for u, v in zip(a.get(), b.get()))
     identical = identical and all(
         np.array_equal(a.download(np.float64, len(b)), b, equal_nan=True) for a, b in zip(mb, mp) if a is not None)
@@ -536,7 +552,7 @@ def main():
         line["adjoint"] = adjoint_leg()
         print(f"[bench] adjoint {line['adjoint']}", file=sys.stderr, flush=True)
     if "batch" in legs:
-        line["batch"] = batch_leg(a.batch_jobs)
+        line["batch"] = batch_leg(a.batch_jobs, a.batch_max_turns)
         print(f"[bench] batch {line['batch']}", file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:
         try:
