@@ -198,16 +198,19 @@ __global__ void k_birkhoff_hist(const double *__restrict__ hist, HistLayout h, u
     if (w >= ws.count) return;
     const size_t nn = n;
     const int cx = 2 * p, cp = 2 * p + 1;
-    const double mx = means ? means[cx * nn + i] : 0.0;
-    const double mp = means ? means[cp * nn + i] : 0.0;
+    double mc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (means)
+        for (int q = 0; q < 4; q++) mc[q] = means[q * nn + i];
+    const double mx = mc[cx], mp = mc[cp];
     double acc = 0.0, th_prev = 0.0;
     bool bad = false;
     for (uint32_t m = 0; m < ws.len[w]; m++) {
         const uint64_t sm = ws.from[w] + m;
         const double a0 = hist[hat(h, sm, 0, i)], a1 = hist[hat(h, sm, 1, i)];
         const double a2 = hist[hat(h, sm, 2, i)], a3 = hist[hat(h, sm, 3, i)];
-        // any NaN among the four coordinates of the window
-        bad |= any_nan(a0, a1, a2, a3);
+        // any NaN among the four coordinates of the window after the mean
+        // removal (dynamic_indicator.cpp:228-241 on the mean-removed vectors)
+        bad |= any_nan(a0 - mc[0], a1 - mc[1], a2 - mc[2], a3 - mc[3]);
         const double xv = (p ? a2 : a0) - mx;
         const double pv = (p ? a3 : a1) - mp;
         const double th = atan2(pv, xv);
@@ -245,9 +248,15 @@ __global__ void __launch_bounds__(256) k_birkhoff_rows(const double *__restrict_
     const double mp = means ? means[cp * nn + i] : 0.0;
     const double *r = rows + (size_t)i * 4 * L + ws.from[w];
     const uint32_t len = ws.len[w];
-    // means of the rows that are all finite prove that no sample of the
-    // particle is NaN: then only this plane's two rows are read
+    // NaN rule on the mean-removed samples (as k_birkhoff_hist): a NaN mean
+    // makes every sample of its coordinate NaN, so the tune is NaN without
+    // reading the rows; four finite means prove that no sample is NaN, and
+    // only this plane's two rows are read
     const double mo0 = means ? means[(2 - cx) * nn + i] : 0.0, mo1 = means ? means[(3 - cx) * nn + i] : 0.0;
+    if (any_nan(mx, mp, mo0, mo1)) {
+        if (lane == 0) out[((size_t)w * 2 + p) * nn + i] = __builtin_nan("");
+        return;  // whole wavefronts
+    }
     const bool scan_other = !means || !(isfinite(mx) && isfinite(mp) && isfinite(mo0) && isfinite(mo1));
     const double *rx = r + cx * L, *rp = r + cp * L, *ro0 = r + (2 - cx) * L, *ro1 = r + (3 - cx) * L;
     double acc = 0.0, carry = 0.0;
@@ -264,7 +273,7 @@ __global__ void __launch_bounds__(256) k_birkhoff_rows(const double *__restrict_
             if (m < len) {
                 a0[u] = rx[m];
                 a1[u] = rp[m];
-                if (scan_other) bad |= any_nan(ro0[m], ro1[m], 0.0, 0.0);
+                if (scan_other) bad |= any_nan(ro0[m] - mo0, ro1[m] - mo1, 0.0, 0.0);
             }
         }
 #pragma unroll
@@ -273,7 +282,7 @@ __global__ void __launch_bounds__(256) k_birkhoff_rows(const double *__restrict_
             const bool valid = m < len;
             double th = 0.0;
             if (valid) {
-                bad |= any_nan(a0[u], a1[u], 0.0, 0.0);
+                bad |= any_nan(a0[u] - mx, a1[u] - mp, 0.0, 0.0);
                 th = atan2(a1[u] - mp, a0[u] - mx);
             }
             double prev = __shfl_up(th, 1);
@@ -393,14 +402,17 @@ __global__ void __launch_bounds__(256) k_fft_tune_pow2(FftArgs a)
     const uint32_t logn = 31 - __builtin_clz(N);
     const size_t nn = a.n;
     const int cx = 2 * plane, cp = 2 * plane + 1;
-    const double mx = a.means ? a.means[cx * nn + particle] : 0.0;
-    const double mp = a.means ? a.means[cp * nn + particle] : 0.0;
+    double mc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (a.means)
+        for (int c = 0; c < 4; c++) mc[c] = a.means[c * nn + particle];
+    const double mx = mc[cx], mp = mc[cp];
     if (threadIdx.x == 0) any_nan_flag = 0;
     __syncthreads();
     bool nan_seen = false;
     for (uint32_t m = threadIdx.x; m < N; m += blockDim.x) {
         const uint64_t sm = a.from[window] + m;
-        for (int c = 0; c < 4; c++) nan_seen |= __builtin_isnan(a.hist[hat(a.lay, sm, c, particle)]);
+        // the NaN rule applies to the mean-removed samples of all four coordinates
+        for (int c = 0; c < 4; c++) nan_seen |= __builtin_isnan(a.hist[hat(a.lay, sm, c, particle)] - mc[c]);
         const double h = a.hann[m];
         const uint32_t r = __brev(m) >> (32 - logn);
         re[r] = (a.hist[hat(a.lay, sm, cx, particle)] - mx) * h;
@@ -701,6 +713,13 @@ __global__ void __launch_bounds__(kStockThreads) k_fft_tune_stockham(FftRowArgs 
     const double mp = a.means ? a.means[(2 * plane + 1) * nn + pp] : 0.0;
     const double mo0 = a.means ? a.means[(2 * (1 - plane)) * nn + pp] : 0.0;
     const double mo1 = a.means ? a.means[(2 * (1 - plane) + 1) * nn + pp] : 0.0;
+    double *o = a.out + ((size_t)a.window * 2 + plane) * nn + pp;
+    if (any_nan(mx, mp, mo0, mo1)) {
+        // a NaN mean makes every sample of its coordinate NaN after the mean
+        // removal: the tune is NaN, nothing needs reading
+        if (threadIdx.x == 0) *o = __builtin_nan("");
+        return;  // the whole workgroup
+    }
     if (threadIdx.x == 0) any_nan_flag = 0;
     __syncthreads();
     // A NaN in any of the four coordinates of the window marks both planes.
@@ -736,7 +755,6 @@ __global__ void __launch_bounds__(kStockThreads) k_fft_tune_stockham(FftRowArgs 
     }
     if constexpr (R8 > 0 && REM == 2) stockham_pass<LOGN, 4, false>(buf, Ns, a.tw, load);
     if constexpr (R8 > 0 && REM == 1) stockham_pass<LOGN, 2, false>(buf, Ns, a.tw, load);
-    double *o = a.out + ((size_t)a.window * 2 + plane) * nn + pp;
     if (any_nan_flag) {
         if (threadIdx.x == 0) *o = __builtin_nan("");
         return;
@@ -777,8 +795,10 @@ __global__ void __launch_bounds__(256) k_fft_tune_dft(FftArgs a, double *__restr
     const uint32_t N = a.N;
     const size_t nn = a.n;
     const int cx = 2 * plane, cp = 2 * plane + 1;
-    const double mx = a.means ? a.means[cx * nn + particle] : 0.0;
-    const double mp = a.means ? a.means[cp * nn + particle] : 0.0;
+    double mc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (a.means)
+        for (int c = 0; c < 4; c++) mc[c] = a.means[c * nn + particle];
+    const double mx = mc[cx], mp = mc[cp];
     double *mag = mag_scratch + (size_t)blockIdx.x * N;
     if (threadIdx.x == 0) any_nan_flag = 0;
     // each thread owns outputs k = t, t + 256, ...; at most 64 per thread here
@@ -793,7 +813,7 @@ __global__ void __launch_bounds__(256) k_fft_tune_dft(FftArgs a, double *__restr
         bool nan_seen = false;
         for (uint32_t m = threadIdx.x; m < cn; m += blockDim.x) {
             const uint64_t sm = a.from[window] + c0 + m;
-            for (int c = 0; c < 4; c++) nan_seen |= __builtin_isnan(a.hist[hat(a.lay, sm, c, particle)]);
+            for (int c = 0; c < 4; c++) nan_seen |= __builtin_isnan(a.hist[hat(a.lay, sm, c, particle)] - mc[c]);
             const double h = a.hann[c0 + m];
             zr[m] = (a.hist[hat(a.lay, sm, cx, particle)] - mx) * h;
             zi[m] = (a.hist[hat(a.lay, sm, cp, particle)] - mp) * h;

@@ -185,12 +185,16 @@ def test_rows_history_and_tunes_equal_sample_major():
         np.testing.assert_allclose(b, a, rtol=0, atol=2e-14, equal_nan=True)
 
 
-def test_fft_rows_nan_rule_on_given_rows():
+@pytest.mark.parametrize("layout", ["rows", "sample_major"])
+@pytest.mark.parametrize("kind", ["fft", "birkhoff"])
+def test_tunes_nan_rule_on_given_samples(kind, layout):
     """A NaN in any of the four coordinates inside a window makes both planes'
-    FFT tunes NaN, also when only the other plane holds it (rows written
+    tunes NaN, also when only the other plane holds it (samples written
     directly, not by the tracker, which sets all four together); with the mean
-    removal, a NaN anywhere in a coordinate makes its mean, and so the tunes,
-    NaN. Finite rows give finite tunes either way."""
+    removal, a NaN anywhere in a coordinate makes its mean, and so every
+    mean-removed sample and both tunes, NaN (dynamic_indicator.cpp:228-241
+    checks the mean-removed vectors). Finite samples give finite tunes either
+    way, for the FFT and the Birkhoff tunes and both history layouts."""
     L, n_samples = 2064, 2049
     t = np.arange(L)
     base = np.stack([np.cos(2 * np.pi * 0.21 * t), -np.sin(2 * np.pi * 0.21 * t),
@@ -199,9 +203,13 @@ def test_fft_rows_nan_rule_on_given_rows():
     rows[1, 3, 10] = np.nan                        # py inside the window
     rows[2, 2, 2048:] = np.nan                     # y after the window only (sample 2048 counts in the mean)
     rows[3, 0, 2047] = np.nan                      # x at the window's last sample
-    dev = C.DeviceBuffer.from_array(np.ascontiguousarray(rows))
+    if layout == "rows":
+        dev, row_len = C.DeviceBuffer.from_array(np.ascontiguousarray(rows)), L
+    else:                                          # [sample][coord][particle]
+        hist = np.ascontiguousarray(rows[:, :, :n_samples].transpose(2, 1, 0))
+        dev, row_len = C.DeviceBuffer.from_array(hist), None
     for mean in (False, True):
-        out = C.history_tunes(dev, 4, n_samples, [(0, 2047)], "fft", subtract_mean=mean, row_len=L)[0]
+        out = C.history_tunes(dev, 4, n_samples, [(0, 2047)], kind, subtract_mean=mean, row_len=row_len)[0]
         want_nan = np.array([False, True, mean, True])
         for plane in (0, 1):
             assert np.array_equal(np.isnan(out[plane]), want_nan), (mean, plane)
