@@ -3,6 +3,7 @@
 Exact mode is compared bit for bit (coordinates, NaN pattern and survival
 turns) with the oracle on the same inputs and tables; fast mode (fused
 multiply-adds) within the stated tolerances."""
+import ctypes
 import os
 
 import numpy as np
@@ -312,3 +313,39 @@ def test_autotune_leaves_inputs_and_results_unchanged():
         assert_bit_exact(ref, got)
     finally:
         C.check(C.lib().hm_autotune_clear())
+
+
+def test_zero_particles_are_a_no_op():
+    """Every particle-array entry point accepts n = 0 (the reference's calls over
+    empty vectors return without work) and launches nothing: NULL particle
+    arrays are not touched, the status word stays clear."""
+    L = C.lib()
+    nul = None
+    tab = C.DeviceBuffer.from_array(C.build_table(np.full(16, 0.3), np.full(16, 0.4)))
+    cfg = C.LaunchCfg(0, 0, 0, 0)
+    spec = C.IndicatorSpec()
+    f = np.zeros(1, np.uint32)
+    ln = np.full(1, 8, np.uint32)
+    calls = {
+        "hm_track": lambda: L.hm_track(nul, nul, nul, nul, nul, 0, tab.ptr, 16, 8, 0.5, 10.0, float("nan"), 0,
+                                       C.MODE_EXACT, ctypes.byref(cfg), nul),
+        "hm_track_kicked": lambda: L.hm_track_kicked(nul, nul, nul, nul, nul, 0, tab.ptr, 16, 8, 0.5, 10.0, 1e-8, 0,
+                                                     C.MODE_EXACT, 1, 0, nul),
+        "hm_track_indicators": lambda: L.hm_track_indicators(nul, nul, nul, nul, nul, 0, tab.ptr, 16, 8, 0.5, 10.0,
+                                                             C.MODE_EXACT, ctypes.byref(spec), nul),
+        "hm_track_history": lambda: L.hm_track_history(nul, nul, nul, nul, nul, 0, tab.ptr, 16, 8, 0.5, 10.0, 0, nul,
+                                                       nul),
+        "hm_track_rows": lambda: L.hm_track_rows(nul, nul, nul, nul, nul, 0, tab.ptr, 16, 8, 0.5, 10.0, 0, nul, 16,
+                                                 nul, nul),
+        "hm_fft_tunes_rows": lambda: L.hm_fft_tunes_rows(nul, 0, 16, 8, nul, C.ptr(f), C.ptr(ln), 1, nul, nul),
+        "hm_birkhoff_tunes_rows": lambda: L.hm_birkhoff_tunes_rows(nul, 0, 16, 8, nul, C.ptr(f), C.ptr(ln + f - 1),
+                                                                   1, nul, nul),
+        "hm_observable": lambda: L.hm_observable(C.OBS_RADIUS, nul, nul, nul, nul, 0, nul, nul),
+    }
+    st = C.U32(0)
+    C.check(L.hm_take_status(ctypes.byref(st), None))  # clear what earlier calls left
+    for name, call in calls.items():
+        assert call() == 0, (name, C.lib().hm_last_error())
+    C.check(L.hm_stream_synchronize(None))
+    C.check(L.hm_take_status(ctypes.byref(st), None))
+    assert st.value == 0
