@@ -16,6 +16,7 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <unordered_set>
 #include <vector>
 
@@ -151,8 +152,24 @@ void random_4d_kick(double &x, double &px, double &y, double &py, double module,
 // of n_base ghosts per add_ghost call (src/henon.cu:737-817); every accessor
 // that returns per-particle arrays returns all entries, as the reference's GPU
 // class does (src/henon.cu:1444-1690), and tracking advances all of them.
+// One coordinate array as numpy hands it over (converted to contiguous float64
+// if it is anything else, lists included): read with one copy, not element by
+// element through the sequence protocol.
+using Coords = py::array_t<double, py::array::c_style | py::array::forcecast>;
+std::vector<double> coords_vector(const Coords &a)
+{
+    if (a.ndim() != 1) throw py::type_error("particles_4d: x, px, y, py must be one-dimensional");
+    std::vector<double> v((size_t)a.shape(0));
+    if (!v.empty()) std::memcpy(v.data(), a.data(), v.size() * sizeof(double));
+    return v;
+}
+
 class Particles4D {
   public:
+    Particles4D(const Coords &x, const Coords &px, const Coords &y, const Coords &py)
+        : Particles4D(coords_vector(x), coords_vector(px), coords_vector(y), coords_vector(py))
+    {
+    }
     Particles4D(const std::vector<double> &x, const std::vector<double> &px, const std::vector<double> &y,
                 const std::vector<double> &py)
     {
@@ -1279,7 +1296,20 @@ struct PyHenonTracker : HenonTracker {
     }
 };
 
-template <class T> py::array as_array(const std::vector<T> &v) { return py::array(py::cast(v)); }
+// The arrays the reference's `py::array out = py::cast(vector)` gives
+// (src/bind.cpp:256-275): float64 for double vectors, int64 for the integer ones
+// (numpy's type for a list of Python ints), float64 when empty; filled directly
+// instead of through a Python list.
+template <class T> py::array as_array(const std::vector<T> &v)
+{
+    using Out = std::conditional_t<std::is_floating_point<T>::value, double, int64_t>;
+    if (v.empty()) return py::array_t<double>(0);
+    py::array_t<Out> a((py::ssize_t)v.size());
+    Out *o = a.mutable_data();
+    if (std::is_same<T, Out>::value) std::memcpy(o, v.data(), v.size() * sizeof(T));
+    else for (size_t i = 0; i < v.size(); ++i) o[i] = static_cast<Out>(v[i]);
+    return a;
+}
 
 }  // namespace
 
@@ -1310,6 +1340,7 @@ PYBIND11_MODULE(henon_map_engine, m)
     m.def("birkhoff_weights", &birkhoff_weights, "Birkhoff weights", py::arg("n_weights"));
 
     py::class_<Particles4D, PyParticles4D>(m, "particles_4d")
+        .def(py::init<const Coords &, const Coords &, const Coords &, const Coords &>())
         .def(py::init<const std::vector<double> &, const std::vector<double> &, const std::vector<double> &,
                       const std::vector<double> &>())
         .def("reset", &Particles4D::reset)
@@ -1352,6 +1383,7 @@ PYBIND11_MODULE(henon_map_engine, m)
         .def_readonly("global_steps", &Particles4D::global_steps);
 
     py::class_<Particles4DGpu, Particles4D>(m, "particles_4d_gpu")
+        .def(py::init<const Coords &, const Coords &, const Coords &, const Coords &>())
         .def(py::init<const std::vector<double> &, const std::vector<double> &, const std::vector<double> &,
                       const std::vector<double> &>());
 

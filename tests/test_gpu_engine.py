@@ -277,3 +277,26 @@ def test_engine_track_batch_and_autotune():
     assert np.all(p.get_steps() == 0)
     assert np.array_equal(p.get_x(), np.linspace(0.01, 0.4, 5000))
     hm._capi.check(hm._capi.lib().hm_autotune_clear())
+
+
+def test_particle_arrays_in_and_out(hm):
+    """Constructors take numpy arrays (any dtype, converted to float64) and
+    lists alike; getters return numpy arrays of the dtypes the reference's
+    `py::cast(vector)` gives (src/bind.cpp:256-275): float64, and int64 for the
+    integer vectors; a coordinate array that is not one-dimensional is a
+    TypeError."""
+    from henon_map_cpp_amd import henon_map_engine as E
+
+    x = np.linspace(0.01, 0.2, 7)
+    z = np.zeros(7)
+    a = E.particles_4d_gpu(x, z, x, z)
+    b = E.particles_4d_gpu(list(x), list(z), list(x), list(z))
+    c = E.particles_4d_gpu(x.astype(np.float32), z, x, z)
+    assert a.get_x().dtype == np.float64 and a.get_steps().dtype == np.int64 and a.get_valid().dtype == np.int64
+    assert np.array_equal(a.get_x(), x) and np.array_equal(b.get_x(), x) and np.array_equal(b.get_py(), z)
+    assert np.array_equal(c.get_x(), x.astype(np.float32).astype(np.float64))
+    assert np.array_equal(a.get_steps(), np.zeros(7)) and np.array_equal(a.get_valid(), np.ones(7))
+    with pytest.raises(TypeError):
+        E.particles_4d_gpu(x.reshape(7, 1), z, x, z)
+    with pytest.raises(ValueError):
+        E.particles_4d_gpu(x, z[:3], x, z)
