@@ -834,6 +834,16 @@ class HenonTrackerGpu : public HenonTracker {
 
 
 // ---------------------------------------------------------------- constructs
+// An (n, 4) numpy array of 4-vectors, read with one copy.
+std::vector<double> flatten_n4(const Coords &a, size_t n)
+{
+    if (a.ndim() != 2 || (size_t)a.shape(0) != n) throw std::invalid_argument("expected one 4-vector per particle");
+    if (a.shape(1) != 4) throw std::invalid_argument("expected 4-vectors");
+    std::vector<double> f(4 * n);
+    if (n) std::memcpy(f.data(), a.data(), f.size() * sizeof(double));
+    return f;
+}
+
 std::vector<double> flatten_n4(const std::vector<std::vector<double>> &v, size_t n)
 {
     if (v.size() != n) throw std::invalid_argument("expected one 4-vector per particle");
@@ -933,7 +943,19 @@ class Vector4D {
         require_device();
         set_vectors(rv);
     }
+    explicit Vector4D(const Coords &a) : N(a.ndim() == 2 ? (size_t)a.shape(0) : 0), d(N * 32)
+    {
+        if (a.ndim() != 2) throw py::type_error("vector_4d_gpu: expected an array of 4-vectors");
+        require_device();
+        set_array(a);
+    }
     void set_vectors(const std::vector<std::vector<double>> &rv) { d.upload(flatten_n4(rv, N)); }
+    // one 4-vector per particle, or one 4-vector for all of them
+    void set_array(const Coords &a)
+    {
+        if (a.ndim() == 1) set_vector(coords_vector(a));
+        else d.upload(flatten_n4(a, N));
+    }
     void set_vector(const std::vector<double> &v)
     {
         if (v.size() != 4) throw std::invalid_argument("set_vectors: expected 4 entries");
@@ -1480,7 +1502,9 @@ PYBIND11_MODULE(henon_map_engine, m)
         .def("get_vector", &Matrix4DVector::get_vector, py::arg("vector"));
 
     py::class_<Vector4D>(m, "vector_4d_gpu")
+        .def(py::init<const Coords &>())
         .def(py::init<const std::vector<std::vector<double>> &>())
+        .def("set_vectors", &Vector4D::set_array, py::arg("vectors"))
         .def("set_vectors", &Vector4D::set_vectors, py::arg("vectors"))
         .def("set_vectors", &Vector4D::set_vector, py::arg("vectors"))
         .def("multiply", &Vector4D::multiply, py::arg("matrix"))

@@ -300,3 +300,22 @@ def test_particle_arrays_in_and_out(hm):
         E.particles_4d_gpu(x.reshape(7, 1), z, x, z)
     with pytest.raises(ValueError):
         E.particles_4d_gpu(x, z[:3], x, z)
+
+
+def test_vector_arrays_in_and_out(hm):
+    """vector_4d_gpu takes an (n, 4) array or a list of 4-vectors; set_vectors
+    takes either, or one 4-vector for every particle (src/bind.cpp:411-420)."""
+    from henon_map_cpp_amd import henon_map_engine as E
+
+    v = np.arange(12, dtype=np.float64).reshape(3, 4)
+    a = E.vector_4d_gpu(v)
+    b = E.vector_4d_gpu([list(r) for r in v])
+    assert np.array_equal(a.get_vectors(), v) and np.array_equal(b.get_vectors(), v)
+    a.set_vectors([1.0, 2.0, 3.0, 4.0])
+    assert np.array_equal(a.get_vectors(), np.tile([1.0, 2.0, 3.0, 4.0], (3, 1)))
+    a.set_vectors(v[::-1].copy())
+    assert np.array_equal(a.get_vectors(), v[::-1])
+    with pytest.raises(ValueError):
+        a.set_vectors(np.zeros((2, 4)))
+    with pytest.raises(ValueError):
+        a.set_vectors(np.zeros((3, 3)))
