@@ -443,6 +443,10 @@ class HenonTracker {
         // src/henon.cu:2706-2715
         if (inverse) {
             if (n_turns > p.global_steps) throw std::runtime_error("The number of turns is too large.");
+            // particles tracked forward by a tracker with a longer table: the
+            // inverse turns would index past this tracker's table (the
+            // reference reads out of bounds there)
+            if (p.global_steps > allowed_steps) throw std::runtime_error("The number of turns is too large.");
         } else {
             if ((uint64_t)n_turns + p.global_steps > allowed_steps) throw std::runtime_error("The number of turns is too large.");
         }
@@ -638,6 +642,11 @@ class HenonTracker {
             }
             uint32_t flags = 0;
             check(hm_take_status(&flags, nullptr));
+            // a stopped lane's frozen tail would be stored as samples: the
+            // tunes are not valid then
+            if (flags & 1u)
+                throw std::runtime_error("tunes: a particle's turn index left the modulation table; it was stopped "
+                                         "there (the reference would read out of bounds)");
         }
         if (!inverse) p.global_steps += n_turns;
         else p.global_steps -= n_turns;

@@ -133,7 +133,9 @@ __device__ __forceinline__ void track_wave(const TrackArgs &A, uint32_t base, ui
     bool same = true;
 #pragma unroll
     for (int q = 0; q < P; ++q) same = same && __all(!alive0[q] || s[q] == s0);
-    const bool in_table = INV ? (s0 >= n_turns) : ((uint64_t)s0 + n_turns <= (uint64_t)tab_len);
+    // forward reads records s0 .. s0+n_turns-1, inverse s0-1 down to s0-n_turns:
+    // both ends must lie inside [0, tab_len), else the per-lane path flags it
+    const bool in_table = INV ? (s0 >= n_turns && s0 <= tab_len) : ((uint64_t)s0 + n_turns <= (uint64_t)tab_len);
 
     if (same && in_table && n_turns > 0) {
         // the table is read-only for the whole launch: reading it through the

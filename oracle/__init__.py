@@ -496,6 +496,91 @@ def megno_construct_add(l1, l2, l3, a, b, idx):
     lib().or_megno_construct_add(_p(l1), _p(l2), _p(l3), _p(a), _p(b), l1.size, idx)
 
 
+class _SampleWeights:
+    """birkhoff_weights(turn_samples[j]) concatenated, with their offsets (the
+    per-sample weight arrays of henon.cu:2302-2334 and :2428-2474)."""
+
+    def __init__(self, turn_samples):
+        self.ts = np.ascontiguousarray(turn_samples, dtype=np.uintp)
+        ws = [birkhoff_weights(int(t)) for t in self.ts]
+        self.off = np.ascontiguousarray(np.cumsum([0] + [len(w) for w in ws[:-1]]), dtype=np.uintp)
+        self.w = np.ascontiguousarray(np.concatenate(ws) if ws else np.zeros(1))
+
+
+class MegnoConstruct:
+    """megno_construct (henon.cu:2223-2299): idx starts at 1, add(a, b) runs
+    kernel_megno's formula, get_values() is layer 3."""
+
+    def __init__(self, n):
+        self.n = n
+        self.reset()
+
+    def reset(self):
+        self.idx = 1
+        self.l1, self.l2, self.l3 = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
+
+    def add(self, a, b):
+        megno_construct_add(self.l1, self.l2, self.l3, np.reshape(a, (self.n, 16)), np.reshape(b, (self.n, 16)),
+                            self.idx)
+        self.idx += 1
+
+    def get_values(self):
+        return self.l3.copy()
+
+
+class MegnoBirkhoffConstruct:
+    """megno_birkhoff_construct (henon.cu:2302-2426): layer 1 as megno_construct,
+    layer 2 per turn sample j: += L1 w_j[idx-1] / idx while idx <= T_j."""
+
+    def __init__(self, n, turn_samples):
+        self.n = n
+        self.sw = _SampleWeights(turn_samples)
+        self.reset()
+
+    def reset(self):
+        self.idx = 1
+        self.l1 = np.zeros(self.n)
+        self.l2 = np.zeros((len(self.sw.ts), self.n))
+
+    def add(self, a, b):
+        a, b = _f64(np.reshape(a, (self.n, 16))), _f64(np.reshape(b, (self.n, 16)))
+        lib().or_megno_birkhoff_add(_p(self.l1), _p(self.l2), _p(self.sw.w), _p(self.sw.off), _p(self.sw.ts),
+                                    len(self.sw.ts), _p(a), _p(b), self.n, self.idx)
+        self.idx += 1
+
+    def get_values(self):
+        return self.l2.copy()
+
+
+class TuneBirkhoffConstruct:
+    """tune_birkhoff_construct (henon.cu:2428-2655): first_add stores the phase
+    angles, each add accumulates the wrapped phase advance with w_j[idx] into
+    tune 1 (idx < T_j) or w_j[idx - T_j] into tune 2 (T_j <= idx < 2 T_j)."""
+
+    def __init__(self, n, turn_samples):
+        self.n = n
+        self.sw = _SampleWeights(turn_samples)
+        self.sx, self.sy = np.zeros(n), np.zeros(n)
+        self.reset()
+
+    def reset(self):
+        self.idx = 1
+        ns = len(self.sw.ts)
+        self.t1x, self.t2x = np.zeros((ns, self.n)), np.zeros((ns, self.n))
+        self.t1y, self.t2y = np.zeros((ns, self.n)), np.zeros((ns, self.n))
+
+    def first_add(self, x, px, y, py):
+        x, px, y, py = (_f64(a) for a in (x, px, y, py))
+        lib().or_tune_birkhoff_init(_p(x), _p(px), _p(y), _p(py), _p(self.sx), _p(self.sy), self.n)
+
+    def add(self, x, px, y, py):
+        x, px, y, py = (_f64(a) for a in (x, px, y, py))
+        lib().or_tune_birkhoff_add(_p(x), _p(px), _p(y), _p(py), _p(self.sx), _p(self.sy), _p(self.t1x),
+                                   _p(self.t2x), _p(self.t1y), _p(self.t2y), _p(self.sw.w), _p(self.sw.off),
+                                   _p(self.sw.ts), len(self.sw.ts), self.n, self.idx)
+        self.idx += 1
+
+
 def matvec(m, v):
     m = _f64(m)
     v = _f64(v).copy()

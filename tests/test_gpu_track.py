@@ -148,6 +148,35 @@ def test_out_of_table_index_is_flagged_not_read():
     assert C.take_status() == 0
 
 
+def test_inverse_past_table_end_is_flagged_not_read():
+    """Inverse turns read records steps-1 downwards: particles whose turn index
+    lies past the end of a shorter table are stopped unchanged and flagged."""
+    x, px, y, py = W.ball(128, 0.2, seed=5)
+    tables, tab = make_tables(100, 0.31, 0.32, "sps", 0.0)
+    dev = C.DeviceParticles(x, px, y, py, np.full(128, 105, dtype=np.uint32))
+    tdev = C.DeviceBuffer.from_array(tab)
+    C.track(dev, tdev, len(tab), 10, 0.0, 10.0, inverse=True)  # indices 104..95: the first is past the end
+    assert C.take_status() & 1
+    got = dev.get()
+    assert np.all(got[4] == 105)
+    for a, b in zip(got[:4], (x, px, y, py)):
+        assert np.array_equal(a, b)
+    assert C.take_status() == 0
+
+
+def test_inverse_after_longer_table_raises():
+    """A particle set advanced past a tracker's table cannot be tracked back by it."""
+    import henon_map_cpp_amd as H
+
+    x, px, y, py = W.ball(64, 0.1, seed=6)
+    p = H.particles(x, px, y, py)
+    long_t = H.henon_tracker(200, 0.31, 0.32, "sps")
+    short_t = H.henon_tracker(100, 0.31, 0.32, "sps")
+    long_t.track(p, 150, 0.0, 10.0)
+    with pytest.raises(RuntimeError, match="too large"):
+        short_t.track(p, 10, 0.0, 10.0, inverse=True)
+
+
 @pytest.mark.parametrize("every", [1, 7, 250])
 def test_loss_compaction_is_bit_identical(every):
     c = W.MODULATED_DA
