@@ -430,33 +430,88 @@ def batch_leg(n_jobs, max_turns=100_000):
             "speedup": tp / tb, "identical_to_plain": identical}
 
 
+def host_cpu_info():
+    """What the host offers: the reference's thread count
+    (std::thread::hardware_concurrency() = os.cpu_count()), the CPUs this
+    process may run on, the cgroup CPU quota, the box's CPU share
+    (OMP_NUM_THREADS) and the CPU model."""
+    import subprocess
+
+    info = {"hardware_concurrency": os.cpu_count()}
+    try:
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True).stdout.strip())
+    except Exception:
+        info["nproc"] = None
+    info["affinity"] = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except Exception:
+        pass
+    info["cgroup_cpu_quota"] = quota
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    info["cpu_share"] = int(omp) if omp.isdigit() and int(omp) > 0 else None
+    model = None
+    try:
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True).stdout.splitlines():
+            if line.lower().startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    info["model"] = model
+    return info
+
+
 def cpu_baseline(stride, turns):
     """The oracle (C++ restatement of the reference CPU path, compiled like the
-    reference's CPU build) on every `stride`-th particle of the same grid, with
-    the reference's threading model (one std::thread per core, strided)."""
+    reference's CPU build: -O3, no FMA contraction, no -march) on every
+    `stride`-th particle of the same grid, with the reference's threading model
+    (one std::thread per hardware thread, particles strided over threads,
+    henon.cu:2701-2752). The thread count is hardware_concurrency() limited to
+    the CPUs this process can use (affinity, cgroup quota, the box's CPU share):
+    more threads than that only time-slice. A one-thread run on a smaller sample
+    gives the per-core rate; `projected_full_host_value` scales it to every
+    hardware thread (linear scaling, a projection, not a measurement)."""
     import oracle as O
     from henon_map_cpp_amd import workloads as W
 
     O.build()
     c = W.MODULATED_DA
-    x, px, y, py = (a[::stride].copy() for a in W.da_grid(1000))
+    info = host_cpu_info()
+    limits = [v for v in (info["hardware_concurrency"], info["affinity"], info["cgroup_cpu_quota"],
+                          info["cpu_share"]) if v]
+    threads = max(1, int(min(limits)))
     coef, freq = W.seeded_line_coefficients(), W.line_frequencies()
     t = O.Tracker(turns, c["tune_x"], c["tune_y"], "custom", epsilon=c["epsilon"], coef=coef, freq=freq)
-    p = O.Particles(x, px, y, py)
-    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16)
-    t0 = time.perf_counter()
-    t.track(p, turns, c["mu"], barrier=c["barrier"], n_threads=cores)
-    dt = time.perf_counter() - t0
-    live = int(p.steps.astype(np.int64).sum())
+
+    def run(step, n_threads):
+        x, px, y, py = (a[::step].copy() for a in W.da_grid(1000))
+        p = O.Particles(x, px, y, py)
+        t0 = time.perf_counter()
+        t.track(p, turns, c["mu"], barrier=c["barrier"], n_threads=n_threads)
+        dt = time.perf_counter() - t0
+        return p.n, int(p.steps.astype(np.int64).sum()), dt
+
+    n, live, dt = run(stride, threads)
+    n1, live1, dt1 = run(stride * threads, 1)
+    per_core = n1 * turns / dt1
     return {
-        "value": p.n * turns / dt,
+        "value": n * turns / dt,
         "unit": "particle-turns/s",
-        "cores": cores,
+        "cores": threads,
         "kind": "port",
-        "sample": f"every {stride}th particle of the 1000x1000 grid ({p.n} particles) x {turns} turns, "
-                  f"mu=0.5, eps=32; live particle-turns/s {live / dt:.4g}; {dt:.1f} s",
+        "sample": f"every {stride}th particle of the 1000x1000 grid ({n} particles) x {turns} turns, mu=0.5, "
+                  f"eps=32, {threads} threads: {dt:.1f} s, live particle-turns/s {live / dt:.4g}; one thread on "
+                  f"every {stride * threads}th particle ({n1} particles): {dt1:.1f} s",
         "live_value": live / dt,
         "seconds": dt,
+        "one_thread_value": per_core,
+        "thread_scaling": (n * turns / dt) / per_core,
+        "projected_full_host_value": per_core * info["hardware_concurrency"],
+        "host": info,
     }
 
 

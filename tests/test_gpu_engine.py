@@ -166,17 +166,17 @@ def test_tune_all_notebook(hm, golden):
     assert df["tune_x_birkhoff"][0][0] == pytest.approx(e["birkhoff_window_0_500_x"], abs=1e-14)
     assert df["tune_x_birkhoff"][1][0] == pytest.approx(e["birkhoff_window_500_1000_x"], abs=1e-14)
     # the last row is the full signal (samples 1..n for the FFT, all for Birkhoff)
-    assert df["tune_x_fft"][2][0] == pytest.approx(e["fft_full_x"], abs=1e-14)
-    assert df["tune_y_fft"][2][0] == pytest.approx(e["fft_full_y"], abs=1e-14)
-    assert df["tune_x_birkhoff"][2][0] == pytest.approx(e["birkhoff_full_x"], abs=1e-14)
+    assert df["tune_x_fft"][2][0] == pytest.approx(e["fft_full_x"], abs=4e-15)
+    assert df["tune_y_fft"][2][0] == pytest.approx(e["fft_full_y"], abs=4e-15)
+    assert df["tune_x_birkhoff"][2][0] == pytest.approx(e["birkhoff_full_x"], abs=4e-15)
     # the appended (0, n_turns) window sits just before it in the engine's rows
     t2 = hm.henon_tracker(10000, 0.168, 0.201, "sps")
     p2 = hm.particles(pp, np.zeros_like(pp), pp, np.zeros_like(pp))
     rb, rf = t2.tracker.all_tunes(p2.particles, 10000, 0.0, 10.0, float("nan"), False, [], [])
     assert rf.shape == (2, 4)
-    assert rf[0, 0] == pytest.approx(e["fft_window_0_10000_x"], abs=1e-14)
-    assert rf[0, 1] == pytest.approx(e["fft_window_0_10000_y"], abs=1e-14)
-    assert rb[0, 0] == pytest.approx(e["birkhoff_full_x"], abs=1e-14)
+    assert rf[0, 0] == pytest.approx(e["fft_window_0_10000_x"], abs=4e-15)
+    assert rf[0, 1] == pytest.approx(e["fft_window_0_10000_y"], abs=4e-15)
+    assert rb[0, 0] == pytest.approx(e["birkhoff_full_x"], abs=4e-15)
 
 
 def test_api_megno_and_birkhoff_tunes_match_oracle(hm):
