@@ -80,6 +80,7 @@ SIGNATURES = {
     "hm_build_table": (I, [P, P, SZ, P]),
     "hm_birkhoff_weights": (I, [U32, P]),
     "hm_track": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, D, D, I, I, ctypes.POINTER(LaunchCfg), P]),
+    "hm_track_noalloc": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, D, D, I, I, ctypes.POINTER(LaunchCfg), P]),
     "hm_track_batch": (I, [ctypes.POINTER(Job), SZ, I, ctypes.POINTER(BatchCfg), P]),
     "hm_autotune_track": (I, [P, P, P, P, P, SZ, P, SZ, U32, U32, D, D, I, ctypes.POINTER(LaunchCfg),
                               ctypes.POINTER(D), P]),

@@ -66,7 +66,8 @@ inline void launch_check(const char *name)
 }
 
 // Library-owned device words, per device: the status flags word (see
-// hm_take_status) and growable scratch buffers, one per slot: 0 tracking live
+// hm_take_status; static device memory, never allocated) and growable scratch
+// buffers, one per slot: 0 tracking live
 // lists, 1 Birkhoff weights of the indicator kernels, 2 tune-window weights,
 // 3 DFT magnitudes, 4 batched-job lists and plans, 5-6 autotuner trial state,
 // 7 reciprocal table, 8 FFT signal rows. Growing a slot synchronises the

@@ -77,24 +77,32 @@ LaunchTimer::~LaunchTimer()
 namespace {
 std::mutex g_ws_mu;
 struct DeviceScratch {
-    uint32_t *status = nullptr;
     void *slots[kWorkspaceSlots] = {};
     size_t sizes[kWorkspaceSlots] = {};
 };
 std::map<int, DeviceScratch> g_scratch;
+
+// The status flags word lives in static device memory: each device holds its
+// own zero-initialised copy from module load, so no call allocates for it.
+// Host code uses the address hipGetSymbolAddress gives on each device.
+__device__ __align__(256) uint32_t g_status_word[64];
+constexpr int kMaxDevices = 64;
+std::mutex g_status_mu;
+uint32_t *g_status_ptr[kMaxDevices] = {};
 }  // namespace
 
 uint32_t *status_word()
 {
     int dev = 0;
     hip_check(hipGetDevice(&dev), "hipGetDevice");
-    std::lock_guard<std::mutex> lk(g_ws_mu);
-    auto &d = g_scratch[dev];
-    if (!d.status) {
-        hip_check(hipMalloc(&d.status, 256), "hipMalloc(status)");
-        hip_check(hipMemset(d.status, 0, 256), "hipMemset(status)");
+    if (dev < 0 || dev >= kMaxDevices) throw Error(HM_ERR_ARG, "device index beyond the library's limit of 64");
+    std::lock_guard<std::mutex> lk(g_status_mu);
+    if (!g_status_ptr[dev]) {
+        void *p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(g_status_word)), "hipGetSymbolAddress(status)");
+        g_status_ptr[dev] = static_cast<uint32_t *>(p);
     }
-    return d.status;
+    return g_status_ptr[dev];
 }
 
 uint32_t device_simds()

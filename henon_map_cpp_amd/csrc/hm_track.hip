@@ -213,42 +213,72 @@ static uint32_t *pinned_counts(size_t n)
     return buf;
 }
 
+// Argument checks and the launch description shared by hm_track and
+// hm_track_noalloc. Returns false when there is nothing to do.
+static bool prepare_track(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n,
+                          const double *table, size_t table_len, uint32_t n_turns, double mu, double barrier,
+                          double kick_module, int mode, const hm_launch_cfg *cfg, const char *who, TrackLaunch &a,
+                          int &every)
+{
+    const std::string w(who);
+    if (!std::isnan(kick_module))
+        throw Error(HM_ERR_ARG, w + ": a finite kick_module needs hm_track_kicked (seed and draw index)");
+    if (mode != HM_MODE_EXACT && mode != HM_MODE_FAST) throw Error(HM_ERR_ARG, w + ": unknown mode");
+    if (n == 0 || n_turns == 0) return false;
+    if (!x || !px || !y || !py || !steps || !table) throw Error(HM_ERR_ARG, w + ": NULL pointer");
+    if (n > 0x7fffffffull || table_len > 0xffffffffull) throw Error(HM_ERR_ARG, w + ": size exceeds 2^31");
+    hm_launch_cfg tuned{};
+    if (!cfg && tuned_config(mode, mu != 0.0, n, &tuned)) cfg = &tuned;
+    const int block = cfg && cfg->block ? cfg->block : kDefaultBlock;
+    if (block < 64 || block > 256 || block % 64) throw Error(HM_ERR_ARG, w + ": block must be 64, 128, 192 or 256");
+    every = cfg && cfg->compact_every ? cfg->compact_every : kDefaultCompactEvery;
+    // 1: one particle per lane; 2: one or two (mixed launch). The default is
+    // one in exact mode and mixed in fast mode, the faster of the two for each
+    // on the 1000x1000-grid, 1e5-turn scan.
+    int pair_mode = cfg ? cfg->particles_per_lane : 0;
+    if (pair_mode < 0 || pair_mode > 2) throw Error(HM_ERR_ARG, w + ": particles_per_lane must be 0, 1 or 2");
+    if (pair_mode == 0) pair_mode = mode == HM_MODE_EXACT ? 1 : 2;
+    a = TrackLaunch{};
+    a.x = x; a.px = px; a.y = y; a.py = py; a.steps = steps;
+    a.tab = reinterpret_cast<const TabEntry *>(table);
+    a.tab_len = (uint32_t)table_len;
+    a.mu = mu;
+    a.b2 = barrier * barrier;  // henon.cu:2717, :3173
+    a.block = block;
+    a.slots = pair_mode == 1 ? 0u : resident_slots();
+    a.staging = cfg ? cfg->table_staging : HM_STAGING_AUTO;
+    if (a.staging < HM_STAGING_AUTO || a.staging > HM_STAGING_RING)
+        throw Error(HM_ERR_ARG, w + ": table_staging must be 0 (auto), 1 (scalar) or 2 (ring)");
+    a.simds = device_simds();
+    return true;
+}
+
+// Segment lengths: kFirstSegment turns, doubling up to `every`.
+static std::vector<uint32_t> segment_plan(uint32_t n_turns, int every)
+{
+    std::vector<uint32_t> seg;
+    for (uint32_t done = 0, len = std::min<uint32_t>(kFirstSegment, (uint32_t)every); done < n_turns;) {
+        const uint32_t t = std::min(len, n_turns - done);
+        seg.push_back(t);
+        done += t;
+        len = std::min<uint32_t>(len * 2u, (uint32_t)every);
+    }
+    return seg;
+}
+
 extern "C" int hm_track(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n,
                         const double *table, size_t table_len, uint32_t n_turns, double mu, double barrier,
                         double kick_module, int inverse, int mode, const hm_launch_cfg *cfg, void *stream)
 {
     return guarded([&] {
-        if (!std::isnan(kick_module))
-            throw Error(HM_ERR_ARG, "track: a finite kick_module needs hm_track_kicked (seed and draw index)");
-        if (mode != HM_MODE_EXACT && mode != HM_MODE_FAST) throw Error(HM_ERR_ARG, "track: unknown mode");
-        if (n == 0 || n_turns == 0) return;
-        if (!x || !px || !y || !py || !steps || !table) throw Error(HM_ERR_ARG, "track: NULL pointer");
-        if (n > 0x7fffffffull || table_len > 0xffffffffull) throw Error(HM_ERR_ARG, "track: size exceeds 2^31");
-        hm_launch_cfg tuned{};
-        if (!cfg && tuned_config(mode, mu != 0.0, n, &tuned)) cfg = &tuned;
-        const int block = cfg && cfg->block ? cfg->block : kDefaultBlock;
-        if (block < 64 || block > 256 || block % 64) throw Error(HM_ERR_ARG, "track: block must be 64, 128, 192 or 256");
-        const int every = cfg && cfg->compact_every ? cfg->compact_every : kDefaultCompactEvery;
-        // 1: one particle per lane; 2: one or two (mixed launch). The default is
-        // one in exact mode and mixed in fast mode, the faster of the two for each
-        // on the 1000x1000-grid, 1e5-turn scan.
-        int pair_mode = cfg ? cfg->particles_per_lane : 0;
-        if (pair_mode < 0 || pair_mode > 2) throw Error(HM_ERR_ARG, "track: particles_per_lane must be 0, 1 or 2");
-        if (pair_mode == 0) pair_mode = mode == HM_MODE_EXACT ? 1 : 2;
-        hipStream_t s = as_stream(stream);
-        const double b2 = barrier * barrier;  // henon.cu:2717, :3173
-        const bool oct = mu != 0.0;           // henon.cu:445-448
-        const TabEntry *tab = reinterpret_cast<const TabEntry *>(table);
-        const uint32_t N = (uint32_t)n;
-
         TrackLaunch a{};
-        a.x = x; a.px = px; a.y = y; a.py = py; a.steps = steps;
-        a.tab = tab; a.tab_len = (uint32_t)table_len; a.mu = mu; a.b2 = b2; a.block = block;
-        a.slots = pair_mode == 1 ? 0u : resident_slots();
-        a.staging = cfg ? cfg->table_staging : HM_STAGING_AUTO;
-        if (a.staging < HM_STAGING_AUTO || a.staging > HM_STAGING_RING)
-            throw Error(HM_ERR_ARG, "track: table_staging must be 0 (auto), 1 (scalar) or 2 (ring)");
-        a.simds = device_simds();
+        int every = 0;
+        if (!prepare_track(x, px, y, py, steps, n, table, table_len, n_turns, mu, barrier, kick_module, mode, cfg,
+                           "track", a, every))
+            return;
+        hipStream_t s = as_stream(stream);
+        const bool oct = mu != 0.0;  // henon.cu:445-448
+        const uint32_t N = (uint32_t)n;
 
         if (every < 0 || (uint32_t)every >= n_turns) {
             a.n_arg = a.grid_bound = N;
@@ -263,13 +293,7 @@ extern "C" int hm_track(double *x, double *px, double *y, double *py, uint32_t *
         // for the segment that feeds the launch. (Letting late segments grow past
         // `every` saves launches but carries more dead lanes: 10% slower on the
         // 1e5-turn scan.)
-        std::vector<uint32_t> seg;
-        for (uint32_t done = 0, len = std::min<uint32_t>(kFirstSegment, (uint32_t)every); done < n_turns;) {
-            const uint32_t t = std::min(len, n_turns - done);
-            seg.push_back(t);
-            done += t;
-            len = std::min<uint32_t>(len * 2u, (uint32_t)every);
-        }
+        const std::vector<uint32_t> seg = segment_plan(n_turns, every);
         const size_t S = seg.size();
         uint32_t *lists = static_cast<uint32_t *>(workspace(sizeof(uint32_t) * (2 * (size_t)N + S + 64), 0));
         uint32_t *counts = lists + 2 * (size_t)N;  // counts[i]: length of segment i's input list (i >= 1)
@@ -309,6 +333,95 @@ extern "C" int hm_track(double *x, double *px, double *y, double *py, uint32_t *
             throw;
         }
         cleanup();
+    });
+}
+
+// ---------------------------------------------------------------- allocation-free tracking
+// hm_track_noalloc keeps its live-index lists and per-segment counts in this
+// file-scope array instead of the library's growable workspace. Each device
+// holds its own copy (8.3 MiB) from module load until the process ends. Host
+// code never passes the array's name to a kernel: the name is only a host-side
+// stand-in; the device address comes from hipGetSymbolAddress, per device.
+constexpr uint32_t kNoAllocMaxParticles = 1u << 20;
+constexpr uint32_t kNoAllocMaxSegments = 1u << 16;
+__device__ __align__(256) uint32_t scratch[2 * (size_t)kNoAllocMaxParticles + kNoAllocMaxSegments + 64];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static uint32_t *g_noalloc_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up on the first
+// call there.
+static uint32_t *noalloc_scratch()
+{
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) throw Error(HM_ERR_ARG, "track_noalloc: device index beyond 64");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        uint32_t *ptr = nullptr;
+        hip_check(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_noalloc_ptr[dev] = ptr;
+    }
+    return g_noalloc_ptr[dev];
+}
+
+__global__ void k_zero_words(uint32_t *__restrict__ p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+extern "C" int hm_track_noalloc(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n,
+                                const double *table, size_t table_len, uint32_t n_turns, double mu, double barrier,
+                                double kick_module, int inverse, int mode, const hm_launch_cfg *cfg, void *stream)
+{
+    return guarded([&] {
+        if (n > kNoAllocMaxParticles)
+            throw Error(HM_ERR_ARG, "track_noalloc: at most 1048576 particles per call (hm_track takes more)");
+        TrackLaunch a{};
+        int every = 0;
+        if (!prepare_track(x, px, y, py, steps, n, table, table_len, n_turns, mu, barrier, kick_module, mode, cfg,
+                           "track_noalloc", a, every))
+            return;
+        hipStream_t s = as_stream(stream);
+        const bool oct = mu != 0.0;
+        const uint32_t N = (uint32_t)n;
+        const TrackFn launch = pick_track(mode, inverse != 0, oct);
+        if (every < 0 || (uint32_t)every >= n_turns) {
+            a.n_arg = a.grid_bound = N;
+            a.n_turns = n_turns;
+            launch(a, s);
+            return;
+        }
+        // hm_track's segments, each launched over all n particles: a segment
+        // reads its live count on the device and wavefronts past it return at
+        // once, so the host never waits for a count
+        uint32_t S = 0;
+        for (uint32_t done = 0, len = std::min<uint32_t>(kFirstSegment, (uint32_t)every); done < n_turns; ++S) {
+            done += std::min(len, n_turns - done);
+            len = std::min<uint32_t>(len * 2u, (uint32_t)every);
+        }
+        if (S > kNoAllocMaxSegments)
+            throw Error(HM_ERR_ARG, "track_noalloc: more than 65536 compaction segments (raise compact_every)");
+        uint32_t *lists = noalloc_scratch();
+        uint32_t *counts = lists + 2 * (size_t)kNoAllocMaxParticles;  // counts[i]: segment i's input length
+        // the previous call's counts are still there: clear them with a kernel
+        hipLaunchKernelGGL(k_zero_words, dim3((S + 1 + 255) / 256), dim3(256), 0, s, counts, S + 1);
+        launch_check("k_zero_words");
+        uint32_t len = std::min<uint32_t>(kFirstSegment, (uint32_t)every), done = 0;
+        for (uint32_t i = 0; i < S; ++i) {
+            const bool last = i + 1 == S;
+            a.n_turns = std::min(len, n_turns - done);
+            a.idx_in = i == 0 ? nullptr : (i % 2 ? lists : lists + N);
+            a.count_in = i == 0 ? nullptr : counts + i;
+            a.n_arg = a.grid_bound = N;
+            a.idx_out = last ? nullptr : (i % 2 ? lists + N : lists);
+            a.count_out = last ? nullptr : counts + i + 1;
+            launch(a, s);
+            done += a.n_turns;
+            len = std::min<uint32_t>(len * 2u, (uint32_t)every);
+        }
     });
 }
 

@@ -128,6 +128,22 @@ int hm_track(double *x, double *px, double *y, double *py, uint32_t *steps, size
              const double *table, size_t table_len, uint32_t n_turns, double mu, double barrier,
              double kick_module, int inverse, int mode, const hm_launch_cfg *cfg, void *stream);
 
+/* hm_track that allocates nothing. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc). Same arguments, results
+ * (bit for bit) and status bit as hm_track; it calls no hipMalloc, hipFree or
+ * other allocating routine and never waits on the host. Its live-index lists
+ * and segment counts sit in a static device array of 8.3 MiB that each device
+ * holds from library load until the process ends, which bounds n to
+ * 1048576 particles and the compaction segments of one call to 65536
+ * (n_turns / compact_every); larger calls return HM_ERR_ARG. Every call on a
+ * device shares that device's array: calls that overlap in time on one device
+ * (several streams or host threads) race and must be serialised by the
+ * caller. With hm_profile_enable(1) launches are bracketed by HIP events. */
+int hm_track_noalloc(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n,
+                     const double *table, size_t table_len, uint32_t n_turns, double mu, double barrier,
+                     double kick_module, int inverse, int mode, const hm_launch_cfg *cfg, void *stream);
+
 /* hm_track with the random 4D kick of src/henon.cu:362-405, 420-542: every
  * turn, before the forward map (after the inverse map), each particle moves by
  * kick_module (finite) along a direction drawn from four standard normals. The
