@@ -4,7 +4,7 @@
 //  * k_means          : per-coordinate mean of each particle's samples
 //  * k_birkhoff_hist  : Birkhoff tunes of windows of the stored samples
 //  * k_fft_tune_pow2  : Hanning-windowed FFT tune, power-of-two length, radix-2 in LDS
-//  * k_fft_tune_dft   : same for any length, direct DFT with exactly reduced twiddles
+//  * other lengths    : the chirp-z transform of hm_chirpz.hip
 // Two history layouts, addressed through strides (HistLayout):
 //  * sample-major hist[sample][coord][particle] (coord = x, px, y, py): a
 //    tracking wavefront writes one contiguous row per turn (k_history);
@@ -21,21 +21,12 @@
 
 #include "hm_device.h"
 #include "hm_internal.h"
+#include "hm_tune.h"
 
 namespace hm {
 
 constexpr int kMaxWindows = 64;
 
-// Element (sample m, coordinate c, particle p) of a history at m*ms + c*cs + p*ps.
-struct HistLayout {
-    uint64_t ms, cs, ps;
-};
-__host__ __device__ inline HistLayout sample_major(uint64_t n) { return HistLayout{4 * n, n, 1}; }
-__host__ __device__ inline HistLayout particle_rows(uint64_t L) { return HistLayout{1, L, 4 * L}; }
-__device__ __forceinline__ size_t hat(const HistLayout &h, uint64_t m, uint64_t c, uint64_t p)
-{
-    return (size_t)(m * h.ms + c * h.cs + p * h.ps);
-}
 
 // ---------------------------------------------------------------- history
 template <bool INV, bool OCT>
@@ -316,32 +307,6 @@ struct FftArgs {
     uint32_t out_window0;  // first window index in `out`
 };
 
-__device__ __forceinline__ double interp_A(double a, double b, double c)
-{
-    // dynamic_indicator.cpp:134-137
-    return (-(a + b * c) * (a - b) + b * sqrt(c * c * ((a + b) * (a + b)) - 2 * a * b * (2 * (c * c) - c - 1))) /
-           ((a * a) + (b * b) + 2 * a * b * c);
-}
-
-// dynamic_indicator.cpp:139-176 on magnitudes mag[0..N): first maximum wins
-// `mag(k)` gives the magnitude of spectrum entry k.
-template <class Mag>
-__device__ double interp_tune_at(Mag mag, uint32_t N, uint32_t index)
-{
-    const double Nd = (double)N;
-    if (index == 0) return 1.0;
-    if (index == N - 1) return 0.0;
-    uint32_t i1, i2;
-    if (mag(index - 1) > mag(index + 1)) {
-        i1 = index - 1; i2 = index; index = index - 1;
-    } else {
-        i1 = index; i2 = index + 1;
-    }
-    const double value = ((double)index / Nd) +
-                         (1.0 / (2.0 * M_PI)) * asin(interp_A(mag(i1), mag(i2), cos(2.0 * M_PI / Nd)) *
-                                                     sin(2.0 * M_PI / Nd));
-    return fabs(1.0 - value);
-}
 __device__ double interp_tune(const double *mag, uint32_t N, uint32_t index)
 {
     return interp_tune_at([&](uint32_t k) { return mag[k]; }, N, index);
@@ -584,52 +549,6 @@ __global__ void __launch_bounds__(256) k_fft_tune_rows(FftRowArgs a)
 // twiddle table), takes the R-point DFT and writes element r to
 // (j / Ns) Ns R + k + r Ns; after the last pass the spectrum is in natural
 // order.
-struct Cplx {
-    double re, im;
-};
-__device__ __forceinline__ Cplx cadd(Cplx a, Cplx b) { return Cplx{a.re + b.re, a.im + b.im}; }
-__device__ __forceinline__ Cplx csub(Cplx a, Cplx b) { return Cplx{a.re - b.re, a.im - b.im}; }
-__device__ __forceinline__ Cplx cmul(Cplx a, Cplx w) { return Cplx{a.re * w.re - a.im * w.im, a.re * w.im + a.im * w.re}; }
-__device__ __forceinline__ Cplx mul_mi(Cplx a) { return Cplx{a.im, -a.re}; }  // times -i
-
-// forward DFT of sizes 2, 4, 8 (exp(-2 pi i m k / R))
-__device__ __forceinline__ void dft2(Cplx *v)
-{
-    const Cplx a = v[0], b = v[1];
-    v[0] = cadd(a, b);
-    v[1] = csub(a, b);
-}
-__device__ __forceinline__ void dft4(Cplx &a, Cplx &b, Cplx &c, Cplx &d)
-{
-    const Cplx s0 = cadd(a, c), s1 = csub(a, c), s2 = cadd(b, d), s3 = mul_mi(csub(b, d));
-    a = cadd(s0, s2);
-    c = csub(s0, s2);
-    b = cadd(s1, s3);
-    d = csub(s1, s3);
-}
-__device__ __forceinline__ void dft8(Cplx *v)
-{
-    // even and odd halves, then X[k] = E[k] + w8^k O[k], X[k+4] = E[k] - w8^k O[k]
-    Cplx e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
-    Cplx o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
-    dft4(e0, e1, e2, e3);
-    dft4(o0, o1, o2, o3);
-    constexpr double c = 0.70710678118654752440;  // sqrt(2) / 2
-    const Cplx t1 = Cplx{c * (o1.re + o1.im), c * (o1.im - o1.re)};   // (1 - i)/sqrt2
-    const Cplx t2 = mul_mi(o2);                                        // -i
-    const Cplx t3 = Cplx{c * (o3.im - o3.re), -c * (o3.re + o3.im)};  // (-1 - i)/sqrt2
-    v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
-    v[1] = cadd(e1, t1); v[5] = csub(e1, t1);
-    v[2] = cadd(e2, t2); v[6] = csub(e2, t2);
-    v[3] = cadd(e3, t3); v[7] = csub(e3, t3);
-}
-template <int R>
-__device__ __forceinline__ void dft_r(Cplx *v)
-{
-    if constexpr (R == 8) dft8(v);
-    else if constexpr (R == 4) dft4(v[0], v[1], v[2], v[3]);
-    else dft2(v);
-}
 
 constexpr int kStockThreads = 256;
 
@@ -780,85 +699,6 @@ __global__ void __launch_bounds__(kStockThreads) k_fft_tune_stockham(FftRowArgs 
     if (threadIdx.x == 0) *o = interp_tune_at(magnitude, N, bi);
 }
 
-// Direct DFT for any N: X_k = sum_m z_m exp(-2 pi i k m / N), twiddle index
-// k*m mod N kept exact; the windowed signal is staged through LDS in chunks.
-__global__ void __launch_bounds__(256) k_fft_tune_dft(FftArgs a, double *__restrict__ mag_scratch)
-{
-    constexpr uint32_t CH = 2048;
-    __shared__ double zr[CH], zi[CH];
-    __shared__ double sv[256];
-    __shared__ uint32_t si[256];
-    __shared__ int any_nan_flag;
-    uint32_t particle, plane, window;
-    signal_of(blockIdx.x, a.n * 2 * a.count, particle, plane, window, a.n);
-    if (window == 0xffffffffu) return;
-    const uint32_t N = a.N;
-    const size_t nn = a.n;
-    const int cx = 2 * plane, cp = 2 * plane + 1;
-    double mc[4] = {0.0, 0.0, 0.0, 0.0};
-    if (a.means)
-        for (int c = 0; c < 4; c++) mc[c] = a.means[c * nn + particle];
-    const double mx = mc[cx], mp = mc[cp];
-    double *mag = mag_scratch + (size_t)blockIdx.x * N;
-    if (threadIdx.x == 0) any_nan_flag = 0;
-    // each thread owns outputs k = t, t + 256, ...; at most 64 per thread here
-    constexpr int KMAX = 64;
-    double xr[KMAX], xi[KMAX];
-    uint32_t q[KMAX];
-    const uint32_t nk = (N + blockDim.x - 1) / blockDim.x;
-    for (int j = 0; j < KMAX; j++) { xr[j] = 0.0; xi[j] = 0.0; q[j] = 0; }
-    for (uint32_t c0 = 0; c0 < N; c0 += CH) {
-        const uint32_t cn = min(CH, N - c0);
-        __syncthreads();
-        bool nan_seen = false;
-        for (uint32_t m = threadIdx.x; m < cn; m += blockDim.x) {
-            const uint64_t sm = a.from[window] + c0 + m;
-            for (int c = 0; c < 4; c++) nan_seen |= __builtin_isnan(a.hist[hat(a.lay, sm, c, particle)] - mc[c]);
-            const double h = a.hann[c0 + m];
-            zr[m] = (a.hist[hat(a.lay, sm, cx, particle)] - mx) * h;
-            zi[m] = (a.hist[hat(a.lay, sm, cp, particle)] - mp) * h;
-        }
-        if (nan_seen) atomicOr(&any_nan_flag, 1);
-        __syncthreads();
-        for (uint32_t j = 0; j < nk && j < (uint32_t)KMAX; j++) {
-            const uint32_t k = threadIdx.x + j * blockDim.x;
-            if (k >= N) break;
-            uint32_t qq = (uint32_t)(((uint64_t)k * c0) % N);
-            double ar = xr[j], ai = xi[j];
-            for (uint32_t m = 0; m < cn; m++) {
-                const double wr = a.tw[2 * qq], wi = a.tw[2 * qq + 1];
-                ar += zr[m] * wr - zi[m] * wi;
-                ai += zr[m] * wi + zi[m] * wr;
-                qq += k;
-                if (qq >= N) qq -= N;
-            }
-            xr[j] = ar;
-            xi[j] = ai;
-        }
-    }
-    __syncthreads();
-    const size_t out_o = ((size_t)(a.out_window0 + window) * 2 + plane) * nn + particle;
-    if (any_nan_flag) {
-        if (threadIdx.x == 0) a.out[out_o] = __builtin_nan("");
-        return;
-    }
-    double best = -1.0;
-    uint32_t best_i = 0xffffffffu;
-    for (uint32_t j = 0; j < nk && j < (uint32_t)KMAX; j++) {
-        const uint32_t k = threadIdx.x + j * blockDim.x;
-        if (k >= N) break;
-        const double mg = sqrt(xr[j] * xr[j] + xi[j] * xi[j]);
-        mag[k] = mg;
-        if (mg > best || best_i == 0xffffffffu) { best = mg; best_i = k; }
-    }
-    __threadfence_block();
-    __syncthreads();
-    double bv;
-    uint32_t bi;
-    block_argmax(best, best_i, sv, si, bv, bi);
-    if (threadIdx.x == 0) a.out[out_o] = interp_tune(mag, N, bi);
-}
-
 // Host tables per length N: Hanning window (dynamic_indicator.cpp:47-52, glibc)
 // and twiddles exp(-2 pi i q / N) in long double, cached on the device.
 struct FftTables {
@@ -908,20 +748,25 @@ static FftTables fft_tables(uint32_t N, hipStream_t s)
     std::lock_guard<std::mutex> lk(mu);
     auto it = cache.find({dev, N});
     if (it != cache.end()) return it->second;
-    std::vector<double> h(N), tw(2 * (size_t)N);
+    // the twiddles serve the power-of-two kernels only (the chirp-z transform
+    // keeps its own per length)
+    const bool pow2 = (N & (N - 1)) == 0 && N <= 8192;
+    std::vector<double> h(N), tw(pow2 ? 2 * (size_t)N : 0);
     for (uint32_t i = 0; i < N; i++) h[i] = 2 * std::pow((std::sin(M_PI * i / (N))), 2);
     const long double two_pi = 6.283185307179586476925286766559005768L;
-    for (uint32_t q = 0; q < N; q++) {
+    for (uint32_t q = 0; pow2 && q < N; q++) {
         const long double ang = two_pi * (long double)q / (long double)N;
         tw[2 * q] = (double)cosl(ang);
         tw[2 * q + 1] = (double)-sinl(ang);
     }
     FftTables t;
     hip_check(hipMalloc(&t.hann, N * sizeof(double)), "hipMalloc(hann)");
-    hip_check(hipMalloc(&t.tw, 2 * (size_t)N * sizeof(double)), "hipMalloc(twiddles)");
     hip_check(hipMemcpyAsync(t.hann, h.data(), N * sizeof(double), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
-    hip_check(hipMemcpyAsync(t.tw, tw.data(), 2 * (size_t)N * sizeof(double), hipMemcpyHostToDevice, s),
-              "hipMemcpyAsync");
+    if (pow2) {
+        hip_check(hipMalloc(&t.tw, 2 * (size_t)N * sizeof(double)), "hipMalloc(twiddles)");
+        hip_check(hipMemcpyAsync(t.tw, tw.data(), 2 * (size_t)N * sizeof(double), hipMemcpyHostToDevice, s),
+                  "hipMemcpyAsync");
+    }
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     cache[{dev, N}] = t;
     return t;
@@ -1128,8 +973,13 @@ static void fft_impl(const double *hist, HistLayout lay, bool rows_input, size_t
         const uint32_t N = kv.first;
         const FftTables t = fft_tables(N, s);
         const bool pow2 = (N & (N - 1)) == 0 && N <= 8192;
-        if (!pow2 && N > 64 * 256) throw Error(HM_ERR_UNSUPPORTED, "fft_tunes: window longer than 16384 samples");
         const auto &ws = kv.second;
+        const bool lds_pow2 = pow2 && 2 * (size_t)N * sizeof(double) + kRowsStaticLds <= lds_per_block();
+        if (!lds_pow2) {
+            // any other length: the chirp-z transform, one window at a time
+            for (uint32_t w : ws) fft_tune_chirpz(hist, lay, (uint32_t)n, means, from[w], N, t.hann, out, w, s);
+            continue;
+        }
         for (size_t b0 = 0; b0 < ws.size();) {
             // consecutive output windows only (one launch writes a contiguous window range)
             size_t b1 = b0 + 1;
@@ -1171,17 +1021,12 @@ static void fft_impl(const double *hist, HistLayout lay, bool rows_input, size_t
                         }
                     }
                 }
-            } else if (pow2 && 2 * (size_t)N * sizeof(double) + kRowsStaticLds <= lds_per_block()) {
+            } else {
                 // longer windows: one signal per workgroup (2 N doubles of LDS)
                 allow_dynamic_lds((const void *)k_fft_tune_pow2, 2 * N * sizeof(double));
                 LaunchTimer timer("k_fft_tune_pow2", s);
                 hipLaunchKernelGGL(k_fft_tune_pow2, dim3(blocks), dim3(256), 2 * N * sizeof(double), s, a);
                 launch_check("k_fft_tune_pow2");
-            } else {
-                double *mag = static_cast<double *>(workspace((size_t)blocks * N * sizeof(double), 3));
-                LaunchTimer timer("k_fft_tune_dft", s);
-                hipLaunchKernelGGL(k_fft_tune_dft, dim3(blocks), dim3(256), 0, s, a, mag);
-                launch_check("k_fft_tune_dft");
             }
             b0 = b1;
         }
