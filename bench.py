@@ -47,11 +47,13 @@ def parse():
     ap.add_argument("--cpu-sample-stride", type=int, default=13)
     ap.add_argument("--single-mode", action="store_true", help="time only --mode (profiling runs)")
     ap.add_argument("--compact-every", type=int, default=0)
-    ap.add_argument("--legs", default="indicators,reversibility,tunes,adjoint,batch",
+    ap.add_argument("--legs", default="indicators,reversibility,tunes,tune_all,adjoint,batch",
                     help="extra measurements after the headline: indicators, adjoint, batch (comma list, or none)")
-    ap.add_argument("--batch-jobs", type=int, default=512)
-    ap.add_argument("--batch-max-turns", type=int, default=100_000,
-                    help="upper end of the batch leg's log-uniform turn counts (the full job mix: 1000000)")
+    ap.add_argument("--batch-jobs", type=int, default=4096)
+    ap.add_argument("--batch-max-turns", type=int, default=1_000_000,
+                    help="upper end of the batch leg's log-uniform turn counts")
+    ap.add_argument("--batch-plain-stride", type=int, default=16,
+                    help="one call per job is timed on every k-th job of the mix (a seeded random sample)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run `--steps` device-to-device copies of MIB MiB (the known-size copy that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters), then exit")
@@ -258,8 +260,7 @@ def tunes_leg():
         fft = C.history_tunes(rows, n, T, [(0, T - 1)], "fft", row_len=L, means=means)
         birk = C.history_tunes(rows, n, T, [(0, T - 1)], "birkhoff", row_len=L, means=means)
         h_ms = C.profile_read("k_history_rows")[0]
-        f_ms = sum(C.profile_read(k)[0] for k in ("k_signal_rows", "k_fft_tune_rows", "k_fft_tune_pow2",
-                                                  "k_fft_tune_dft"))
+        f_ms = sum(C.profile_read(k)[0] for k in ("k_signal_rows", "k_fft_tune_rows", "k_fft_tune_pow2"))
         b_ms = C.profile_read("k_birkhoff_hist")[0]
         C.profile_enable(False)
         n_read = int(np.all(np.isfinite(means.download(np.float64, 4 * n).reshape(4, n)), axis=0).sum())
@@ -278,6 +279,52 @@ def tunes_leg():
         "birkhoff_ms": b_ms, "birkhoff_GBps": rbytes / (b_ms / 1e3) / 1e9,
         "hbm_peak_GBps": HBM_PEAK_GBPS,
         "finite_fft_tunes": float(np.mean(np.isfinite(fft))), "finite_birkhoff_tunes": float(np.mean(np.isfinite(birk))),
+    }
+
+
+def tune_all_leg(n=90_000, n_turns=10_000, reps=2):
+    """The reference's own tune workload (tests/new_tune.ipynb cells 1-3): n
+    copies of (0.1, 0, 0.1, 0), henon_tracker(1e4, 0.168, 0.201, "sps") (eps=0),
+    tune_all(p, 1e4, mu=0, from_idx=[0, 500], to_idx=[500, 1000]), through the
+    drop-in engine call (trajectory rows, means, Birkhoff and FFT tunes of four
+    windows: 501, 501, 10001 and 10000 samples, results back on the host). The
+    notebook reports 38.33 s on the reference's CPU path. FFT bytes: the two
+    coordinates of a plane read once per window (16 B per sample per plane)."""
+    import henon_map_cpp_amd as hm
+    from henon_map_cpp_amd import _capi as C
+
+    pp = np.full(n, 0.1)
+    z = np.zeros(n)
+    t = hm.henon_tracker(n_turns, 0.168, 0.201, "sps")
+    best = None
+    for _ in range(reps):
+        p = hm.particles(pp, z, pp, z)
+        C.check(C.lib().hm_stream_synchronize(None))
+        C.profile_reset()
+        C.profile_enable(True)
+        t0 = time.perf_counter()
+        rb, rf = t.tracker.all_tunes(p.particles, n_turns, 0.0, 10.0, float("nan"), False, [0, 500], [500, 1000])
+        dt = time.perf_counter() - t0
+        ks = {k: C.profile_read(k)[0] for k in ("k_history_rows", "k_birkhoff_hist", "k_fft_tune_rows",
+                                                "k_cz_columns", "k_cz_rows", "k_cz_final", "k_cz_tune")}
+        C.profile_enable(False)
+        if best is None or dt < best[0]:
+            best = (dt, ks, np.asarray(rb), np.asarray(rf))
+    dt, ks, rb, rf = best
+    fft_ms = sum(v for k, v in ks.items() if k.startswith("k_cz") or k == "k_fft_tune_rows")
+    fft_bytes = 16.0 * n * 2 * (501 + 501 + 10001 + 10000)
+    return {
+        "workload": f"tune_all of tests/new_tune.ipynb: {n} particles at (0.1, 0, 0.1, 0), {n_turns} turns, "
+                    "sps eps=0, mu=0, windows [0,500], [500,1000] + (0, 1e4) + full signal",
+        "seconds": dt,
+        "reference_cpu_seconds": 38.33,
+        "speedup_vs_reference_cpu": 38.33 / dt,
+        "particle_turns_per_s": n * n_turns / dt,
+        "kernel_ms": ks,
+        "fft_stage_ms": fft_ms,
+        "fft_algorithmic_GBps": fft_bytes / (fft_ms / 1e3) / 1e9,
+        "history_GBps": 32.0 * n * (n_turns + 1) / (ks["k_history_rows"] / 1e3) / 1e9,
+        "fft_tune_x_full": float(rf[0, -2]), "birkhoff_tune_x_full": float(rb[0, -2]),
     }
 
 
@@ -353,56 +400,63 @@ def adjoint_leg():
     return out
 
 
-def batch_leg(n_jobs, max_turns=100_000):
-    """Uneven independent jobs in one batched launch sequence against one plain
-    hm_track call per job (same GPU, same exact arithmetic), wall time each."""
+def batch_leg(n_jobs, max_turns=1_000_000, plain_stride=16):
+    """Uneven independent jobs (the batched-jobs mix: 4096 jobs, N ~ logU[64,
+    65536], T ~ logU[1e3, 1e6], 30 % with MEGNO) in one batched launch sequence,
+    against one plain call per job (hm_track, or hm_track_indicators for the
+    MEGNO jobs; same GPU, same exact arithmetic), wall time each. The whole mix
+    runs batched; the one-call-per-job rate is measured on every
+    `plain_stride`-th job (a seeded random sample of the mix: about 190 s for
+    all 4096 jobs), which is also run batched on its own to compare results
+    and times job for job."""
     from henon_map_cpp_amd import _capi as C
     from henon_map_cpp_amd import workloads as W
 
     c = dict(W.BATCHED_JOBS, t_range=(W.BATCHED_JOBS["t_range"][0], max_turns))
-    c["name"] = c["name"].replace("logU[1e3, 1e5]", f"logU[1e3, {max_turns:.0e}]")
+    c["name"] = c["name"].replace("logU[1e3, 1e6]", f"logU[1e3, {max_turns:.0e}]")
     params = W.batch_job_params(n_jobs, c)
     coef, freq = W.seeded_line_coefficients(), W.line_frequencies()
+    t0 = time.perf_counter()
     ics, tabs = [], []
-    for (n, T, qx, qy, eps, mu, seed) in params:
+    for k, (n, T, qx, qy, eps, mu, seed) in enumerate(params):
         ics.append(np.stack(W.ball(n, c["r_max"], seed=seed)))
         wx = C.lines_modulation(qx, eps, coef, freq, 0, T)
         wy = C.lines_modulation(qy, eps, coef, freq, 0, T)
         tabs.append(C.DeviceBuffer.from_array(C.build_table(wx, wy)))
-    pt = sum(p[0] * p[1] for p in params)
+        if k and k % 1024 == 0:
+            print(f"[bench] batch: tables of {k} jobs, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    megno_job = [W.batch_megno_job(k) for k in range(n_jobs)]
+    renorm = c["megno_renorm_every"]
 
-    def fresh():
-        return [C.DeviceParticles(*x) for x in ics]
-
-    megno_job = [k % 10 < 3 for k in range(n_jobs)]  # 30% of the jobs also accumulate MEGNO
-
-    def batched(graph=False):
-        parts = fresh()
-        megs = [C.DeviceBuffer(8 * p[0]) if m else None for p, m in zip(params, megno_job)]
-        jobs = [(pp, tb, p[1], p[1], p[5], c["barrier"], mg, 100)
-                for pp, tb, p, mg in zip(parts, tabs, params, megs)]
+    def batched(sel, graph=False):
+        parts = [C.DeviceParticles(*ics[k]) for k in sel]
+        megs = [C.DeviceBuffer(8 * params[k][0]) if megno_job[k] else None for k in sel]
+        jobs = [(pp, tabs[k], params[k][1], params[k][1], params[k][5], c["barrier"], mg, renorm)
+                for pp, k, mg in zip(parts, sel, megs)]
         _sync()
         t0 = time.perf_counter()
-        C.track_batch(jobs, C.MODE_EXACT, C.BatchCfg(0, 0, 0, 1) if graph else None)
+        cfg = C.BatchCfg(tuned.block, tuned.compact_every, tuned.waves, 1) if graph else None
+        C.track_batch(jobs, C.MODE_EXACT, cfg)
         C.check(C.lib().hm_stream_synchronize(None))
         return time.perf_counter() - t0, parts, megs
 
-    def plain():
-        parts = fresh()
+    def plain(sel):
+        parts = [C.DeviceParticles(*ics[k]) for k in sel]
         megs = []
         _sync()
         t0 = time.perf_counter()
-        for k, (pp, tb, p, m) in enumerate(zip(parts, tabs, params, megno_job)):
-            if k and k % 512 == 0:
+        for q, (pp, k) in enumerate(zip(parts, sel)):
+            n, T, mu = params[k][0], params[k][1], params[k][5]
+            if q and q % 64 == 0:
                 C.check(C.lib().hm_stream_synchronize(None))
-                print(f"[bench] batch: {k} of {n_jobs} plain calls, {time.perf_counter() - t0:.1f} s", file=sys.stderr,
-                      flush=True)
-            if m:
-                r = C.track_indicators(pp, tb, p[1], p[1], p[5], c["barrier"], mode=C.MODE_EXACT, renorm_every=100,
-                                       samples=[p[1]])
+                print(f"[bench] batch: {q} of {len(sel)} plain calls, {time.perf_counter() - t0:.1f} s",
+                      file=sys.stderr, flush=True)
+            if megno_job[k]:
+                r = C.track_indicators(pp, tabs[k], T, T, mu, c["barrier"], mode=C.MODE_EXACT, renorm_every=renorm,
+                                       samples=[T])
                 megs.append(r["megno"][0])
             else:
-                C.track(pp, tb, p[1], p[1], p[5], c["barrier"], False, C.MODE_EXACT)
+                C.track(pp, tabs[k], T, T, mu, c["barrier"], False, C.MODE_EXACT)
                 megs.append(None)
         C.check(C.lib().hm_stream_synchronize(None))
         return time.perf_counter() - t0, parts, megs
@@ -410,24 +464,50 @@ def batch_leg(n_jobs, max_turns=100_000):
     def note(what, t):
         print(f"[bench] batch: {what} {t:.2f} s", file=sys.stderr, flush=True)
 
-    note("warm-up", batched()[0])
+    every = list(range(n_jobs))
+    sub = every[::plain_stride]
+    pt = sum(params[k][0] * params[k][1] for k in every)
+    pt_sub = sum(params[k][0] * params[k][1] for k in sub)
+    # the autotuner picks block, wavefronts and compaction interval for this
+    # job mix (trials on copies, every job cut to 4000 turns); the timed
+    # batched calls below pass no configuration and so use its choice
+    C.check(C.lib().hm_autotune_batch_clear())
+    parts = [C.DeviceParticles(*ics[k]) for k in every]
+    megs = [C.DeviceBuffer(8 * params[k][0]) if megno_job[k] else None for k in every]
+    t0 = time.perf_counter()
+    tuned, trial_ms = C.autotune_batch([(pp, tabs[k], params[k][1], params[k][1], params[k][5], c["barrier"], mg, renorm)
+                                        for pp, k, mg in zip(parts, every, megs)], C.MODE_EXACT, trial_turns=4000)
+    tune_s = time.perf_counter() - t0
+    del parts, megs
+    note(f"autotune (block {tuned.block}, compact_every {tuned.compact_every}, waves {tuned.waves})", tune_s)
     runs = []
     for _ in range(2):
-        runs.append(batched())
-        note("batched", runs[-1][0])
-    tb, pb, mb = min(runs, key=lambda r: r[0])
-    tg = batched(graph=True)[0]
-    note("batched, graph replay", tg)
-    tp, pp, mp = plain()
-    note("one call per job", tp)
+        runs.append(batched(every)[0])
+        note("batched, whole mix", runs[-1])
+    tb = min(runs)
+    batched(every, graph=True)  # records the launch sequence
+    tg = batched(every, graph=True)[0]
+    note("batched, graph replay, whole mix", tg)
+    tbs, pb, mb = batched(sub)
+    note(f"batched, every {plain_stride}th job", tbs)
+    tp, pp, mp = plain(sub)
+    note(f"one call per job, every {plain_stride}th job", tp)
     identical = all(np.array_equal(u, v, equal_nan=True) for a, b in zip(pb, pp) for u, v in zip(a.get(), b.get()))
     identical = identical and all(
         np.array_equal(a.download(np.float64, len(b)), b, equal_nan=True) for a, b in zip(mb, mp) if a is not None)
-    return {"workload": c["name"] + ", 30% of the jobs with MEGNO (renormalised every 100 turns)",
+    return {"workload": c["name"] + f", 30% of the jobs with MEGNO (renormalised every {renorm} turns)",
             "jobs": n_jobs, "megno_jobs": int(sum(megno_job)), "nominal_particle_turns": pt,
-            "batched_s": tb, "batched_graph_s": tg, "plain_s": tp, "batched_value": pt / tb,
-            "plain_value": pt / tp,
-            "speedup": tp / tb, "identical_to_plain": identical}
+            "autotuned": {"block": tuned.block, "compact_every": tuned.compact_every, "waves": tuned.waves,
+                          "trial_ms": trial_ms, "autotune_s": tune_s},
+            "batched_s": tb, "batched_graph_replay_s": tg, "batched_value": pt / tb,
+            "plain_sample": f"every {plain_stride}th job: {len(sub)} jobs, {pt_sub:.4g} nominal particle-turns",
+            "plain_s": tp, "plain_value": pt_sub / tp,
+            # the sample is a uniform random draw of the jobs, so n_jobs / len(sub)
+            # times its one-call-per-job time estimates the whole mix's
+            "plain_whole_mix_estimate_s": tp * n_jobs / len(sub),
+            "speedup": tp * n_jobs / len(sub) / tb,
+            "sample_batched_s": tbs, "sample_speedup": tp / tbs,
+            "identical_to_plain": identical}
 
 
 def host_cpu_info():
@@ -607,11 +687,14 @@ def main():
     if "tunes" in legs:
         line["tunes"] = tunes_leg()
         print(f"[bench] tunes {line['tunes']}", file=sys.stderr, flush=True)
+    if "tune_all" in legs:
+        line["tune_all"] = tune_all_leg()
+        print(f"[bench] tune_all {line['tune_all']}", file=sys.stderr, flush=True)
     if "adjoint" in legs:
         line["adjoint"] = adjoint_leg()
         print(f"[bench] adjoint {line['adjoint']}", file=sys.stderr, flush=True)
     if "batch" in legs:
-        line["batch"] = batch_leg(a.batch_jobs, a.batch_max_turns)
+        line["batch"] = batch_leg(a.batch_jobs, a.batch_max_turns, a.batch_plain_stride)
         print(f"[bench] batch {line['batch']}", file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:
         try:

@@ -36,6 +36,7 @@ try:
     from .henon_map_engine import storage_4d_gpu as cpp_storage_4d_gpu
     from .henon_map_engine import tune_birkhoff_construct as cpp_tune_birkhoff
     from .henon_map_engine import track_batch as cpp_track_batch
+    from .henon_map_engine import autotune_batch as cpp_autotune_batch
     _ENGINE_ERROR = None
 except (ImportError, OSError, AttributeError) as _e:  # not built (or stale): entry points raise on use
     _ENGINE_ERROR = _e
@@ -47,14 +48,14 @@ except (ImportError, OSError, AttributeError) as _e:  # not built (or stale): en
     cpp_birkhoff_weights = cpp_henon_tracker = cpp_henon_tracker_gpu = particles_4d = particles_4d_gpu = _missing
     cpp_get_tunes = lbc = lbcm = cpp_matrix_4d_vector = cpp_matrix_4d_vector_gpu = _missing
     cpp_megno_birkhoff = cpp_megno = storage_4d = vector_4d_gpu = cpp_storage_4d_gpu = cpp_tune_birkhoff = _missing
-    cpp_track_batch = _missing
+    cpp_track_batch = cpp_autotune_batch = _missing
 
 import pandas as pd  # noqa: E402
 
 __all__ = ["gpu_available", "birkhoff_weights", "get_tunes", "particles", "henon_tracker", "matrix_4d_vector",
            "vector_4d", "lyapunov_birkhoff_construct", "lyapunov_birkhoff_construct_multi", "megno_construct",
            "megno_birkhoff_construct", "tune_birkhoff_construct", "storage", "storage_gpu", "particles_4d",
-           "particles_4d_gpu", "storage_4d", "vector_4d_gpu", "track_batch"]
+           "particles_4d_gpu", "storage_4d", "vector_4d_gpu", "track_batch", "autotune_batch"]
 
 
 def track_batch(jobs, mode="exact"):
@@ -62,6 +63,15 @@ def track_batch(jobs, mode="exact"):
     (henon_tracker, particles, n_turns, mu, barrier); each job advances exactly
     as tracker.track(particles, n_turns, mu, barrier) would."""
     cpp_track_batch([(t.tracker, p.particles, int(n), float(mu), float(b)) for t, p, n, mu, b in jobs], mode)
+
+
+def autotune_batch(jobs, mode="exact", trial_turns=4000):
+    """Times batched launch configurations on copies of the jobs' particles
+    (each job cut to trial_turns turns; the particles are not advanced) and
+    makes later track_batch calls of the same job mix use the fastest; returns
+    it as a dict. Results do not depend on the configuration."""
+    return dict(cpp_autotune_batch([(t.tracker, p.particles, int(n), float(mu), float(b)) for t, p, n, mu, b in jobs],
+                                   mode, int(trial_turns)))
 
 
 def gpu_available():

@@ -82,6 +82,8 @@ SIGNATURES = {
     "hm_track": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, D, D, I, I, ctypes.POINTER(LaunchCfg), P]),
     "hm_track_noalloc": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, D, D, I, I, ctypes.POINTER(LaunchCfg), P]),
     "hm_track_batch": (I, [ctypes.POINTER(Job), SZ, I, ctypes.POINTER(BatchCfg), P]),
+    "hm_autotune_batch": (I, [ctypes.POINTER(Job), SZ, I, U32, ctypes.POINTER(BatchCfg), ctypes.POINTER(D), P]),
+    "hm_autotune_batch_clear": (I, []),
     "hm_autotune_track": (I, [P, P, P, P, P, SZ, P, SZ, U32, U32, D, D, I, ctypes.POINTER(LaunchCfg),
                               ctypes.POINTER(D), P]),
     "hm_autotune_clear": (I, []),
@@ -285,9 +287,7 @@ def track(parts: DeviceParticles, table_dev: DeviceBuffer, table_len: int, n_tur
                          int(inverse), mode, c, stream))
 
 
-def track_batch(jobs, mode=MODE_EXACT, cfg: BatchCfg | None = None, stream=None):
-    """jobs: list of (DeviceParticles, table DeviceBuffer, table_len, n_turns, mu, barrier[, megno
-    DeviceBuffer, renorm_every]); with a megno buffer the job also accumulates the MEGNO."""
+def _job_array(jobs):
     arr = (Job * max(len(jobs), 1))()
     for k, job in enumerate(jobs):
         parts, tdev, tlen, n_turns, mu, barrier = job[:6]
@@ -295,8 +295,24 @@ def track_batch(jobs, mode=MODE_EXACT, cfg: BatchCfg | None = None, stream=None)
         renorm = job[7] if len(job) > 7 else 0
         x, px, y, py, st = parts.ptrs()
         arr[k] = Job(x, px, y, py, st, parts.n, tdev.ptr, tlen, n_turns, renorm, mu, barrier, float("nan"), megno)
+    return arr
+
+
+def track_batch(jobs, mode=MODE_EXACT, cfg: BatchCfg | None = None, stream=None):
+    """jobs: list of (DeviceParticles, table DeviceBuffer, table_len, n_turns, mu, barrier[, megno
+    DeviceBuffer, renorm_every]); with a megno buffer the job also accumulates the MEGNO."""
     c = ctypes.byref(cfg) if cfg is not None else None
-    check(lib().hm_track_batch(arr, len(jobs), mode, c, stream))
+    check(lib().hm_track_batch(_job_array(jobs), len(jobs), mode, c, stream))
+
+
+def autotune_batch(jobs, mode=MODE_EXACT, trial_turns=4000, stream=None):
+    """Times batch configurations on copies of the jobs (each cut to trial_turns
+    turns); returns (BatchCfg, ms). Later track_batch(..., cfg=None) calls of
+    the same job-mix class use it."""
+    best, ms = BatchCfg(), D()
+    check(lib().hm_autotune_batch(_job_array(jobs), len(jobs), mode, trial_turns, ctypes.byref(best),
+                                  ctypes.byref(ms), stream))
+    return best, ms.value
 
 
 def autotune_track(parts: DeviceParticles, table_dev: DeviceBuffer, table_len: int, mu: float, barrier: float,

@@ -1263,10 +1263,9 @@ class Storage4DGpu {
 // Many independent (tracker, particles, n_turns, mu, barrier) jobs in one
 // batched launch sequence (hm_track_batch); each job's particles advance as
 // with tracker.track(particles, n_turns, mu, barrier, inverse=False).
-void track_batch(const py::list &jobs, const std::string &mode)
+static std::vector<hm_job> batch_jobs(const py::list &jobs, std::vector<std::pair<Particles4D *, unsigned int>> &adv)
 {
     std::vector<hm_job> js;
-    std::vector<std::pair<Particles4D *, unsigned int>> adv;
     std::unordered_set<const Particles4D *> seen;
     for (const py::handle &h : jobs) {
         const py::tuple t = h.cast<py::tuple>();
@@ -1278,6 +1277,35 @@ void track_batch(const py::list &jobs, const std::string &mode)
         js.push_back(tr.batch_job(p, n_turns, t[3].cast<double>(), t[4].cast<double>()));
         adv.emplace_back(&p, n_turns);
     }
+    return js;
+}
+
+// Batch autotuner (hm_autotune_batch): times launch configurations on copies of
+// the jobs' particles, each job cut to trial_turns turns, and makes later
+// track_batch calls of the same job-mix class use the fastest. The particles
+// are not advanced.
+py::dict autotune_batch(const py::list &jobs, const std::string &mode, unsigned int trial_turns)
+{
+    std::vector<std::pair<Particles4D *, unsigned int>> adv;
+    const std::vector<hm_job> js = batch_jobs(jobs, adv);
+    hm_batch_cfg best{};
+    double ms = 0.0;
+    {
+        py::gil_scoped_release nogil;
+        check(hm_autotune_batch(js.data(), js.size(), mode_of(mode), trial_turns, &best, &ms, nullptr));
+    }
+    py::dict d;
+    d["block"] = best.block;
+    d["compact_every"] = best.compact_every;
+    d["waves"] = best.waves;
+    d["trial_ms"] = ms;
+    return d;
+}
+
+void track_batch(const py::list &jobs, const std::string &mode)
+{
+    std::vector<std::pair<Particles4D *, unsigned int>> adv;
+    const std::vector<hm_job> js = batch_jobs(jobs, adv);
     const int m = mode_of(mode);
     {
         py::gil_scoped_release nogil;
@@ -1452,6 +1480,8 @@ PYBIND11_MODULE(henon_map_engine, m)
             py::arg("frequencies") = std::vector<double>());
 
     m.def("get_tunes", &get_tunes, "Get tunes", py::arg("x"), py::arg("px"));
+    m.def("autotune_batch", &autotune_batch, "Pick the fastest batch launch configuration for this job mix",
+          py::arg("jobs"), py::arg("mode") = "exact", py::arg("trial_turns") = 4000);
     m.def("track_batch", &track_batch, "Track many (tracker, particles, n_turns, mu, barrier) jobs in one batch",
           py::arg("jobs"), py::arg("mode") = "exact");
 

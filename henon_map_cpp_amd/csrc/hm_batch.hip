@@ -23,6 +23,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
 #include <vector>
 
 #include "hm_device.h"
@@ -331,6 +335,239 @@ using namespace hm;
 
 static constexpr uint32_t kBatchFirstSegment = 125;
 
+namespace {
+
+// Launch configurations found by hm_autotune_batch, per device, mode and
+// job-mix class: floor(log2) of the job count, of the particles in total and of
+// the longest job's turns, and the share of MEGNO particles in quarters.
+struct BatchKey {
+    int dev, mode, jobs_bits, n_bits, t_bits, megno_q;
+    bool operator<(const BatchKey &o) const
+    {
+        return std::tie(dev, mode, jobs_bits, n_bits, t_bits, megno_q) <
+               std::tie(o.dev, o.mode, o.jobs_bits, o.n_bits, o.t_bits, o.megno_q);
+    }
+};
+std::mutex g_batch_tune_mu;
+std::map<BatchKey, hm_batch_cfg> g_batch_tuned;
+
+int floor_log2(uint64_t v)
+{
+    int b = 0;
+    while ((uint64_t(2) << b) <= v) ++b;
+    return b;
+}
+
+BatchKey batch_key(const hm_job *jobs, size_t n_jobs, int mode)
+{
+    uint64_t total = 0, megno = 0;
+    uint32_t max_t = 0;
+    for (size_t k = 0; k < n_jobs; ++k) {
+        total += jobs[k].n;
+        if (jobs[k].megno) megno += jobs[k].n;
+        if (jobs[k].n) max_t = std::max(max_t, jobs[k].n_turns);
+    }
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    const int q = total ? (int)std::min<uint64_t>(3, (4 * megno) / total) : 0;
+    return BatchKey{dev, mode, floor_log2(n_jobs), floor_log2(std::max<uint64_t>(total, 1)),
+                    floor_log2(std::max<uint32_t>(max_t, 1)), q};
+}
+
+// Instantiated graphs of earlier calls, keyed by everything the recorded launch
+// sequence depends on (the job descriptors are read from device memory, so
+// particle and table pointers, mu and barriers may change between replays).
+struct GraphKey {
+    int dev, mode, block;
+    uint32_t grid, every, max_turns, n_plain;
+    size_t n_jobs;
+    uint64_t total_n, megno_n;
+    const void *ws, *mstate, *inv_idx;
+    bool operator<(const GraphKey &o) const
+    {
+        return std::tie(dev, mode, block, grid, every, max_turns, n_plain, n_jobs, total_n, megno_n, ws, mstate,
+                        inv_idx) < std::tie(o.dev, o.mode, o.block, o.grid, o.every, o.max_turns, o.n_plain,
+                                            o.n_jobs, o.total_n, o.megno_n, o.ws, o.mstate, o.inv_idx);
+    }
+};
+constexpr size_t kGraphCache = 8;
+std::mutex g_graph_mu;
+std::vector<std::pair<GraphKey, hipGraphExec_t>> g_graphs;  // oldest first
+
+// The batched launch sequence for a resolved configuration.
+void batch_run(const hm_job *jobs, size_t n_jobs, int mode, int block, int every, uint32_t waves, bool graph,
+               hipStream_t s)
+{
+    // tracking-only jobs first, then the MEGNO jobs (each kernel takes one range)
+    std::vector<size_t> order;
+    order.reserve(n_jobs);
+    for (size_t k = 0; k < n_jobs; ++k)
+        if (!jobs[k].megno) order.push_back(k);
+    const uint32_t n_plain = (uint32_t)order.size();
+    for (size_t k = 0; k < n_jobs; ++k)
+        if (jobs[k].megno) order.push_back(k);
+    std::vector<JobDesc> desc(n_jobs);
+    uint64_t total_n = 0, megno_n = 0;
+    uint32_t max_turns = 0;
+    for (size_t r = 0; r < n_jobs; ++r) {
+        const size_t k = order[r];
+        const hm_job &j = jobs[k];
+        if (j.n > 0x7fffffffull) throw Error(HM_ERR_ARG, "batch: a job exceeds 2^31 particles");
+        if (j.n && (!j.x || !j.px || !j.y || !j.py || !j.steps || !j.table))
+            throw Error(HM_ERR_ARG, "batch: NULL pointer in job " + std::to_string(k));
+        if (j.table_len > 0xffffffffull) throw Error(HM_ERR_ARG, "batch: table exceeds 2^32 turns");
+        if (!std::isnan(j.kick_module)) throw Error(HM_ERR_UNSUPPORTED, "batch: the random kick is not available");
+        JobDesc &d = desc[r];
+        d.x = j.x; d.px = j.px; d.y = j.y; d.py = j.py; d.steps = j.steps;
+        d.megno = j.megno;
+        d.renorm = j.renorm_every;
+        d.state = nullptr;
+        if (j.megno) {
+            d.state = reinterpret_cast<double *>((uintptr_t)(megno_n * kMegnoState * sizeof(double)));  // offset
+            megno_n += j.n;
+        }
+        d.tab = reinterpret_cast<const TabEntry *>(j.table);
+        d.n = (uint32_t)j.n;
+        d.n_turns = j.n ? j.n_turns : 0;
+        d.tab_len = (uint32_t)j.table_len;
+        d.list_off = (uint32_t)total_n;
+        d.mu = j.mu;
+        d.b2 = j.barrier * j.barrier;  // henon.cu:2717
+        total_n += j.n;
+        if (total_n > 0xffffffffull) throw Error(HM_ERR_ARG, "batch: more than 2^32 particles in total");
+        max_turns = std::max(max_turns, d.n_turns);
+    }
+    if (max_turns == 0) return;
+
+    // segment schedule
+    std::vector<std::pair<uint32_t, uint32_t>> seg;  // (start, length)
+    for (uint64_t done = 0, len = std::min<uint32_t>(kBatchFirstSegment, (uint32_t)every); done < max_turns;) {
+        seg.emplace_back((uint32_t)done, (uint32_t)len);
+        done += len;
+        len = std::min<uint64_t>(len * 2, (uint64_t)every);
+    }
+
+    // device buffers (workspace slot 4): descriptors | lists x2 | counts x2 | scan | queue
+    auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t b_desc = align(sizeof(JobDesc) * n_jobs);
+    const size_t b_list = align(sizeof(uint32_t) * std::max<uint64_t>(total_n, 1));
+    const size_t b_cnt = align(sizeof(uint32_t) * n_jobs);
+    const size_t b_off = align(sizeof(uint32_t) * (n_jobs + 1));
+    const size_t b_q = align(sizeof(BatchQueue));
+    char *ws = static_cast<char *>(workspace(b_desc + 2 * b_list + 2 * b_cnt + b_off + b_q, 4));
+    double *mstate = megno_n ? static_cast<double *>(workspace(megno_n * kMegnoState * sizeof(double), 9))
+                             : nullptr;
+    for (auto &d : desc)
+        if (d.megno) d.state = reinterpret_cast<double *>(reinterpret_cast<char *>(mstate) +
+                                                          reinterpret_cast<uintptr_t>(d.state));
+    const double *inv_idx = megno_n ? reciprocal_table(max_turns, s) : nullptr;
+    JobDesc *d_desc = reinterpret_cast<JobDesc *>(ws);
+    uint32_t *lists[2] = {reinterpret_cast<uint32_t *>(ws + b_desc),
+                          reinterpret_cast<uint32_t *>(ws + b_desc + b_list)};
+    uint32_t *counts[2] = {reinterpret_cast<uint32_t *>(ws + b_desc + 2 * b_list),
+                           reinterpret_cast<uint32_t *>(ws + b_desc + 2 * b_list + b_cnt)};
+    uint32_t *d_off = reinterpret_cast<uint32_t *>(ws + b_desc + 2 * b_list + 2 * b_cnt);
+    BatchQueue *d_q = reinterpret_cast<BatchQueue *>(ws + b_desc + 2 * b_list + 2 * b_cnt + b_off);
+
+    hip_check(hipMemcpyAsync(d_desc, desc.data(), sizeof(JobDesc) * n_jobs, hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync(jobs)");
+    // the descriptors are read from pageable memory: make sure the copy has
+    // consumed them before `desc` goes out of scope
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+
+    waves = std::max<uint32_t>(waves, (uint32_t)block / 64u);
+    const uint32_t grid = (waves * 64u + (uint32_t)block - 1) / (uint32_t)block;
+    uint32_t *status = status_word();
+    auto enqueue = [&](hipStream_t q) {
+        if (megno_n) {
+            hipLaunchKernelGGL(k_megno_init, dim3((uint32_t)((megno_n + 255) / 256)), dim3(256), 0, q, mstate,
+                               (uint64_t)megno_n);
+            launch_check("k_megno_init");
+        }
+        for (size_t i = 0; i < seg.size(); ++i) {
+            BatchSeg S{d_desc, (uint32_t)n_jobs, n_plain, inv_idx, seg[i].first, seg[i].second,
+                       i == 0 ? 1u : 0u, lists[i % 2], lists[(i + 1) % 2], counts[i % 2], counts[(i + 1) % 2],
+                       d_off, d_q, status};
+            hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(kPlanThreads), 0, q, S);
+            launch_check("k_batch_plan");
+            if (megno_n) {
+                if (mode == HM_MODE_EXACT)
+                    hipLaunchKernelGGL(k_batch_megno<kExact>, dim3(grid), dim3(block), 0, q, S);
+                else
+                    hipLaunchKernelGGL(k_batch_megno<kFast>, dim3(grid), dim3(block), 0, q, S);
+                launch_check("k_batch_megno");
+            }
+            if (mode == HM_MODE_EXACT)
+                hipLaunchKernelGGL(k_batch_track<kExact>, dim3(grid), dim3(block), 0, q, S);
+            else
+                hipLaunchKernelGGL(k_batch_track<kFast>, dim3(grid), dim3(block), 0, q, S);
+            launch_check("k_batch_track");
+        }
+    };
+    if (!graph) {
+        LaunchTimer timer("k_batch_track", s);
+        enqueue(s);
+        return;
+    }
+    // record the whole segment sequence once per job shape and replay it
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    const GraphKey key{dev, mode, block, grid, (uint32_t)every, max_turns, n_plain, n_jobs, total_n, megno_n,
+                       ws, mstate, inv_idx};
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    hipGraphExec_t ge = nullptr;
+    for (auto &e : g_graphs)
+        if (!(e.first < key) && !(key < e.first)) ge = e.second;
+    if (!ge) {
+        hipStream_t cs = nullptr;
+        hipGraph_t g = nullptr;
+        hip_check(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate");
+        try {
+            hip_check(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            enqueue(cs);
+            hip_check(hipStreamEndCapture(cs, &g), "hipStreamEndCapture");
+            hip_check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+        } catch (...) {
+            if (ge) (void)hipGraphExecDestroy(ge);
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipStreamDestroy(cs);
+            throw;
+        }
+        (void)hipGraphDestroy(g);
+        (void)hipStreamDestroy(cs);
+        if (g_graphs.size() == kGraphCache) {
+            (void)hipGraphExecDestroy(g_graphs.front().second);
+            g_graphs.erase(g_graphs.begin());
+        }
+        g_graphs.emplace_back(key, ge);
+    }
+    {
+        LaunchTimer timer("k_batch_track", s);
+        hip_check(hipGraphLaunch(ge, s), "hipGraphLaunch");
+    }
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+struct BatchCfgResolved {
+    int block, every;
+    uint32_t waves;
+    bool graph;
+};
+
+BatchCfgResolved resolve(const hm_batch_cfg *cfg)
+{
+    BatchCfgResolved r;
+    r.every = cfg && cfg->compact_every ? cfg->compact_every : 1000;
+    if (r.every <= 0) throw Error(HM_ERR_ARG, "batch: compact_every must be positive");
+    r.block = cfg && cfg->block ? cfg->block : 256;
+    if (r.block < 64 || r.block > 256 || r.block % 64) throw Error(HM_ERR_ARG, "batch: block must be 64, 128, 192 or 256");
+    r.waves = cfg && cfg->waves > 0 ? (uint32_t)cfg->waves : batch_resident_waves();
+    r.graph = cfg && cfg->use_graph;
+    return r;
+}
+
+}  // namespace
+
 extern "C" int hm_track_batch(const hm_job *jobs, size_t n_jobs, int mode, const hm_batch_cfg *cfg, void *stream)
 {
     return guarded([&] {
@@ -338,148 +575,111 @@ extern "C" int hm_track_batch(const hm_job *jobs, size_t n_jobs, int mode, const
         if (n_jobs == 0) return;
         if (!jobs) throw Error(HM_ERR_ARG, "batch: NULL job array");
         if (n_jobs > (1u << 24)) throw Error(HM_ERR_ARG, "batch: too many jobs");
-        const int every = cfg && cfg->compact_every ? cfg->compact_every : 1000;
-        if (every <= 0) throw Error(HM_ERR_ARG, "batch: compact_every must be positive");
-        const int block = cfg && cfg->block ? cfg->block : 256;
-        if (block < 64 || block > 256 || block % 64) throw Error(HM_ERR_ARG, "batch: block must be 64, 128, 192 or 256");
-        uint32_t waves = cfg && cfg->waves > 0 ? (uint32_t)cfg->waves : batch_resident_waves();
-        const bool graph = cfg && cfg->use_graph;
-
-        // tracking-only jobs first, then the MEGNO jobs (each kernel takes one range)
-        std::vector<size_t> order;
-        order.reserve(n_jobs);
-        for (size_t k = 0; k < n_jobs; ++k)
-            if (!jobs[k].megno) order.push_back(k);
-        const uint32_t n_plain = (uint32_t)order.size();
-        for (size_t k = 0; k < n_jobs; ++k)
-            if (jobs[k].megno) order.push_back(k);
-        std::vector<JobDesc> desc(n_jobs);
-        uint64_t total_n = 0, megno_n = 0;
-        uint32_t max_turns = 0;
-        for (size_t r = 0; r < n_jobs; ++r) {
-            const size_t k = order[r];
-            const hm_job &j = jobs[k];
-            if (j.n > 0x7fffffffull) throw Error(HM_ERR_ARG, "batch: a job exceeds 2^31 particles");
-            if (j.n && (!j.x || !j.px || !j.y || !j.py || !j.steps || !j.table))
-                throw Error(HM_ERR_ARG, "batch: NULL pointer in job " + std::to_string(k));
-            if (j.table_len > 0xffffffffull) throw Error(HM_ERR_ARG, "batch: table exceeds 2^32 turns");
-            if (!std::isnan(j.kick_module)) throw Error(HM_ERR_UNSUPPORTED, "batch: the random kick is not available");
-            JobDesc &d = desc[r];
-            d.x = j.x; d.px = j.px; d.y = j.y; d.py = j.py; d.steps = j.steps;
-            d.megno = j.megno;
-            d.renorm = j.renorm_every;
-            d.state = nullptr;
-            if (j.megno) {
-                d.state = reinterpret_cast<double *>((uintptr_t)(megno_n * kMegnoState * sizeof(double)));  // offset
-                megno_n += j.n;
+        hm_batch_cfg tuned{};
+        if (!cfg) {
+            const BatchKey k = batch_key(jobs, n_jobs, mode);
+            std::lock_guard<std::mutex> lk(g_batch_tune_mu);
+            auto it = g_batch_tuned.find(k);
+            if (it != g_batch_tuned.end()) {
+                tuned = it->second;
+                cfg = &tuned;
             }
-            d.tab = reinterpret_cast<const TabEntry *>(j.table);
-            d.n = (uint32_t)j.n;
-            d.n_turns = j.n ? j.n_turns : 0;
-            d.tab_len = (uint32_t)j.table_len;
-            d.list_off = (uint32_t)total_n;
-            d.mu = j.mu;
-            d.b2 = j.barrier * j.barrier;  // henon.cu:2717
-            total_n += j.n;
-            if (total_n > 0xffffffffull) throw Error(HM_ERR_ARG, "batch: more than 2^32 particles in total");
-            max_turns = std::max(max_turns, d.n_turns);
         }
-        if (max_turns == 0) return;
+        const BatchCfgResolved r = resolve(cfg);
+        batch_run(jobs, n_jobs, mode, r.block, r.every, r.waves, r.graph, as_stream(stream));
+    });
+}
 
-        // segment schedule
-        std::vector<std::pair<uint32_t, uint32_t>> seg;  // (start, length)
-        for (uint64_t done = 0, len = std::min<uint32_t>(kBatchFirstSegment, (uint32_t)every); done < max_turns;) {
-            seg.emplace_back((uint32_t)done, (uint32_t)len);
-            done += len;
-            len = std::min<uint64_t>(len * 2, (uint64_t)every);
-        }
-
-        // device buffers (workspace slot 4): descriptors | lists x2 | counts x2 | scan | queue
-        auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
-        const size_t b_desc = align(sizeof(JobDesc) * n_jobs);
-        const size_t b_list = align(sizeof(uint32_t) * std::max<uint64_t>(total_n, 1));
-        const size_t b_cnt = align(sizeof(uint32_t) * n_jobs);
-        const size_t b_off = align(sizeof(uint32_t) * (n_jobs + 1));
-        const size_t b_q = align(sizeof(BatchQueue));
-        char *ws = static_cast<char *>(workspace(b_desc + 2 * b_list + 2 * b_cnt + b_off + b_q, 4));
-        double *mstate = megno_n ? static_cast<double *>(workspace(megno_n * kMegnoState * sizeof(double), 9))
-                                 : nullptr;
-        for (auto &d : desc)
-            if (d.megno) d.state = reinterpret_cast<double *>(reinterpret_cast<char *>(mstate) +
-                                                              reinterpret_cast<uintptr_t>(d.state));
-        const double *inv_idx = megno_n ? reciprocal_table(max_turns, as_stream(stream)) : nullptr;
-        JobDesc *d_desc = reinterpret_cast<JobDesc *>(ws);
-        uint32_t *lists[2] = {reinterpret_cast<uint32_t *>(ws + b_desc),
-                              reinterpret_cast<uint32_t *>(ws + b_desc + b_list)};
-        uint32_t *counts[2] = {reinterpret_cast<uint32_t *>(ws + b_desc + 2 * b_list),
-                               reinterpret_cast<uint32_t *>(ws + b_desc + 2 * b_list + b_cnt)};
-        uint32_t *d_off = reinterpret_cast<uint32_t *>(ws + b_desc + 2 * b_list + 2 * b_cnt);
-        BatchQueue *d_q = reinterpret_cast<BatchQueue *>(ws + b_desc + 2 * b_list + 2 * b_cnt + b_off);
-
+extern "C" int hm_autotune_batch(const hm_job *jobs, size_t n_jobs, int mode, uint32_t trial_turns,
+                                 hm_batch_cfg *best, double *best_ms, void *stream)
+{
+    return guarded([&] {
+        if (!best) throw Error(HM_ERR_ARG, "autotune_batch: NULL result");
+        if (mode != HM_MODE_EXACT && mode != HM_MODE_FAST) throw Error(HM_ERR_ARG, "autotune_batch: unknown mode");
+        if (n_jobs == 0 || !jobs || trial_turns == 0) throw Error(HM_ERR_ARG, "autotune_batch: empty trial");
+        if (n_jobs > (1u << 24)) throw Error(HM_ERR_ARG, "autotune_batch: too many jobs");
         hipStream_t s = as_stream(stream);
-        hip_check(hipMemcpyAsync(d_desc, desc.data(), sizeof(JobDesc) * n_jobs, hipMemcpyHostToDevice, s),
-                  "hipMemcpyAsync(jobs)");
-        // the descriptors are read from pageable memory: make sure the copy has
-        // consumed them before `desc` goes out of scope
-        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-
-        waves = std::max<uint32_t>(waves, (uint32_t)block / 64u);
-        const uint32_t grid = (waves * 64u + (uint32_t)block - 1) / (uint32_t)block;
-        uint32_t *status = status_word();
-        auto enqueue = [&](hipStream_t q) {
-            if (megno_n) {
-                hipLaunchKernelGGL(k_megno_init, dim3((uint32_t)((megno_n + 255) / 256)), dim3(256), 0, q, mstate,
-                                   (uint64_t)megno_n);
-                launch_check("k_megno_init");
-            }
-            for (size_t i = 0; i < seg.size(); ++i) {
-                BatchSeg S{d_desc, (uint32_t)n_jobs, n_plain, inv_idx, seg[i].first, seg[i].second,
-                           i == 0 ? 1u : 0u, lists[i % 2], lists[(i + 1) % 2], counts[i % 2], counts[(i + 1) % 2],
-                           d_off, d_q, status};
-                hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(kPlanThreads), 0, q, S);
-                launch_check("k_batch_plan");
-                if (megno_n) {
-                    if (mode == HM_MODE_EXACT)
-                        hipLaunchKernelGGL(k_batch_megno<kExact>, dim3(grid), dim3(block), 0, q, S);
-                    else
-                        hipLaunchKernelGGL(k_batch_megno<kFast>, dim3(grid), dim3(block), 0, q, S);
-                    launch_check("k_batch_megno");
-                }
-                if (mode == HM_MODE_EXACT)
-                    hipLaunchKernelGGL(k_batch_track<kExact>, dim3(grid), dim3(block), 0, q, S);
-                else
-                    hipLaunchKernelGGL(k_batch_track<kFast>, dim3(grid), dim3(block), 0, q, S);
-                launch_check("k_batch_track");
-            }
+        // trial copies of every job's particles (and MEGNO outputs), each job
+        // cut to at most trial_turns turns; the caller's arrays are only read
+        std::vector<hm_job> trial(jobs, jobs + n_jobs);
+        std::vector<void *> bufs;
+        auto release = [&] {
+            for (void *b : bufs) (void)hipFree(b);
         };
-        if (!graph) {
-            LaunchTimer timer("k_batch_track", s);
-            enqueue(s);
-            return;
-        }
-        // record the whole segment sequence once and replay it as one graph
-        hipStream_t cs = nullptr;
-        hipGraph_t g = nullptr;
-        hipGraphExec_t ge = nullptr;
-        hip_check(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate");
         try {
-            hip_check(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-            enqueue(cs);
-            hip_check(hipStreamEndCapture(cs, &g), "hipStreamEndCapture");
-            hip_check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "hipGraphInstantiate");
-            {
-                LaunchTimer timer("k_batch_track", s);
-                hip_check(hipGraphLaunch(ge, s), "hipGraphLaunch");
+            for (size_t k = 0; k < n_jobs; ++k) {
+                hm_job &t = trial[k];
+                const size_t n = jobs[k].n;
+                t.n_turns = std::min(jobs[k].n_turns, trial_turns);
+                if (!n) continue;
+                void *b = nullptr;
+                hip_check(hipMalloc(&b, n * (4 * sizeof(double) + sizeof(uint32_t)) + (jobs[k].megno ? 8 * n + 8 : 0)),
+                          "hipMalloc(autotune_batch)");
+                bufs.push_back(b);
+                double *d = static_cast<double *>(b);
+                t.x = d; t.px = d + n; t.y = d + 2 * n; t.py = d + 3 * n;
+                t.steps = reinterpret_cast<uint32_t *>(d + 4 * n);
+                if (jobs[k].megno) t.megno = reinterpret_cast<double *>(t.steps + n + (n & 1));
             }
-            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+            auto restore = [&] {
+                for (size_t k = 0; k < n_jobs; ++k) {
+                    const size_t n = jobs[k].n;
+                    if (!n) continue;
+                    hip_check(hipMemcpyAsync(trial[k].x, jobs[k].x, n * 8, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+                    hip_check(hipMemcpyAsync(trial[k].px, jobs[k].px, n * 8, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+                    hip_check(hipMemcpyAsync(trial[k].y, jobs[k].y, n * 8, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+                    hip_check(hipMemcpyAsync(trial[k].py, jobs[k].py, n * 8, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+                    hip_check(hipMemcpyAsync(trial[k].steps, jobs[k].steps, n * 4, hipMemcpyDeviceToDevice, s),
+                              "hipMemcpyAsync");
+                }
+            };
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            hip_check(hipEventCreate(&e0), "hipEventCreate");
+            hip_check(hipEventCreate(&e1), "hipEventCreate");
+            const uint32_t resident = batch_resident_waves();
+            double best_t = 0.0;
+            hm_batch_cfg best_c{};
+            bool have = false;
+            // first the defaults untimed (tables into L2, code objects loaded)
+            restore();
+            batch_run(trial.data(), n_jobs, mode, 256, 1000, resident, false, s);
+            for (int block : {64, 256})
+                for (int every : {250, 1000, 4000})
+                    for (uint32_t waves : {resident, resident / 2}) {
+                        restore();
+                        hip_check(hipEventRecord(e0, s), "hipEventRecord");
+                        batch_run(trial.data(), n_jobs, mode, block, every, waves, false, s);
+                        hip_check(hipEventRecord(e1, s), "hipEventRecord");
+                        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                        float ms = 0.0f;
+                        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                        if (!have || ms < best_t) {
+                            have = true;
+                            best_t = ms;
+                            best_c = hm_batch_cfg{block, every, (int32_t)waves, 0};
+                        }
+                    }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            *best = best_c;
+            if (best_ms) *best_ms = best_t;
+            const BatchKey key = batch_key(jobs, n_jobs, mode);
+            std::lock_guard<std::mutex> lk(g_batch_tune_mu);
+            g_batch_tuned[key] = best_c;
         } catch (...) {
-            if (ge) (void)hipGraphExecDestroy(ge);
-            if (g) (void)hipGraphDestroy(g);
-            (void)hipStreamDestroy(cs);
+            (void)hipStreamSynchronize(s);
+            release();
             throw;
         }
-        (void)hipGraphExecDestroy(ge);
-        (void)hipGraphDestroy(g);
-        (void)hipStreamDestroy(cs);
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        release();
+    });
+}
+
+extern "C" int hm_autotune_batch_clear(void)
+{
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_batch_tune_mu);
+        g_batch_tuned.clear();
     });
 }

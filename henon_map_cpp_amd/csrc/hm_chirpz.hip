@@ -55,6 +55,7 @@ struct CzArgs {
     const Cplx *tw;       // [M]: exp(-2 pi i q / M)
     const Cplx *chirp;    // [N]: exp(-i pi (m^2 mod 2N) / N)
     const Cplx *bhat;     // [M1][M2]: B[k1 + M1 k2] at k1 M2 + k2
+    const Cplx *twm;      // [M1][M2]: W_M^{k1 n2} at k1 M2 + n2 (the step twiddles, coalesced)
     Cplx *Y;              // scratch [nsig][M1][M2]
     double *mags;         // scratch [nsig][N]
     uint32_t *flags;      // [nsig]: 1 when a sample of the window is NaN
@@ -90,8 +91,15 @@ __device__ __forceinline__ void cz_pass(Cplx *buf, uint32_t G, uint32_t logL, ui
             for (int r = 0; r < R; ++r) v[q][r] = src[r * B];
             const uint32_t k = j & (Ns - 1);
             if (k != 0) {
+                // one table load; the higher powers by products (relative
+                // error below R ulps)
+                const Cplx w1 = tw[k << (logM - lognsr)];
+                Cplx w = w1;
 #pragma unroll
-                for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], tw[(k * r) << (logM - lognsr)]);
+                for (int r = 1; r < R; ++r) {
+                    v[q][r] = cmul(v[q][r], w);
+                    if (r + 1 < R) w = cmul(w, w1);
+                }
             }
             dft_r<R>(v[q]);
         }
@@ -136,7 +144,7 @@ __device__ void cz_fft(Cplx *buf, uint32_t G, uint32_t logL, uint32_t GS, const 
 // Columns n2 in [c0, c0 + C) of signal `sig`; LDS holds column c at
 // buf[c (M1 + 1) ..] (one pad entry against bank conflicts).
 template <uint32_t PMAX>
-__global__ void __launch_bounds__(kCzThreads) k_cz_columns(CzArgs a)
+__global__ void __launch_bounds__(kCzThreads, PMAX <= 2048 ? 2 : 1) k_cz_columns(CzArgs a)
 {
     extern __shared__ Cplx buf[];
     const uint32_t per_sig = 1u << (a.logM2 - a.logC);
@@ -184,13 +192,14 @@ __global__ void __launch_bounds__(kCzThreads) k_cz_columns(CzArgs a)
     Cplx *Y = a.Y + ((size_t)sig << a.logM);
     for (uint32_t e = threadIdx.x; e < (C << a.logM1); e += kCzThreads) {
         const uint32_t c = e & (C - 1), k1 = e >> a.logC, n2 = c0 + c;
-        Y[((size_t)k1 << a.logM2) + n2] = cmul(buf[c * GS + k1], a.tw[n2 * k1]);
+        const size_t o = ((size_t)k1 << a.logM2) + n2;
+        Y[o] = cmul(buf[c * GS + k1], a.twm[o]);
     }
 }
 
 // Rows k1 in [r0, r0 + RR) of signal `sig`, contiguous in LDS.
 template <uint32_t PMAX>
-__global__ void __launch_bounds__(kCzThreads) k_cz_rows(CzArgs a)
+__global__ void __launch_bounds__(kCzThreads, PMAX <= 2048 ? 2 : 1) k_cz_rows(CzArgs a)
 {
     extern __shared__ Cplx buf[];
     const uint32_t per_sig = 1u << (a.logM1 - a.logRR);
@@ -212,16 +221,14 @@ __global__ void __launch_bounds__(kCzThreads) k_cz_rows(CzArgs a)
     }
     __syncthreads();
     cz_fft<PMAX>(buf, 1u << a.logRR, a.logM2, M2, a.tw, a.logM);
-    for (uint32_t e = threadIdx.x; e < P; e += kCzThreads) {
-        const uint32_t n2 = e & (M2 - 1), k1 = r0 + (e >> a.logM2);
-        Y[e] = cmul(buf[e], a.tw[n2 * k1]);
-    }
+    const Cplx *tm = a.twm + ((size_t)r0 << a.logM2);
+    for (uint32_t e = threadIdx.x; e < P; e += kCzThreads) Y[e] = cmul(buf[e], tm[e]);
 }
 
 // Columns n2 in [c0, c0 + C): the last M1-point transforms and the magnitudes
 // of entries m = n2 + M2 n1 < N.
 template <uint32_t PMAX>
-__global__ void __launch_bounds__(kCzThreads) k_cz_final(CzArgs a)
+__global__ void __launch_bounds__(kCzThreads, PMAX <= 2048 ? 2 : 1) k_cz_final(CzArgs a)
 {
     extern __shared__ Cplx buf[];
     const uint32_t per_sig = 1u << (a.logM2 - a.logC);
@@ -295,7 +302,7 @@ __global__ void k_cz_zero(uint32_t *__restrict__ p, uint32_t n)
 // ---------------------------------------------------------------- host plan
 struct CzPlan {
     uint32_t logM = 0, logM1 = 0, logM2 = 0;
-    Cplx *tw = nullptr, *bhat = nullptr, *chirp = nullptr;
+    Cplx *tw = nullptr, *bhat = nullptr, *chirp = nullptr, *twm = nullptr;
 };
 
 // In-place forward DFT (radix 2, decimation in time) in long double, for the
@@ -362,6 +369,12 @@ static CzPlan cz_plan(uint32_t N, hipStream_t s)
             const std::complex<long double> z = b[k1 + M1 * k2];
             bhat[k1 * M2 + k2] = Cplx{(double)z.real(), (double)z.imag()};
         }
+    // step twiddles W_M^{k1 n2} (k1 < M1, n2 < M2), reduced exactly mod M
+    std::vector<Cplx> twm(M);
+    for (size_t k1 = 0; k1 < M1; ++k1)
+        for (size_t n2 = 0; n2 < M2; ++n2) twm[k1 * M2 + n2] = tw[(k1 * n2) & (M - 1)];
+    hip_check(hipMalloc(&p.twm, M * sizeof(Cplx)), "hipMalloc(chirp-z step twiddles)");
+    hip_check(hipMemcpyAsync(p.twm, twm.data(), M * sizeof(Cplx), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
     hip_check(hipMalloc(&p.tw, M * sizeof(Cplx)), "hipMalloc(chirp-z twiddles)");
     hip_check(hipMalloc(&p.bhat, M * sizeof(Cplx)), "hipMalloc(chirp-z filter)");
     hip_check(hipMalloc(&p.chirp, N * sizeof(Cplx)), "hipMalloc(chirp)");
@@ -420,14 +433,16 @@ void fft_tune_chirpz(const double *hist, HistLayout lay, uint32_t n, const doubl
     if (N > kCzMaxN) throw Error(HM_ERR_UNSUPPORTED, "fft_tunes: window longer than 33554432 samples");
     if (n == 0) return;
     const CzPlan p = cz_plan(N, s);
-    const uint32_t PMAX = std::max(p.logM1, p.logM2) <= 12 ? 4096u : 8192u;
-    uint32_t logPM = PMAX == 4096 ? 12 : 13;
+    // 2048-point workgroups (32 KiB of LDS, four resident per CU) while both
+    // factors fit, else 8192
+    const uint32_t PMAX = std::max(p.logM1, p.logM2) <= 11 ? 2048u : 8192u;
+    const uint32_t logPM = PMAX == 2048 ? 11 : 13;
     CzArgs a{};
     a.hist = hist; a.lay = lay; a.means = means; a.n = n; a.from = from; a.N = N;
     a.logM = p.logM; a.logM1 = p.logM1; a.logM2 = p.logM2;
     a.logC = std::min(p.logM2, logPM - p.logM1);
     a.logRR = std::min(p.logM1, logPM - p.logM2);
-    a.hann = hann; a.tw = p.tw; a.chirp = p.chirp; a.bhat = p.bhat; a.out = out; a.window = window;
+    a.hann = hann; a.tw = p.tw; a.chirp = p.chirp; a.bhat = p.bhat; a.twm = p.twm; a.out = out; a.window = window;
     const size_t lds_cols = ((size_t)1 << a.logC) * (((size_t)1 << p.logM1) + 1) * sizeof(Cplx);
     const size_t lds_rows = ((size_t)1 << (a.logRR + p.logM2)) * sizeof(Cplx);
     // scratch per particle (both planes): the M-point work arrays, magnitudes, flags
@@ -448,7 +463,7 @@ void fft_tune_chirpz(const double *hist, HistLayout lay, uint32_t n, const doubl
         a.nsig = 2 * nb;
         hipLaunchKernelGGL(k_cz_zero, dim3((a.nsig + 255) / 256), dim3(256), 0, s, a.flags, a.nsig);
         launch_check("k_cz_zero");
-        if (PMAX == 4096) cz_launch<4096>(a, lds_cols, lds_rows, s);
+        if (PMAX == 2048) cz_launch<2048>(a, lds_cols, lds_rows, s);
         else cz_launch<8192>(a, lds_cols, lds_rows, s);
         LaunchTimer timer("k_cz_tune", s);
         hipLaunchKernelGGL(k_cz_tune, dim3(a.nsig), dim3(kCzThreads), 0, s, a);

@@ -64,10 +64,16 @@ CHAOS_INDICATORS = dict(name="chaos indicators: 256x256 grid + 65536 ball points
 ADJOINT_GRADIENTS = dict(name="adjoint gradients: 65536 points, r ~ U[0.05, 0.4], 1e4 turns, mu=0.5, eps=16",
                          n=65_536, r_min=0.05, r_max=0.4, n_turns=10_000, tune_x=0.168, tune_y=0.201, epsilon=16.0,
                          mu=0.5, barrier=float(np.sqrt(10.0)), every=(8, 64, 256, 1024), seed=4)
-BATCHED_JOBS = dict(name="batched uneven jobs: N ~ logU[64, 65536], T ~ logU[1e3, 1e5], tunes U[0.15, 0.32], "
+BATCHED_JOBS = dict(name="batched uneven jobs: N ~ logU[64, 65536], T ~ logU[1e3, 1e6], tunes U[0.15, 0.32], "
                          "eps U[0, 64], mu U[-1, 1], r < 0.6, barrier r^2 > 10",
-                    n_range=(64, 65_536), t_range=(1_000, 100_000), tune_range=(0.15, 0.32), eps_range=(0.0, 64.0),
-                    mu_range=(-1.0, 1.0), r_max=0.6, barrier=float(np.sqrt(10.0)), seed=5)
+                    n_jobs=4096, n_range=(64, 65_536), t_range=(1_000, 1_000_000), tune_range=(0.15, 0.32),
+                    eps_range=(0.0, 64.0), mu_range=(-1.0, 1.0), r_max=0.6, barrier=float(np.sqrt(10.0)), seed=5,
+                    megno_renorm_every=100)
+
+
+def batch_megno_job(k: int) -> bool:
+    """Jobs k with k mod 10 < 3 (30 % of the mix) also accumulate the MEGNO."""
+    return k % 10 < 3
 
 
 def indicator_points(c=CHAOS_INDICATORS):

@@ -210,6 +210,25 @@ typedef struct hm_batch_cfg {
  * one particle set per call (henon_tracker::track, src/henon.cu:2701-2752,
  * :3155-3185). jobs: host array. Asynchronous on `stream` unless use_graph. */
 int hm_track_batch(const hm_job *jobs, size_t n_jobs, int mode, const hm_batch_cfg *cfg, void *stream);
+/* With cfg == NULL, hm_track_batch uses the configuration hm_autotune_batch
+ * chose for this device, mode and job-mix class (floor(log2) of the job count,
+ * of the particles in total and of the longest job's turns, and the share of
+ * MEGNO particles in quarters), else the defaults. With use_graph, the recorded
+ * launch sequence is kept (up to 8 job shapes per process) and replayed by later
+ * calls with the same job shapes; the descriptors (particles, tables, mu,
+ * barriers) are re-read from device memory on every call. */
+
+/* Batch autotuner (no reference counterpart). Times hm_track_batch launch
+ * configurations (block 64 / 256, compact_every 250 / 1000 / 4000, one or half
+ * a round of resident wavefronts) on copies of the jobs' particles with every
+ * job cut to at most trial_turns turns, returns the fastest in *best (its time
+ * in *best_ms) and remembers it for the job-mix class. The caller's arrays are
+ * not modified; every configuration gives bit-identical results. Allocates the
+ * copies (device memory) for the duration of the call. Synchronises `stream`. */
+int hm_autotune_batch(const hm_job *jobs, size_t n_jobs, int mode, uint32_t trial_turns, hm_batch_cfg *best,
+                      double *best_ms, void *stream);
+/* Forget every remembered batch configuration. */
+int hm_autotune_batch_clear(void);
 
 /* ---- readback observables ------------------------------------------------ */
 enum {

@@ -134,3 +134,39 @@ def test_batch_megno_jobs_equal_indicator_runs(mode):
         else:
             C.track(parts, tdev, j["T"], j["T"], j["mu"], j["barrier"], False, mode)
         assert same(got, parts.get()), k
+
+
+def test_batch_autotune_leaves_inputs_and_results_unchanged():
+    """The batch autotuner times configurations on copies (the caller's
+    particles stay as given); a later call without a configuration uses the
+    chosen one and gives the results of the defaults bit for bit."""
+    jobs = make_jobs(40, seed=16)
+    dev = []
+    for j in jobs:
+        dev.append((C.DeviceParticles(*j["x0"]), C.DeviceBuffer.from_array(C.build_table(j["wx"], j["wy"])), j["T"],
+                    j["T"], j["mu"], j["barrier"]))
+    before = [d[0].get() for d in dev]
+    C.check(C.lib().hm_autotune_batch_clear())
+    best, ms = C.autotune_batch(dev, C.MODE_EXACT, trial_turns=500)
+    assert best.block in (64, 256) and best.compact_every in (250, 1000, 4000) and best.waves > 0 and ms > 0
+    assert all(same(a, d[0].get()) for a, d in zip(before, dev))
+    C.track_batch(dev, C.MODE_EXACT)  # the tuned configuration
+    got = [d[0].get() for d in dev]
+    C.check(C.lib().hm_autotune_batch_clear())
+    ref = run_batch(jobs, C.MODE_EXACT, C.BatchCfg(256, 1000, 0, 0))
+    assert all(same(a, b) for a, b in zip(got, ref))
+
+
+def test_batch_graph_replay_with_new_particles():
+    """A recorded launch sequence is replayed for later calls with the same job
+    shapes: the second call (new initial conditions, new mu and barriers, same
+    particle and turn counts) must still equal separate launches."""
+    jobs = make_jobs(20, seed=17)
+    first = run_batch(jobs, C.MODE_EXACT, C.BatchCfg(0, 0, 0, 1))
+    assert all(same(a, b) for a, b in zip(first, run_single(jobs, C.MODE_EXACT)))
+    rng = np.random.default_rng(18)
+    for j in jobs:
+        j["x0"] = j["x0"] * rng.uniform(0.5, 1.0)
+        j["mu"] = float(rng.uniform(-1, 1))
+    second = run_batch(jobs, C.MODE_EXACT, C.BatchCfg(0, 0, 0, 1))
+    assert all(same(a, b) for a, b in zip(second, run_single(jobs, C.MODE_EXACT)))

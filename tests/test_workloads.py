@@ -44,7 +44,7 @@ def test_batch_job_params():
     assert p == W.batch_job_params(2000)
     n = np.array([q[0] for q in p])
     t = np.array([q[1] for q in p])
-    assert n.min() >= 64 and n.max() <= 65_536 and t.min() >= 1_000 and t.max() <= 100_000
+    assert n.min() >= 64 and n.max() <= 65_536 and t.min() >= 1_000 and t.max() <= 1_000_000
     # log-uniform: the median sits near the geometric mean of the range
     assert abs(np.log(np.median(n)) - np.log(np.sqrt(64 * 65_536))) < 0.3
     assert all(0.15 <= q[2] <= 0.32 and 0.15 <= q[3] <= 0.32 and 0 <= q[4] <= 64 and -1 <= q[5] <= 1 for q in p)
