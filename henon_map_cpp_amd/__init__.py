@@ -210,9 +210,16 @@ class henon_tracker:
         return dict(self.tracker.autotune(particles.particles, mu, barrier, mode, warm_turns, trial_turns))
 
     def megno(self, particles, n_turns, mu, barrier=10.0, kick_module=np.nan, inverse=False,
-              turn_samples=np.array([]), n_threads=-1):
+              turn_samples=np.array([]), n_threads=-1, carry_off_diagonal=True):
+        """henon_map_cpp/__init__.py megno over henon_tracker::megno. As there,
+        n_threads (-1: the host's hardware threads) fixes which particles
+        share a tangent matrix: each of n_threads chains takes the particles
+        t, t + n_threads, ... and starts each from the previous one's matrix
+        with only the diagonal reset (carry_off_diagonal=True, the reference's
+        results); carry_off_diagonal=False starts every particle from the
+        identity (results independent of n_threads)."""
         return np.asarray(self.tracker.megno(particles.particles, n_turns, mu, barrier, kick_module, inverse,
-                                             [int(v) for v in turn_samples], n_threads))
+                                             [int(v) for v in turn_samples], n_threads, carry_off_diagonal))
 
     def tune_birkhoff(self, particles, n_turns, mu, barrier=10.0, kick_module=np.nan, inverse=False,
                       from_idx=np.array([]), to_idx=np.array([])):

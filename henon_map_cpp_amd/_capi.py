@@ -89,7 +89,7 @@ SIGNATURES = {
     "hm_autotune_clear": (I, []),
     "hm_observable": (I, [I, P, P, P, P, SZ, P, P]),
     "hm_tangent_matrix": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, I, P, P]),
-    "hm_megno_trace": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, D, P, SZ, P, P]),
+    "hm_megno_trace": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, D, I, P, SZ, P, P, P, P]),
     "hm_track_indicators": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, D, I, ctypes.POINTER(IndicatorSpec), P]),
     "hm_tangent_apply": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, I, P]),
     "hm_matrix_left_multiply": (I, [P, P, SZ, P]),
@@ -345,12 +345,18 @@ def profile_read(name: str):
 
 
 def megno_trace(parts: DeviceParticles, table_dev: DeviceBuffer, table_len: int, n_turns: int, mu: float,
-                barrier: float, samples, stream=None):
+                barrier: float, samples, stream=None, inverse=False, matrix_in=None, matrix_out=False):
+    """Trace-form MEGNO [n][n_samples]; with matrix_out=True also the final
+    matrices [n][4][4], returned as (megno, matrices)."""
     s = np.ascontiguousarray(samples, dtype=np.uint32)
     out = DeviceBuffer(8 * parts.n * max(s.size, 1))
-    check(lib().hm_megno_trace(*parts.ptrs(), parts.n, table_dev.ptr, table_len, n_turns, mu, barrier, ptr(s),
-                               s.size, out.ptr, stream))
-    return out.download(np.float64, parts.n * s.size).reshape(parts.n, s.size)
+    mi = DeviceBuffer.from_array(np.ascontiguousarray(matrix_in, dtype=np.float64)) if matrix_in is not None else None
+    mo = DeviceBuffer(8 * 16 * parts.n) if matrix_out else None
+    check(lib().hm_megno_trace(*parts.ptrs(), parts.n, table_dev.ptr, table_len, n_turns, mu, barrier, int(inverse),
+                               ptr(s), s.size, mi.ptr if mi is not None else None, mo.ptr if mo is not None else None,
+                               out.ptr, stream))
+    m = out.download(np.float64, parts.n * s.size).reshape(parts.n, s.size)
+    return (m, mo.download(np.float64, 16 * parts.n).reshape(parts.n, 4, 4)) if matrix_out else m
 
 
 def track_indicators(parts: DeviceParticles, table_dev: DeviceBuffer, table_len: int, n_turns: int, mu: float,

@@ -18,6 +18,7 @@
 #include <string>
 #include <type_traits>
 #include <unordered_set>
+#include <thread>
 #include <vector>
 
 #include "henon_hip.h"
@@ -542,39 +543,97 @@ class HenonTracker {
 
 
     // henon_tracker::megno (src/henon.cu:2755-2887) on the GPU: trace form,
-    // per-particle rows of turn samples.
+    // per-particle rows of turn samples, J at table index k of the call and the
+    // map (forward, or inverse with inverse=True) at the particle's own index.
+    // The reference's threads each take the particles j = t, t + n_threads, ...
+    // in turn and carry the tangent matrix from one to the next with only its
+    // diagonal reset to 1 (henon.cu:2794-2803), so a particle's result depends
+    // on the particle before it in its thread (a lost one makes every later one
+    // NaN). With carry_off_diagonal (the default) that chain is reproduced: a
+    // first pass gives each particle's product P of its tangent matrices from the
+    // identity, the host chains M_start(next) = P M_start(prev) with the
+    // diagonal set to 1 (the reference's running product up to rounding), and
+    // the MEGNO pass starts every particle from its M_start. n_threads = -1 is
+    // std::thread::hardware_concurrency(), as in the reference.
     py::array_t<double> megno(Particles4D &p, unsigned int n_turns, double mu, double barrier, double kick_module,
-                              bool inverse, std::vector<unsigned int> turn_samples, int n_threads)
+                              bool inverse, std::vector<unsigned int> turn_samples, int n_threads,
+                              bool carry_off_diagonal)
     {
-        (void)n_threads;
         if (!std::isnan(kick_module)) throw NotImplemented("megno: the random kick is not available");
-        if (inverse) throw NotImplemented("megno: inverse tracking is not available for the trace-form MEGNO");
+        if (n_threads == 0 || n_threads < -1) throw std::invalid_argument("megno: n_threads must be positive or -1");
         check_turns(p, n_turns, inverse);
         if (turn_samples.empty()) turn_samples.push_back(n_turns);
+        for (unsigned int t : turn_samples)
+            if (t > n_turns) throw std::invalid_argument("megno: turn sample beyond n_turns");
+        const size_t n = p.n_particles;
+        const size_t nt = n_threads == -1 ? std::max(1u, std::thread::hardware_concurrency()) : (size_t)n_threads;
         std::vector<uint32_t> ts(turn_samples.begin(), turn_samples.end());
-        py::array_t<double> out({(py::ssize_t)p.n_particles, (py::ssize_t)ts.size()});
-        for (size_t q0 = 0; q0 < ts.size(); q0 += HM_MAX_SAMPLES) {
-            const size_t nq = std::min<size_t>(HM_MAX_SAMPLES, ts.size() - q0);
-            DevBuf o(p.n_particles * nq * sizeof(double));
-            // every pass restarts from the same state: keep a copy
-            DevBuf sx(p.n_particles * 8), spx(p.n_particles * 8), sy(p.n_particles * 8), spy(p.n_particles * 8),
-                ss(p.n_particles * 4);
-            const bool last = q0 + nq >= ts.size();
-            if (!last) snapshot(p, sx, spx, sy, spy, ss);
+        py::array_t<double> out({(py::ssize_t)n, (py::ssize_t)ts.size()});
+        if (n == 0) {
+            p.global_steps = inverse ? p.global_steps - n_turns : p.global_steps + n_turns;
+            return out;
+        }
+        // every pass restarts from the same state: keep a copy
+        DevBuf sx(n * 8), spx(n * 8), sy(n * 8), spy(n * 8), ss(n * 4);
+        snapshot(p, sx, spx, sy, spy, ss);
+        const bool chain = carry_off_diagonal && nt < n;
+        DevBuf m_start(chain ? n * 128 : 8);
+        const int inv = inverse ? 1 : 0;
+        if (chain) {
+            DevBuf prod(n * 128), scratch(n * 8);
             {
                 py::gil_scoped_release nogil;
                 check(hm_megno_trace(p.dx.as<double>(), p.dpx.as<double>(), p.dy.as<double>(), p.dpy.as<double>(),
-                                     p.dsteps.as<uint32_t>(), p.n_particles, dtable.as<double>(), allowed_steps,
-                                     n_turns, mu, barrier, ts.data() + q0, nq, o.as<double>(), nullptr));
+                                     p.dsteps.as<uint32_t>(), n, dtable.as<double>(), allowed_steps, n_turns, mu,
+                                     barrier, inv, nullptr, 0, nullptr, prod.as<double>(), scratch.as<double>(),
+                                     nullptr));
                 sync();
             }
-            auto v = o.download<double>(p.n_particles * nq);
-            auto r = out.mutable_unchecked<2>();
-            for (size_t i = 0; i < p.n_particles; i++)
-                for (size_t q = 0; q < nq; q++) r(i, q0 + q) = v[i * nq + q];
-            if (!last) restore(p, sx, spx, sy, spy, ss);
+            restore(p, sx, spx, sy, spy, ss);
+            const std::vector<double> P = prod.download<double>(n * 16);
+            std::vector<double> M(n * 16);
+            for (size_t c = 0; c < nt; ++c) {
+                double cur[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+                for (size_t j = c; j < n; j += nt) {
+                    cur[0] = cur[5] = cur[10] = cur[15] = 1.0;
+                    std::copy(cur, cur + 16, M.begin() + 16 * j);
+                    // multiply_matrices' order (henon.cu:251-265): += from 0.0 over k
+                    double next[16];
+                    for (int r = 0; r < 4; ++r)
+                        for (int q = 0; q < 4; ++q) {
+                            double acc = 0.0;
+                            for (int k = 0; k < 4; ++k) acc += P[16 * j + 4 * r + k] * cur[4 * k + q];
+                            next[4 * r + q] = acc;
+                        }
+                    std::copy(next, next + 16, cur);
+                }
+            }
+            m_start.upload(M);
         }
-        p.global_steps += n_turns;
+        for (size_t q0 = 0; q0 < ts.size(); q0 += HM_MAX_SAMPLES) {
+            const size_t nq = std::min<size_t>(HM_MAX_SAMPLES, ts.size() - q0);
+            DevBuf o(n * nq * sizeof(double));
+            {
+                py::gil_scoped_release nogil;
+                check(hm_megno_trace(p.dx.as<double>(), p.dpx.as<double>(), p.dy.as<double>(), p.dpy.as<double>(),
+                                     p.dsteps.as<uint32_t>(), n, dtable.as<double>(), allowed_steps, n_turns, mu,
+                                     barrier, inv, ts.data() + q0, nq, chain ? m_start.as<double>() : nullptr,
+                                     nullptr, o.as<double>(), nullptr));
+                sync();
+            }
+            auto v = o.download<double>(n * nq);
+            auto r = out.mutable_unchecked<2>();
+            for (size_t i = 0; i < n; i++)
+                for (size_t q = 0; q < nq; q++) r(i, q0 + q) = v[i * nq + q];
+            if (q0 + nq < ts.size()) restore(p, sx, spx, sy, spy, ss);
+        }
+        uint32_t flags = 0;
+        check(hm_take_status(&flags, nullptr));
+        if (flags & 1u)
+            throw std::runtime_error("megno: a particle's turn index left the modulation table; it was stopped there "
+                                     "(the reference would read out of bounds)");
+        if (!inverse) p.global_steps += n_turns;
+        else p.global_steps -= n_turns;
         return out;
     }
 
@@ -1490,7 +1549,7 @@ PYBIND11_MODULE(henon_map_engine, m)
         cls.def("megno", &C::megno, "Compute megno", py::arg("particles"), py::arg("n_turns"), py::arg("mu"),
                 py::arg("barrier") = 100.0, py::arg("kick_module") = std::numeric_limits<double>::quiet_NaN(),
                 py::arg("inverse") = false, py::arg("turn_samples") = std::vector<unsigned int>(),
-                py::arg("n_threads") = -1)
+                py::arg("n_threads") = -1, py::arg("carry_off_diagonal") = true)
             .def("birkhoff_tunes",
                  [](C &s, Particles4D &p, unsigned n, double mu, double barrier, double kick, bool inverse,
                     std::vector<unsigned> f, std::vector<unsigned> t) {

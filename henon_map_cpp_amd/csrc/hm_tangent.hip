@@ -34,6 +34,8 @@ struct MegnoTraceArgs {
     uint32_t samples[kMaxSamples];
     uint32_t n_samples;
     double *out;  // [n][n_samples], the reference's per-particle row layout
+    const double *m_in;  // [n][16] initial matrices, or NULL (identity)
+    double *m_out;       // [n][16] final matrices, or NULL
     uint32_t *status;
 };
 
@@ -60,7 +62,7 @@ __device__ __forceinline__ void mat_mul_full(const double *a, double *m)
 // steps. The two prefix sums are accumulated online in the same order.
 // Every particle starts from the identity (the reference reuses the
 // off-diagonal entries of the previous particle its CPU thread handled).
-template <bool OCT>
+template <bool OCT, bool INV>
 __global__ void __launch_bounds__(256) k_megno_trace(MegnoTraceArgs a)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -69,7 +71,7 @@ __global__ void __launch_bounds__(256) k_megno_trace(MegnoTraceArgs a)
     uint32_t s = a.steps[i];
     double m[16];
 #pragma unroll
-    for (int q = 0; q < 16; q++) m[q] = (q % 5 == 0) ? 1.0 : 0.0;
+    for (int q = 0; q < 16; q++) m[q] = a.m_in ? a.m_in[(size_t)i * 16 + q] : ((q % 5 == 0) ? 1.0 : 0.0);
     double trace_1 = 4.0, trace_2 = 4.0, s1 = 0.0, s2 = 0.0;
     double res[kMaxSamples];
 #pragma unroll
@@ -92,13 +94,14 @@ __global__ void __launch_bounds__(256) k_megno_trace(MegnoTraceArgs a)
         for (int q = 0; q < kMaxSamples; q++)
             if (q < (int)a.n_samples && a.samples[q] == k) res[q] = s2 / (double)a.samples[q];
         if (alive) {
-            if (s >= a.tab_len) {
+            // the map's own table index: steps forward, steps - 1 inverse
+            if (INV ? (s == 0 || s > a.tab_len) : s >= a.tab_len) {
                 atomicOr(a.status, 1u);
                 alive = false;
             } else {
-                map_step<kExact, false, OCT>(x, px, y, py, a.mu, m3mu, a.tab[s]);
+                map_step<kExact, INV, OCT>(x, px, y, py, a.mu, m3mu, a.tab[INV ? s - 1 : s]);
                 if (radius2<kExact>(x, px, y, py) < a.b2) {
-                    s += 1;
+                    s = INV ? s - 1 : s + 1;
                 } else {
                     x = px = y = py = __builtin_nan("");
                     alive = false;
@@ -107,6 +110,9 @@ __global__ void __launch_bounds__(256) k_megno_trace(MegnoTraceArgs a)
         }
     }
     a.x[i] = x; a.px[i] = px; a.y[i] = y; a.py[i] = py; a.steps[i] = s;
+    if (a.m_out)
+#pragma unroll
+        for (int q = 0; q < 16; q++) a.m_out[(size_t)i * 16 + q] = m[q];
 #pragma unroll
     for (int q = 0; q < kMaxSamples; q++)
         if (q < (int)a.n_samples) a.out[(size_t)i * a.n_samples + q] = res[q];
@@ -588,7 +594,8 @@ void host_birkhoff_weights(uint32_t n, double *out)
 
 extern "C" int hm_megno_trace(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n,
                               const double *table, size_t table_len, uint32_t n_turns, double mu, double barrier,
-                              const uint32_t *turn_samples, size_t n_samples, double *out, void *stream)
+                              int inverse, const uint32_t *turn_samples, size_t n_samples, const double *matrix_in,
+                              double *matrix_out, double *out, void *stream)
 {
     return guarded([&] {
         if (n_samples > (size_t)kMaxSamples) throw Error(HM_ERR_ARG, "megno: at most 16 turn samples");
@@ -603,10 +610,17 @@ extern "C" int hm_megno_trace(double *x, double *px, double *y, double *py, uint
         a.mu = mu; a.b2 = barrier * barrier;
         for (size_t q = 0; q < n_samples; q++) a.samples[q] = turn_samples[q];
         a.n_samples = (uint32_t)n_samples; a.out = out; a.status = status_word();
+        a.m_in = matrix_in; a.m_out = matrix_out;
         hipStream_t s = as_stream(stream);
         LaunchTimer timer("k_megno_trace", s);
-        if (mu != 0.0) hipLaunchKernelGGL(k_megno_trace<true>, dim3(grid_of(n)), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL(k_megno_trace<false>, dim3(grid_of(n)), dim3(256), 0, s, a);
+        const dim3 g(grid_of(n)), b(256);
+        if (inverse) {
+            if (mu != 0.0) hipLaunchKernelGGL((k_megno_trace<true, true>), g, b, 0, s, a);
+            else hipLaunchKernelGGL((k_megno_trace<false, true>), g, b, 0, s, a);
+        } else {
+            if (mu != 0.0) hipLaunchKernelGGL((k_megno_trace<true, false>), g, b, 0, s, a);
+            else hipLaunchKernelGGL((k_megno_trace<false, false>), g, b, 0, s, a);
+        }
         launch_check("k_megno_trace");
     });
 }

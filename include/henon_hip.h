@@ -290,13 +290,18 @@ int hm_tangent_matrix(const double *x, const double *px, const double *y, const 
 /* ---- tangent map, MEGNO, FLI, Birkhoff ----------------------------------- */
 /* henon_tracker::megno (src/henon.cu:2755-2887): per turn k, J at the pre-step
  * point with table index k, M <- J M, a_k = (k+1) log10(tr M_{k+1}/tr M_k), then
- * the map step (table index steps); out[i][q] = S2[turn_samples[q]] /
- * turn_samples[q] with S2 the running sum of (running sum of a_k)/(k+1); a sample
- * equal to n_turns reads 0 (the reference's never-written slot). Every particle
- * starts from the identity. At most 16 samples. out: device double[n][n_samples]. */
+ * the map step (table index steps, or steps - 1 with inverse = 1: the inverse
+ * map, as the reference does); out[i][q] = S2[turn_samples[q]] / turn_samples[q]
+ * with S2 the running sum of (running sum of a_k)/(k+1); a sample equal to
+ * n_turns reads 0 (the reference's never-written slot). M starts from
+ * matrix_in[i] (device double[n][16]) or, when NULL, the identity; the two
+ * traces start at 4 either way, as in the reference. matrix_out (device
+ * double[n][16], or NULL) receives the final M. At most 16 samples.
+ * out: device double[n][n_samples]. */
 int hm_megno_trace(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n, const double *table,
-                   size_t table_len, uint32_t n_turns, double mu, double barrier, const uint32_t *turn_samples,
-                   size_t n_samples, double *out, void *stream);
+                   size_t table_len, uint32_t n_turns, double mu, double barrier, int inverse,
+                   const uint32_t *turn_samples, size_t n_samples, const double *matrix_in, double *matrix_out,
+                   double *out, void *stream);
 
 #define HM_MAX_SAMPLES 16
 #define HM_MAX_BIRKHOFF 4

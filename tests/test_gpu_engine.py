@@ -184,7 +184,8 @@ def test_api_megno_and_birkhoff_tunes_match_oracle(hm):
     x, y = rng.uniform(0, 0.4, 256), rng.uniform(0, 0.4, 256)
     z = np.zeros(256)
     t = hm.henon_tracker(800, 0.01, 0.012, "sps", epsilon=4.0)
-    got = t.megno(hm.particles(x, z, y, z), 700, 0.3, turn_samples=np.array([10, 300, 699]))
+    got = t.megno(hm.particles(x, z, y, z), 700, 0.3, turn_samples=np.array([10, 300, 699]),
+                  carry_off_diagonal=False)
     ot = O.Tracker(800, 0.01, 0.012, "sps", epsilon=4.0)
     want = ot.megno(O.Particles(x, z, y, z), 700, 0.3, barrier=10.0, turn_samples=[10, 300, 699],
                     carry_off_diagonal=False)
@@ -195,6 +196,44 @@ def test_api_megno_and_birkhoff_tunes_match_oracle(hm):
     want2 = ot2.birkhoff_tunes(O.Particles(x, z, y, z), 800, 0.3, barrier=10.0, from_idx=[0, 100], to_idx=[400, 800])
     np.testing.assert_allclose(np.stack(df["tune_x"].to_list()), want2[:, [0, 2, 4]].T, atol=2e-15, equal_nan=True)
     np.testing.assert_allclose(np.stack(df["tune_y"].to_list()), want2[:, [1, 3, 5]].T, atol=2e-15, equal_nan=True)
+
+
+@pytest.mark.parametrize("n_threads,inverse", [(1, False), (7, False), (16, True), (300, False)])
+def test_api_megno_per_thread_carry_matches_oracle(hm, n_threads, inverse):
+    """henon.cu:2782-2803: the reference's threads take particles t,
+    t + n_threads, ... and start each from the previous one's tangent matrix
+    with only the diagonal reset, so results depend on n_threads and a lost
+    particle turns the later ones of its thread NaN; with inverse=True the map
+    steps backwards while J keeps the forward form at table index k. The GPU
+    chains the per-particle products on the host (the running product up to
+    rounding): equal to the oracle's literal restatement within 1e-10, NaN at
+    the same places. Small tunes and 40 turns keep many traces positive (the
+    trace form's logarithm is NaN otherwise)."""
+    rng = np.random.default_rng(43)
+    n = 257
+    x, y = rng.uniform(0, 0.05, n), rng.uniform(0, 0.05, n)
+    x[100] = y[100] = 3.0  # lost at once
+    px = rng.uniform(-0.01, 0.01, n)
+    z = np.zeros(n)
+    t = hm.henon_tracker(1200, 0.01, 0.012, "sps", epsilon=4.0)
+    p = hm.particles(x, px, y, z)
+    ot = O.Tracker(1200, 0.01, 0.012, "sps", epsilon=4.0)
+    op = O.Particles(x, px, y, z)
+    if inverse:  # move forward first so that the inverse turns exist
+        t.track(p, 40, 0.3)
+        ot.track(op, 40, 0.3, barrier=10.0)
+    samples = [5, 15, 39, 40]
+    got = t.megno(p, 40, 0.3, inverse=inverse, turn_samples=np.array(samples), n_threads=n_threads)
+    want = ot.megno(op, 40, 0.3, barrier=10.0, inverse=inverse, turn_samples=samples, n_threads=n_threads,
+                    carry_off_diagonal=True)
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    assert np.isfinite(want).sum() > 100
+    np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-12, equal_nan=True)
+    assert np.array_equal(p.get_x(), op.x, equal_nan=True) and np.array_equal(p.get_steps(), op.steps)
+    if n_threads < n and not inverse:
+        ident = t.megno(hm.particles(x, px, y, z), 40, 0.3, turn_samples=np.array(samples), n_threads=n_threads,
+                        carry_off_diagonal=False)
+        assert not np.array_equal(ident, got, equal_nan=True)  # the carry changes the results
 
 
 def test_get_tunes_and_storage(hm):

@@ -102,3 +102,31 @@ def test_fli_matches_finite_difference_growth():
     t.track(b, 200, 0.3, barrier=10.0)
     d = np.array([a.x[0] - b.x[0], a.px[0] - b.px[0], a.y[0] - b.y[0], a.py[0] - b.py[0]]) / (2 * h)
     assert r["fli"][0, 0] == pytest.approx(np.log10(np.linalg.norm(d)) / 200, abs=1e-6)
+
+
+def test_megno_thread_carry_properties():
+    """The reference CPU megno (henon.cu:2782-2803) keeps one tangent matrix per
+    thread and resets only its diagonal between the thread's particles: with a
+    thread per particle nothing is carried (equal to the identity start); with
+    one thread a lost particle's NaN matrix makes every later particle NaN, and
+    a chain after a surviving particle differs from the identity start. (Small
+    tunes and few turns keep the traces of the trace form positive, so its
+    logarithms are finite.)"""
+    tables = O.Tables(O.sps_modulation(0.01, 4.0, 0, 400), O.sps_modulation(0.012, 4.0, 0, 400))
+    x0 = ([0.02, 1.3, 0.03, 0.04], [0.0] * 4, [0.02, 1.3, 0.025, 0.01], [0.0] * 4)  # particle 1 is lost
+    t = O.Tracker.from_tables(tables)
+
+    def run(nt, carry):
+        return t.megno(O.Particles(*x0), 20, 0.3, barrier=2.0, turn_samples=[5, 15], n_threads=nt,
+                       carry_off_diagonal=carry)
+
+    ident = run(4, False)
+    np.testing.assert_array_equal(run(4, True), ident)
+    assert np.isnan(ident[1]).all() and np.isfinite(ident[[0, 2, 3]]).all()
+    one = run(1, True)
+    np.testing.assert_array_equal(one[0], ident[0])
+    assert np.isnan(one[1:]).all()
+    two = run(2, True)  # chains {0, 2} and {1, 3}
+    np.testing.assert_array_equal(two[:2], ident[:2])
+    assert np.isfinite(two[2, 0]) and abs(two[2, 0] - ident[2, 0]) > 1e-3
+    assert np.isnan(two[3]).all()
