@@ -252,12 +252,18 @@ class henon_tracker:
 
     # ---- additions with no reference counterpart
     def tangent_indicators(self, particles, n_turns, mu, barrier=10.0, renorm_every=100, samples=(),
-                           fli_direction=(1.0, 0.0, 0.0, 0.0), birkhoff=(), history=(0, 0), mode="exact"):
-        """Fused tracking + tangent map + MEGNO + FLI + Birkhoff tunes (one kernel)."""
+                           fli_direction=(1.0, 0.0, 0.0, 0.0), birkhoff=(), history=(0, 0), mode="exact",
+                           megno_birkhoff=(), tune_birkhoff=()):
+        """Fused tracking + tangent map + MEGNO + FLI + Birkhoff tunes (one kernel
+        per 16 samples / 4 windows of each kind). megno_birkhoff and
+        tune_birkhoff list turn samples T of megno_birkhoff_construct and
+        tune_birkhoff_construct fed every turn of the call: the results equal
+        those constructs' get_values() / get_tune1_x() ... after the same turns."""
         return self.tracker.tangent_indicators(particles.particles, n_turns, mu, barrier, renorm_every,
                                                [int(v) for v in samples], list(fli_direction),
                                                [(int(a), int(b)) for a, b in birkhoff],
-                                               (int(history[0]), int(history[1])), mode)
+                                               (int(history[0]), int(history[1])), mode,
+                                               [int(v) for v in megno_birkhoff], [int(v) for v in tune_birkhoff])
 
     def reversibility_error(self, particles, t_list, mu, barrier=10.0, mode="exact"):
         """|x0 - B^t F^t x0| for each t (sorted), shape (len(t_list), n)."""

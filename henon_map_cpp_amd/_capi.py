@@ -57,7 +57,10 @@ class IndicatorSpec(ctypes.Structure):
                 ("log_scale", ctypes.c_void_p), ("n_birkhoff", ctypes.c_uint32),
                 ("birk_from", ctypes.c_uint32 * MAX_BIRKHOFF), ("birk_to", ctypes.c_uint32 * MAX_BIRKHOFF),
                 ("birk_out", ctypes.c_void_p), ("hist_from", ctypes.c_uint32), ("hist_len", ctypes.c_uint32),
-                ("hist", ctypes.c_void_p)]
+                ("hist", ctypes.c_void_p), ("n_megno_birkhoff", ctypes.c_uint32),
+                ("megno_birkhoff_samples", ctypes.c_uint32 * MAX_BIRKHOFF), ("megno_birkhoff_out", ctypes.c_void_p),
+                ("n_tune_birkhoff", ctypes.c_uint32), ("tune_birkhoff_samples", ctypes.c_uint32 * MAX_BIRKHOFF),
+                ("tune_birkhoff_out", ctypes.c_void_p)]
 
 
 # name -> (restype, argtypes); mirrors include/henon_hip.h
@@ -361,10 +364,12 @@ def megno_trace(parts: DeviceParticles, table_dev: DeviceBuffer, table_len: int,
 
 def track_indicators(parts: DeviceParticles, table_dev: DeviceBuffer, table_len: int, n_turns: int, mu: float,
                      barrier: float, mode=MODE_EXACT, renorm_every=0, samples=(), fli_dir=(1.0, 0.0, 0.0, 0.0),
-                     matrix=None, log_scale=None, birkhoff=(), hist=None, stream=None):
+                     matrix=None, log_scale=None, birkhoff=(), hist=None, stream=None, megno_birkhoff=(),
+                     tune_birkhoff=()):
     """Fused indicators; returns numpy results (megno, fli [n_samples][n], matrix
     [n][4][4] when `matrix` is given (array of initial matrices or True for the
-    identity), log_scale [n], birkhoff [windows][2][n], hist [len][4][n])."""
+    identity), log_scale [n], birkhoff [windows][2][n], hist [len][4][n],
+    megno_birkhoff [len][n], tune_birkhoff [len][4][n])."""
     n = parts.n
     spec = IndicatorSpec()
     spec.renorm_every = renorm_every
@@ -397,6 +402,20 @@ def track_indicators(parts: DeviceParticles, table_dev: DeviceBuffer, table_len:
         hd = DeviceBuffer(8 * n * 4 * hl)
         keep.append(hd)
         spec.hist_from, spec.hist_len, spec.hist = h0, hl, hd.ptr
+    spec.n_megno_birkhoff = len(megno_birkhoff)
+    for q, v in enumerate(megno_birkhoff):
+        spec.megno_birkhoff_samples[q] = v
+    if len(megno_birkhoff):
+        mbo = DeviceBuffer(8 * n * len(megno_birkhoff))
+        keep.append(mbo)
+        spec.megno_birkhoff_out = mbo.ptr
+    spec.n_tune_birkhoff = len(tune_birkhoff)
+    for q, v in enumerate(tune_birkhoff):
+        spec.tune_birkhoff_samples[q] = v
+    if len(tune_birkhoff):
+        tbo = DeviceBuffer(8 * n * 4 * len(tune_birkhoff))
+        keep.append(tbo)
+        spec.tune_birkhoff_out = tbo.ptr
     check(lib().hm_track_indicators(*parts.ptrs(), n, table_dev.ptr, table_len, n_turns, mu, barrier, mode,
                                     ctypes.byref(spec), stream))
     if len(samples):
@@ -409,6 +428,10 @@ def track_indicators(parts: DeviceParticles, table_dev: DeviceBuffer, table_len:
         out["birkhoff"] = bo.download(np.float64, n * 2 * len(birkhoff)).reshape(len(birkhoff), 2, n)
     if hist is not None:
         out["hist"] = hd.download(np.float64, n * 4 * hist[1]).reshape(hist[1], 4, n)
+    if len(megno_birkhoff):
+        out["megno_birkhoff"] = mbo.download(np.float64, n * len(megno_birkhoff)).reshape(len(megno_birkhoff), n)
+    if len(tune_birkhoff):
+        out["tune_birkhoff"] = tbo.download(np.float64, n * 4 * len(tune_birkhoff)).reshape(len(tune_birkhoff), 4, n)
     return out
 
 

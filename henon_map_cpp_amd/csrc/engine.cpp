@@ -722,40 +722,70 @@ class HenonTracker {
     py::dict tangent_indicators(Particles4D &p, unsigned int n_turns, double mu, double barrier,
                                 unsigned int renorm_every, std::vector<unsigned int> samples,
                                 std::vector<double> fli_direction, std::vector<std::pair<unsigned, unsigned>> birkhoff,
-                                std::pair<unsigned, unsigned> history, const std::string &mode)
+                                std::pair<unsigned, unsigned> history, const std::string &mode,
+                                std::vector<unsigned int> megno_birkhoff, std::vector<unsigned int> tune_birkhoff)
     {
         check_turns(p, n_turns, false);
-        if (samples.size() > HM_MAX_SAMPLES) throw std::invalid_argument("tangent_indicators: at most 16 samples");
-        if (birkhoff.size() > HM_MAX_BIRKHOFF) throw std::invalid_argument("tangent_indicators: at most 4 Birkhoff windows");
         if (fli_direction.size() != 4) throw std::invalid_argument("tangent_indicators: fli_direction needs 4 entries");
         const size_t n = p.n_particles, S = samples.size(), B = birkhoff.size();
-        hm_indicator_spec spec{};
-        spec.renorm_every = renorm_every;
-        spec.n_samples = (uint32_t)S;
-        for (size_t q = 0; q < S; q++) spec.samples[q] = samples[q];
-        for (int q = 0; q < 4; q++) spec.fli_dir[q] = fli_direction[q];
+        const size_t MB = megno_birkhoff.size(), TB = tune_birkhoff.size();
+        // The kernel takes 16 samples and 4 windows of each kind per launch: more
+        // run as further passes over the same turns from the same start
+        auto chunks = [](size_t v, size_t per) { return (v + per - 1) / per; };
+        const size_t passes = std::max<size_t>(
+            {1, chunks(S, HM_MAX_SAMPLES), chunks(B, HM_MAX_BIRKHOFF), chunks(MB, HM_MAX_BIRKHOFF),
+             chunks(TB, HM_MAX_BIRKHOFF)});
         DevBuf mg(n * S * 8), fl(n * S * 8), mat(n * 128), ls(n * 8), bo(n * 2 * B * 8),
-            hi(n * 32 * (size_t)history.second);
+            hi(n * 32 * (size_t)history.second), mbo(n * MB * 8), tbo(n * 4 * TB * 8);
         std::vector<double> eye(16 * n, 0.0);
         for (size_t i = 0; i < n; i++) eye[16 * i] = eye[16 * i + 5] = eye[16 * i + 10] = eye[16 * i + 15] = 1.0;
-        mat.upload(eye);
-        ls.upload(std::vector<double>(n, 0.0));
-        if (S) { spec.megno = mg.as<double>(); spec.fli = fl.as<double>(); }
-        spec.matrix = mat.as<double>();
-        spec.log_scale = ls.as<double>();
-        spec.n_birkhoff = (uint32_t)B;
-        for (size_t w = 0; w < B; w++) { spec.birk_from[w] = birkhoff[w].first; spec.birk_to[w] = birkhoff[w].second; }
-        if (B) spec.birk_out = bo.as<double>();
-        spec.hist_from = history.first;
-        spec.hist_len = history.second;
-        if (history.second) spec.hist = hi.as<double>();
-        {
+        DevBuf sx(passes > 1 ? n * 8 : 8), spx(passes > 1 ? n * 8 : 8), sy(passes > 1 ? n * 8 : 8),
+            spy(passes > 1 ? n * 8 : 8), ss(passes > 1 ? n * 4 : 8);
+        if (passes > 1) snapshot(p, sx, spx, sy, spy, ss);
+        for (size_t pass = 0; pass < passes; ++pass) {
+            if (pass) restore(p, sx, spx, sy, spy, ss);
+            mat.upload(eye);
+            ls.upload(std::vector<double>(n, 0.0));
+            hm_indicator_spec spec{};
+            spec.renorm_every = renorm_every;
+            for (int q = 0; q < 4; q++) spec.fli_dir[q] = fli_direction[q];
+            spec.matrix = mat.as<double>();
+            spec.log_scale = ls.as<double>();
+            const size_t s0 = std::min(S, pass * HM_MAX_SAMPLES), s1 = std::min(S, s0 + HM_MAX_SAMPLES);
+            spec.n_samples = (uint32_t)(s1 - s0);
+            for (size_t q = s0; q < s1; q++) spec.samples[q - s0] = samples[q];
+            if (s1 > s0) { spec.megno = mg.as<double>() + s0 * n; spec.fli = fl.as<double>() + s0 * n; }
+            const size_t b0 = std::min(B, pass * HM_MAX_BIRKHOFF), b1 = std::min(B, b0 + HM_MAX_BIRKHOFF);
+            spec.n_birkhoff = (uint32_t)(b1 - b0);
+            for (size_t w = b0; w < b1; w++) {
+                spec.birk_from[w - b0] = birkhoff[w].first;
+                spec.birk_to[w - b0] = birkhoff[w].second;
+            }
+            if (b1 > b0) spec.birk_out = bo.as<double>() + b0 * 2 * n;
+            const size_t m0 = std::min(MB, pass * HM_MAX_BIRKHOFF), m1 = std::min(MB, m0 + HM_MAX_BIRKHOFF);
+            spec.n_megno_birkhoff = (uint32_t)(m1 - m0);
+            for (size_t w = m0; w < m1; w++) spec.megno_birkhoff_samples[w - m0] = megno_birkhoff[w];
+            if (m1 > m0) spec.megno_birkhoff_out = mbo.as<double>() + m0 * n;
+            const size_t t0 = std::min(TB, pass * HM_MAX_BIRKHOFF), t1 = std::min(TB, t0 + HM_MAX_BIRKHOFF);
+            spec.n_tune_birkhoff = (uint32_t)(t1 - t0);
+            for (size_t w = t0; w < t1; w++) spec.tune_birkhoff_samples[w - t0] = tune_birkhoff[w];
+            if (t1 > t0) spec.tune_birkhoff_out = tbo.as<double>() + t0 * 4 * n;
+            if (pass == 0 && history.second) {
+                spec.hist_from = history.first;
+                spec.hist_len = history.second;
+                spec.hist = hi.as<double>();
+            }
             py::gil_scoped_release nogil;
             check(hm_track_indicators(p.dx.as<double>(), p.dpx.as<double>(), p.dy.as<double>(), p.dpy.as<double>(),
                                       p.dsteps.as<uint32_t>(), n, dtable.as<double>(), allowed_steps, n_turns, mu,
                                       barrier, mode_of(mode), &spec, nullptr));
             sync();
         }
+        uint32_t flags = 0;
+        check(hm_take_status(&flags, nullptr));
+        if (flags & 1u)
+            throw std::runtime_error("tangent_indicators: a particle's turn index left the modulation table; it was "
+                                     "stopped there (the reference would read out of bounds)");
         p.global_steps += n_turns;
         py::dict d;
         auto arr = [](const DevBuf &b, std::vector<py::ssize_t> shape) {
@@ -771,6 +801,8 @@ class HenonTracker {
         d["matrix"] = arr(mat, {(py::ssize_t)n, 4, 4});
         d["log_scale"] = arr(ls, {(py::ssize_t)n});
         d["birkhoff"] = arr(bo, {(py::ssize_t)B, 2, (py::ssize_t)n});
+        d["megno_birkhoff"] = arr(mbo, {(py::ssize_t)MB, (py::ssize_t)n});
+        d["tune_birkhoff"] = arr(tbo, {(py::ssize_t)TB, 4, (py::ssize_t)n});
         if (history.second) d["history"] = arr(hi, {(py::ssize_t)history.second, 4, (py::ssize_t)n});
         return d;
     }
@@ -1573,7 +1605,9 @@ PYBIND11_MODULE(henon_map_engine, m)
                  py::arg("renorm_every") = 100u, py::arg("samples") = std::vector<unsigned int>(),
                  py::arg("fli_direction") = std::vector<double>{1.0, 0.0, 0.0, 0.0},
                  py::arg("birkhoff") = std::vector<std::pair<unsigned, unsigned>>(),
-                 py::arg("history") = std::pair<unsigned, unsigned>(0u, 0u), py::arg("mode") = "exact")
+                 py::arg("history") = std::pair<unsigned, unsigned>(0u, 0u), py::arg("mode") = "exact",
+                 py::arg("megno_birkhoff") = std::vector<unsigned int>(),
+                 py::arg("tune_birkhoff") = std::vector<unsigned int>())
             .def("reversibility_error", &C::reversibility_error, "Reversibility error", py::arg("particles"),
                  py::arg("t_list"), py::arg("mu"), py::arg("barrier") = 100.0, py::arg("mode") = "exact");
     };

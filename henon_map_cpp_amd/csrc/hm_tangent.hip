@@ -139,6 +139,11 @@ struct IndicatorArgs {
     double *birk_out;       // [n_birk][2][n] tunes
     uint32_t hist_from, hist_len;
     double *hist;           // [hist_len][4][n]
+    // construct-form turn-sample windows (megno_birkhoff / tune_birkhoff)
+    uint32_t mb_T[kMaxBirk], mb_off[kMaxBirk], n_mb;
+    double *mb_out;         // [n_mb][n]
+    uint32_t tb_T[kMaxBirk], tb_off[kMaxBirk], n_tb;
+    double *tb_out;         // [n_tb][4][n]
     const double *inv_idx;  // [n_turns]: 1.0 / (k + 1), correctly rounded (host table)
     uint32_t *status;
 };
@@ -216,6 +221,19 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
         thx = atan2(px, x);
         thy = atan2(py, y);
     }
+    // construct-form accumulators: megno_birkhoff L2_j, tune_birkhoff
+    // (tune1 x, tune2 x, tune1 y, tune2 y)_j and the stored angles
+    double mbacc[kMaxBirk], tbacc[kMaxBirk][4];
+#pragma unroll
+    for (int w = 0; w < kMaxBirk; w++) {
+        mbacc[w] = 0.0;
+        tbacc[w][0] = tbacc[w][1] = tbacc[w][2] = tbacc[w][3] = 0.0;
+    }
+    double ctx = 0.0, cty = 0.0;
+    if (EXTRA && a.n_tb) {
+        ctx = atan2(px, x);
+        cty = atan2(py, y);
+    }
     if (EXTRA && in_range && a.hist_len && a.hist_from == 0) {
         const size_t n = a.n;
         a.hist[0 * n + i] = x; a.hist[1 * n + i] = px; a.hist[2 * n + i] = y; a.hist[3 * n + i] = py;
@@ -251,6 +269,13 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
         l1 = l1 + (0.5 * log10(fa2 / fb2)) * w_prev;
         l2 = l2 + inv_prev * l1;
         fb2 = fa2;
+        if (EXTRA) {
+            // megno_birkhoff_construct: L2_j += L1 w_j[idx-1] / idx, idx <= T_j
+#pragma unroll
+            for (int w = 0; w < kMaxBirk; w++)
+                if (w < (int)a.n_mb && idx_prev <= a.mb_T[w])
+                    mbacc[w] = mbacc[w] + (l1 * a.birk_w[a.mb_off[w] + idx_prev - 1]) * inv_prev;
+        }
         if (idx_prev == next_megno) {
             if (in_range && a.megno) a.megno[(size_t)sample_m * a.n + i] = inv_prev * l2;
             ++sample_m;
@@ -379,12 +404,36 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
                 }
             }
         }
+        if (EXTRA && a.n_tb) {
+            // tune_birkhoff_construct's add after this turn (add index idx)
+            const double nx = atan2(px, x), ny = atan2(py, y);
+            double dx = nx - ctx, dy = ny - cty;
+            dx = dx > 0 ? dx : dx + 2 * M_PI;
+            dy = dy > 0 ? dy : dy + 2 * M_PI;
+            ctx = nx;
+            cty = ny;
+#pragma unroll
+            for (int w = 0; w < kMaxBirk; w++) {
+                if (w < (int)a.n_tb) {
+                    const uint32_t T = a.tb_T[w];
+                    if (idx < T) {
+                        const double wt = a.birk_w[a.tb_off[w] + idx];
+                        tbacc[w][0] = tbacc[w][0] + dx * wt;
+                        tbacc[w][2] = tbacc[w][2] + dy * wt;
+                    } else if ((uint64_t)idx < 2ull * T) {
+                        const double wt = a.birk_w[a.tb_off[w] + idx - T];
+                        tbacc[w][1] = tbacc[w][1] + dx * wt;
+                        tbacc[w][3] = tbacc[w][3] + dy * wt;
+                    }
+                }
+            }
+        }
         if (EXTRA && in_range && a.hist_len && idx >= a.hist_from && idx < a.hist_from + a.hist_len) {
             const size_t n = a.n, base = (size_t)(idx - a.hist_from) * 4 * n;
             a.hist[base + 0 * n + i] = x; a.hist[base + 1 * n + i] = px;
             a.hist[base + 2 * n + i] = y; a.hist[base + 3 * n + i] = py;
         }
-        if (!__any(alive) && !(EXTRA && (a.hist_len || a.n_birk))) break;
+        if (!__any(alive) && !(EXTRA && (a.hist_len || a.n_birk || a.n_mb || a.n_tb))) break;
     }
     // the last turn's MEGNO sums
     if (pending) megno_update();
@@ -409,6 +458,10 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
             a.birk_out[((size_t)w * 2 + 0) * a.n + i] = 1.0 - bacc[w][0] / (2 * M_PI);
             a.birk_out[((size_t)w * 2 + 1) * a.n + i] = 1.0 - bacc[w][1] / (2 * M_PI);
         }
+        if (EXTRA && w < (int)a.n_mb) a.mb_out[(size_t)w * a.n + i] = mbacc[w];
+        if (EXTRA && w < (int)a.n_tb)
+#pragma unroll
+            for (int c = 0; c < 4; c++) a.tb_out[((size_t)w * 4 + c) * a.n + i] = tbacc[w][c];
     }
 }
 
@@ -669,6 +722,16 @@ extern "C" int hm_track_indicators(double *x, double *px, double *y, double *py,
         if (spec->n_birkhoff && !spec->birk_out) throw Error(HM_ERR_ARG, "indicators: birk_out is NULL");
         if (spec->hist_len && (!spec->hist || (uint64_t)spec->hist_from + spec->hist_len > (uint64_t)n_turns + 1))
             throw Error(HM_ERR_ARG, "indicators: trajectory window must lie in samples 0..n_turns");
+        if (spec->n_megno_birkhoff > (uint32_t)kMaxBirk || spec->n_tune_birkhoff > (uint32_t)kMaxBirk)
+            throw Error(HM_ERR_ARG, "indicators: at most 4 megno_birkhoff and 4 tune_birkhoff turn samples");
+        if (spec->n_megno_birkhoff && !spec->megno_birkhoff_out)
+            throw Error(HM_ERR_ARG, "indicators: megno_birkhoff_out is NULL");
+        if (spec->n_tune_birkhoff && !spec->tune_birkhoff_out)
+            throw Error(HM_ERR_ARG, "indicators: tune_birkhoff_out is NULL");
+        for (uint32_t w = 0; w < spec->n_megno_birkhoff; w++)
+            if (spec->megno_birkhoff_samples[w] == 0) throw Error(HM_ERR_ARG, "indicators: turn samples must be >= 1");
+        for (uint32_t w = 0; w < spec->n_tune_birkhoff; w++)
+            if (spec->tune_birkhoff_samples[w] == 0) throw Error(HM_ERR_ARG, "indicators: turn samples must be >= 1");
         if (n == 0 || n_turns == 0) return;
         if (!x || !px || !y || !py || !steps || !table) throw Error(HM_ERR_ARG, "indicators: NULL pointer");
         hipStream_t s = as_stream(stream);
@@ -682,7 +745,11 @@ extern "C" int hm_track_indicators(double *x, double *px, double *y, double *py,
         if (a.fli_dir[0] == 0 && a.fli_dir[1] == 0 && a.fli_dir[2] == 0 && a.fli_dir[3] == 0) a.fli_dir[0] = 1.0;
         a.megno = spec->megno; a.fli = spec->fli; a.matrix = spec->matrix; a.log_scale = spec->log_scale;
         a.n_birk = spec->n_birkhoff; a.birk_out = spec->birk_out;
-        if (a.n_birk) {
+        a.n_mb = spec->n_megno_birkhoff; a.mb_out = spec->megno_birkhoff_out;
+        a.n_tb = spec->n_tune_birkhoff; a.tb_out = spec->tune_birkhoff_out;
+        if (a.n_birk || a.n_mb || a.n_tb) {
+            // one table of weights: the history windows', then birkhoff_weights(T_j)
+            // of every construct-form turn sample
             size_t total = 0;
             for (uint32_t w = 0; w < a.n_birk; w++) {
                 a.birk_from[w] = spec->birk_from[w];
@@ -690,10 +757,23 @@ extern "C" int hm_track_indicators(double *x, double *px, double *y, double *py,
                 a.birk_off[w] = (uint32_t)total;
                 total += spec->birk_to[w] - spec->birk_from[w];
             }
-            std::vector<double> w(total);
+            for (uint32_t w = 0; w < a.n_mb; w++) {
+                a.mb_T[w] = spec->megno_birkhoff_samples[w];
+                a.mb_off[w] = (uint32_t)total;
+                total += a.mb_T[w];
+            }
+            for (uint32_t w = 0; w < a.n_tb; w++) {
+                a.tb_T[w] = spec->tune_birkhoff_samples[w];
+                a.tb_off[w] = (uint32_t)total;
+                total += a.tb_T[w];
+            }
+            if (total > 0xffffffffull) throw Error(HM_ERR_ARG, "indicators: weight tables exceed 2^32 entries");
+            std::vector<double> w(std::max<size_t>(total, 1));
             for (uint32_t q = 0; q < a.n_birk; q++)
                 host_birkhoff_weights(a.birk_to[q] - a.birk_from[q], w.data() + a.birk_off[q]);
-            double *dw = static_cast<double *>(workspace(total * sizeof(double), 1));
+            for (uint32_t q = 0; q < a.n_mb; q++) host_birkhoff_weights(a.mb_T[q], w.data() + a.mb_off[q]);
+            for (uint32_t q = 0; q < a.n_tb; q++) host_birkhoff_weights(a.tb_T[q], w.data() + a.tb_off[q]);
+            double *dw = static_cast<double *>(workspace(w.size() * sizeof(double), 1));
             hip_check(hipMemcpyAsync(dw, w.data(), total * sizeof(double), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
             hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
             a.birk_w = dw;
@@ -705,7 +785,7 @@ extern "C" int hm_track_indicators(double *x, double *px, double *y, double *py,
         LaunchTimer timer("k_indicators", s);
         // Birkhoff windows and the trajectory window are compiled into their own
         // variants, so the MEGNO/FLI-only kernel carries none of their code
-        const bool extra = a.n_birk || a.hist_len;
+        const bool extra = a.n_birk || a.hist_len || a.n_mb || a.n_tb;
         using K = void (*)(IndicatorArgs);
         K k = nullptr;
         if (mode == HM_MODE_EXACT) {

@@ -321,6 +321,21 @@ typedef struct hm_indicator_spec {
     double *birk_out;                  /* device [n_birkhoff][2][n]: tune x, tune y */
     uint32_t hist_from, hist_len;      /* trajectory window: samples [hist_from, hist_from+hist_len) */
     double *hist;                      /* device [hist_len][4][n] (x, px, y, py) */
+    /* megno_birkhoff_construct (src/henon.cu:2366-2402) fed every turn of this
+     * call, idx = 1, 2, ...: for turn sample T_j, L2_j += L1 w_j[idx-1] / idx
+     * while idx <= T_j, w_j = birkhoff_weights(T_j), L1 the MEGNO recursion's
+     * first layer; out [n_megno_birkhoff][n] holds L2_j (get_values()). */
+    uint32_t n_megno_birkhoff;
+    uint32_t megno_birkhoff_samples[HM_MAX_BIRKHOFF];
+    double *megno_birkhoff_out;
+    /* tune_birkhoff_construct (src/henon.cu:2533-2584), first_add at the
+     * initial state and add after every turn: with the wrapped phase advance
+     * d (d <= 0 gets + 2 pi), tune1_j += d w_j[idx] while idx < T_j, tune2_j +=
+     * d w_j[idx - T_j] while T_j <= idx < 2 T_j; out [n_tune_birkhoff][4][n]:
+     * tune1 x, tune2 x, tune1 y, tune2 y (the getters' sums). */
+    uint32_t n_tune_birkhoff;
+    uint32_t tune_birkhoff_samples[HM_MAX_BIRKHOFF];
+    double *tune_birkhoff_out;
 } hm_indicator_spec;
 
 /* Fused tracking + tangent map + indicators, one thread per particle with the
