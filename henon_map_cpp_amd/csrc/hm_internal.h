@@ -70,7 +70,8 @@ inline void launch_check(const char *name)
 // buffers, one per slot: 0 tracking live
 // lists, 1 Birkhoff weights of the indicator kernels, 2 tune-window weights,
 // 3 DFT magnitudes, 4 batched-job lists and plans, 5-6 autotuner trial state,
-// 7 reciprocal table, 8 FFT signal rows. Growing a slot synchronises the
+// 7 reciprocal table, 8 FFT signal rows, 9 batched MEGNO state, 10-11
+// indicator-kernel state and live lists. Growing a slot synchronises the
 // device. Calls that overlap in time on one device share these buffers.
 constexpr int kWorkspaceSlots = 12;
 uint32_t *status_word();

@@ -146,7 +146,20 @@ struct IndicatorArgs {
     double *tb_out;         // [n_tb][4][n]
     const double *inv_idx;  // [n_turns]: 1.0 / (k + 1), correctly rounded (host table)
     uint32_t *status;
+    // Segments with loss compaction: this launch runs turns [k0, k1) of the call
+    // over the live list idx_in (count on the device; NULL: every particle) and
+    // appends its survivors to idx_out; between launches each live particle's
+    // tangent matrix and sums wait in state[j] (kIndState doubles). A single
+    // launch has k0 = 0, k1 = n_turns, no lists and no state.
+    uint32_t k0, k1;
+    const uint32_t *idx_in, *count_in;
+    uint32_t *idx_out, *count_out;
+    double *state;
+    uint32_t first, last;
 };
+
+// Per-particle state between segments: M (16), log10 scale, L1, L2, |M|_F^2.
+constexpr int kIndState = 20;
 
 
 // One thread per particle; the table index of turn k is steps+k for both the
@@ -183,8 +196,11 @@ __device__ __forceinline__ bool forward_turn_out(double &x, double &px, double &
 template <int MODE, bool OCT, bool EXTRA>
 __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in_range = i < a.n;
+    const uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;  // position in the live list
+    const uint32_t n_live = a.count_in ? *a.count_in : a.n;
+    const bool in_range = li < n_live;
+    const uint32_t i = in_range ? (a.idx_in ? a.idx_in[li] : li) : 0;  // the particle
+    const uint32_t k0 = a.k0, k1 = a.k1;
     double x = __builtin_nan(""), px = x, y = x, py = x;
     uint32_t s = 0;
     if (in_range) {
@@ -192,20 +208,27 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
         s = a.steps[i];
     }
     double m[16];
-    if (a.matrix && in_range) {
+    double lscale = 0.0, l1 = 0.0, l2 = 0.0, f0;
+    if (!a.first && in_range) {
+        const double *st = a.state + (size_t)i * kIndState;
 #pragma unroll
-        for (int q = 0; q < 16; q++) m[q] = a.matrix[(size_t)i * 16 + q];
+        for (int q = 0; q < 16; q++) m[q] = st[q];
+        lscale = st[16]; l1 = st[17]; l2 = st[18]; f0 = st[19];
     } else {
+        if (a.matrix && in_range) {
 #pragma unroll
-        for (int q = 0; q < 16; q++) m[q] = (q % 5 == 0) ? 1.0 : 0.0;
+            for (int q = 0; q < 16; q++) m[q] = a.matrix[(size_t)i * 16 + q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 16; q++) m[q] = (q % 5 == 0) ? 1.0 : 0.0;
+        }
+        lscale = (a.log_scale && in_range) ? a.log_scale[i] : 0.0;
+        f0 = frob2_fused(m);
     }
-    double lscale = (a.log_scale && in_range) ? a.log_scale[i] : 0.0;
-    const double f0 = frob2_fused(m);
-    double l1 = 0.0, l2 = 0.0;
     const double mu3 = a.mu * 3, mu6 = a.mu * 6, m3mu = -3.0 * a.mu;
     const bool alive0 = in_range && !any_nan(x, px, y, py);
     bool alive = alive0;
-    uint32_t done = a.n_turns;  // turns survived
+    uint32_t done = k1;  // the turn a particle is lost in (k1: survived the launch)
     double v0[4];
     {
         const double nn = sqrt(((a.fli_dir[0] * a.fli_dir[0] + a.fli_dir[1] * a.fli_dir[1]) +
@@ -242,19 +265,22 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
     const uint64_t live = __ballot(alive0);
     uint32_t s0 = 0;
     if (live) s0 = __builtin_amdgcn_readlane(s, __builtin_ctzll(live));
-    const bool uniform = live && __all(!alive0 || s == s0) && (uint64_t)s0 + a.n_turns <= (uint64_t)a.tab_len;
-    const ConstPtr<TabEntry> tp = as_const(a.tab + s0);
+    const bool uniform = live && __all(!alive0 || s == s0) && (uint64_t)s0 + (k1 - k0) <= (uint64_t)a.tab_len;
+    // record of call turn k at tp + k (s0 = the shared index at turn k0)
+    const ConstPtr<TabEntry> tp = as_const(a.tab + s0) - k0;
     const ConstPtr<double> inv_idx = as_const(a.inv_idx);
 
+    // samples already behind (taken by earlier segments)
     uint32_t sample_q = 0;  // FLI samples (taken at their turn)
-    uint32_t sample_m = 0;  // MEGNO samples (taken once the turn's sums are in)
-    uint32_t next_sample = a.n_samples ? a.samples[0] : 0;
+    while (sample_q < a.n_samples && a.samples[sample_q] <= k0) ++sample_q;
+    uint32_t sample_m = sample_q;  // MEGNO samples (taken once the turn's sums are in)
+    uint32_t next_sample = sample_q < a.n_samples ? a.samples[sample_q] : 0;
     uint32_t next_megno = next_sample;
-    uint32_t k = 0;
+    uint32_t k = k0;
     // squared Frobenius norm of M before the turn; the MEGNO ratio
     // |M_{k+1}|/|M_k| is taken as sqrt(fa2 / fb2): one division, no square root
     double fb2 = f0;
-    uint32_t renorm_in = a.renorm_every;
+    uint32_t renorm_in = a.renorm_every ? a.renorm_every - k0 % a.renorm_every : 0;
     double xx = x * x, yy = y * y;  // carried between turns in exact mode
     const double b2s = screen_bound(a.b2);
     // The MEGNO sums of turn k (norm of M_{k+1}, ratio, log10, L1, L2) are
@@ -283,7 +309,7 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
         }
         pending = false;
     };
-    for (; k < a.n_turns; ++k) {
+    for (; k < k1; ++k) {
         if (!EXTRA && uniform && pending) {
             // Plain turns up to the next event (an FLI sample at its turn, a
             // MEGNO sample at the top of the turn after it, a renormalisation
@@ -292,7 +318,7 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
             // map interleave. The next turn's table record and reciprocal are
             // requested once this turn's have arrived (scalar loads complete
             // together), a turn ahead of their use.
-            uint32_t stop = a.n_turns;
+            uint32_t stop = k1;
             if (sample_q < a.n_samples) stop = min(stop, next_sample - 1);
             if (sample_m < a.n_samples) stop = min(stop, next_megno);
             if (renorm_in) stop = min(stop, k + renorm_in - 1);
@@ -305,7 +331,7 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
                 for (uint32_t kk = k; kk < stop; ++kk) {
                     __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this turn's entries
                     __builtin_amdgcn_sched_barrier(0);
-                    const uint32_t kn = min(kk + 1, a.n_turns - 1);
+                    const uint32_t kn = min(kk + 1, k1 - 1);
                     const TabEntry tn = load_record(tp + kn);
                     const double inv_n = inv_idx[kn];
                     {
@@ -332,7 +358,7 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
                 }
                 if (renorm_in) renorm_in -= stop - k;
                 k = stop;
-                if (k == a.n_turns || !__any(alive)) break;
+                if (k == k1 || !__any(alive)) break;
             }
         }
         if (pending) megno_update();
@@ -340,7 +366,7 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
         if (uniform) {
             t = load_record(tp + k);
         } else {
-            const uint64_t ti = (uint64_t)s + k;
+            const uint64_t ti = (uint64_t)s + (k - k0);
             if (alive && ti >= a.tab_len) {
                 atomicOr(a.status, 1u);
                 alive = false;
@@ -437,16 +463,37 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
     }
     // the last turn's MEGNO sums
     if (pending) megno_update();
+    // survivors of a segment that is not the last carry on in the next launch
+    const bool carry = !a.last && alive;
+    if (a.idx_out) {
+        const uint64_t mlive = __ballot(in_range && carry);
+        if (mlive) {
+            const uint32_t lane = __lane_id();
+            const int leader = __builtin_ctzll(mlive);
+            uint32_t slot = 0;
+            if ((int)lane == leader) slot = atomicAdd(a.count_out, (uint32_t)__popcll(mlive));
+            slot = __builtin_amdgcn_readlane(slot, leader);
+            const uint64_t bit = 1ull << lane;
+            if (mlive & bit) a.idx_out[slot + __popcll(mlive & (bit - 1ull))] = i;
+        }
+    }
     if (!in_range) return;
-    // samples not reached because every particle of the wavefront was lost
+    if (alive0) {
+        a.x[i] = x; a.px[i] = px; a.y[i] = y; a.py[i] = py;
+        a.steps[i] = s + (done - k0);
+    }
+    if (carry) {
+        double *st = a.state + (size_t)i * kIndState;
+#pragma unroll
+        for (int q = 0; q < 16; q++) st[q] = m[q];
+        st[16] = lscale; st[17] = l1; st[18] = l2; st[19] = fb2;
+        return;
+    }
+    // samples not reached because the particle (or its whole wavefront) was lost
     for (; sample_q < a.n_samples; ++sample_q)
         if (a.fli) a.fli[(size_t)sample_q * a.n + i] = __builtin_nan("");
     for (; sample_m < a.n_samples; ++sample_m)
         if (a.megno) a.megno[(size_t)sample_m * a.n + i] = __builtin_nan("");
-    if (alive0) {
-        a.x[i] = x; a.px[i] = px; a.y[i] = y; a.py[i] = py;
-        a.steps[i] = s + done;
-    }
     if (a.matrix) {
 #pragma unroll
         for (int q = 0; q < 16; q++) a.matrix[(size_t)i * 16 + q] = m[q];
@@ -699,6 +746,28 @@ const double *hm::reciprocal_table(uint32_t n, hipStream_t s)
     return d;
 }
 
+// Loss compaction of the indicator kernel: the first segment's length and the
+// longest segment.
+static constexpr uint32_t kIndFirstSegment = 125;
+static constexpr uint32_t kIndCompactEvery = 1000;
+
+// Pinned host words receiving the per-segment live counts of k_indicators.
+static uint32_t *indicator_counts(size_t n)
+{
+    static std::mutex mu;
+    static uint32_t *buf = nullptr;
+    static size_t cap = 0;
+    std::lock_guard<std::mutex> lk(mu);
+    if (cap < n) {
+        if (buf) hip_check(hipHostFree(buf), "hipHostFree");
+        buf = nullptr;
+        cap = 0;
+        hip_check(hipHostMalloc((void **)&buf, n * sizeof(uint32_t), hipHostMallocDefault), "hipHostMalloc");
+        cap = n;
+    }
+    return buf;
+}
+
 extern "C" int hm_track_indicators(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n,
                                    const double *table, size_t table_len, uint32_t n_turns, double mu,
                                    double barrier, int mode, const hm_indicator_spec *spec, void *stream)
@@ -795,8 +864,72 @@ extern "C" int hm_track_indicators(double *x, double *px, double *y, double *py,
             if (oct) k = extra ? k_indicators<kFast, true, true> : k_indicators<kFast, true, false>;
             else k = extra ? k_indicators<kFast, false, true> : k_indicators<kFast, false, false>;
         }
-        hipLaunchKernelGGL(k, dim3(grid_of(n)), dim3(256), 0, s, a);
-        launch_check("k_indicators");
+        a.first = a.last = 1;
+        a.k0 = 0;
+        a.k1 = n_turns;
+        if (extra || n_turns <= kIndFirstSegment || n > 0x7fffffffull) {
+            hipLaunchKernelGGL(k, dim3(grid_of(n)), dim3(256), 0, s, a);
+            launch_check("k_indicators");
+            return;
+        }
+        // Segments of kIndFirstSegment turns doubling up to kIndCompactEvery,
+        // each over the particles still alive: the survivors' matrices and sums
+        // wait in `state` between launches, and a launch is sized from the live
+        // count of the segment two back (read by the host while the GPU runs),
+        // so lost particles stop occupying lanes.
+        std::vector<uint32_t> seg;
+        for (uint32_t done = 0, len = kIndFirstSegment; done < n_turns;) {
+            const uint32_t t = std::min(len, n_turns - done);
+            seg.push_back(t);
+            done += t;
+            len = std::min<uint32_t>(len * 2u, kIndCompactEvery);
+        }
+        const size_t S = seg.size(), N = n;
+        double *state = static_cast<double *>(workspace(N * kIndState * sizeof(double), 10));
+        uint32_t *lists = static_cast<uint32_t *>(workspace(sizeof(uint32_t) * (2 * N + S + 64), 11));
+        uint32_t *counts = lists + 2 * N;  // counts[i]: live entries entering segment i (i >= 1)
+        uint32_t *host = indicator_counts(S + 1);
+        hip_check(hipMemsetAsync(counts, 0, sizeof(uint32_t) * (S + 1), s), "hipMemsetAsync");
+        std::vector<hipEvent_t> ev(S, nullptr);
+        auto cleanup = [&] {
+            for (auto e : ev)
+                if (e) (void)hipEventDestroy(e);
+        };
+        try {
+            uint64_t bound = N;
+            uint32_t start = 0;
+            for (size_t q = 0; q < S; ++q) {
+                if (q >= 2) {
+                    hip_check(hipEventSynchronize(ev[q - 2]), "hipEventSynchronize");
+                    bound = std::min<uint64_t>(bound, host[q - 1]);
+                }
+                const bool last = q + 1 == S;
+                a.k0 = start;
+                a.k1 = start + seg[q];
+                a.first = q == 0;
+                a.last = last;
+                a.idx_in = q == 0 ? nullptr : (q % 2 ? lists : lists + N);
+                a.count_in = q == 0 ? nullptr : counts + q;
+                a.idx_out = last ? nullptr : (q % 2 ? lists + N : lists);
+                a.count_out = last ? nullptr : counts + q + 1;
+                a.state = state;
+                if (bound > 0) {
+                    hipLaunchKernelGGL(k, dim3(grid_of(bound)), dim3(256), 0, s, a);
+                    launch_check("k_indicators");
+                }
+                if (!last) {
+                    hip_check(hipMemcpyAsync(host + q + 1, counts + q + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+                              "hipMemcpyAsync");
+                    hip_check(hipEventCreateWithFlags(&ev[q], hipEventDisableTiming), "hipEventCreate");
+                    hip_check(hipEventRecord(ev[q], s), "hipEventRecord");
+                }
+                start += seg[q];
+            }
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
     });
 }
 
