@@ -33,7 +33,7 @@ HIP_FLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 HIP_SOURCES = ["hm_runtime.hip", "hm_track.hip", "hm_tangent.hip", "hm_fft.hip", "hm_adjoint.hip", "hm_batch.hip",
-               "hm_ghost.hip", "hm_kick.hip", "hm_chirpz.hip"]
+               "hm_ghost.hip", "hm_kick.hip", "hm_chirpz.hip", "hm_fft_mixed.hip"]
 
 LIB_NAME = "libhenon_hip.so"
 ENGINE_SOURCES = ["engine.cpp"]

@@ -4,7 +4,7 @@
 //  * k_means          : per-coordinate mean of each particle's samples
 //  * k_birkhoff_hist  : Birkhoff tunes of windows of the stored samples
 //  * k_fft_tune_pow2  : Hanning-windowed FFT tune, power-of-two length, radix-2 in LDS
-//  * other lengths    : the chirp-z transform of hm_chirpz.hip
+//  * other lengths    : mixed radix (hm_fft_mixed.hip) or chirp-z (hm_chirpz.hip)
 // Two history layouts, addressed through strides (HistLayout):
 //  * sample-major hist[sample][coord][particle] (coord = x, px, y, py): a
 //    tracking wavefront writes one contiguous row per turn (k_history);
@@ -976,8 +976,11 @@ static void fft_impl(const double *hist, HistLayout lay, bool rows_input, size_t
         const auto &ws = kv.second;
         const bool lds_pow2 = pow2 && 2 * (size_t)N * sizeof(double) + kRowsStaticLds <= lds_per_block();
         if (!lds_pow2) {
-            // any other length: the chirp-z transform, one window at a time
-            for (uint32_t w : ws) fft_tune_chirpz(hist, lay, (uint32_t)n, means, from[w], N, t.hann, out, w, s);
+            // any other length, one window at a time: the mixed-radix transform
+            // when N has only the factors 2, 3, 5, 7, else the chirp-z transform
+            for (uint32_t w : ws)
+                if (!fft_tune_mixed(hist, lay, (uint32_t)n, means, from[w], N, t.hann, out, w, s))
+                    fft_tune_chirpz(hist, lay, (uint32_t)n, means, from[w], N, t.hann, out, w, s);
             continue;
         }
         for (size_t b0 = 0; b0 < ws.size();) {

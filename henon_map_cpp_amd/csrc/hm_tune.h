@@ -104,4 +104,11 @@ __device__ double interp_tune_at(Mag mag, uint32_t N, uint32_t index)
 void fft_tune_chirpz(const double *hist, HistLayout lay, uint32_t n, const double *means, uint32_t from,
                      uint32_t N, const double *hann, double *out, uint32_t window, hipStream_t s);
 
+// The same for a length whose prime factors are 2, 3, 5 and 7 only and that
+// splits into two factors of at most 2048 (hm_fft_mixed.hip): a mixed-radix
+// four-step transform of N itself. Returns false (nothing launched) for any
+// other length.
+bool fft_tune_mixed(const double *hist, HistLayout lay, uint32_t n, const double *means, uint32_t from, uint32_t N,
+                    const double *hann, double *out, uint32_t window, hipStream_t s);
+
 }  // namespace hm

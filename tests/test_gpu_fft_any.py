@@ -1,4 +1,5 @@
-"""FFT tunes of windows of any length (the chirp-z path) through the C ABI and
+"""FFT tunes of windows of any length (the mixed-radix and chirp-z paths)
+through the C ABI and
 the drop-in API: against the oracle's long-double DFT where that finishes in
 seconds (N up to 10001), against numpy's FFT magnitudes with the oracle's peak
 interpolation above that, against single tones, with NaN samples, in particle
@@ -48,7 +49,20 @@ def test_chirpz_matches_oracle_dft(N):
         np.testing.assert_allclose(got[0, :, i], want, rtol=0, atol=1e-12, err_msg=f"N={N} particle {i}")
 
 
-@pytest.mark.parametrize("N", [16411, 65537, 100000, 100001])
+@pytest.mark.parametrize("N", [6, 12, 45, 49, 210, 343, 1000, 1500, 2187, 3125, 5040, 9604])
+def test_mixed_radix_lengths_match_oracle_dft(N):
+    """Lengths with only the factors 2, 3, 5, 7 take the mixed-radix
+    transform (radices 8, 4, 2, 3, 5, 7 in every combination here)."""
+    n = 3 if N > 4000 else 8
+    h = quasi_periodic(N, n, seed=N + 1)
+    d = C.DeviceBuffer.from_array(np.ascontiguousarray(h))
+    got = C.history_tunes(d, n, N, [(0, N - 1)], "fft")
+    for i in range(n):
+        want = [O.fft_tune(h[:, 0, i], h[:, 1, i]), O.fft_tune(h[:, 2, i], h[:, 3, i])]
+        np.testing.assert_allclose(got[0, :, i], want, rtol=0, atol=1e-12, err_msg=f"N={N} particle {i}")
+
+
+@pytest.mark.parametrize("N", [16384, 16411, 65537, 100000, 100001, 131072])
 def test_chirpz_long_windows_match_numpy_fft(N):
     n = 4
     h = quasi_periodic(N, n, seed=N)
