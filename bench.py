@@ -307,7 +307,7 @@ def tune_all_leg(n=90_000, n_turns=10_000, reps=2):
         dt = time.perf_counter() - t0
         ks = {k: C.profile_read(k)[0] for k in ("k_history_rows", "k_birkhoff_hist", "k_fft_tune_rows",
                                                 "k_cz_columns", "k_cz_rows", "k_cz_final", "k_cz_tune",
-                                                "k_mr_columns", "k_mr_rows", "k_mr_tune")}
+                                                "k_mr_columns", "k_mr_rows", "k_mr_final", "k_mr_tune")}
         C.profile_enable(False)
         if best is None or dt < best[0]:
             best = (dt, ks, np.asarray(rb), np.asarray(rf))

@@ -977,7 +977,8 @@ static void fft_impl(const double *hist, HistLayout lay, bool rows_input, size_t
         const bool lds_pow2 = pow2 && 2 * (size_t)N * sizeof(double) + kRowsStaticLds <= lds_per_block();
         if (!lds_pow2) {
             // any other length, one window at a time: the mixed-radix transform
-            // when N has only the factors 2, 3, 5, 7, else the chirp-z transform
+            // of N when it has only the factors 2, 3, 5, 7, else chirp-z over a
+            // smooth length, and the power-of-two chirp-z past 2^21 samples
             for (uint32_t w : ws)
                 if (!fft_tune_mixed(hist, lay, (uint32_t)n, means, from[w], N, t.hann, out, w, s))
                     fft_tune_chirpz(hist, lay, (uint32_t)n, means, from[w], N, t.hann, out, w, s);

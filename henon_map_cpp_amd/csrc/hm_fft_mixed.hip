@@ -13,10 +13,28 @@
 // Sub-transforms are Stockham passes of radix 8, 4, 2, 3, 5, 7 (runtime plan)
 // over several columns or rows per workgroup. HBM traffic per signal: the
 // samples once, 16 N bytes of scratch each way, 8 N bytes of magnitudes each way.
-// Other lengths take the chirp-z transform (hm_chirpz.hip).
+//
+// Other lengths run Bluestein's chirp-z transform over the smallest
+// {2,3,5,7}-smooth length M >= 2N - 1 that splits the same way (with
+// c_m = exp(-i pi m^2 / N): X_k = c_k sum_m (x_m c_m) conj(c_{k-m}), a cyclic
+// convolution of length M with the fixed filter B = DFT_M(conj(c)), computed
+// once per N on the host in long double). The same kernels with the chirp:
+//   k_mr_columns<1> : chirped samples (zero from N on), M1-point FFTs, twiddle
+//   k_mr_rows<1>    : per row k1 the M2-point FFT (the spectrum, entry
+//                     k1 + M1 k2), times B, conjugate, M2-point FFT, twiddle
+//                     W_M^{k1 j1}: the inverse transform as the conjugate of a
+//                     forward one (input index k1 + M1 k2, output j1 + M2 j2)
+//   k_mr_final      : per column j1 the M1-point FFT over k1 (giving j2), the
+//                     magnitudes of the entries j = j1 + M2 j2 < N
+//   k_mr_tune<1>    : first maximum and interpolation in natural order
+// |X_k| = |conv_k| / M; the scale is left out since the first maximum and the
+// interpolation (homogeneous of degree 0 in the magnitudes) do not see it.
+// Windows too long for two 2048-point factors take the power-of-two chirp-z
+// transform (hm_chirpz.hip).
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstdlib>
 #include <map>
@@ -45,11 +63,14 @@ struct MrArgs {
     HistLayout lay;
     const double *means;  // [4][n] or NULL
     uint32_t n, p0, nsig, from, N;
-    uint32_t N1, N2, C, RR;  // C columns per workgroup (columns kernel), RR rows (rows kernel)
+    uint32_t M;              // transform length: N, or the chirp-z length
+    uint32_t N1, N2, C, RR;  // M = N1 x N2; C columns per workgroup (columns kernels), RR rows (rows kernel)
     MrPlan p1, p2;
     const double *hann;      // [N]
     const Cplx *tw1, *tw2;   // W_{N1}^q, W_{N2}^q
-    const Cplx *twm;         // [N1][N2]: W_N^{k1 n2}
+    const Cplx *twm;         // [N1][N2]: W_M^{k1 n2}
+    const Cplx *chirp;       // chirp-z: [N] exp(-i pi (m^2 mod 2N) / N)
+    const Cplx *bhat;        // chirp-z: [N1][N2], B[k1 + N1 k2] at k1 N2 + k2
     Cplx *Y;                 // scratch [nsig][N1][N2]
     double *mags;            // scratch [nsig][N]
     uint32_t *flags;         // [nsig]
@@ -207,6 +228,7 @@ __device__ __forceinline__ void mr_fft(Cplx *buf, uint32_t G, const MrPlan &p, u
 
 __device__ __forceinline__ uint32_t mr_virtual(uint32_t b, uint32_t per) { return (b % 8) * per + b / 8; }
 
+template <bool CZ>
 __global__ void __launch_bounds__(kMrThreads, 3) k_mr_columns(MrArgs a)
 {
     extern __shared__ Cplx buf[];
@@ -235,20 +257,25 @@ __global__ void __launch_bounds__(kMrThreads, 3) k_mr_columns(MrArgs a)
     for (uint32_t e = threadIdx.x; e < C * N1; e += kMrThreads) {
         const uint32_t n1 = e / C, c = e - n1 * C;
         const uint32_t m = n1 * N2 + c0 + c;
-        const uint64_t sm = (uint64_t)a.from + m;
-        const double vx = a.hist[hat(a.lay, sm, cx, particle)] - mc[cx];
-        const double vp = a.hist[hat(a.lay, sm, cp, particle)] - mc[cp];
-        nan_seen |= __builtin_isnan(vx) || __builtin_isnan(vp);
-        if (scan_other)
-            nan_seen |= __builtin_isnan(a.hist[hat(a.lay, sm, co0, particle)] - mc[co0]) ||
-                        __builtin_isnan(a.hist[hat(a.lay, sm, co1, particle)] - mc[co1]);
-        const double h = a.hann[m];
-        buf[c * GS + n1] = Cplx{vx * h, vp * h};
+        Cplx z{0.0, 0.0};
+        if (!CZ || m < a.N) {
+            const uint64_t sm = (uint64_t)a.from + m;
+            const double vx = a.hist[hat(a.lay, sm, cx, particle)] - mc[cx];
+            const double vp = a.hist[hat(a.lay, sm, cp, particle)] - mc[cp];
+            nan_seen |= __builtin_isnan(vx) || __builtin_isnan(vp);
+            if (scan_other)
+                nan_seen |= __builtin_isnan(a.hist[hat(a.lay, sm, co0, particle)] - mc[co0]) ||
+                            __builtin_isnan(a.hist[hat(a.lay, sm, co1, particle)] - mc[co1]);
+            const double h = a.hann[m];
+            z = Cplx{vx * h, vp * h};
+            if constexpr (CZ) z = cmul(z, a.chirp[m]);
+        }
+        buf[c * GS + n1] = z;
     }
     if (__any(nan_seen) && (threadIdx.x & 63) == 0) atomicOr(a.flags + sig, 1u);
     __syncthreads();
     mr_fft(buf, C, a.p1, GS, a.tw1);
-    Cplx *Y = a.Y + (size_t)sig * a.N;
+    Cplx *Y = a.Y + (size_t)sig * a.M;
     for (uint32_t e = threadIdx.x; e < C * N1; e += kMrThreads) {
         const uint32_t k1 = e / C, c = e - k1 * C;
         const size_t o = (size_t)k1 * N2 + c0 + c;
@@ -256,6 +283,7 @@ __global__ void __launch_bounds__(kMrThreads, 3) k_mr_columns(MrArgs a)
     }
 }
 
+template <bool CZ>
 __global__ void __launch_bounds__(kMrThreads, 3) k_mr_rows(MrArgs a)
 {
     extern __shared__ Cplx buf[];
@@ -265,18 +293,62 @@ __global__ void __launch_bounds__(kMrThreads, 3) k_mr_rows(MrArgs a)
     if (v >= total) return;
     const uint32_t sig = v / per_sig, r0 = (v % per_sig) * a.RR;
     const uint32_t N2 = a.N2, P = a.RR * N2;
-    const Cplx *Y = a.Y + (size_t)sig * a.N + (size_t)r0 * N2;
+    Cplx *Y = a.Y + (size_t)sig * a.M + (size_t)r0 * N2;
     for (uint32_t e = threadIdx.x; e < P; e += kMrThreads) buf[e] = Y[e];
     __syncthreads();
     mr_fft(buf, a.RR, a.p2, N2, a.tw2);
-    // spectrum entry k1 + N1 k2 at row k1, position k2
-    double *mags = a.mags + (size_t)sig * a.N + (size_t)r0 * N2;
-    for (uint32_t e = threadIdx.x; e < P; e += kMrThreads) {
-        const Cplx z = buf[e];
-        mags[e] = sqrt(z.re * z.re + z.im * z.im);
+    if constexpr (CZ) {
+        // spectrum entry k1 + N1 k2 times B, conjugated: the inverse transform
+        // becomes a forward one over k2 (then k1, in k_mr_final)
+        const Cplx *bh = a.bhat + (size_t)r0 * N2;
+        for (uint32_t e = threadIdx.x; e < P; e += kMrThreads) {
+            const Cplx z = cmul(buf[e], bh[e]);
+            buf[e] = Cplx{z.re, -z.im};
+        }
+        __syncthreads();
+        mr_fft(buf, a.RR, a.p2, N2, a.tw2);
+        const Cplx *tm = a.twm + (size_t)r0 * N2;
+        for (uint32_t e = threadIdx.x; e < P; e += kMrThreads) Y[e] = cmul(buf[e], tm[e]);
+    } else {
+        // spectrum entry k1 + N1 k2 at row k1, position k2
+        double *mags = a.mags + (size_t)sig * a.N + (size_t)r0 * N2;
+        for (uint32_t e = threadIdx.x; e < P; e += kMrThreads) {
+            const Cplx z = buf[e];
+            mags[e] = sqrt(z.re * z.re + z.im * z.im);
+        }
     }
 }
 
+// Chirp-z only: columns j1 in [c0, c0 + C), the M1-point transforms over k1
+// (output j2) and the magnitudes of the entries j = j1 + N2 j2 < N.
+__global__ void __launch_bounds__(kMrThreads, 3) k_mr_final(MrArgs a)
+{
+    extern __shared__ Cplx buf[];
+    const uint32_t per_sig = a.N2 / a.C;
+    const uint32_t total = a.nsig * per_sig;
+    const uint32_t v = mr_virtual(blockIdx.x, gridDim.x / 8);
+    if (v >= total) return;
+    const uint32_t sig = v / per_sig, c0 = (v % per_sig) * a.C;
+    const uint32_t C = a.C, N1 = a.N1, N2 = a.N2, GS = N1 + 1;
+    const Cplx *Y = a.Y + (size_t)sig * a.M;
+    for (uint32_t e = threadIdx.x; e < C * N1; e += kMrThreads) {
+        const uint32_t k1 = e / C, c = e - k1 * C;
+        buf[c * GS + k1] = Y[(size_t)k1 * N2 + c0 + c];
+    }
+    __syncthreads();
+    mr_fft(buf, C, a.p1, GS, a.tw1);
+    double *mags = a.mags + (size_t)sig * a.N;
+    for (uint32_t e = threadIdx.x; e < C * N1; e += kMrThreads) {
+        const uint32_t j2 = e / C, c = e - j2 * C;
+        const uint32_t j = c0 + c + j2 * N2;
+        if (j < a.N) {
+            const Cplx z = buf[c * GS + j2];
+            mags[j] = sqrt(z.re * z.re + z.im * z.im);
+        }
+    }
+}
+
+template <bool NATURAL>
 __global__ void __launch_bounds__(kMrThreads) k_mr_tune(MrArgs a)
 {
     __shared__ double sv[kMrThreads / 64];
@@ -291,13 +363,17 @@ __global__ void __launch_bounds__(kMrThreads) k_mr_tune(MrArgs a)
     }
     const double *mags = a.mags + (size_t)sig * a.N;
     const uint32_t N1 = a.N1, N2 = a.N2;
-    // first maximum in the spectrum's order: the lowest k = k1 + N1 k2 among
-    // equal magnitudes, whatever the storage order (k1 N2 + k2)
+    // first maximum in the spectrum's order: the lowest k among equal
+    // magnitudes, whatever the storage order (k1 N2 + k2 for k = k1 + N1 k2)
     double v = -1.0;
     uint32_t i = 0xffffffffu;
     for (uint32_t p = threadIdx.x; p < a.N; p += kMrThreads) {
         const double mg = mags[p];
-        const uint32_t k1 = p / N2, k = k1 + N1 * (p - k1 * N2);
+        uint32_t k = p;
+        if constexpr (!NATURAL) {
+            const uint32_t k1 = p / N2;
+            k = k1 + N1 * (p - k1 * N2);
+        }
         if (mg > v || (mg == v && k < i) || i == 0xffffffffu) { v = mg; i = k; }
     }
 #pragma unroll
@@ -313,7 +389,8 @@ __global__ void __launch_bounds__(kMrThreads) k_mr_tune(MrArgs a)
         uint32_t bi = si[0];
         for (uint32_t q = 1; q < kMrThreads / 64; ++q)
             if (sv[q] > bv || (sv[q] == bv && si[q] < bi)) { bv = sv[q]; bi = si[q]; }
-        *o = interp_tune_at([&](uint32_t k) { return mags[(size_t)(k % N1) * N2 + k / N1]; }, a.N, bi);
+        if constexpr (NATURAL) *o = interp_tune_at([&](uint32_t k) { return mags[k]; }, a.N, bi);
+        else *o = interp_tune_at([&](uint32_t k) { return mags[(size_t)(k % N1) * N2 + k / N1]; }, a.N, bi);
     }
 }
 
@@ -325,9 +402,10 @@ __global__ void k_mr_zero(uint32_t *__restrict__ p, uint32_t n)
 
 // ---------------------------------------------------------------- host plan
 struct MrTables {
-    uint32_t N1 = 0, N2 = 0;
+    uint32_t M = 0, N1 = 0, N2 = 0;  // M = 0: no plan for this length
     MrPlan p1{}, p2{};
     Cplx *tw1 = nullptr, *tw2 = nullptr, *twm = nullptr;
+    Cplx *chirp = nullptr, *bhat = nullptr;  // chirp-z plans only
 };
 
 // Radix plan of a {2,3,5,7}-smooth length: radix 8 while it divides, then 4,
@@ -356,16 +434,69 @@ static bool split(uint32_t N, uint32_t &N1, uint32_t &N2)
     return N1 != 0;
 }
 
-static Cplx *upload_roots(const std::vector<Cplx> &v, hipStream_t s)
+// Both factors of a split length have radix plans.
+static bool plan_length(uint32_t M, MrTables &t)
 {
-    Cplx *d = nullptr;
-    hip_check(hipMalloc(&d, v.size() * sizeof(Cplx)), "hipMalloc(mixed-radix twiddles)");
-    hip_check(hipMemcpyAsync(d, v.data(), v.size() * sizeof(Cplx), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    uint32_t r = M;
+    for (uint32_t q : {2u, 3u, 5u, 7u})
+        while (r % q == 0) r /= q;
+    if (r != 1) return false;
+    uint32_t N1, N2;
+    if (!split(M, N1, N2) || !radix_plan(N1, t.p1) || !radix_plan(N2, t.p2)) return false;
+    t.M = M; t.N1 = N1; t.N2 = N2;
+    return true;
+}
+
+template <class T>
+static T *upload(const std::vector<T> &v, hipStream_t s, const char *what)
+{
+    T *d = nullptr;
+    hip_check(hipMalloc(&d, v.size() * sizeof(T)), what);
+    hip_check(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
     return d;
 }
 
-// Tables per length (cached per device): nullptr result when N is not
-// {2,3,5,7}-smooth or does not split into two factors a workgroup holds.
+using CplxL = std::complex<long double>;
+
+// Forward DFT of a {2,3,5,7}-smooth length in long double (the chirp-z filter
+// spectrum): recursive decimation in time by the smallest prime factor r of
+// n = r m, X[k + m q] = sum_s W_n^{s (k + m q)} Y_s[k], Y_s the DFT of x[s + r j].
+// root[e] = W_M^e; a sub-length n uses W_n^e = root[e M / n].
+static void host_dft_rec(const CplxL *in, size_t stride, CplxL *out, size_t n, const std::vector<CplxL> &root,
+                         size_t M, std::vector<CplxL> &tmp)
+{
+    if (n == 1) { out[0] = in[0]; return; }
+    size_t r = 2;
+    while (n % r) ++r;
+    const size_t m = n / r, rs = M / n;
+    for (size_t s = 0; s < r; ++s) host_dft_rec(in + s * stride, stride * r, out + s * m, m, root, M, tmp);
+    CplxL *t = tmp.data();
+    for (size_t k = 0; k < m; ++k)
+        for (size_t q = 0; q < r; ++q) {
+            const size_t kk = k + m * q;
+            CplxL acc = out[k];
+            for (size_t s = 1; s < r; ++s) acc += out[s * m + k] * root[((s * kk) % n) * rs];
+            t[kk] = acc;
+        }
+    std::copy(t, t + n, out);
+}
+
+static std::vector<CplxL> host_dft(const std::vector<CplxL> &x)
+{
+    const size_t M = x.size();
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    std::vector<CplxL> root(M), out(M), tmp(M);
+    for (size_t q = 0; q < M; ++q) {
+        const long double ang = two_pi * (long double)q / (long double)M;
+        root[q] = CplxL(cosl(ang), -sinl(ang));
+    }
+    host_dft_rec(x.data(), 1, out.data(), M, root, M, tmp);
+    return out;
+}
+
+// Tables of one length (cached per device): the direct plan of a
+// {2,3,5,7}-smooth N, else the chirp-z plan over the smallest smooth M >=
+// 2N - 1 that splits; M = 0 when neither exists (M would exceed kMrPmax^2).
 static const MrTables *mr_tables(uint32_t N, hipStream_t s)
 {
     static std::mutex mu;
@@ -374,16 +505,21 @@ static const MrTables *mr_tables(uint32_t N, hipStream_t s)
     hip_check(hipGetDevice(&dev), "hipGetDevice");
     std::lock_guard<std::mutex> lk(mu);
     auto it = cache.find({dev, N});
-    if (it != cache.end()) return it->second.N1 ? &it->second : nullptr;
+    if (it != cache.end()) return it->second.M ? &it->second : nullptr;
     MrTables t;
-    uint32_t N1, N2;
-    if (!split(N, N1, N2) || !radix_plan(N1, t.p1) || !radix_plan(N2, t.p2)) {
-        cache[{dev, N}] = MrTables{};
-        return nullptr;
+    bool cz = false;
+    if (!plan_length(N, t)) {
+        cz = true;
+        const uint64_t lo = 2ull * N - 1, hi = (uint64_t)kMrPmax * kMrPmax;
+        for (uint64_t M = lo; M <= hi; ++M)
+            if (plan_length((uint32_t)M, t)) break;
+        if (!t.M) {
+            cache[{dev, N}] = MrTables{};
+            return nullptr;
+        }
     }
-    t.N1 = N1;
-    t.N2 = N2;
-    const long double two_pi = 6.283185307179586476925286766559005768L;
+    const uint32_t M = t.M, N1 = t.N1, N2 = t.N2;
+    const long double two_pi = 6.283185307179586476925286766559005768L, pi = two_pi / 2;
     auto roots = [&](uint32_t L) {
         std::vector<Cplx> v(L);
         for (uint32_t q = 0; q < L; ++q) {
@@ -392,13 +528,35 @@ static const MrTables *mr_tables(uint32_t N, hipStream_t s)
         }
         return v;
     };
-    const std::vector<Cplx> r1 = roots(N1), r2 = roots(N2), rN = roots(N);
-    std::vector<Cplx> m((size_t)N);
+    const std::vector<Cplx> r1 = roots(N1), r2 = roots(N2), rM = roots(M);
+    // step twiddles W_M^{k1 n2}, reduced exactly mod M
+    std::vector<Cplx> m((size_t)M);
     for (uint64_t k1 = 0; k1 < N1; ++k1)
-        for (uint64_t n2 = 0; n2 < N2; ++n2) m[k1 * N2 + n2] = rN[(k1 * n2) % N];
-    t.tw1 = upload_roots(r1, s);
-    t.tw2 = upload_roots(r2, s);
-    t.twm = upload_roots(m, s);
+        for (uint64_t n2 = 0; n2 < N2; ++n2) m[k1 * N2 + n2] = rM[(k1 * n2) % M];
+    t.tw1 = upload(r1, s, "hipMalloc(mixed-radix twiddles)");
+    t.tw2 = upload(r2, s, "hipMalloc(mixed-radix twiddles)");
+    t.twm = upload(m, s, "hipMalloc(mixed-radix step twiddles)");
+    if (cz) {
+        // chirp c_m = exp(-i pi (m^2 mod 2N) / N) (m^2 exact in 64 bits), filter
+        // b_j = conj(c_j) for |j| < N (j mod M), B = DFT_M(b) stored [k1][k2]
+        std::vector<Cplx> chirp(N), bhat((size_t)M);
+        std::vector<CplxL> b((size_t)M, CplxL(0, 0));
+        for (uint64_t q = 0; q < N; ++q) {
+            const long double ang = pi * (long double)((q * q) % (2ull * N)) / (long double)N;
+            const long double c = cosl(ang), sn = sinl(ang);
+            chirp[q] = Cplx{(double)c, (double)-sn};
+            b[q] = CplxL(c, sn);
+            if (q > 0) b[M - q] = b[q];
+        }
+        const std::vector<CplxL> B = host_dft(b);
+        for (size_t k1 = 0; k1 < N1; ++k1)
+            for (size_t k2 = 0; k2 < N2; ++k2) {
+                const CplxL z = B[k1 + (size_t)N1 * k2];
+                bhat[k1 * N2 + k2] = Cplx{(double)z.real(), (double)z.imag()};
+            }
+        t.chirp = upload(chirp, s, "hipMalloc(chirp)");
+        t.bhat = upload(bhat, s, "hipMalloc(chirp-z filter)");
+    }
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     cache[{dev, N}] = t;
     return &cache[{dev, N}];
@@ -422,53 +580,78 @@ static uint32_t divisor_at_most(uint32_t v, uint32_t cap)
     return 1;
 }
 
+template <class K>
+static void set_lds(K kernel, size_t bytes)
+{
+    hip_check(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes),
+              "hipFuncSetAttribute");
+}
+
 bool fft_tune_mixed(const double *hist, HistLayout lay, uint32_t n, const double *means, uint32_t from, uint32_t N,
                     const double *hann, double *out, uint32_t window, hipStream_t s)
 {
-    if (N < 4) return false;
+    if (N < 2) return false;
     const MrTables *t = mr_tables(N, s);
     if (!t) return false;
     if (n == 0) return true;
+    const bool cz = t->chirp != nullptr;
     MrArgs a{};
-    a.hist = hist; a.lay = lay; a.means = means; a.n = n; a.from = from; a.N = N;
+    a.hist = hist; a.lay = lay; a.means = means; a.n = n; a.from = from; a.N = N; a.M = t->M;
     a.N1 = t->N1; a.N2 = t->N2; a.p1 = t->p1; a.p2 = t->p2;
     a.C = divisor_at_most(a.N2, kMrPmax / a.N1);
     a.RR = divisor_at_most(a.N1, kMrPmax / a.N2);
-    a.hann = hann; a.tw1 = t->tw1; a.tw2 = t->tw2; a.twm = t->twm; a.out = out; a.window = window;
+    a.hann = hann; a.tw1 = t->tw1; a.tw2 = t->tw2; a.twm = t->twm; a.chirp = t->chirp; a.bhat = t->bhat;
+    a.out = out; a.window = window;
     const size_t lds_cols = (size_t)a.C * (a.N1 + 1) * sizeof(Cplx);
     const size_t lds_rows = (size_t)a.RR * a.N2 * sizeof(Cplx);
-    const uint64_t per_particle = 2 * ((uint64_t)sizeof(Cplx) * N + 8ull * N) + 8;
+    // scratch per particle (both planes): the M-point work arrays, magnitudes, flags
+    const uint64_t per_particle = 2 * ((uint64_t)sizeof(Cplx) * a.M + 8ull * N) + 8;
+    // at most 2^22 particles per batch keeps every grid below 2^24 workgroups
     const uint32_t batch =
         (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({(uint64_t)n, mr_budget() / per_particle, 1ull << 22}));
     const size_t nsig_max = 2 * (size_t)batch;
-    const size_t y_bytes = nsig_max * (size_t)N * sizeof(Cplx), mag_bytes = nsig_max * (size_t)N * 8;
+    const size_t y_bytes = nsig_max * (size_t)a.M * sizeof(Cplx), mag_bytes = nsig_max * (size_t)N * 8;
     char *ws = static_cast<char *>(workspace(y_bytes + mag_bytes + nsig_max * 4 + 256, 3));
     a.Y = reinterpret_cast<Cplx *>(ws);
     a.mags = reinterpret_cast<double *>(ws + y_bytes);
     a.flags = reinterpret_cast<uint32_t *>(ws + y_bytes + mag_bytes);
-    hip_check(hipFuncSetAttribute((const void *)k_mr_columns, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds_cols), "hipFuncSetAttribute");
-    hip_check(hipFuncSetAttribute((const void *)k_mr_rows, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds_rows), "hipFuncSetAttribute");
+    if (cz) {
+        set_lds(k_mr_columns<true>, lds_cols);
+        set_lds(k_mr_rows<true>, lds_rows);
+        set_lds(k_mr_final, lds_cols);
+    } else {
+        set_lds(k_mr_columns<false>, lds_cols);
+        set_lds(k_mr_rows<false>, lds_rows);
+    }
     for (uint32_t p0 = 0; p0 < n; p0 += batch) {
         const uint32_t nb = std::min(batch, n - p0);
         a.p0 = p0;
         a.nsig = 2 * nb;
         hipLaunchKernelGGL(k_mr_zero, dim3((a.nsig + 255) / 256), dim3(256), 0, s, a.flags, a.nsig);
         launch_check("k_mr_zero");
+        // grids of 8 x (workgroups per XCD); see mr_virtual
         const uint32_t cols = a.nsig * (a.N2 / a.C), rows = a.nsig * (a.N1 / a.RR);
+        const dim3 gc((cols + 7) / 8 * 8), gr((rows + 7) / 8 * 8);
         {
             LaunchTimer timer("k_mr_columns", s);
-            hipLaunchKernelGGL(k_mr_columns, dim3((cols + 7) / 8 * 8), dim3(kMrThreads), lds_cols, s, a);
+            if (cz) hipLaunchKernelGGL(k_mr_columns<true>, gc, dim3(kMrThreads), lds_cols, s, a);
+            else hipLaunchKernelGGL(k_mr_columns<false>, gc, dim3(kMrThreads), lds_cols, s, a);
             launch_check("k_mr_columns");
         }
         {
             LaunchTimer timer("k_mr_rows", s);
-            hipLaunchKernelGGL(k_mr_rows, dim3((rows + 7) / 8 * 8), dim3(kMrThreads), lds_rows, s, a);
+            if (cz) hipLaunchKernelGGL(k_mr_rows<true>, gr, dim3(kMrThreads), lds_rows, s, a);
+            else hipLaunchKernelGGL(k_mr_rows<false>, gr, dim3(kMrThreads), lds_rows, s, a);
             launch_check("k_mr_rows");
         }
+        if (cz) {
+            LaunchTimer timer("k_mr_final", s);
+            hipLaunchKernelGGL(k_mr_final, gc, dim3(kMrThreads), lds_cols, s, a);
+            launch_check("k_mr_final");
+        }
         LaunchTimer timer("k_mr_tune", s);
-        hipLaunchKernelGGL(k_mr_tune, dim3(a.nsig), dim3(kMrThreads), 0, s, a);
+        if (cz) hipLaunchKernelGGL(k_mr_tune<true>, dim3(a.nsig), dim3(kMrThreads), 0, s, a);
+        else hipLaunchKernelGGL(k_mr_tune<false>, dim3(a.nsig), dim3(kMrThreads), 0, s, a);
         launch_check("k_mr_tune");
     }
     return true;

@@ -99,15 +99,16 @@ __device__ double interp_tune_at(Mag mag, uint32_t N, uint32_t index)
 // FFT tunes of one window of any length N >= 2 (hm_chirpz.hip): Bluestein's
 // chirp-z transform over a power-of-two length M >= 2N - 1, computed as two
 // LDS-resident passes of sub-transforms (M = M1 x M2) with HBM scratch between
-// them. hist/lay/means/from as the other tune kernels; hann: device [N];
+// them; for the windows fft_tune_mixed does not take. hist/lay/means/from as the other tune kernels; hann: device [N];
 // out: device [.. window][2][n] (tune x, tune y), window `window`.
 void fft_tune_chirpz(const double *hist, HistLayout lay, uint32_t n, const double *means, uint32_t from,
                      uint32_t N, const double *hann, double *out, uint32_t window, hipStream_t s);
 
-// The same for a length whose prime factors are 2, 3, 5 and 7 only and that
-// splits into two factors of at most 2048 (hm_fft_mixed.hip): a mixed-radix
-// four-step transform of N itself. Returns false (nothing launched) for any
-// other length.
+// The same by mixed-radix four-step transforms (hm_fft_mixed.hip): of N itself
+// when its prime factors are 2, 3, 5 and 7 only and it splits into two factors
+// of at most 2048, else Bluestein's chirp-z over the smallest such length
+// M >= 2N - 1. Returns false (nothing launched) when no such M exists
+// (N > 2^21 roughly).
 bool fft_tune_mixed(const double *hist, HistLayout lay, uint32_t n, const double *means, uint32_t from, uint32_t N,
                     const double *hann, double *out, uint32_t window, hipStream_t s);
 

@@ -1,6 +1,5 @@
-"""FFT tunes of windows of any length (the mixed-radix and chirp-z paths)
-through the C ABI and
-the drop-in API: against the oracle's long-double DFT where that finishes in
+"""FFT tunes of windows of any length (the mixed-radix transform, chirp-z
+over smooth and power-of-two lengths) through the C ABI and the drop-in API: against the oracle's long-double DFT where that finishes in
 seconds (N up to 10001), against numpy's FFT magnitudes with the oracle's peak
 interpolation above that, against single tones, with NaN samples, in particle
 batches, and all_tunes over 1e5 turns (windows of 100000 and 100001 samples)."""
@@ -62,9 +61,12 @@ def test_mixed_radix_lengths_match_oracle_dft(N):
         np.testing.assert_allclose(got[0, :, i], want, rtol=0, atol=1e-12, err_msg=f"N={N} particle {i}")
 
 
-@pytest.mark.parametrize("N", [16384, 16411, 65537, 100000, 100001, 131072])
+@pytest.mark.parametrize("N", [16384, 16411, 65537, 100000, 100001, 131072, 2_100_001])
 def test_chirpz_long_windows_match_numpy_fft(N):
-    n = 4
+    """Up to 2^21 samples the chirp-z transform runs over a {2,3,5,7}-smooth
+    length; 2100001 samples (no smooth length of two 2048-point factors is
+    long enough) take the power-of-two chirp-z path."""
+    n = 4 if N < 1_000_000 else 2
     h = quasi_periodic(N, n, seed=N)
     d = C.DeviceBuffer.from_array(np.ascontiguousarray(h))
     got = C.history_tunes(d, n, N, [(0, N - 1)], "fft")
