@@ -233,7 +233,8 @@ def reversibility_leg():
 def tunes_leg():
     """Trajectory store + 2048-sample FFT tunes and Birkhoff tunes of the
     chaos-indicator particle set (256x256 grid + 65536 ball points), as
-    all_tunes runs them: the HBM-bound stages. The trajectory is stored as
+    all_tunes runs them (both tunes from one read of the rows: hm_tunes_rows),
+    next to the two stages run separately. The trajectory is stored as
     particle rows (hm_track_rows), which also sums the per-coordinate means that
     both tune stages remove; the FFT reads the rows in place. Bytes: 32 B per
     particle-sample written by the store; 32 B per particle-sample read by each
@@ -259,16 +260,20 @@ def tunes_leg():
         rows, means = C.track_rows(parts, tdev, T, T - 1, c["mu"], c["barrier"], means=True)  # samples 0..2047
         fft = C.history_tunes(rows, n, T, [(0, T - 1)], "fft", row_len=L, means=means)
         birk = C.history_tunes(rows, n, T, [(0, T - 1)], "birkhoff", row_len=L, means=means)
+        birk2, fft2 = C.tunes_rows(rows, n, T, [(0, T - 1)], L, means=means)
         h_ms = C.profile_read("k_history_rows")[0]
         f_ms = sum(C.profile_read(k)[0] for k in ("k_signal_rows", "k_fft_tune_rows", "k_fft_tune_pow2"))
         b_ms = C.profile_read("k_birkhoff_hist")[0]
+        u_ms = C.profile_read("k_tunes_fused")[0]
         C.profile_enable(False)
+        assert np.array_equal(fft2, fft, equal_nan=True)
+        assert np.allclose(birk2, birk, rtol=0, atol=1e-14, equal_nan=True)
         n_read = int(np.all(np.isfinite(means.download(np.float64, 4 * n).reshape(4, n)), axis=0).sum())
         rows.free()
         means.free()
-        if best is None or h_ms + f_ms + b_ms < sum(best):
-            best = (h_ms, f_ms, b_ms)
-    h_ms, f_ms, b_ms = best
+        if best is None or h_ms + f_ms + b_ms + u_ms < sum(best):
+            best = (h_ms, f_ms, b_ms, u_ms)
+    h_ms, f_ms, b_ms, u_ms = best
     nbytes, rbytes = 32.0 * n * T, 32.0 * n_read * T
     fft_flops = 2 * 5 * T * np.log2(T) * n_read
     return {
@@ -277,6 +282,8 @@ def tunes_leg():
         "particles_read": n_read,
         "fft_ms": f_ms, "fft_GBps": rbytes / (f_ms / 1e3) / 1e9, "fft_TFLOPs": fft_flops / (f_ms / 1e3) / 1e12,
         "birkhoff_ms": b_ms, "birkhoff_GBps": rbytes / (b_ms / 1e3) / 1e9,
+        "fused_ms": u_ms, "fused_GBps": rbytes / (u_ms / 1e3) / 1e9,
+        "fused_vs_separate": (f_ms + b_ms) / u_ms,
         "hbm_peak_GBps": HBM_PEAK_GBPS,
         "finite_fft_tunes": float(np.mean(np.isfinite(fft))), "finite_birkhoff_tunes": float(np.mean(np.isfinite(birk))),
     }

@@ -111,6 +111,7 @@ SIGNATURES = {
     "hm_rows_means": (I, [P, SZ, SZ, U32, P, P]),
     "hm_birkhoff_tunes_rows": (I, [P, SZ, SZ, U32, P, P, P, SZ, P, P]),
     "hm_fft_tunes_rows": (I, [P, SZ, SZ, U32, P, P, P, SZ, P, P]),
+    "hm_tunes_rows": (I, [P, SZ, SZ, U32, P, P, P, SZ, P, P, P]),
     "hm_adjoint_forward": (I, [P, P, P, P, SZ, P, SZ, U32, D, D, U32, P, P, P, P, P, I, P]),
     "hm_track_kicked": (I, [P, P, P, P, P, SZ, P, SZ, U32, D, D, D, I, I, ctypes.c_uint64, ctypes.c_uint64, P]),
     "hm_ghost_renormalize": (I, [P, P, P, P, SZ, SZ, D, P]),
@@ -493,3 +494,19 @@ def history_tunes(hist: DeviceBuffer, n: int, n_samples: int, windows, kind="fft
             check(L.hm_birkhoff_tunes_rows(hist.ptr, n, row_len, n_samples, mp, ptr(f), ptr(t), len(windows),
                                            out.ptr, stream))
     return out.download(np.float64, n * 2 * len(windows)).reshape(len(windows), 2, n)
+
+
+def tunes_rows(rows: DeviceBuffer, n: int, n_samples: int, windows, row_len: int, means: DeviceBuffer | None = None,
+               stream=None):
+    """Birkhoff and FFT tunes of the same windows [(from, to)] of particle
+    rows in one call (hm_tunes_rows): numpy ([n_windows][2][n], [n_windows][2][n])."""
+    f = np.ascontiguousarray([w[0] for w in windows], dtype=np.uint32)
+    t = np.ascontiguousarray([w[1] for w in windows], dtype=np.uint32)
+    ob = DeviceBuffer(8 * n * 2 * len(windows))
+    of = DeviceBuffer(8 * n * 2 * len(windows))
+    mp = means.ptr if means is not None else None
+    check(lib().hm_tunes_rows(rows.ptr, n, row_len, n_samples, mp, ptr(f), ptr(t), len(windows), ob.ptr, of.ptr,
+                              stream))
+    shape = (len(windows), 2, n)
+    return (ob.download(np.float64, n * 2 * len(windows)).reshape(shape),
+            of.download(np.float64, n * 2 * len(windows)).reshape(shape))

@@ -683,11 +683,21 @@ class HenonTracker {
                                     allowed_steps, n_turns, mu, barrier, inverse ? 1 : 0, hist.as<double>(), row_len,
                                     all ? means.as<double>() : nullptr, nullptr));
                 const double *mp = all ? means.as<double>() : nullptr;
-                check(hm_birkhoff_tunes_rows(hist.as<double>(), nc, row_len, L, mp, bf.data(), bt.data(), NW,
-                                             ob.as<double>(), nullptr));
-                if (all)
-                    check(hm_fft_tunes_rows(hist.as<double>(), nc, row_len, L, mp, ff.data(), fl.data(), NW,
-                                            of.as<double>(), nullptr));
+                if (all) {
+                    // the given windows and (0, n_turns): both tunes from one read
+                    // of each window; the full signal differs between the two
+                    // (Birkhoff samples 0..n_turns, FFT 1..n_turns)
+                    const size_t last = NW - 1;
+                    check(hm_tunes_rows(hist.as<double>(), nc, row_len, L, mp, bf.data(), bt.data(), last,
+                                        ob.as<double>(), of.as<double>(), nullptr));
+                    check(hm_birkhoff_tunes_rows(hist.as<double>(), nc, row_len, L, mp, bf.data() + last,
+                                                 bt.data() + last, 1, ob.as<double>() + last * 2 * nc, nullptr));
+                    check(hm_fft_tunes_rows(hist.as<double>(), nc, row_len, L, mp, ff.data() + last, fl.data() + last,
+                                            1, of.as<double>() + last * 2 * nc, nullptr));
+                } else {
+                    check(hm_birkhoff_tunes_rows(hist.as<double>(), nc, row_len, L, mp, bf.data(), bt.data(), NW,
+                                                 ob.as<double>(), nullptr));
+                }
                 auto vb = ob.download<double>(nc * 2 * NW);
                 for (size_t w = 0; w < NW; w++)
                     for (int pl = 0; pl < 2; pl++)

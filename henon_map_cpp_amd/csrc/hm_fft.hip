@@ -457,6 +457,8 @@ struct FftRowArgs {
     const double *tw;      // [N] complex (cos, -sin)
     double *out;           // [window][plane][n]
     uint32_t window;
+    const double *bw;      // Birkhoff weights [N - 1] (fused kernel only)
+    double *bout;          // Birkhoff tunes [window][plane][n] (fused kernel only)
 };
 
 // Both planes of one particle per workgroup, radix-2 decimation-in-time in LDS
@@ -613,14 +615,21 @@ __device__ __forceinline__ void argmax_first(double v, uint32_t i, double *sv, u
 
 // One workgroup per (particle, plane) of a batch of signal rows (FftRowArgs):
 // window, mean removal and Hanning at the load, the FFT, magnitudes, first
-// maximum, interpolated tune. N = 2^LOGN, 8 <= N <= 4096.
-template <int LOGN>
+// maximum, interpolated tune. N = 2^LOGN, 8 <= N <= 2048.
+// BIRK: the Birkhoff tune of the same window from the same read of the rows
+// (dynamic_indicator.cpp:81-131 as k_birkhoff_rows): the first pass's loads
+// also give each sample's angle, the angles go through the LDS buffer before
+// the transform uses it (the angle of sample m - 1 comes from its neighbour),
+// and the weighted wrapped differences are summed per thread, then across the
+// workgroup in a fixed order (within rounding of the sample-order sum).
+template <int LOGN, bool BIRK>
 __global__ void __launch_bounds__(kStockThreads) k_fft_tune_stockham(FftRowArgs a)
 {
     constexpr uint32_t N = 1u << LOGN;
     __shared__ Cplx buf[N];
     __shared__ double sv[kStockThreads / 64];
     __shared__ uint32_t si[kStockThreads / 64];
+    __shared__ double sb[kStockThreads / 64];
     __shared__ int any_nan_flag;
     const uint32_t b = blockIdx.x >> 1, plane = blockIdx.x & 1;
     if (b >= a.nb) return;
@@ -633,10 +642,14 @@ __global__ void __launch_bounds__(kStockThreads) k_fft_tune_stockham(FftRowArgs 
     const double mo0 = a.means ? a.means[(2 * (1 - plane)) * nn + pp] : 0.0;
     const double mo1 = a.means ? a.means[(2 * (1 - plane) + 1) * nn + pp] : 0.0;
     double *o = a.out + ((size_t)a.window * 2 + plane) * nn + pp;
+    double *ob = BIRK ? a.bout + ((size_t)a.window * 2 + plane) * nn + pp : nullptr;
     if (any_nan(mx, mp, mo0, mo1)) {
         // a NaN mean makes every sample of its coordinate NaN after the mean
         // removal: the tune is NaN, nothing needs reading
-        if (threadIdx.x == 0) *o = __builtin_nan("");
+        if (threadIdx.x == 0) {
+            *o = __builtin_nan("");
+            if constexpr (BIRK) *ob = __builtin_nan("");
+        }
         return;  // the whole workgroup
     }
     if (threadIdx.x == 0) any_nan_flag = 0;
@@ -661,9 +674,57 @@ __global__ void __launch_bounds__(kStockThreads) k_fft_tune_stockham(FftRowArgs 
     };
     constexpr int R8 = LOGN / 3, REM = LOGN % 3;
     constexpr int R1 = R8 > 0 ? 8 : (REM == 2 ? 4 : 2);
-    // first pass from memory (the same pass twice: a uniform branch)
-    if (scan_other) stockham_pass<LOGN, R1, true>(buf, 1, a.tw, load4);
-    else stockham_pass<LOGN, R1, true>(buf, 1, a.tw, load2);
+    double bacc = 0.0;
+    if constexpr (BIRK) {
+        // first radix-8 pass (Ns = 1, no twiddles) with the angles: thread j
+        // takes samples m = j + r B (B = N / 8 <= 256, one butterfly per thread)
+        static_assert(R1 == 8 && N / 8 <= kStockThreads, "fused kernel: one radix-8 butterfly per thread");
+        constexpr uint32_t B = N / 8;
+        double *th = reinterpret_cast<double *>(buf);  // N angles; buf is not in use yet
+        const uint32_t j = threadIdx.x;
+        Cplx v[8];
+        double t[8];
+        if (j < B) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const uint32_t m = j + r * B;
+                const double vx = rx[m] - mx, vp = rp[m] - mp;
+                nan_seen |= any_nan(vx, vp, 0.0, 0.0);
+                if (scan_other) nan_seen |= any_nan(oth0[m] - mo0, oth1[m] - mo1, 0.0, 0.0);
+                t[r] = atan2(vp, vx);
+                th[m] = t[r];
+                v[r] = Cplx{vx, vp};
+            }
+        }
+        __syncthreads();
+        if (j < B) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const uint32_t m = j + r * B;
+                if (m > 0) {
+                    double d = t[r] - th[m - 1];
+                    if (d < 0.0) d += 2.0 * M_PI;
+                    bacc = bacc + a.bw[m - 1] * d;
+                }
+            }
+        }
+        __syncthreads();  // every angle read before the transform overwrites buf
+        if (j < B) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const double h = a.hann[j + r * B];
+                v[r] = Cplx{v[r].re * h, v[r].im * h};
+            }
+            dft_r<8>(v);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) buf[j * 8 + r] = v[r];
+        }
+        __syncthreads();
+    } else {
+        // first pass from memory (the same pass twice: a uniform branch)
+        if (scan_other) stockham_pass<LOGN, R1, true>(buf, 1, a.tw, load4);
+        else stockham_pass<LOGN, R1, true>(buf, 1, a.tw, load2);
+    }
     uint32_t Ns = R1;
     auto &load = load2;  // unused by the later passes
     if (nan_seen) atomicOr(&any_nan_flag, 1);
@@ -675,8 +736,17 @@ __global__ void __launch_bounds__(kStockThreads) k_fft_tune_stockham(FftRowArgs 
     if constexpr (R8 > 0 && REM == 2) stockham_pass<LOGN, 4, false>(buf, Ns, a.tw, load);
     if constexpr (R8 > 0 && REM == 1) stockham_pass<LOGN, 2, false>(buf, Ns, a.tw, load);
     if (any_nan_flag) {
-        if (threadIdx.x == 0) *o = __builtin_nan("");
+        if (threadIdx.x == 0) {
+            *o = __builtin_nan("");
+            if constexpr (BIRK) *ob = __builtin_nan("");
+        }
         return;
+    }
+    if constexpr (BIRK) {
+        // wavefront sums; argmax_first's barrier publishes them
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) bacc = bacc + __shfl_xor(bacc, off);
+        if ((threadIdx.x & 63) == 0) sb[threadIdx.x >> 6] = bacc;
     }
     // magnitudes and their first maximum; the interpolation recomputes the
     // three magnitudes it needs from the spectrum, which stays in LDS
@@ -696,14 +766,24 @@ __global__ void __launch_bounds__(kStockThreads) k_fft_tune_stockham(FftRowArgs 
     }
     uint32_t bi;
     argmax_first(best, best_i, sv, si, bi);
-    if (threadIdx.x == 0) *o = interp_tune_at(magnitude, N, bi);
+    if (threadIdx.x == 0) {
+        *o = interp_tune_at(magnitude, N, bi);
+        if constexpr (BIRK) {
+            double acc = sb[0];
+            for (uint32_t q = 1; q < kStockThreads / 64; ++q) acc = acc + sb[q];
+            *ob = 1 - acc / (2 * M_PI);
+        }
+    }
 }
 
 // Host tables per length N: Hanning window (dynamic_indicator.cpp:47-52, glibc)
 // and twiddles exp(-2 pi i q / N) in long double, cached on the device.
 struct FftTables {
     double *hann = nullptr, *tw = nullptr;
+    double *bw = nullptr;  // Birkhoff weights of N - 1 differences (the fused kernel's)
 };
+
+static void host_birkhoff_weights(uint32_t n, double *out);
 
 // Signal rows staged per launch pair (bytes); particles are taken in batches that fit.
 static constexpr uint64_t kRowBudget = 4ull << 30;
@@ -765,6 +845,13 @@ static FftTables fft_tables(uint32_t N, hipStream_t s)
     if (pow2) {
         hip_check(hipMalloc(&t.tw, 2 * (size_t)N * sizeof(double)), "hipMalloc(twiddles)");
         hip_check(hipMemcpyAsync(t.tw, tw.data(), 2 * (size_t)N * sizeof(double), hipMemcpyHostToDevice, s),
+                  "hipMemcpyAsync");
+    }
+    if (pow2 && N >= 8 && N <= 2048) {
+        std::vector<double> bw(N - 1);
+        host_birkhoff_weights(N - 1, bw.data());
+        hip_check(hipMalloc(&t.bw, (N - 1) * sizeof(double)), "hipMalloc(Birkhoff weights)");
+        hip_check(hipMemcpyAsync(t.bw, bw.data(), (N - 1) * sizeof(double), hipMemcpyHostToDevice, s),
                   "hipMemcpyAsync");
     }
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
@@ -931,21 +1018,31 @@ extern "C" int hm_birkhoff_tunes_rows(const double *rows, size_t n, size_t row_l
 
 // Launches the FFT of a batch of signal rows: the Stockham kernel for
 // 8 <= N <= 2048 (its signal fits the static LDS), else the radix-2 row kernel.
+template <bool BIRK>
+static void launch_stockham(const FftRowArgs &r, uint32_t logn, hipStream_t s)
+{
+    const dim3 g2(2 * r.nb);
+    switch (logn) {
+    case 3: hipLaunchKernelGGL((k_fft_tune_stockham<3, BIRK>), g2, dim3(kStockThreads), 0, s, r); break;
+    case 4: hipLaunchKernelGGL((k_fft_tune_stockham<4, BIRK>), g2, dim3(kStockThreads), 0, s, r); break;
+    case 5: hipLaunchKernelGGL((k_fft_tune_stockham<5, BIRK>), g2, dim3(kStockThreads), 0, s, r); break;
+    case 6: hipLaunchKernelGGL((k_fft_tune_stockham<6, BIRK>), g2, dim3(kStockThreads), 0, s, r); break;
+    case 7: hipLaunchKernelGGL((k_fft_tune_stockham<7, BIRK>), g2, dim3(kStockThreads), 0, s, r); break;
+    case 8: hipLaunchKernelGGL((k_fft_tune_stockham<8, BIRK>), g2, dim3(kStockThreads), 0, s, r); break;
+    case 9: hipLaunchKernelGGL((k_fft_tune_stockham<9, BIRK>), g2, dim3(kStockThreads), 0, s, r); break;
+    case 10: hipLaunchKernelGGL((k_fft_tune_stockham<10, BIRK>), g2, dim3(kStockThreads), 0, s, r); break;
+    default: hipLaunchKernelGGL((k_fft_tune_stockham<11, BIRK>), g2, dim3(kStockThreads), 0, s, r); break;
+    }
+}
+
 static void launch_row_fft(const FftRowArgs &r, uint32_t N, hipStream_t s)
 {
     const uint32_t logn = 31 - __builtin_clz(N);
-    const dim3 g2(2 * r.nb);
     LaunchTimer timer("k_fft_tune_rows", s);
     switch (logn) {
-    case 3: hipLaunchKernelGGL(k_fft_tune_stockham<3>, g2, dim3(kStockThreads), 0, s, r); break;
-    case 4: hipLaunchKernelGGL(k_fft_tune_stockham<4>, g2, dim3(kStockThreads), 0, s, r); break;
-    case 5: hipLaunchKernelGGL(k_fft_tune_stockham<5>, g2, dim3(kStockThreads), 0, s, r); break;
-    case 6: hipLaunchKernelGGL(k_fft_tune_stockham<6>, g2, dim3(kStockThreads), 0, s, r); break;
-    case 7: hipLaunchKernelGGL(k_fft_tune_stockham<7>, g2, dim3(kStockThreads), 0, s, r); break;
-    case 8: hipLaunchKernelGGL(k_fft_tune_stockham<8>, g2, dim3(kStockThreads), 0, s, r); break;
-    case 9: hipLaunchKernelGGL(k_fft_tune_stockham<9>, g2, dim3(kStockThreads), 0, s, r); break;
-    case 10: hipLaunchKernelGGL(k_fft_tune_stockham<10>, g2, dim3(kStockThreads), 0, s, r); break;
-    case 11: hipLaunchKernelGGL(k_fft_tune_stockham<11>, g2, dim3(kStockThreads), 0, s, r); break;
+    case 3: case 4: case 5: case 6: case 7: case 8: case 9: case 10: case 11:
+        launch_stockham<false>(r, logn, s);
+        break;
     default:
         allow_dynamic_lds((const void *)k_fft_tune_rows, 4 * N * sizeof(double));
         hipLaunchKernelGGL(k_fft_tune_rows, dim3(r.nb), dim3(256), 4 * N * sizeof(double), s, r);
@@ -1053,5 +1150,37 @@ extern "C" int hm_fft_tunes_rows(const double *rows, size_t n, size_t row_len, u
         if (n_samples > row_len) throw Error(HM_ERR_RANGE, "fft_tunes: more samples than row_len");
         fft_impl(rows, particle_rows(row_len), true, n, n_samples, means, from, len, n_windows, out,
                  as_stream(stream));
+    });
+}
+
+extern "C" int hm_tunes_rows(const double *rows, size_t n, size_t row_len, uint32_t n_samples, const double *means,
+                             const uint32_t *from, const uint32_t *to, size_t n_windows, double *birkhoff_out,
+                             double *fft_out, void *stream)
+{
+    return guarded([&] {
+        if (n == 0 || n_windows == 0) return;
+        if (!rows || !from || !to || !birkhoff_out || !fft_out) throw Error(HM_ERR_ARG, "tunes: NULL pointer");
+        if (n_samples > row_len) throw Error(HM_ERR_RANGE, "tunes: more samples than row_len");
+        if (n > 0x3fffffffull) throw Error(HM_ERR_ARG, "tunes: too many particles");
+        hipStream_t s = as_stream(stream);
+        const HistLayout lay = particle_rows(row_len);
+        for (size_t w = 0; w < n_windows; w++) {
+            const uint32_t f = from[w], t = to[w];
+            if (f >= n_samples || t >= n_samples) throw Error(HM_ERR_RANGE, "compute_tune: from and to must be valid indices");
+            if (t < f + 1) throw Error(HM_ERR_ARG, "tunes: a window needs at least two samples");
+            const uint32_t N = t - f + 1;
+            if ((N & (N - 1)) == 0 && N >= 8 && N <= 2048) {
+                // both tunes from one read of the window
+                const FftTables tb = fft_tables(N, s);
+                FftRowArgs r{rows, lay.ps, lay.cs, 0, f, means, (uint32_t)n, 0, (uint32_t)n, N, tb.hann, tb.tw,
+                             fft_out, (uint32_t)w, tb.bw, birkhoff_out};
+                LaunchTimer timer("k_tunes_fused", s);
+                launch_stockham<true>(r, 31 - __builtin_clz(N), s);
+                launch_check("k_tunes_fused");
+            } else {
+                birkhoff_impl(rows, lay, n, n_samples, means, &from[w], &to[w], 1, birkhoff_out + w * 2 * n, s);
+                fft_impl(rows, lay, true, n, n_samples, means, &from[w], &N, 1, fft_out + w * 2 * n, s);
+            }
+        }
     });
 }

@@ -438,6 +438,16 @@ int hm_birkhoff_tunes_rows(const double *rows, size_t n, size_t row_len, uint32_
                            const uint32_t *from, const uint32_t *to, size_t n_windows, double *out, void *stream);
 int hm_fft_tunes_rows(const double *rows, size_t n, size_t row_len, uint32_t n_samples, const double *means,
                       const uint32_t *from, const uint32_t *len, size_t n_windows, double *out, void *stream);
+/* Birkhoff and FFT tunes of the same windows [from, to] of particle rows, as
+ * all_tunes computes both (src/henon.cu:2969-3078: birkhoff_tunes' windows and
+ * the FFT of samples from..to): equal to hm_birkhoff_tunes_rows (within
+ * rounding: another summation order) and hm_fft_tunes_rows (bit for bit) with
+ * len = to - from + 1. A window of 8, 16, ..., 2048 samples is read once for
+ * both (one workgroup per particle and plane); other lengths run the two
+ * separately. birkhoff_out, fft_out: device double[n_windows][2][n]. */
+int hm_tunes_rows(const double *rows, size_t n, size_t row_len, uint32_t n_samples, const double *means,
+                  const uint32_t *from, const uint32_t *to, size_t n_windows, double *birkhoff_out, double *fft_out,
+                  void *stream);
 
 /* ---- reverse-mode derivatives ---------------------------------------------- */
 /* No counterpart in the reference, whose only derivative is the one-turn tangent
