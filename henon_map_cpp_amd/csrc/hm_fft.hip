@@ -683,16 +683,22 @@ __global__ void __launch_bounds__(kStockThreads) k_fft_tune_stockham(FftRowArgs 
         double *th = reinterpret_cast<double *>(buf);  // N angles; buf is not in use yet
         const uint32_t j = threadIdx.x;
         Cplx v[8];
-        double t[8];
+        double hw[8], bw[8];
         if (j < B) {
+            // every load of the pass first (samples, Hanning factors, weights)
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
                 const uint32_t m = j + r * B;
-                const double vx = rx[m] - mx, vp = rp[m] - mp;
-                nan_seen |= any_nan(vx, vp, 0.0, 0.0);
+                v[r] = Cplx{rx[m], rp[m]};
+                hw[r] = a.hann[m];
+                bw[r] = m > 0 ? a.bw[m - 1] : 0.0;
                 if (scan_other) nan_seen |= any_nan(oth0[m] - mo0, oth1[m] - mo1, 0.0, 0.0);
-                t[r] = atan2(vp, vx);
-                th[m] = t[r];
+            }
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const double vx = v[r].re - mx, vp = v[r].im - mp;
+                nan_seen |= any_nan(vx, vp, 0.0, 0.0);
+                th[j + r * B] = atan2(vp, vx);
                 v[r] = Cplx{vx, vp};
             }
         }
@@ -702,19 +708,16 @@ __global__ void __launch_bounds__(kStockThreads) k_fft_tune_stockham(FftRowArgs 
             for (int r = 0; r < 8; ++r) {
                 const uint32_t m = j + r * B;
                 if (m > 0) {
-                    double d = t[r] - th[m - 1];
+                    double d = th[m] - th[m - 1];
                     if (d < 0.0) d += 2.0 * M_PI;
-                    bacc = bacc + a.bw[m - 1] * d;
+                    bacc = bacc + bw[r] * d;
                 }
             }
         }
         __syncthreads();  // every angle read before the transform overwrites buf
         if (j < B) {
 #pragma unroll
-            for (int r = 0; r < 8; ++r) {
-                const double h = a.hann[j + r * B];
-                v[r] = Cplx{v[r].re * h, v[r].im * h};
-            }
+            for (int r = 0; r < 8; ++r) v[r] = Cplx{v[r].re * hw[r], v[r].im * hw[r]};
             dft_r<8>(v);
 #pragma unroll
             for (int r = 0; r < 8; ++r) buf[j * 8 + r] = v[r];
