@@ -554,6 +554,11 @@ __global__ void __launch_bounds__(256) k_fft_tune_rows(FftRowArgs a)
 
 constexpr int kStockThreads = 256;
 
+// LDS position of spectrum entry i: one pad entry after every 16, so that the
+// stores of the first pass (eight entries of 16 bytes apart per thread) fall
+// into distinct banks.
+__device__ __forceinline__ uint32_t stk_pos(uint32_t i) { return i + (i >> 4); }
+
 // One pass: butterflies j = t, t + 256, ... of N / R; inputs from LDS (or, for
 // the first pass, from `load`), outputs to LDS, a barrier between the two.
 template <int LOGN, int R, bool FIRST, class Load>
@@ -567,7 +572,7 @@ __device__ __forceinline__ void stockham_pass(Cplx *buf, uint32_t Ns, const doub
         const uint32_t j = threadIdx.x + q * kStockThreads;
         if (j < B) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = FIRST ? load(j + r * B) : buf[j + r * B];
+            for (int r = 0; r < R; ++r) v[q][r] = FIRST ? load(j + r * B) : buf[stk_pos(j + r * B)];
             const uint32_t k = j % Ns;
             if (!FIRST && k != 0) {
                 const uint32_t step = k * (N / (Ns * R));
@@ -587,7 +592,7 @@ __device__ __forceinline__ void stockham_pass(Cplx *buf, uint32_t Ns, const doub
         if (j < B) {
             const uint32_t k = j % Ns, base = (j / Ns) * Ns * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[base + r * Ns] = v[q][r];
+            for (int r = 0; r < R; ++r) buf[stk_pos(base + r * Ns)] = v[q][r];
         }
     }
     __syncthreads();
@@ -626,7 +631,7 @@ template <int LOGN, bool BIRK>
 __global__ void __launch_bounds__(kStockThreads) k_fft_tune_stockham(FftRowArgs a)
 {
     constexpr uint32_t N = 1u << LOGN;
-    __shared__ Cplx buf[N];
+    __shared__ Cplx buf[N + (N - 1) / 16 + 1];  // entry i at stk_pos(i)
     __shared__ double sv[kStockThreads / 64];
     __shared__ uint32_t si[kStockThreads / 64];
     __shared__ double sb[kStockThreads / 64];
@@ -720,7 +725,7 @@ __global__ void __launch_bounds__(kStockThreads) k_fft_tune_stockham(FftRowArgs 
             for (int r = 0; r < 8; ++r) v[r] = Cplx{v[r].re * hw[r], v[r].im * hw[r]};
             dft_r<8>(v);
 #pragma unroll
-            for (int r = 0; r < 8; ++r) buf[j * 8 + r] = v[r];
+            for (int r = 0; r < 8; ++r) buf[stk_pos(j * 8 + r)] = v[r];
         }
         __syncthreads();
     } else {
@@ -754,7 +759,7 @@ __global__ void __launch_bounds__(kStockThreads) k_fft_tune_stockham(FftRowArgs 
     // magnitudes and their first maximum; the interpolation recomputes the
     // three magnitudes it needs from the spectrum, which stays in LDS
     auto magnitude = [&](uint32_t k) {
-        const Cplx z = buf[k];
+        const Cplx z = buf[stk_pos(k)];
         return sqrt(z.re * z.re + z.im * z.im);
     };
     double best = -1.0;

@@ -47,7 +47,7 @@
 
 namespace hm {
 
-constexpr uint32_t kMrThreads = 256;
+constexpr uint32_t kMrThreads = 512;
 constexpr uint32_t kMrPmax = 2048;  // points of one workgroup's transforms (32 KiB)
 constexpr int kMrMaxPasses = 12;
 
@@ -159,8 +159,17 @@ __device__ __forceinline__ void dft_any(Cplx *v)
     else dft_prime<R>(v);
 }
 
+// LDS position of entry i of a signal: one pad entry after every 16, so that
+// the power-of-two strides of the early passes' stores (R entries of 16 bytes
+// apart when Ns = 1) fall into distinct banks.
+__device__ __forceinline__ uint32_t lp(uint32_t i) { return i + (i >> 4); }
+// LDS entries a signal of L entries spans (lp(L - 1) + 1); the odd variant for
+// the columns kernels, whose loads step across signals.
+__host__ __device__ constexpr uint32_t lds_len(uint32_t L) { return L + (L - 1) / 16 + 1; }
+__host__ __device__ constexpr uint32_t lds_len_odd(uint32_t L) { return lds_len(L) | 1u; }
+
 // One radix-R Stockham pass over G signals of length L in LDS (signal g at
-// buf[g GS ..]); Ns = product of the earlier radices. Butterfly j reads
+// buf[g GS ..], entry i at lp(i)); Ns = product of the earlier radices. Butterfly j reads
 // x[j + r L/R], twiddles by W_L^{k r L/(Ns R)} (k = j mod Ns; one table load, the
 // higher powers by products), takes the R-point DFT and writes element r to
 // (j - k) R + k + r Ns.
@@ -171,16 +180,16 @@ __device__ __forceinline__ void mr_pass(Cplx *buf, uint32_t G, uint32_t L, uint3
     constexpr uint32_t PER = (kMrPmax / R + kMrThreads - 1) / kMrThreads;
     const uint32_t B = L / R, total = G * B, step = L / (Ns * R);
     Cplx v[PER][R];
-    uint32_t base[PER];
+    uint32_t goff[PER], base[PER];
 #pragma unroll
     for (uint32_t q = 0; q < PER; ++q) {
         const uint32_t J = threadIdx.x + q * kMrThreads;
         base[q] = 0xffffffffu;
         if (J < total) {
             const uint32_t g = J / B, j = J - g * B;
-            const Cplx *src = buf + g * GS + j;
+            const Cplx *src = buf + g * GS;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = src[r * B];
+            for (int r = 0; r < R; ++r) v[q][r] = src[lp(j + r * B)];
             const uint32_t k = j % Ns;
             if (k != 0) {
                 const Cplx w1 = tw[k * step];
@@ -192,17 +201,18 @@ __device__ __forceinline__ void mr_pass(Cplx *buf, uint32_t G, uint32_t L, uint3
                 }
             }
             dft_any<R>(v[q]);
-            base[q] = g * GS + (j - k) * R + k;
+            goff[q] = g * GS;
+            base[q] = (j - k) * R + k;
         }
     }
     __syncthreads();
 #pragma unroll
     for (uint32_t q = 0; q < PER; ++q) {
         if (base[q] != 0xffffffffu) {
-            Cplx *dst = buf + base[q];
+            Cplx *dst = buf + goff[q];
             const uint32_t ns = Ns;
 #pragma unroll
-            for (int r = 0; r < R; ++r) dst[r * ns] = v[q][r];
+            for (int r = 0; r < R; ++r) dst[lp(base[q] + r * ns)] = v[q][r];
         }
     }
     __syncthreads();
@@ -229,7 +239,7 @@ __device__ __forceinline__ void mr_fft(Cplx *buf, uint32_t G, const MrPlan &p, u
 __device__ __forceinline__ uint32_t mr_virtual(uint32_t b, uint32_t per) { return (b % 8) * per + b / 8; }
 
 template <bool CZ>
-__global__ void __launch_bounds__(kMrThreads, 3) k_mr_columns(MrArgs a)
+__global__ void __launch_bounds__(kMrThreads, 4) k_mr_columns(MrArgs a)
 {
     extern __shared__ Cplx buf[];
     const uint32_t per_sig = a.N2 / a.C;
@@ -237,7 +247,7 @@ __global__ void __launch_bounds__(kMrThreads, 3) k_mr_columns(MrArgs a)
     const uint32_t v = mr_virtual(blockIdx.x, gridDim.x / 8);
     if (v >= total) return;
     const uint32_t sig = v / per_sig, c0 = (v % per_sig) * a.C;
-    const uint32_t C = a.C, N1 = a.N1, N2 = a.N2, GS = N1 + 1;
+    const uint32_t C = a.C, N1 = a.N1, N2 = a.N2, GS = lds_len_odd(N1);
     const uint32_t particle = a.p0 + sig / 2, plane = sig % 2;
     const size_t nn = a.n;
     double mc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -270,7 +280,7 @@ __global__ void __launch_bounds__(kMrThreads, 3) k_mr_columns(MrArgs a)
             z = Cplx{vx * h, vp * h};
             if constexpr (CZ) z = cmul(z, a.chirp[m]);
         }
-        buf[c * GS + n1] = z;
+        buf[c * GS + lp(n1)] = z;
     }
     if (__any(nan_seen) && (threadIdx.x & 63) == 0) atomicOr(a.flags + sig, 1u);
     __syncthreads();
@@ -279,12 +289,12 @@ __global__ void __launch_bounds__(kMrThreads, 3) k_mr_columns(MrArgs a)
     for (uint32_t e = threadIdx.x; e < C * N1; e += kMrThreads) {
         const uint32_t k1 = e / C, c = e - k1 * C;
         const size_t o = (size_t)k1 * N2 + c0 + c;
-        Y[o] = cmul(buf[c * GS + k1], a.twm[o]);
+        Y[o] = cmul(buf[c * GS + lp(k1)], a.twm[o]);
     }
 }
 
 template <bool CZ>
-__global__ void __launch_bounds__(kMrThreads, 3) k_mr_rows(MrArgs a)
+__global__ void __launch_bounds__(kMrThreads, 4) k_mr_rows(MrArgs a)
 {
     extern __shared__ Cplx buf[];
     const uint32_t per_sig = a.N1 / a.RR;
@@ -292,28 +302,34 @@ __global__ void __launch_bounds__(kMrThreads, 3) k_mr_rows(MrArgs a)
     const uint32_t v = mr_virtual(blockIdx.x, gridDim.x / 8);
     if (v >= total) return;
     const uint32_t sig = v / per_sig, r0 = (v % per_sig) * a.RR;
-    const uint32_t N2 = a.N2, P = a.RR * N2;
+    const uint32_t N2 = a.N2, P = a.RR * N2, GS = lds_len(N2);
+    // row g entry i (global offset e = g N2 + i) at LDS g GS + lp(i)
+    auto at = [&](uint32_t e) {
+        const uint32_t g = e / N2;
+        return g * GS + lp(e - g * N2);
+    };
     Cplx *Y = a.Y + (size_t)sig * a.M + (size_t)r0 * N2;
-    for (uint32_t e = threadIdx.x; e < P; e += kMrThreads) buf[e] = Y[e];
+    for (uint32_t e = threadIdx.x; e < P; e += kMrThreads) buf[at(e)] = Y[e];
     __syncthreads();
-    mr_fft(buf, a.RR, a.p2, N2, a.tw2);
+    mr_fft(buf, a.RR, a.p2, GS, a.tw2);
     if constexpr (CZ) {
         // spectrum entry k1 + N1 k2 times B, conjugated: the inverse transform
         // becomes a forward one over k2 (then k1, in k_mr_final)
         const Cplx *bh = a.bhat + (size_t)r0 * N2;
         for (uint32_t e = threadIdx.x; e < P; e += kMrThreads) {
-            const Cplx z = cmul(buf[e], bh[e]);
-            buf[e] = Cplx{z.re, -z.im};
+            const uint32_t l = at(e);
+            const Cplx z = cmul(buf[l], bh[e]);
+            buf[l] = Cplx{z.re, -z.im};
         }
         __syncthreads();
-        mr_fft(buf, a.RR, a.p2, N2, a.tw2);
+        mr_fft(buf, a.RR, a.p2, GS, a.tw2);
         const Cplx *tm = a.twm + (size_t)r0 * N2;
-        for (uint32_t e = threadIdx.x; e < P; e += kMrThreads) Y[e] = cmul(buf[e], tm[e]);
+        for (uint32_t e = threadIdx.x; e < P; e += kMrThreads) Y[e] = cmul(buf[at(e)], tm[e]);
     } else {
         // spectrum entry k1 + N1 k2 at row k1, position k2
         double *mags = a.mags + (size_t)sig * a.N + (size_t)r0 * N2;
         for (uint32_t e = threadIdx.x; e < P; e += kMrThreads) {
-            const Cplx z = buf[e];
+            const Cplx z = buf[at(e)];
             mags[e] = sqrt(z.re * z.re + z.im * z.im);
         }
     }
@@ -321,7 +337,7 @@ __global__ void __launch_bounds__(kMrThreads, 3) k_mr_rows(MrArgs a)
 
 // Chirp-z only: columns j1 in [c0, c0 + C), the M1-point transforms over k1
 // (output j2) and the magnitudes of the entries j = j1 + N2 j2 < N.
-__global__ void __launch_bounds__(kMrThreads, 3) k_mr_final(MrArgs a)
+__global__ void __launch_bounds__(kMrThreads, 4) k_mr_final(MrArgs a)
 {
     extern __shared__ Cplx buf[];
     const uint32_t per_sig = a.N2 / a.C;
@@ -329,11 +345,11 @@ __global__ void __launch_bounds__(kMrThreads, 3) k_mr_final(MrArgs a)
     const uint32_t v = mr_virtual(blockIdx.x, gridDim.x / 8);
     if (v >= total) return;
     const uint32_t sig = v / per_sig, c0 = (v % per_sig) * a.C;
-    const uint32_t C = a.C, N1 = a.N1, N2 = a.N2, GS = N1 + 1;
+    const uint32_t C = a.C, N1 = a.N1, N2 = a.N2, GS = lds_len_odd(N1);
     const Cplx *Y = a.Y + (size_t)sig * a.M;
     for (uint32_t e = threadIdx.x; e < C * N1; e += kMrThreads) {
         const uint32_t k1 = e / C, c = e - k1 * C;
-        buf[c * GS + k1] = Y[(size_t)k1 * N2 + c0 + c];
+        buf[c * GS + lp(k1)] = Y[(size_t)k1 * N2 + c0 + c];
     }
     __syncthreads();
     mr_fft(buf, C, a.p1, GS, a.tw1);
@@ -342,7 +358,7 @@ __global__ void __launch_bounds__(kMrThreads, 3) k_mr_final(MrArgs a)
         const uint32_t j2 = e / C, c = e - j2 * C;
         const uint32_t j = c0 + c + j2 * N2;
         if (j < a.N) {
-            const Cplx z = buf[c * GS + j2];
+            const Cplx z = buf[c * GS + lp(j2)];
             mags[j] = sqrt(z.re * z.re + z.im * z.im);
         }
     }
@@ -602,8 +618,8 @@ bool fft_tune_mixed(const double *hist, HistLayout lay, uint32_t n, const double
     a.RR = divisor_at_most(a.N1, kMrPmax / a.N2);
     a.hann = hann; a.tw1 = t->tw1; a.tw2 = t->tw2; a.twm = t->twm; a.chirp = t->chirp; a.bhat = t->bhat;
     a.out = out; a.window = window;
-    const size_t lds_cols = (size_t)a.C * (a.N1 + 1) * sizeof(Cplx);
-    const size_t lds_rows = (size_t)a.RR * a.N2 * sizeof(Cplx);
+    const size_t lds_cols = (size_t)a.C * lds_len_odd(a.N1) * sizeof(Cplx);
+    const size_t lds_rows = (size_t)a.RR * lds_len(a.N2) * sizeof(Cplx);
     // scratch per particle (both planes): the M-point work arrays, magnitudes, flags
     const uint64_t per_particle = 2 * ((uint64_t)sizeof(Cplx) * a.M + 8ull * N) + 8;
     // at most 2^22 particles per batch keeps every grid below 2^24 workgroups
