@@ -169,7 +169,7 @@ def indicators_leg(steps, warmup, mode_name="exact"):
             C.profile_reset()
             C.profile_enable(True)
         C.track_indicators(parts, tdev, T, T, c["mu"], c["barrier"],
-                           mode=C.MODE_FAST if mode_name == "fast" else C.MODE_EXACT,
+                           mode={"fast": C.MODE_FAST, "exact_orbit": C.MODE_EXACT_ORBIT}.get(mode_name, C.MODE_EXACT),
                            renorm_every=c["renorm_every"], samples=c["samples"], matrix=True)
         if it >= warmup:
             live += int(parts.get()[4].astype(np.int64).sum())
@@ -685,9 +685,10 @@ def main():
     legs = [x for x in a.legs.split(",") if x and x != "none"]
     if "indicators" in legs:
         line["tangent_megno"] = indicators_leg(2, 1)
-        fast = indicators_leg(1, 0, "fast")
-        line["tangent_megno"]["other_mode"] = {"fast": {k: fast[k] for k in ("value", "live_value", "frac",
-                                                                            "kernel_ms_per_step")}}
+        other = {m: indicators_leg(1, 0, m) for m in ("exact_orbit", "fast")}
+        line["tangent_megno"]["other_mode"] = {m: {k: r[k] for k in ("value", "live_value", "frac",
+                                                                     "kernel_ms_per_step")}
+                                               for m, r in other.items()}
         print(f"[bench] tangent/MEGNO {line['tangent_megno']}", file=sys.stderr, flush=True)
     if "reversibility" in legs:
         line["reversibility"] = reversibility_leg()

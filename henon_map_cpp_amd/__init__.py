@@ -258,7 +258,10 @@ class henon_tracker:
         per 16 samples / 4 windows of each kind). megno_birkhoff and
         tune_birkhoff list turn samples T of megno_birkhoff_construct and
         tune_birkhoff_construct fed every turn of the call: the results equal
-        those constructs' get_values() / get_tune1_x() ... after the same turns."""
+        those constructs' get_values() / get_tune1_x() ... after the same turns.
+        mode: "exact", "fast", or "exact_orbit" (the orbit, losses and Birkhoff
+        tunes of "exact" bit for bit, the tangent products of "fast": MEGNO and
+        FLI to rounding)."""
         return self.tracker.tangent_indicators(particles.particles, n_turns, mu, barrier, renorm_every,
                                                [int(v) for v in samples], list(fli_direction),
                                                [(int(a), int(b)) for a, b in birkhoff],

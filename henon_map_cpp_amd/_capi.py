@@ -15,7 +15,7 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libhenon_hip.so")
 
 HM_OK, HM_ERR_ARG, HM_ERR_RANGE, HM_ERR_HIP, HM_ERR_UNSUPPORTED, HM_ERR_ALLOC = range(6)
-MODE_EXACT, MODE_FAST = 0, 1
+MODE_EXACT, MODE_FAST, MODE_EXACT_ORBIT = 0, 1, 2  # exact orbit: hm_track_indicators only
 OBS_RADIUS, OBS_ACTION, OBS_ACTION_X, OBS_ACTION_Y, OBS_ANGLE_X, OBS_ANGLE_Y = range(6)
 
 D = ctypes.c_double

@@ -417,7 +417,8 @@ int mode_of(const std::string &m)
 {
     if (m == "exact") return HM_MODE_EXACT;
     if (m == "fast") return HM_MODE_FAST;
-    throw std::invalid_argument("mode must be 'exact' or 'fast'");
+    if (m == "exact_orbit") return HM_MODE_EXACT_ORBIT;  // tangent_indicators only
+    throw std::invalid_argument("mode must be 'exact' or 'fast' (or 'exact_orbit' for tangent_indicators)");
 }
 
 // ---------------------------------------------------------------- tracker

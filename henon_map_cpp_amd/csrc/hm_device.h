@@ -20,7 +20,11 @@ struct __attribute__((aligned(32))) TabEntry {
     double sx, cx, sy, cy;
 };
 
-enum Mode : int { kExact = 0, kFast = 1 };
+// kExactOrbit (indicator kernels only): the map and loss test of kExact, the
+// tangent products and norms of kFast.
+enum Mode : int { kExact = 0, kFast = 1, kExactOrbit = 2 };
+constexpr int map_mode(int m) { return m == kFast ? kFast : kExact; }
+constexpr int tangent_mode(int m) { return m == kExact ? kExact : kFast; }
 
 // Pointer into the constant address space: for data that no wavefront writes
 // during the launch. Wave-uniform loads through it become scalar loads.

@@ -196,6 +196,7 @@ __device__ __forceinline__ bool forward_turn_out(double &x, double &px, double &
 template <int MODE, bool OCT, bool EXTRA>
 __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
 {
+    constexpr int MAP = map_mode(MODE), TAN = tangent_mode(MODE);
     const uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;  // position in the live list
     const uint32_t n_live = a.count_in ? *a.count_in : a.n;
     const bool in_range = li < n_live;
@@ -340,9 +341,9 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
                         l2 = l2 + inv_prev * l1;
                         fb2 = fa2;
                     }
-                    const Jac jc = jac_coeffs<MODE>(x, y, mu3, mu6);
-                    mat_mul_structured<MODE>(jc, t, m);
-                    const bool out = forward_turn_out<MODE, OCT>(x, px, y, py, xx, yy, a.mu, m3mu, t, a.b2, b2s);
+                    const Jac jc = jac_coeffs<TAN>(x, y, mu3, mu6);
+                    mat_mul_structured<TAN>(jc, t, m);
+                    const bool out = forward_turn_out<MAP, OCT>(x, px, y, py, xx, yy, a.mu, m3mu, t, a.b2, b2s);
                     done = (alive & out) ? kk : done;
                     alive = alive & !out;
                     w_prev = (double)(kk + 1);
@@ -374,9 +375,9 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
             }
             t = a.tab[ti < a.tab_len ? ti : a.tab_len - 1];
         }
-        const Jac jc = jac_coeffs<MODE>(x, y, mu3, mu6);
-        mat_mul_structured<MODE>(jc, t, m);
-        if (alive && forward_turn_lost<MODE, OCT>(x, px, y, py, xx, yy, a.mu, m3mu, t, a.b2, b2s)) {
+        const Jac jc = jac_coeffs<TAN>(x, y, mu3, mu6);
+        mat_mul_structured<TAN>(jc, t, m);
+        if (alive && forward_turn_lost<MAP, OCT>(x, px, y, py, xx, yy, a.mu, m3mu, t, a.b2, b2s)) {
             x = px = y = py = xx = yy = __builtin_nan("");
 #pragma unroll
             for (int q = 0; q < 16; q++) m[q] = __builtin_nan("");
@@ -406,7 +407,7 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
             renorm_in = a.renorm_every;
             megno_update();
             if (alive) {
-                const double fa = sqrt(frob2<MODE>(m));
+                const double fa = sqrt(frob2<TAN>(m));
                 const double inv = 1.0 / fa;
 #pragma unroll
                 for (int q = 0; q < 16; q++) m[q] = m[q] * inv;
@@ -774,7 +775,8 @@ extern "C" int hm_track_indicators(double *x, double *px, double *y, double *py,
 {
     return guarded([&] {
         if (!spec) throw Error(HM_ERR_ARG, "indicators: spec is NULL");
-        if (mode != HM_MODE_EXACT && mode != HM_MODE_FAST) throw Error(HM_ERR_ARG, "indicators: unknown mode");
+        if (mode != HM_MODE_EXACT && mode != HM_MODE_FAST && mode != HM_MODE_EXACT_ORBIT)
+            throw Error(HM_ERR_ARG, "indicators: unknown mode");
         if (spec->n_samples > (uint32_t)kMaxSamples) throw Error(HM_ERR_ARG, "indicators: at most 16 samples");
         if (spec->n_birkhoff > (uint32_t)kMaxBirk) throw Error(HM_ERR_ARG, "indicators: at most 4 Birkhoff windows");
         for (uint32_t q = 0; q < spec->n_samples; q++) {
@@ -860,6 +862,9 @@ extern "C" int hm_track_indicators(double *x, double *px, double *y, double *py,
         if (mode == HM_MODE_EXACT) {
             if (oct) k = extra ? k_indicators<kExact, true, true> : k_indicators<kExact, true, false>;
             else k = extra ? k_indicators<kExact, false, true> : k_indicators<kExact, false, false>;
+        } else if (mode == HM_MODE_EXACT_ORBIT) {
+            if (oct) k = extra ? k_indicators<kExactOrbit, true, true> : k_indicators<kExactOrbit, true, false>;
+            else k = extra ? k_indicators<kExactOrbit, false, true> : k_indicators<kExactOrbit, false, false>;
         } else {
             if (oct) k = extra ? k_indicators<kFast, true, true> : k_indicators<kFast, true, false>;
             else k = extra ? k_indicators<kFast, false, true> : k_indicators<kFast, false, false>;

@@ -40,7 +40,13 @@ enum {
     HM_ERR_ALLOC = 5         /* device or host allocation failed */
 };
 
-enum { HM_MODE_EXACT = 0, HM_MODE_FAST = 1 };
+/* HM_MODE_EXACT: bit-exact with the reference CPU path (no fused multiply-adds).
+ * HM_MODE_FAST: fused multiply-adds everywhere.
+ * HM_MODE_EXACT_ORBIT (hm_track_indicators only; no reference counterpart):
+ *   the map and the loss test of HM_MODE_EXACT (trajectory, losses and Birkhoff
+ *   tunes bit-exact), the tangent products and norms of HM_MODE_FAST (MEGNO and
+ *   FLI to rounding). */
+enum { HM_MODE_EXACT = 0, HM_MODE_FAST = 1, HM_MODE_EXACT_ORBIT = 2 };
 
 /* Last error message of the calling thread (never NULL).
  * Replaces check_cuda_errors' print-and-exit(1) (src/henon.cu:17-25). */
@@ -346,7 +352,8 @@ typedef struct hm_indicator_spec {
  * (src/henon.cu:2259-2283: L1 += log10(|M_k+1|_F/|M_k|_F) idx, L2 += L1/idx,
  * value L2/idx); FLI = log10|M_t v0| / t; Birkhoff tunes as
  * dynamic_indicator.cpp:81-131 over the (n_turns+1)-sample trajectory; the
- * trajectory window feeds hm_fft_tunes. mode: HM_MODE_EXACT or HM_MODE_FAST. */
+ * trajectory window feeds hm_fft_tunes. mode: HM_MODE_EXACT, HM_MODE_FAST or
+ * HM_MODE_EXACT_ORBIT. */
 int hm_track_indicators(double *x, double *px, double *y, double *py, uint32_t *steps, size_t n,
                         const double *table, size_t table_len, uint32_t n_turns, double mu, double barrier, int mode,
                         const hm_indicator_spec *spec, void *stream);

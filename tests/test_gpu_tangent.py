@@ -1,7 +1,7 @@
 """GPU parity of the tangent-map kernels through the C ABI: the reference's
 trace-form MEGNO, the fused indicator kernel (exact mode: state and tangent
-matrix bit-exact, logarithm-based outputs within 1e-13), fast mode, and the
-per-call construct kernels."""
+matrix bit-exact, logarithm-based outputs within 1e-13), fast mode, exact-orbit
+mode, and the per-call construct kernels."""
 import os
 
 import numpy as np
@@ -77,6 +77,45 @@ def test_indicators_fast_mode_regular_orbits():
     np.testing.assert_allclose(b["megno"], a["megno"], rtol=1e-9, atol=1e-10)
     np.testing.assert_allclose(b["fli"], a["fli"], rtol=1e-9, atol=1e-10)
     np.testing.assert_allclose(b["birkhoff"], a["birkhoff"], atol=1e-11)
+
+
+@pytest.mark.parametrize("mu", [0.0, 0.5])
+def test_indicators_exact_orbit_mode(mu):
+    """Exact orbit: the state, losses, Birkhoff tunes and trajectory window of
+    exact mode bit for bit (chaotic particles and losses included); MEGNO and
+    FLI from fused tangent products, against the oracle's unfused ones."""
+    ot, tab, tdev = tables(3000, 0.168, 0.201, 32.0, "custom")
+    x, px, y, py = W.ball(2000, 0.5, seed=26)
+    samples = [1, 10, 1000, 3000]
+    birk = [(0, 3000), (100, 2148)]
+    de, do = C.DeviceParticles(x, px, y, py), C.DeviceParticles(x, px, y, py)
+    a = C.track_indicators(de, tdev, 3000, 3000, mu, 10.0, C.MODE_EXACT, 100, samples, (1, 1, 0, 0),
+                           matrix=True, birkhoff=birk)
+    b = C.track_indicators(do, tdev, 3000, 3000, mu, 10.0, C.MODE_EXACT_ORBIT, 100, samples, (1, 1, 0, 0),
+                           matrix=True, birkhoff=birk)
+    for u, v in zip(de.get(), do.get()):
+        assert np.array_equal(u, v, equal_nan=True)
+    assert np.array_equal(a["birkhoff"], b["birkhoff"], equal_nan=True)
+    lost = np.isnan(de.get()[0])
+    assert lost.any() and (~lost).any()
+    assert np.array_equal(np.isnan(a["megno"]), np.isnan(b["megno"]))
+    assert not np.array_equal(a["matrix"], b["matrix"], equal_nan=True)  # the products are fused
+    # regular orbits: fused and unfused products agree closely; chaotic ones
+    # amplify the rounding difference like any other perturbation of M
+    want = O.indicators(O.Particles(x, px, y, py), ot, 3000, mu, 10.0, 100, samples, (1, 1, 0, 0),
+                        birkhoff=birk, n_threads=NT)
+    regular = np.all(np.isfinite(want["megno"]), axis=0) & (np.abs(want["megno"][-1]) < 2.5)
+    assert regular.sum() > 100
+    for key in ("megno", "fli"):
+        np.testing.assert_allclose(b[key][:, regular], want[key][:, regular], rtol=1e-9, atol=1e-10)
+
+
+def test_exact_orbit_mode_is_indicators_only():
+    tab = C.DeviceBuffer.from_array(C.build_table(C.sps_modulation(0.168, 0.0, 0, 10),
+                                                  C.sps_modulation(0.201, 0.0, 0, 10)))
+    p = C.DeviceParticles(*W.ball(8, 0.1, seed=1))
+    with pytest.raises(ValueError, match="unknown mode"):
+        C.track(p, tab, 10, 10, 0.0, 10.0, mode=C.MODE_EXACT_ORBIT)
 
 
 def test_history_window():
