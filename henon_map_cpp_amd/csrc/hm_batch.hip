@@ -230,6 +230,51 @@ __device__ __forceinline__ void megno_chunk(const BatchSeg &S, ConstPtr<JobDesc>
     double xx = x * x, yy = y * y;  // carried between turns in exact mode
     const double b2s = screen_bound(b2);
     for (uint32_t k = 0; k < turns && live0; ++k) {
+        if (uniform && pending) {
+            // Plain turns up to the next renormalisation turn, as k_indicators
+            // runs them: one straight block per turn (the previous turn's MEGNO
+            // sums, the matrix product and the map interleave), the next turn's
+            // table record and reciprocal requested a turn ahead. A particle
+            // lost in the run keeps its loss turn and becomes NaN when the run
+            // ends; its MEGNO is NaN whatever the sums hold.
+            uint32_t stop = turns;
+            if (renorm_in) stop = min(stop, k + renorm_in - 1);
+            if (stop > k) {
+                const bool alive_start = alive;
+                TabEntry t = load_record(tp + k);
+                double inv_k = inv_idx[S.seg_start + k];
+                for (uint32_t kk = k; kk < stop; ++kk) {
+                    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this turn's entries
+                    __builtin_amdgcn_sched_barrier(0);
+                    const uint32_t kn = min(kk + 1, turns - 1);
+                    const TabEntry tn = load_record(tp + kn);
+                    const double inv_n = inv_idx[S.seg_start + kn];
+                    {
+                        const double fa2 = frob2_fused(m);
+                        l1 = l1 + (0.5 * log10(fa2 / fb2)) * w_prev;
+                        l2 = l2 + inv_prev * l1;
+                        fb2 = fa2;
+                    }
+                    const Jac jc = jac_coeffs<MODE>(x, y, mu3, mu6);
+                    mat_mul_structured<MODE>(jc, t, m);
+                    const bool out = forward_turn_out<MODE, OCT>(x, px, y, py, xx, yy, mu, m3mu, t, b2, b2s);
+                    done = (alive & out) ? kk : done;
+                    alive = alive & !out;
+                    w_prev = (double)(S.seg_start + kk + 1);
+                    inv_prev = inv_k;
+                    t = tn;
+                    inv_k = inv_n;
+                }
+                if (alive_start && !alive) {
+                    x = px = y = py = xx = yy = __builtin_nan("");
+#pragma unroll
+                    for (int q = 0; q < 16; q++) m[q] = __builtin_nan("");
+                }
+                if (renorm_in) renorm_in -= stop - k;
+                k = stop;
+                if (k == turns || !__any(alive)) break;
+            }
+        }
         if (pending) megno_update();
         TabEntry t;
         if (uniform) {

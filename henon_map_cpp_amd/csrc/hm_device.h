@@ -215,6 +215,23 @@ __device__ __forceinline__ bool forward_turn_lost(double &x, double &px, double 
     return !(radius2<kFast>(x, px, y, py) < b2);
 }
 
+// forward_turn_lost without branches: both radius tests are evaluated (exact
+// mode: the screen, then the reference's expression), so a plain turn stays one
+// basic block. Same decisions as forward_turn_lost.
+template <int MODE, bool OCT>
+__device__ __forceinline__ bool forward_turn_out(double &x, double &px, double &y, double &py, double &xx, double &yy,
+                                                 double mu, double m3mu, const TabEntry &t, double b2, double b2s)
+{
+    if (MODE == kExact) {
+        const double r2 = fwd_exact_carry<OCT>(x, px, y, py, xx, yy, mu, t);
+        const bool screen = !(r2 < b2s);
+        const bool out = !(exact_radius2(xx, px, yy, py) < b2);
+        return screen & out;
+    }
+    map_step<kFast, false, OCT>(x, px, y, py, mu, m3mu, t);
+    return !(radius2<kFast>(x, px, y, py) < b2);
+}
+
 __device__ __forceinline__ bool any_nan(double x, double px, double y, double py)
 {
     return __builtin_isnan(x) || __builtin_isnan(px) || __builtin_isnan(y) || __builtin_isnan(py);

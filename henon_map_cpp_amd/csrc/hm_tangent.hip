@@ -176,23 +176,6 @@ constexpr int kIndState = 20;
 //   Birkhoff windows (sample indices from < m <= to): acc += w[m-1-from] dtheta_m
 //     with dtheta wrapped into [0, 2pi) (dynamic_indicator.cpp:110-117); tune =
 //     1 - acc / 2pi (NaN once a sample of the window is NaN).
-// forward_turn_lost without branches: both radius tests are evaluated (exact
-// mode: the screen, then the reference's expression), so a plain turn stays one
-// basic block. Same decisions as forward_turn_lost.
-template <int MODE, bool OCT>
-__device__ __forceinline__ bool forward_turn_out(double &x, double &px, double &y, double &py, double &xx, double &yy,
-                                                 double mu, double m3mu, const TabEntry &t, double b2, double b2s)
-{
-    if (MODE == kExact) {
-        const double r2 = fwd_exact_carry<OCT>(x, px, y, py, xx, yy, mu, t);
-        const bool screen = !(r2 < b2s);
-        const bool out = !(exact_radius2(xx, px, yy, py) < b2);
-        return screen & out;
-    }
-    map_step<kFast, false, OCT>(x, px, y, py, mu, m3mu, t);
-    return !(radius2<kFast>(x, px, y, py) < b2);
-}
-
 template <int MODE, bool OCT, bool EXTRA>
 __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
 {
