@@ -1026,8 +1026,12 @@ extern "C" int hm_birkhoff_tunes_rows(const double *rows, size_t n, size_t row_l
 
 // Launches the FFT of a batch of signal rows: the Stockham kernel for
 // 8 <= N <= 2048 (its signal fits the static LDS), else the radix-2 row kernel.
+// Particles per launch of the row kernels (one or two workgroups of 256 threads
+// each): grids stay below 2^31 threads.
+static constexpr uint32_t kRowLaunchParticles = 1u << 22;
+
 template <bool BIRK>
-static void launch_stockham(const FftRowArgs &r, uint32_t logn, hipStream_t s)
+static void launch_stockham_part(const FftRowArgs &r, uint32_t logn, hipStream_t s)
 {
     const dim3 g2(2 * r.nb);
     switch (logn) {
@@ -1043,6 +1047,25 @@ static void launch_stockham(const FftRowArgs &r, uint32_t logn, hipStream_t s)
     }
 }
 
+// The launches of `r` in particle ranges of at most kRowLaunchParticles.
+template <class F>
+static void for_row_parts(const FftRowArgs &r, F launch)
+{
+    for (uint32_t b0 = 0; b0 < r.nb; b0 += kRowLaunchParticles) {
+        FftRowArgs q = r;
+        q.row0 = r.row0 + b0;
+        q.p0 = r.p0 + b0;
+        q.nb = std::min<uint32_t>(kRowLaunchParticles, r.nb - b0);
+        launch(q);
+    }
+}
+
+template <bool BIRK>
+static void launch_stockham(const FftRowArgs &r, uint32_t logn, hipStream_t s)
+{
+    for_row_parts(r, [&](const FftRowArgs &q) { launch_stockham_part<BIRK>(q, logn, s); });
+}
+
 static void launch_row_fft(const FftRowArgs &r, uint32_t N, hipStream_t s)
 {
     const uint32_t logn = 31 - __builtin_clz(N);
@@ -1053,7 +1076,9 @@ static void launch_row_fft(const FftRowArgs &r, uint32_t N, hipStream_t s)
         break;
     default:
         allow_dynamic_lds((const void *)k_fft_tune_rows, 4 * N * sizeof(double));
-        hipLaunchKernelGGL(k_fft_tune_rows, dim3(r.nb), dim3(256), 4 * N * sizeof(double), s, r);
+        for_row_parts(r, [&](const FftRowArgs &q) {
+            hipLaunchKernelGGL(k_fft_tune_rows, dim3(q.nb), dim3(256), 4 * N * sizeof(double), s, q);
+        });
     }
     launch_check("k_fft_tune_rows");
 }

@@ -622,9 +622,12 @@ bool fft_tune_mixed(const double *hist, HistLayout lay, uint32_t n, const double
     const size_t lds_rows = (size_t)a.RR * lds_len(a.N2) * sizeof(Cplx);
     // scratch per particle (both planes): the M-point work arrays, magnitudes, flags
     const uint64_t per_particle = 2 * ((uint64_t)sizeof(Cplx) * a.M + 8ull * N) + 8;
-    // at most 2^22 particles per batch keeps every grid below 2^24 workgroups
+    // every grid below 2^31 threads: 2 signals per particle, up to per_sig
+    // workgroups of kMrThreads per signal
+    const uint64_t per_sig = std::max<uint64_t>({1, a.N2 / a.C, a.N1 / a.RR});
+    const uint64_t max_batch = ((1ull << 31) / kMrThreads) / (2 * per_sig);
     const uint32_t batch =
-        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({(uint64_t)n, mr_budget() / per_particle, 1ull << 22}));
+        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({(uint64_t)n, mr_budget() / per_particle, max_batch}));
     const size_t nsig_max = 2 * (size_t)batch;
     const size_t y_bytes = nsig_max * (size_t)a.M * sizeof(Cplx), mag_bytes = nsig_max * (size_t)N * 8;
     char *ws = static_cast<char *>(workspace(y_bytes + mag_bytes + nsig_max * 4 + 256, 3));
