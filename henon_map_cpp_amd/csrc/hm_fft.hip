@@ -177,13 +177,14 @@ struct WindowSet {
 // (dynamic_indicator.cpp:228-241). out: [window][plane][n].
 // One thread per (particle, plane, window): enough wavefronts to hide the
 // latency of the per-sample atan2 chains.
-__global__ void k_birkhoff_hist(const double *__restrict__ hist, HistLayout h, uint32_t n,
+__global__ void k_birkhoff_hist(const double *__restrict__ hist, HistLayout h, uint32_t n, uint32_t p0, uint32_t nb,
                                 const double *__restrict__ means, WindowSet ws, const double *__restrict__ w_flat,
                                 double *__restrict__ out)
 {
+    // particles p0 .. p0 + nb - 1 of n
     const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t i = (uint32_t)(idx % n);
-    const uint64_t rest = idx / n;
+    const uint32_t i = p0 + (uint32_t)(idx % nb);
+    const uint64_t rest = idx / nb;
     const int p = (int)(rest % 2);
     const uint32_t w = (uint32_t)(rest / 2);
     if (w >= ws.count) return;
@@ -223,13 +224,15 @@ __global__ void k_birkhoff_hist(const double *__restrict__ hist, HistLayout h, u
 // order. Same terms as k_birkhoff_hist, summed in that order instead of
 // sample order (within rounding of it).
 __global__ void __launch_bounds__(256) k_birkhoff_rows(const double *__restrict__ rows, uint64_t L, uint32_t n,
-                                                       const double *__restrict__ means, WindowSet ws,
-                                                       const double *__restrict__ w_flat, double *__restrict__ out)
+                                                       uint32_t p0, uint32_t nb, const double *__restrict__ means,
+                                                       WindowSet ws, const double *__restrict__ w_flat,
+                                                       double *__restrict__ out)
 {
+    // particles p0 .. p0 + nb - 1 of n
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t i = (uint32_t)(wave % n);
-    const uint64_t rest = wave / n;
+    const uint32_t i = p0 + (uint32_t)(wave % nb);
+    const uint64_t rest = wave / nb;
     const int p = (int)(rest % 2);
     const uint32_t w = (uint32_t)(rest / 2);
     if (w >= ws.count) return;  // whole wavefronts
@@ -973,6 +976,7 @@ static void birkhoff_impl(const double *hist, HistLayout lay, size_t n, uint32_t
 {
     if (n == 0 || n_windows == 0) return;
     if (!hist || !out || !from || !to) throw Error(HM_ERR_ARG, "birkhoff: NULL pointer");
+    if (n > 0xffffffffull) throw Error(HM_ERR_ARG, "birkhoff: too many particles");
     for (size_t w0 = 0; w0 < n_windows; w0 += kMaxWindows) {
         WindowSet ws{};
         ws.count = (uint32_t)std::min<size_t>(kMaxWindows, n_windows - w0);
@@ -991,16 +995,22 @@ static void birkhoff_impl(const double *hist, HistLayout lay, size_t n, uint32_t
         double *dw = static_cast<double *>(workspace(std::max<size_t>(total, 1) * sizeof(double), 2));
         hip_check(hipMemcpyAsync(dw, wts.data(), total * sizeof(double), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
         LaunchTimer timer("k_birkhoff_hist", s);
-        if (lay.ms == 1) {
-            // particle rows: a wavefront per (particle, plane, window)
-            const uint64_t waves = (uint64_t)n * 2 * ws.count;
-            hipLaunchKernelGGL(k_birkhoff_rows, dim3((uint32_t)((waves + 3) / 4)), dim3(256), 0, s, hist, lay.cs,
-                               (uint32_t)n, means, ws, (const double *)dw, out + w0 * 2 * n);
-        } else {
-            hipLaunchKernelGGL(k_birkhoff_hist, dim3((uint32_t)((n * 2 * ws.count + 255) / 256)), dim3(256), 0, s,
-                               hist, lay, (uint32_t)n, means, ws, (const double *)dw, out + w0 * 2 * n);
+        // particle ranges of at most 2^22 / windows: grids below 2^31 threads
+        const uint32_t pb = std::max<uint32_t>(1, (1u << 22) / ws.count);
+        for (uint32_t p0 = 0; p0 < (uint32_t)n; p0 += pb) {
+            const uint32_t nb = std::min<uint32_t>(pb, (uint32_t)n - p0);
+            if (lay.ms == 1) {
+                // particle rows: a wavefront per (particle, plane, window)
+                const uint64_t waves = (uint64_t)nb * 2 * ws.count;
+                hipLaunchKernelGGL(k_birkhoff_rows, dim3((uint32_t)((waves + 3) / 4)), dim3(256), 0, s, hist, lay.cs,
+                                   (uint32_t)n, p0, nb, means, ws, (const double *)dw, out + w0 * 2 * n);
+            } else {
+                hipLaunchKernelGGL(k_birkhoff_hist, dim3((uint32_t)(((uint64_t)nb * 2 * ws.count + 255) / 256)),
+                                   dim3(256), 0, s, hist, lay, (uint32_t)n, p0, nb, means, ws, (const double *)dw,
+                                   out + w0 * 2 * n);
+            }
+            launch_check("k_birkhoff_hist");
         }
-        launch_check("k_birkhoff_hist");
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     }
 }

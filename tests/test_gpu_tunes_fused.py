@@ -120,10 +120,11 @@ def test_all_tunes_uses_the_fused_windows():
 
 @pytest.mark.parametrize("N,n", [(8, (1 << 22) + 3), (6, (1 << 21) + 5)])
 def test_many_particles_take_several_launches(N, n):
-    """More particles than one launch takes (4194304 per row-kernel launch,
-    2097152 per mixed-radix batch at 6 samples): the particles on both sides of
-    each launch boundary and a random sample against the oracle, and the first
-    thousand against a call of their own."""
+    """More particles than one launch takes (4194304 per row-kernel launch;
+    at 6 samples 2097152 per mixed-radix batch and per Birkhoff launch of two
+    windows): the particles on both sides of each launch boundary and a random
+    sample against the oracle, and the first thousand against a call of their
+    own."""
     rng = np.random.default_rng(N)
     L = 16
     rows = np.zeros((n, 4, L))
@@ -134,16 +135,18 @@ def test_many_particles_take_several_launches(N, n):
         rows[:, 2 * plane, :N], rows[:, 2 * plane + 1, :N] = z.real, z.imag
     rows[:, :, :N] += 1e-3 * rng.normal(size=(n, 4, N))
     d = C.DeviceBuffer.from_array(rows)
-    birk, fft = C.tunes_rows(d, n, N, [(0, N - 1)], L)
+    windows = [(0, N - 1)] if N == 8 else [(0, N - 1), (1, N - 1)]
+    birk, fft = C.tunes_rows(d, n, N, windows, L)
     boundary = (1 << 22) if N == 8 else (1 << 21)
     check = list(range(boundary - 3, min(n, boundary + 3))) + list(rng.integers(0, n, 64)) + [0, n - 1]
     for i in check:
-        seg = rows[i, :, :N]
-        want_f = [O.fft_tune(seg[0], seg[1]), O.fft_tune(seg[2], seg[3])]
-        want_b = [O.birkhoff_tune(seg[0], seg[1]), O.birkhoff_tune(seg[2], seg[3])]
-        np.testing.assert_allclose(fft[0, :, i], want_f, rtol=0, atol=1e-12, err_msg=f"particle {i}")
-        np.testing.assert_allclose(birk[0, :, i], want_b, rtol=0, atol=1e-14, err_msg=f"particle {i}")
+        for w, (a, b) in enumerate(windows):
+            seg = rows[i, :, a:b + 1]
+            want_f = [O.fft_tune(seg[0], seg[1]), O.fft_tune(seg[2], seg[3])]
+            want_b = [O.birkhoff_tune(seg[0], seg[1]), O.birkhoff_tune(seg[2], seg[3])]
+            np.testing.assert_allclose(fft[w, :, i], want_f, rtol=0, atol=1e-12, err_msg=f"particle {i}")
+            np.testing.assert_allclose(birk[w, :, i], want_b, rtol=0, atol=1e-14, err_msg=f"particle {i}")
     small = C.DeviceBuffer.from_array(np.ascontiguousarray(rows[:1000]))
-    b1, f1 = C.tunes_rows(small, 1000, N, [(0, N - 1)], L)
+    b1, f1 = C.tunes_rows(small, 1000, N, windows, L)
     assert np.array_equal(f1, fft[:, :, :1000]) and np.array_equal(b1, birk[:, :, :1000])
     assert np.isfinite(fft).all() and np.isfinite(birk).all()
