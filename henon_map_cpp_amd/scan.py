@@ -8,9 +8,10 @@ mu, barrier=10), and the survival turns saved as dataset "steps" of
 stability_henon_{omega_x}_{omega_y}_{epsilon}_{mu}_{max_radius}.h5 (decimal
 points written as "d"). Here the same lines run as batched jobs
 (henon_map_cpp_amd.track_batch), as many per launch sequence as their
-modulation tables fit in a memory budget, and each job's steps are saved under
-the same name with the .npy suffix (numpy's format: this image has no HDF5
-writer), the same uint32 array.
+modulation tables fit in a memory budget, and each job's steps are saved as the
+reference saves them: dataset "steps" (uint32, little-endian, contiguous) of
+the .h5 file of that name, written through the HDF5 C library
+(henon_map_cpp_amd.h5; fmt="npy" writes numpy's format instead).
 
 Command line, as run_sim.py (one configuration) or for a whole job list:
     python -m henon_map_cpp_amd.scan --omegax 0.168 --omegay 0.201 --epsilon 32 --mu 0.5
@@ -82,14 +83,34 @@ def initial_conditions(max_radius, num_points):
     return x, np.zeros_like(x), y, np.zeros_like(y)
 
 
+def save_steps(path_stem, steps, fmt="h5"):
+    """run_sim.py:55-59: `steps` as dataset "steps" of path_stem + ".h5" (or
+    numpy's path_stem + ".npy" for fmt="npy"); returns the file name."""
+    if fmt == "h5":
+        from . import h5
+
+        path = path_stem + ".h5"
+        h5.write_dataset(path, "steps", np.asarray(steps, dtype=np.uint32))
+    elif fmt == "npy":
+        path = path_stem + ".npy"
+        np.save(path, np.asarray(steps, dtype=np.uint32))
+    else:
+        raise ValueError("fmt must be 'h5' or 'npy'")
+    return path
+
+
 def run_scan(configs, num_points=100, num_iterations=int(1e8), out_dir=".", mode="exact", table_budget=64 << 30,
-             save=True):
+             save=True, fmt="h5"):
     """Tracks every configuration as in run_sim.py and returns {config: steps}.
     Configurations sharing (omega_x, omega_y, epsilon) share one modulation
     table; jobs are batched while their distinct tables fit in table_budget
     bytes (32 bytes per turn each)."""
     from . import henon_tracker, particles, track_batch
 
+    if save and fmt == "h5":
+        from . import h5
+
+        h5.lib()  # no HDF5 library: fail before tracking
     configs = list(configs)
     per_table = 32 * int(num_iterations)
     if per_table > table_budget:
@@ -107,7 +128,7 @@ def run_scan(configs, num_points=100, num_iterations=int(1e8), out_dir=".", mode
             steps = p.get_steps().astype(np.uint32)
             results[c] = steps
             if save:
-                np.save(os.path.join(out_dir, c.file_stem() + ".npy"), steps)
+                save_steps(os.path.join(out_dir, c.file_stem()), steps, fmt)
         batch.clear()
         trackers.clear()
 
@@ -135,13 +156,14 @@ def main(argv=None):
     parser.add_argument("--jobs", default=None, help="job list (compose_config's format): run every line")
     parser.add_argument("--out_dir", default=".")
     parser.add_argument("--mode", default="exact", choices=("exact", "fast"))
+    parser.add_argument("--format", default="h5", choices=("h5", "npy"))
     args = parser.parse_args(argv)
     if args.jobs:
         configs = read_config(args.jobs)
     else:
         configs = [ScanConfig(args.omegax, args.omegay, args.epsilon, args.mu, args.max_radius)]
     start = datetime.datetime.now()
-    run_scan(configs, args.num_points, args.num_iterations, args.out_dir, args.mode)
+    run_scan(configs, args.num_points, args.num_iterations, args.out_dir, args.mode, fmt=args.format)
     print("Elapsed time: {}".format(datetime.datetime.now() - start))
 
 

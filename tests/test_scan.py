@@ -1,11 +1,13 @@
 """The parameter-scan driver (htcondor_jobs/compose_config.py, run_sim.py):
-job-list format and output names on the CPU; on the GPU, a batched scan equal
-to the oracle's tracking of each configuration."""
+job-list format, output names and the HDF5 "steps" file on the CPU; on the GPU,
+a batched scan equal to the oracle's tracking of each configuration, saved as
+HDF5."""
 import os
 
 import numpy as np
 import pytest
 
+from henon_map_cpp_amd import h5
 from henon_map_cpp_amd import scan as S
 
 
@@ -24,6 +26,27 @@ def test_job_list_and_names(tmp_path):
     np.testing.assert_array_equal(y[::4], np.linspace(0, 0.25, 4))
 
 
+def test_steps_file_is_hdf5(tmp_path):
+    """The saved file is HDF5 (signature, read back through the HDF5 library)
+    with dataset "steps" of unsigned 32-bit integers, as h5py writes
+    run_sim.py's output."""
+    steps = np.array([0, 1, 7, 100000, 4294967295], dtype=np.uint32)
+    stem = os.path.join(tmp_path, S.ScanConfig(0.31, 0.32, 0.0, 0.0, 0.25).file_stem())
+    path = S.save_steps(stem, steps)
+    assert path.endswith("stability_henon_0d31_0d32_0d0_0d0_0d25.h5")
+    with open(path, "rb") as f:
+        assert f.read(8) == b"\x89HDF\r\n\x1a\n"
+    got = h5.read_dataset(path, "steps")
+    assert got.dtype == np.uint32 and np.array_equal(got, steps)
+    grid = np.arange(12, dtype=np.float64).reshape(3, 4) / 7
+    h5.write_dataset(os.path.join(tmp_path, "g.h5"), "x", grid)
+    assert np.array_equal(h5.read_dataset(os.path.join(tmp_path, "g.h5"), "x"), grid)
+    with pytest.raises(OSError):
+        h5.read_dataset(path, "missing")
+    npy = S.save_steps(stem, steps, fmt="npy")
+    assert np.array_equal(np.load(npy), steps)
+
+
 @pytest.mark.gpu
 def test_batched_scan_matches_oracle(tmp_path):
     import oracle as O
@@ -37,6 +60,6 @@ def test_batched_scan_matches_oracle(tmp_path):
         p = O.Particles(*S.initial_conditions(c.max_radius, pts))
         t.track(p, n_turns, c.mu, barrier=10.0)
         assert np.array_equal(res[c], p.steps), c
-        saved = np.load(os.path.join(tmp_path, c.file_stem() + ".npy"))
+        saved = h5.read_dataset(os.path.join(tmp_path, c.file_stem() + ".h5"), "steps")
         assert saved.dtype == np.uint32 and np.array_equal(saved, p.steps)
     assert len(set(tuple(v) for v in res.values())) > 1
