@@ -84,7 +84,7 @@ __device__ __forceinline__ void dft3(Cplx *v)
     constexpr double c = -0.5, s = 0.86602540378443864676;  // cos, sin of 2 pi / 3
     const Cplx a = v[0], t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
     v[0] = cadd(a, t1);
-    const Cplx m1 = Cplx{a.re + c * t1.re, a.im + c * t1.im};
+    const Cplx m1 = Cplx{fma(c, t1.re, a.re), fma(c, t1.im, a.im)};
     const Cplx m2 = Cplx{s * t2.im, -(s * t2.re)};
     v[1] = cadd(m1, m2);
     v[2] = csub(m1, m2);
@@ -139,10 +139,10 @@ __device__ __forceinline__ void dft_prime(Cplx *v)
             const int r = (m * k) % R;
             const int f = r <= H ? r : R - r;  // cos t = cos(2 pi f / R), sin t = +-sin(2 pi f / R)
             const double c = root_cos<R>(f), sn = r <= H ? root_sin<R>(f) : -root_sin<R>(f);
-            p.re = p.re + a[m - 1].re * c;
-            p.im = p.im + a[m - 1].im * c;
-            q.re = q.re + b[m - 1].re * sn;
-            q.im = q.im + b[m - 1].im * sn;
+            p.re = fma(a[m - 1].re, c, p.re);
+            p.im = fma(a[m - 1].im, c, p.im);
+            q.re = fma(b[m - 1].re, sn, q.re);
+            q.im = fma(b[m - 1].im, sn, q.im);
         }
         // X_k = p - i q, X_{R-k} = p + i q
         v[k] = Cplx{p.re + q.im, p.im - q.re};

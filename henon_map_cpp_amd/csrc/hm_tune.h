@@ -27,7 +27,12 @@ struct Cplx {
 };
 __device__ __forceinline__ Cplx cadd(Cplx a, Cplx b) { return Cplx{a.re + b.re, a.im + b.im}; }
 __device__ __forceinline__ Cplx csub(Cplx a, Cplx b) { return Cplx{a.re - b.re, a.im - b.im}; }
-__device__ __forceinline__ Cplx cmul(Cplx a, Cplx w) { return Cplx{a.re * w.re - a.im * w.im, a.re * w.im + a.im * w.re}; }
+// complex product with fused multiply-adds (the tune transforms are held to a
+// tolerance, not to the reference's rounding; the files build with contraction off)
+__device__ __forceinline__ Cplx cmul(Cplx a, Cplx w)
+{
+    return Cplx{fma(a.re, w.re, -(a.im * w.im)), fma(a.re, w.im, a.im * w.re)};
+}
 __device__ __forceinline__ Cplx mul_mi(Cplx a) { return Cplx{a.im, -a.re}; }  // times -i
 
 // forward DFT of sizes 2, 4, 8 (exp(-2 pi i m k / R))
