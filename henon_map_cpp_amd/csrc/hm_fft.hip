@@ -623,7 +623,7 @@ __device__ __forceinline__ void argmax_first(double v, uint32_t i, double *sv, u
 
 // One workgroup per (particle, plane) of a batch of signal rows (FftRowArgs):
 // window, mean removal and Hanning at the load, the FFT, magnitudes, first
-// maximum, interpolated tune. N = 2^LOGN, 8 <= N <= 2048.
+// maximum, interpolated tune. N = 2^LOGN, 8 <= N <= 1024 (2048: k_fft_tune_2048).
 // BIRK: the Birkhoff tune of the same window from the same read of the rows
 // (dynamic_indicator.cpp:81-131 as k_birkhoff_rows): the first pass's loads
 // also give each sample's angle, the angles go through the LDS buffer before
@@ -778,6 +778,181 @@ __global__ void __launch_bounds__(kStockThreads) k_fft_tune_stockham(FftRowArgs 
     uint32_t bi;
     argmax_first(best, best_i, sv, si, bi);
     if (threadIdx.x == 0) {
+        *o = interp_tune_at(magnitude, N, bi);
+        if constexpr (BIRK) {
+            double acc = sb[0];
+            for (uint32_t q = 1; q < kStockThreads / 64; ++q) acc = acc + sb[q];
+            *ob = 1 - acc / (2 * M_PI);
+        }
+    }
+}
+
+// 2048-point windows (the chaos-indicator workload's), four-step with one
+// wavefront per quarter: N = 4 x 512, index m = n2 + 512 n1 in, k = k1 + 4 k2
+// out,
+//   X[k1 + 4 k2] = sum_{n2} W_512^{n2 k2} [W_2048^{n2 k1} sum_{n1} x[n2 + 512 n1] W_4^{n1 k1}].
+// Step 1, straight from the rows: thread t takes n2 = t and t + 256, the radix-4
+// DFT over n1 and the twiddle, and stores entry k1 in quarter k1 of LDS. One
+// barrier. Step 2: wavefront w runs the 512-point transform of quarter w (three
+// radix-8 Stockham passes of one butterfly per lane) with no workgroup barrier:
+// a wavefront's LDS operations take effect in program order. Step 3: the
+// magnitudes, each wavefront's first maximum in the spectrum's order, one
+// barrier, the interpolation. Two barriers against nine of the general kernel.
+// BIRK: the Birkhoff tune from the same loads, as k_fft_tune_stockham<.., true>.
+constexpr uint32_t kQuarter = 512 + 32 + 1;  // LDS entries of a quarter (stk_pos(511) + 1, odd)
+
+template <bool BIRK>
+__global__ void __launch_bounds__(kStockThreads) k_fft_tune_2048(FftRowArgs a)
+{
+    constexpr uint32_t N = 2048;
+    __shared__ Cplx buf[4 * kQuarter];
+    __shared__ double sv[kStockThreads / 64];
+    __shared__ uint32_t si[kStockThreads / 64];
+    __shared__ double sb[kStockThreads / 64];
+    __shared__ int any_nan_flag;
+    const uint32_t b = blockIdx.x >> 1, plane = blockIdx.x & 1;
+    if (b >= a.nb) return;
+    const double *base = a.rows + (size_t)((a.row0 + b) * a.pstride + a.from);
+    const double *rx = base + (2 * plane) * a.cstride, *rp = base + (2 * plane + 1) * a.cstride;
+    const double *oth0 = base + (2 * (1 - plane)) * a.cstride, *oth1 = base + (2 * (1 - plane) + 1) * a.cstride;
+    const size_t nn = a.n, pp = a.p0 + b;
+    const double mx = a.means ? a.means[(2 * plane) * nn + pp] : 0.0;
+    const double mp = a.means ? a.means[(2 * plane + 1) * nn + pp] : 0.0;
+    const double mo0 = a.means ? a.means[(2 * (1 - plane)) * nn + pp] : 0.0;
+    const double mo1 = a.means ? a.means[(2 * (1 - plane) + 1) * nn + pp] : 0.0;
+    double *o = a.out + ((size_t)a.window * 2 + plane) * nn + pp;
+    double *ob = BIRK ? a.bout + ((size_t)a.window * 2 + plane) * nn + pp : nullptr;
+    if (any_nan(mx, mp, mo0, mo1)) {
+        // a NaN mean makes every sample of its coordinate NaN: nothing to read
+        if (threadIdx.x == 0) {
+            *o = __builtin_nan("");
+            if constexpr (BIRK) *ob = __builtin_nan("");
+        }
+        return;  // the whole workgroup
+    }
+    if (threadIdx.x == 0) any_nan_flag = 0;
+    __syncthreads();
+    // four finite means: no sample of the particle is NaN, the other plane's
+    // rows need not be read
+    const bool scan_other = !a.means || !(isfinite(mx) && isfinite(mp) && isfinite(mo0) && isfinite(mo1));
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    bool nan_seen = false;
+    // ---- step 1: samples m = n2 + 512 n1 (n2 = t + 256 h), loads first
+    Cplx v[2][4];
+    double hw[2][4], bw[2][4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int n1 = 0; n1 < 4; ++n1) {
+            const uint32_t m = t + 256 * h + 512 * n1;
+            v[h][n1] = Cplx{rx[m], rp[m]};
+            hw[h][n1] = a.hann[m];
+            bw[h][n1] = (BIRK && m > 0) ? a.bw[m - 1] : 0.0;
+            if (scan_other) nan_seen |= any_nan(oth0[m] - mo0, oth1[m] - mo1, 0.0, 0.0);
+        }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int n1 = 0; n1 < 4; ++n1) {
+            v[h][n1] = Cplx{v[h][n1].re - mx, v[h][n1].im - mp};
+            nan_seen |= any_nan(v[h][n1].re, v[h][n1].im, 0.0, 0.0);
+        }
+    double bacc = 0.0;
+    if constexpr (BIRK) {
+        // the angles through LDS (not in use yet): the angle of sample m - 1
+        double *th = reinterpret_cast<double *>(buf);
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int n1 = 0; n1 < 4; ++n1) th[t + 256 * h + 512 * n1] = atan2(v[h][n1].im, v[h][n1].re);
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int n1 = 0; n1 < 4; ++n1) {
+                const uint32_t m = t + 256 * h + 512 * n1;
+                if (m > 0) {
+                    double d = th[m] - th[m - 1];
+                    if (d < 0.0) d += 2.0 * M_PI;
+                    bacc = bacc + bw[h][n1] * d;
+                }
+            }
+        __syncthreads();  // every angle read before the transform overwrites buf
+    }
+    if (nan_seen) atomicOr(&any_nan_flag, 1);
+    // ---- step 1
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t n2 = t + 256 * h;
+        Cplx *x4 = v[h];
+#pragma unroll
+        for (int n1 = 0; n1 < 4; ++n1) x4[n1] = Cplx{x4[n1].re * hw[h][n1], x4[n1].im * hw[h][n1]};
+        dft4(x4[0], x4[1], x4[2], x4[3]);
+#pragma unroll
+        for (int k1 = 1; k1 < 4; ++k1) {
+            const uint32_t q = (n2 * k1) & (N - 1);
+            x4[k1] = cmul(x4[k1], Cplx{a.tw[2 * q], a.tw[2 * q + 1]});
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) buf[k1 * kQuarter + stk_pos(n2)] = x4[k1];
+    }
+    __syncthreads();
+    if (any_nan_flag) {
+        if (t == 0) {
+            *o = __builtin_nan("");
+            if constexpr (BIRK) *ob = __builtin_nan("");
+        }
+        return;
+    }
+    // ---- step 2: wavefront w, the 512-point transform of quarter w
+    Cplx *qb = buf + w * kQuarter;
+    uint32_t Ns = 1;
+#pragma unroll
+    for (int pass = 0; pass < 3; ++pass) {
+        const uint32_t j = lane, k = j % Ns;
+        Cplx u[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) u[r] = qb[stk_pos(j + 64 * r)];
+        if (k != 0) {
+            // W_512^{k r 512 / (8 Ns)} = W_2048^{4 k r 64 / Ns}, exact table entries
+            const uint32_t step = 4 * k * (64 / Ns);
+#pragma unroll
+            for (int r = 1; r < 8; ++r) {
+                const uint32_t q = (step * r) & (N - 1);
+                u[r] = cmul(u[r], Cplx{a.tw[2 * q], a.tw[2 * q + 1]});
+            }
+        }
+        dft_r<8>(u);
+        __builtin_amdgcn_wave_barrier();  // this pass's reads before its writes
+        const uint32_t dst = (j - k) * 8 + k;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) qb[stk_pos(dst + r * Ns)] = u[r];
+        __builtin_amdgcn_wave_barrier();  // the writes before the next pass's reads
+        Ns *= 8;
+    }
+    // ---- step 3: quarter w holds X[w + 4 k2] at stk_pos(k2)
+    auto magnitude = [&](uint32_t kk) {
+        const Cplx z = buf[(kk & 3) * kQuarter + stk_pos(kk >> 2)];
+        return sqrt(z.re * z.re + z.im * z.im);
+    };
+    double best = -1.0;
+    uint32_t best_i = 0xffffffffu;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const uint32_t k2 = lane + 64 * q;
+        const Cplx z = qb[stk_pos(k2)];
+        const double mg = sqrt(z.re * z.re + z.im * z.im);
+        const uint32_t kk = w + 4 * k2;  // ascending in q: the first of equal values stays
+        if (mg > best || best_i == 0xffffffffu) { best = mg; best_i = kk; }
+    }
+    if constexpr (BIRK) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) bacc = bacc + __shfl_xor(bacc, off);
+        if (lane == 0) sb[w] = bacc;
+    }
+    uint32_t bi;
+    argmax_first(best, best_i, sv, si, bi);  // its barrier also publishes every quarter
+    if (t == 0) {
         *o = interp_tune_at(magnitude, N, bi);
         if constexpr (BIRK) {
             double acc = sb[0];
@@ -1035,7 +1210,8 @@ extern "C" int hm_birkhoff_tunes_rows(const double *rows, size_t n, size_t row_l
 }
 
 // Launches the FFT of a batch of signal rows: the Stockham kernel for
-// 8 <= N <= 2048 (its signal fits the static LDS), else the radix-2 row kernel.
+// 8 <= N <= 1024 and the four-step kernel for 2048 (the signal fits the static
+// LDS), else the radix-2 row kernel.
 // Particles per launch of the row kernels (one or two workgroups of 256 threads
 // each): grids stay below 2^31 threads.
 static constexpr uint32_t kRowLaunchParticles = 1u << 22;
@@ -1053,7 +1229,7 @@ static void launch_stockham_part(const FftRowArgs &r, uint32_t logn, hipStream_t
     case 8: hipLaunchKernelGGL((k_fft_tune_stockham<8, BIRK>), g2, dim3(kStockThreads), 0, s, r); break;
     case 9: hipLaunchKernelGGL((k_fft_tune_stockham<9, BIRK>), g2, dim3(kStockThreads), 0, s, r); break;
     case 10: hipLaunchKernelGGL((k_fft_tune_stockham<10, BIRK>), g2, dim3(kStockThreads), 0, s, r); break;
-    default: hipLaunchKernelGGL((k_fft_tune_stockham<11, BIRK>), g2, dim3(kStockThreads), 0, s, r); break;
+    default: hipLaunchKernelGGL((k_fft_tune_2048<BIRK>), g2, dim3(kStockThreads), 0, s, r); break;
     }
 }
 
