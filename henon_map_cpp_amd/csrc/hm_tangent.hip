@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "hm_device.h"
@@ -158,8 +159,10 @@ struct IndicatorArgs {
     uint32_t first, last;
 };
 
-// Per-particle state between segments: M (16), log10 scale, L1, L2, |M|_F^2.
-constexpr int kIndState = 20;
+// Per-particle state between segments: M (16), log10 scale, L1, L2, |M|_F^2,
+// and for the Birkhoff-window variant the last angles (2) and the window sums
+// (2 per window).
+constexpr int kIndState = 20 + 2 + 2 * kMaxBirk;
 
 
 // One thread per particle; the table index of turn k is steps+k for both the
@@ -176,9 +179,14 @@ constexpr int kIndState = 20;
 //   Birkhoff windows (sample indices from < m <= to): acc += w[m-1-from] dtheta_m
 //     with dtheta wrapped into [0, 2pi) (dynamic_indicator.cpp:110-117); tune =
 //     1 - acc / 2pi (NaN once a sample of the window is NaN).
-template <int MODE, bool OCT, bool EXTRA>
+// Variants V: 0 MEGNO/FLI only; 1 with Birkhoff windows (accumulated inside the
+// straight runs of plain turns, kept in the state between segments); 2 with
+// any of the trajectory window and the construct windows (turn by turn, one
+// launch).
+template <int MODE, bool OCT, int V>
 __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
 {
+    constexpr bool EXTRA = V == 2, BW = V >= 1;
     constexpr int MAP = map_mode(MODE), TAN = tangent_mode(MODE);
     const uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;  // position in the live list
     const uint32_t n_live = a.count_in ? *a.count_in : a.n;
@@ -193,11 +201,20 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
     }
     double m[16];
     double lscale = 0.0, l1 = 0.0, l2 = 0.0, f0;
+    double bacc[kMaxBirk][2];
+#pragma unroll
+    for (int w = 0; w < kMaxBirk; w++) bacc[w][0] = bacc[w][1] = 0.0;
+    double thx = 0.0, thy = 0.0;
     if (!a.first && in_range) {
         const double *st = a.state + (size_t)i * kIndState;
 #pragma unroll
         for (int q = 0; q < 16; q++) m[q] = st[q];
         lscale = st[16]; l1 = st[17]; l2 = st[18]; f0 = st[19];
+        if (BW && a.n_birk) {
+            thx = st[20]; thy = st[21];
+#pragma unroll
+            for (int w = 0; w < kMaxBirk; w++) { bacc[w][0] = st[22 + 2 * w]; bacc[w][1] = st[23 + 2 * w]; }
+        }
     } else {
         if (a.matrix && in_range) {
 #pragma unroll
@@ -220,11 +237,7 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
 #pragma unroll
         for (int q = 0; q < 4; q++) v0[q] = a.fli_dir[q] / nn;
     }
-    double bacc[kMaxBirk][2];
-#pragma unroll
-    for (int w = 0; w < kMaxBirk; w++) bacc[w][0] = bacc[w][1] = 0.0;
-    double thx = 0.0, thy = 0.0;
-    if (EXTRA && a.n_birk) {
+    if (BW && a.n_birk && a.first) {
         thx = atan2(px, x);
         thy = atan2(py, y);
     }
@@ -293,6 +306,23 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
         }
         pending = false;
     };
+    // Birkhoff windows after turn idx - 1 (the sample idx is the state now)
+    auto birkhoff_turn = [&](uint32_t idx) {
+        const double nx = atan2(px, x), ny = atan2(py, y);
+        double dx = nx - thx, dy = ny - thy;
+        if (dx < 0.0) dx += 2.0 * M_PI;
+        if (dy < 0.0) dy += 2.0 * M_PI;
+        thx = nx;
+        thy = ny;
+#pragma unroll
+        for (int w = 0; w < kMaxBirk; w++) {
+            if (w < (int)a.n_birk && idx > a.birk_from[w] && idx <= a.birk_to[w]) {
+                const double wt = a.birk_w[a.birk_off[w] + (idx - 1 - a.birk_from[w])];
+                bacc[w][0] = bacc[w][0] + wt * dx;
+                bacc[w][1] = bacc[w][1] + wt * dy;
+            }
+        }
+    };
     for (; k < k1; ++k) {
         if (!EXTRA && uniform && pending) {
             // Plain turns up to the next event (an FLI sample at its turn, a
@@ -329,6 +359,7 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
                     const bool out = forward_turn_out<MAP, OCT>(x, px, y, py, xx, yy, a.mu, m3mu, t, a.b2, b2s);
                     done = (alive & out) ? kk : done;
                     alive = alive & !out;
+                    if (BW && a.n_birk) birkhoff_turn(kk + 1);
                     w_prev = (double)(kk + 1);
                     inv_prev = inv_k;
                     idx_prev = kk + 1;
@@ -339,6 +370,14 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
                     x = px = y = py = xx = yy = __builtin_nan("");
 #pragma unroll
                     for (int q = 0; q < 16; q++) m[q] = __builtin_nan("");
+                    // the run fed the lost particle's windows with its values after
+                    // the loss; sample done + 1 on is NaN, as are those windows
+                    if (BW && a.n_birk) {
+                        thx = thy = __builtin_nan("");
+#pragma unroll
+                        for (int w = 0; w < kMaxBirk; w++)
+                            if (w < (int)a.n_birk && a.birk_to[w] >= done + 1) bacc[w][0] = bacc[w][1] = __builtin_nan("");
+                    }
                 }
                 if (renorm_in) renorm_in -= stop - k;
                 k = stop;
@@ -398,22 +437,7 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
                 fb2 = frob2_fused(m);
             }
         }
-        if (EXTRA && a.n_birk) {
-            const double nx = atan2(px, x), ny = atan2(py, y);
-            double dx = nx - thx, dy = ny - thy;
-            if (dx < 0.0) dx += 2.0 * M_PI;
-            if (dy < 0.0) dy += 2.0 * M_PI;
-            thx = nx;
-            thy = ny;
-#pragma unroll
-            for (int w = 0; w < kMaxBirk; w++) {
-                if (w < (int)a.n_birk && idx > a.birk_from[w] && idx <= a.birk_to[w]) {
-                    const double wt = a.birk_w[a.birk_off[w] + (idx - 1 - a.birk_from[w])];
-                    bacc[w][0] = bacc[w][0] + wt * dx;
-                    bacc[w][1] = bacc[w][1] + wt * dy;
-                }
-            }
-        }
+        if (BW && a.n_birk) birkhoff_turn(idx);
         if (EXTRA && a.n_tb) {
             // tune_birkhoff_construct's add after this turn (add index idx)
             const double nx = atan2(px, x), ny = atan2(py, y);
@@ -471,6 +495,11 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
 #pragma unroll
         for (int q = 0; q < 16; q++) st[q] = m[q];
         st[16] = lscale; st[17] = l1; st[18] = l2; st[19] = fb2;
+        if (BW && a.n_birk) {
+            st[20] = thx; st[21] = thy;
+#pragma unroll
+            for (int w = 0; w < kMaxBirk; w++) { st[22 + 2 * w] = bacc[w][0]; st[23 + 2 * w] = bacc[w][1]; }
+        }
         return;
     }
     // samples not reached because the particle (or its whole wavefront) was lost
@@ -485,9 +514,13 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
     if (a.log_scale) a.log_scale[i] = lscale;
 #pragma unroll
     for (int w = 0; w < kMaxBirk; w++) {
-        if (EXTRA && w < (int)a.n_birk) {
-            a.birk_out[((size_t)w * 2 + 0) * a.n + i] = 1.0 - bacc[w][0] / (2 * M_PI);
-            a.birk_out[((size_t)w * 2 + 1) * a.n + i] = 1.0 - bacc[w][1] / (2 * M_PI);
+        if (BW && w < (int)a.n_birk) {
+            // a window holding a NaN sample is NaN: the particle was NaN from the
+            // start, or lost at turn done (sample done + 1 on), which a wavefront
+            // that stopped early did not accumulate
+            const bool nan_w = !alive0 || (done < k1 && a.birk_to[w] >= done + 1);
+            a.birk_out[((size_t)w * 2 + 0) * a.n + i] = nan_w ? __builtin_nan("") : 1.0 - bacc[w][0] / (2 * M_PI);
+            a.birk_out[((size_t)w * 2 + 1) * a.n + i] = nan_w ? __builtin_nan("") : 1.0 - bacc[w][1] / (2 * M_PI);
         }
         if (EXTRA && w < (int)a.n_mb) a.mb_out[(size_t)w * a.n + i] = mbacc[w];
         if (EXTRA && w < (int)a.n_tb)
@@ -837,21 +870,21 @@ extern "C" int hm_track_indicators(double *x, double *px, double *y, double *py,
         a.status = status_word();
         const bool oct = mu != 0.0;
         LaunchTimer timer("k_indicators", s);
-        // Birkhoff windows and the trajectory window are compiled into their own
-        // variants, so the MEGNO/FLI-only kernel carries none of their code
-        const bool extra = a.n_birk || a.hist_len || a.n_mb || a.n_tb;
+        // Birkhoff windows, and the trajectory and construct windows, are
+        // compiled into their own variants, so the MEGNO/FLI-only kernel
+        // carries none of their code; only the last runs turn by turn in one
+        // launch
+        const bool extra = a.hist_len || a.n_mb || a.n_tb;
+        const int V = extra ? 2 : (a.n_birk ? 1 : 0);
         using K = void (*)(IndicatorArgs);
-        K k = nullptr;
-        if (mode == HM_MODE_EXACT) {
-            if (oct) k = extra ? k_indicators<kExact, true, true> : k_indicators<kExact, true, false>;
-            else k = extra ? k_indicators<kExact, false, true> : k_indicators<kExact, false, false>;
-        } else if (mode == HM_MODE_EXACT_ORBIT) {
-            if (oct) k = extra ? k_indicators<kExactOrbit, true, true> : k_indicators<kExactOrbit, true, false>;
-            else k = extra ? k_indicators<kExactOrbit, false, true> : k_indicators<kExactOrbit, false, false>;
-        } else {
-            if (oct) k = extra ? k_indicators<kFast, true, true> : k_indicators<kFast, true, false>;
-            else k = extra ? k_indicators<kFast, false, true> : k_indicators<kFast, false, false>;
-        }
+        auto pick = [&](auto mode_tag) -> K {
+            constexpr int M = decltype(mode_tag)::value;
+            if (oct) return V == 2 ? k_indicators<M, true, 2> : V == 1 ? k_indicators<M, true, 1> : k_indicators<M, true, 0>;
+            return V == 2 ? k_indicators<M, false, 2> : V == 1 ? k_indicators<M, false, 1> : k_indicators<M, false, 0>;
+        };
+        K k = mode == HM_MODE_EXACT       ? pick(std::integral_constant<int, kExact>{})
+              : mode == HM_MODE_EXACT_ORBIT ? pick(std::integral_constant<int, kExactOrbit>{})
+                                            : pick(std::integral_constant<int, kFast>{});
         a.first = a.last = 1;
         a.k0 = 0;
         a.k1 = n_turns;

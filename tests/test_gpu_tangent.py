@@ -110,6 +110,35 @@ def test_indicators_exact_orbit_mode(mu):
         np.testing.assert_allclose(b[key][:, regular], want[key][:, regular], rtol=1e-9, atol=1e-10)
 
 
+@pytest.mark.parametrize("renorm", [0, 100])
+def test_birkhoff_window_variant_equals_turn_by_turn(renorm):
+    """Birkhoff windows alone run the segmented kernel with the windows summed
+    inside its straight runs of plain turns (losses inside a run, windows that
+    end inside a run, a NaN start); adding a construct window selects the
+    turn-by-turn kernel. Every output agrees bit for bit."""
+    ot, tab, tdev = tables(3000, 0.168, 0.201, 32.0, "custom")
+    x, px, y, py = W.ball(3000, 0.55, seed=27)
+    x[5] = np.nan
+    samples = [10, 1000, 3000]
+    birk = [(0, 3000), (100, 2148), (1500, 1537), (7, 400)]
+    a = C.track_indicators(C.DeviceParticles(x, px, y, py), tdev, 3000, 3000, 0.5, 10.0, C.MODE_EXACT, renorm,
+                           samples, (1, 1, 0, 0), matrix=True, birkhoff=birk)
+    dv = C.DeviceParticles(x, px, y, py)
+    b = C.track_indicators(dv, tdev, 3000, 3000, 0.5, 10.0, C.MODE_EXACT, renorm, samples, (1, 1, 0, 0),
+                           matrix=True, birkhoff=birk, tune_birkhoff=[50])
+    for key in ("megno", "fli", "matrix", "birkhoff"):
+        assert np.array_equal(a[key], b[key], equal_nan=True), key
+    lost = np.isnan(dv.get()[0])
+    steps = dv.get()[4]
+    # particles lost inside each window's range, and windows finished before the loss
+    assert np.any(lost & (steps >= 1500) & (steps < 1537))
+    assert np.isnan(a["birkhoff"][:, :, 5]).all()
+    ok = ~lost
+    assert np.isfinite(a["birkhoff"][:, :, ok]).all()
+    early = lost & (steps > 500)
+    assert np.isfinite(a["birkhoff"][3, :, early]).all() and np.isnan(a["birkhoff"][0, :, early]).all()
+
+
 def test_exact_orbit_mode_is_indicators_only():
     tab = C.DeviceBuffer.from_array(C.build_table(C.sps_modulation(0.168, 0.0, 0, 10),
                                                   C.sps_modulation(0.201, 0.0, 0, 10)))
