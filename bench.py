@@ -521,8 +521,8 @@ def batch_leg(n_jobs, max_turns=1_000_000, plain_stride=16):
 def host_cpu_info():
     """What the host offers: the reference's thread count
     (std::thread::hardware_concurrency() = os.cpu_count()), the CPUs this
-    process may run on, the cgroup CPU quota, the box's CPU share
-    (OMP_NUM_THREADS) and the CPU model."""
+    process may run on, the cgroup CPU quota, the host's CPU share given in
+    OMP_NUM_THREADS, and the CPU model."""
     import subprocess
 
     info = {"hardware_concurrency": os.cpu_count()}
@@ -559,7 +559,8 @@ def cpu_baseline(stride, turns):
     `stride`-th particle of the same grid, with the reference's threading model
     (one std::thread per hardware thread, particles strided over threads,
     henon.cu:2701-2752). The thread count is hardware_concurrency() limited to
-    the CPUs this process can use (affinity, cgroup quota, the box's CPU share):
+    the CPUs this process can use (affinity, cgroup quota, the CPU share given
+    in OMP_NUM_THREADS):
     more threads than that only time-slice. A one-thread run on a smaller sample
     gives the per-core rate; `projected_full_host_value` scales it to every
     hardware thread (linear scaling, a projection, not a measurement)."""
