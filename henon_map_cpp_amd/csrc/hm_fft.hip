@@ -888,11 +888,11 @@ __global__ void __launch_bounds__(kStockThreads) k_fft_tune_2048(FftRowArgs a)
 #pragma unroll
         for (int n1 = 0; n1 < 4; ++n1) x4[n1] = Cplx{x4[n1].re * hw[h][n1], x4[n1].im * hw[h][n1]};
         dft4(x4[0], x4[1], x4[2], x4[3]);
-#pragma unroll
-        for (int k1 = 1; k1 < 4; ++k1) {
-            const uint32_t q = (n2 * k1) & (N - 1);
-            x4[k1] = cmul(x4[k1], Cplx{a.tw[2 * q], a.tw[2 * q + 1]});
-        }
+        // W_2048^{n2 k1}: one table load, its square and cube by products
+        const Cplx w1{a.tw[2 * n2], a.tw[2 * n2 + 1]}, w2 = cmul(w1, w1);
+        x4[1] = cmul(x4[1], w1);
+        x4[2] = cmul(x4[2], w2);
+        x4[3] = cmul(x4[3], cmul(w2, w1));
 #pragma unroll
         for (int k1 = 0; k1 < 4; ++k1) buf[k1 * kQuarter + stk_pos(n2)] = x4[k1];
     }
@@ -914,12 +914,15 @@ __global__ void __launch_bounds__(kStockThreads) k_fft_tune_2048(FftRowArgs a)
 #pragma unroll
         for (int r = 0; r < 8; ++r) u[r] = qb[stk_pos(j + 64 * r)];
         if (k != 0) {
-            // W_512^{k r 512 / (8 Ns)} = W_2048^{4 k r 64 / Ns}, exact table entries
+            // W_512^{k r 512 / (8 Ns)} = W_2048^{4 k r 64 / Ns}: one table load,
+            // the higher powers by products (relative error below 8 ulps)
             const uint32_t step = 4 * k * (64 / Ns);
+            const Cplx w1{a.tw[2 * step], a.tw[2 * step + 1]};
+            Cplx wr = w1;
 #pragma unroll
             for (int r = 1; r < 8; ++r) {
-                const uint32_t q = (step * r) & (N - 1);
-                u[r] = cmul(u[r], Cplx{a.tw[2 * q], a.tw[2 * q + 1]});
+                u[r] = cmul(u[r], wr);
+                if (r + 1 < 8) wr = cmul(wr, w1);
             }
         }
         dft_r<8>(u);
