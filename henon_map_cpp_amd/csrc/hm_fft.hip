@@ -550,8 +550,9 @@ __global__ void __launch_bounds__(256) k_fft_tune_rows(FftRowArgs a)
 // a final radix-2 or radix-4 pass (N = 2048: 8 8 8 4), butterflies in
 // registers, the signal in LDS between passes. Pass with radix R and span Ns
 // (the product of the earlier radices): butterfly j reads x[j + r N/R],
-// multiplies by w^(k r N/(Ns R)) (k = j mod Ns, w = exp(-2 pi i / N), from the
-// twiddle table), takes the R-point DFT and writes element r to
+// multiplies by w^(k r N/(Ns R)) (k = j mod Ns, w = exp(-2 pi i / N); one
+// twiddle from the table, its powers by products), takes the R-point DFT and
+// writes element r to
 // (j / Ns) Ns R + k + r Ns; after the last pass the spectrum is in natural
 // order.
 
@@ -578,11 +579,15 @@ __device__ __forceinline__ void stockham_pass(Cplx *buf, uint32_t Ns, const doub
             for (int r = 0; r < R; ++r) v[q][r] = FIRST ? load(j + r * B) : buf[stk_pos(j + r * B)];
             const uint32_t k = j % Ns;
             if (!FIRST && k != 0) {
+                // w^r for w = exp(-2 pi i step / N): one table load, the higher
+                // powers by products
                 const uint32_t step = k * (N / (Ns * R));
+                const Cplx w1{tw[2 * step], tw[2 * step + 1]};
+                Cplx wr = w1;
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    const uint32_t qi = step * r;
-                    v[q][r] = cmul(v[q][r], Cplx{tw[2 * qi], tw[2 * qi + 1]});
+                    v[q][r] = cmul(v[q][r], wr);
+                    if (r + 1 < R) wr = cmul(wr, w1);
                 }
             }
             dft_r<R>(v[q]);
