@@ -68,6 +68,7 @@ SIGNATURES = {
     "hm_last_error": (ctypes.c_char_p, []),
     "hm_version": (ctypes.c_char_p, []),
     "hm_device_count": (I, [ctypes.POINTER(I)]),
+    "hm_mem_info": (I, [ctypes.POINTER(SZ), ctypes.POINTER(SZ)]),
     "hm_malloc": (I, [ctypes.POINTER(P), SZ]),
     "hm_free": (I, [P]),
     "hm_memcpy": (I, [P, P, SZ, I, P]),

@@ -670,7 +670,11 @@ class HenonTracker {
         // the FFT reads each particle's samples in place
         const size_t row_len = ((size_t)L + 15) / 16 * 16;
         const size_t per = row_len * 32;
-        const size_t budget = (size_t)8 << 30;
+        // as many particles per chunk as half the free device memory holds
+        // (at least 8 GiB): tracking the rows needs many wavefronts
+        size_t free_b = 0, total_b = 0;
+        check(hm_mem_info(&free_b, &total_b));
+        const size_t budget = std::max<size_t>((size_t)8 << 30, free_b / 2);
         const size_t chunk = std::max<size_t>(1, std::min<size_t>(p.n_particles, budget / per));
         std::vector<double> birk(p.n_particles * 2 * NW), fft(all ? p.n_particles * 2 * NW : 0);
         DevBuf hist(chunk * per), means(all ? chunk * 32 : 8), ob(chunk * 2 * NW * 8), of(chunk * 2 * NW * 8);

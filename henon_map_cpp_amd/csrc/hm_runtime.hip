@@ -199,6 +199,14 @@ int hm_device_count(int *count)
     });
 }
 
+int hm_mem_info(size_t *free_bytes, size_t *total_bytes)
+{
+    return guarded([&] {
+        if (!free_bytes || !total_bytes) throw Error(HM_ERR_ARG, "hm_mem_info: NULL pointer");
+        hip_check(hipMemGetInfo(free_bytes, total_bytes), "hipMemGetInfo");
+    });
+}
+
 int hm_malloc(void **ptr, size_t bytes)
 {
     return guarded([&] {

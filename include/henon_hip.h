@@ -59,6 +59,9 @@ const char *hm_version(void);
 /* Number of visible HIP devices (0 when none). Replaces
  * is_cuda_device_available (src/bind.cpp:28-39). */
 int hm_device_count(int *count);
+/* Free and total device memory of the current device, in bytes (no reference
+ * counterpart: the engine sizes its trajectory chunks from it). */
+int hm_mem_info(size_t *free_bytes, size_t *total_bytes);
 int hm_malloc(void **ptr, size_t bytes);
 int hm_free(void *ptr);
 /* kind: 0 host->device, 1 device->host, 2 device->device. Asynchronous on
