@@ -395,7 +395,7 @@ static uint64_t cz_budget()
         const unsigned long long v = std::strtoull(e, nullptr, 10);
         if (v > 0) return v;
     }
-    return 2ull << 30;
+    return 8ull << 30;  // a few percent of the device memory: large batches, few launches
 }
 
 template <uint32_t PMAX>

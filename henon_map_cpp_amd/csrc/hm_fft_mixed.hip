@@ -585,7 +585,7 @@ static uint64_t mr_budget()
         const unsigned long long v = std::strtoull(e, nullptr, 10);
         if (v > 0) return v;
     }
-    return 2ull << 30;
+    return 8ull << 30;  // a few percent of the device memory: large batches, few launches
 }
 
 // largest divisor of v that is at most cap
