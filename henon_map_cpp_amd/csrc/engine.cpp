@@ -688,25 +688,27 @@ class HenonTracker {
                                     allowed_steps, n_turns, mu, barrier, inverse ? 1 : 0, hist.as<double>(), row_len,
                                     all ? means.as<double>() : nullptr, nullptr));
                 const double *mp = all ? means.as<double>() : nullptr;
+                // the last two Birkhoff windows, (0, n_turns) and the full signal
+                // 0..L-1 = 0..n_turns, are the same: computed once, copied below
+                const size_t last = NW - 1;
                 if (all) {
                     // the given windows and (0, n_turns): both tunes from one read
-                    // of each window; the full signal differs between the two
-                    // (Birkhoff samples 0..n_turns, FFT 1..n_turns)
-                    const size_t last = NW - 1;
+                    // of each window; the full signal's FFT takes samples 1..n_turns
                     check(hm_tunes_rows(hist.as<double>(), nc, row_len, L, mp, bf.data(), bt.data(), last,
                                         ob.as<double>(), of.as<double>(), nullptr));
-                    check(hm_birkhoff_tunes_rows(hist.as<double>(), nc, row_len, L, mp, bf.data() + last,
-                                                 bt.data() + last, 1, ob.as<double>() + last * 2 * nc, nullptr));
                     check(hm_fft_tunes_rows(hist.as<double>(), nc, row_len, L, mp, ff.data() + last, fl.data() + last,
                                             1, of.as<double>() + last * 2 * nc, nullptr));
                 } else {
-                    check(hm_birkhoff_tunes_rows(hist.as<double>(), nc, row_len, L, mp, bf.data(), bt.data(), NW,
+                    check(hm_birkhoff_tunes_rows(hist.as<double>(), nc, row_len, L, mp, bf.data(), bt.data(), last,
                                                  ob.as<double>(), nullptr));
                 }
-                auto vb = ob.download<double>(nc * 2 * NW);
-                for (size_t w = 0; w < NW; w++)
+                auto vb = ob.download<double>(nc * 2 * last);
+                for (size_t w = 0; w < NW; w++) {
+                    const size_t src = std::min(w, last - 1);
                     for (int pl = 0; pl < 2; pl++)
-                        for (size_t i = 0; i < nc; i++) birk[(c0 + i) * 2 * NW + 2 * w + pl] = vb[(w * 2 + pl) * nc + i];
+                        for (size_t i = 0; i < nc; i++)
+                            birk[(c0 + i) * 2 * NW + 2 * w + pl] = vb[(src * 2 + pl) * nc + i];
+                }
                 if (all) {
                     auto vf = of.download<double>(nc * 2 * NW);
                     for (size_t w = 0; w < NW; w++)
