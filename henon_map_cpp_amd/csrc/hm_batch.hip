@@ -202,6 +202,12 @@ __device__ __forceinline__ void megno_chunk(const BatchSeg &S, ConstPtr<JobDesc>
         for (int q = 0; q < 16; q++) m[q] = 0.0;
     }
     const bool alive0 = in && !any_nan(x, px, y, py);
+    // held (some but not all coordinates NaN): not tracked, but its matrix and
+    // sums go on at the held point with table index steps + job turn, as in
+    // k_indicators; it stays in the live list and off the straight runs
+    const bool held = in && !alive0 && !(__builtin_isnan(x) && __builtin_isnan(px) && __builtin_isnan(y) &&
+                                         __builtin_isnan(py));
+    if (held) s += S.seg_start;
     bool alive = alive0;
     uint32_t done = turns;
     const double mu = J->mu, b2 = J->b2, mu3 = mu * 3, mu6 = mu * 6, m3mu = -3.0 * mu;
@@ -210,7 +216,8 @@ __device__ __forceinline__ void megno_chunk(const BatchSeg &S, ConstPtr<JobDesc>
     const uint64_t live0 = __ballot(alive0);
     uint32_t s0 = 0;
     if (live0) s0 = __builtin_amdgcn_readlane(s, __builtin_ctzll(live0));
-    const bool uniform = live0 && __all(!alive0 || s == s0) && (uint64_t)s0 + turns <= (uint64_t)tab_len;
+    const bool uniform = live0 && __all(!alive0 || s == s0) && !__any(held) && (uint64_t)s0 + turns <= (uint64_t)tab_len;
+    const bool any_live = __any(alive0 || held);
     const ConstPtr<TabEntry> tp = as_const(tab + s0);
     const ConstPtr<double> inv_idx = as_const(S.inv_idx);
     const uint32_t renorm = J->renorm;
@@ -229,7 +236,7 @@ __device__ __forceinline__ void megno_chunk(const BatchSeg &S, ConstPtr<JobDesc>
     };
     double xx = x * x, yy = y * y;  // carried between turns in exact mode
     const double b2s = screen_bound(b2);
-    for (uint32_t k = 0; k < turns && live0; ++k) {
+    for (uint32_t k = 0; k < turns && any_live; ++k) {
         if (uniform && pending) {
             // Plain turns up to the next renormalisation turn, as k_indicators
             // runs them: one straight block per turn (the previous turn's MEGNO
@@ -312,21 +319,25 @@ __device__ __forceinline__ void megno_chunk(const BatchSeg &S, ConstPtr<JobDesc>
                 fb2 = frob2_fused(m);
             }
         }
-        if (!__any(alive)) break;
+        if (!__any(alive || held)) break;
     }
     if (pending) megno_update();
     if (alive0) {
         gx[j] = x; gpx[j] = px; gy[j] = y; gpy[j] = py;
         gsteps[j] = s + done;
+    }
+    if (alive0 || held) {
 #pragma unroll
         for (int q = 0; q < 16; q++) st[q] = m[q];
         st[16] = l1; st[17] = l2; st[18] = fb2;
     }
-    // a particle leaves the live list when lost (or never stepped): its MEGNO
-    // is final then; survivors get theirs after the job's last turn
-    if (in && (!alive || last)) J->megno[j] = alive ? inv_idx[n_turns - 1] * l2 : __builtin_nan("");
+    // a particle leaves the live list when lost (or all NaN from the start):
+    // its MEGNO is final then; survivors and held particles get theirs after
+    // the job's last turn
+    const bool keep = alive || held;
+    if (in && (!keep || last)) J->megno[j] = keep ? inv_idx[n_turns - 1] * l2 : __builtin_nan("");
     if (!last) {
-        const uint64_t mlive = __ballot(alive);
+        const uint64_t mlive = __ballot(keep);
         if (mlive) {
             const int leader = __builtin_ctzll(mlive);
             uint32_t slot = 0;

@@ -150,7 +150,7 @@ struct IndicatorArgs {
     // Segments with loss compaction: this launch runs turns [k0, k1) of the call
     // over the live list idx_in (count on the device; NULL: every particle) and
     // appends its survivors to idx_out; between launches each live particle's
-    // tangent matrix and sums wait in state[j] (kIndState doubles). A single
+    // tangent matrix and sums wait in state[j] (ind_state_len(V) doubles). A single
     // launch has k0 = 0, k1 = n_turns, no lists and no state.
     uint32_t k0, k1;
     const uint32_t *idx_in, *count_in;
@@ -159,10 +159,14 @@ struct IndicatorArgs {
     uint32_t first, last;
 };
 
-// Per-particle state between segments: M (16), log10 scale, L1, L2, |M|_F^2,
-// and for the Birkhoff-window variant the last angles (2) and the window sums
-// (2 per window).
-constexpr int kIndState = 20 + 2 + 2 * kMaxBirk;
+// Per-particle state between segments: M (16), log10 scale, L1, L2, |M|_F^2;
+// with Birkhoff windows the last angles (2) and the window sums (2 per window);
+// with construct windows the megno_birkhoff sums (1 per window), the
+// tune_birkhoff sums (4 per window) and their last angles (2).
+constexpr int kIndBirkState = 20;                          // offset of the Birkhoff part
+constexpr int kIndConsState = kIndBirkState + 2 + 2 * kMaxBirk;  // offset of the construct part
+// doubles of state per particle in variant V (below)
+constexpr int ind_state_len(int V) { return V == 2 ? kIndConsState + 5 * kMaxBirk + 2 : V == 1 ? kIndConsState : kIndBirkState; }
 
 
 // One thread per particle; the table index of turn k is steps+k for both the
@@ -179,14 +183,34 @@ constexpr int kIndState = 20 + 2 + 2 * kMaxBirk;
 //   Birkhoff windows (sample indices from < m <= to): acc += w[m-1-from] dtheta_m
 //     with dtheta wrapped into [0, 2pi) (dynamic_indicator.cpp:110-117); tune =
 //     1 - acc / 2pi (NaN once a sample of the window is NaN).
-// Variants V: 0 MEGNO/FLI only; 1 with Birkhoff windows (accumulated inside the
-// straight runs of plain turns, kept in the state between segments); 2 with
-// any of the trajectory window and the construct windows (turn by turn, one
-// launch).
+// Variants V: 0 MEGNO/FLI only; 1 with Birkhoff windows; 2 with any of the
+// trajectory window and the construct windows as well. Every variant runs the
+// straight runs of plain turns and loss-compaction segments (its sums kept in
+// the state between them); a particle lost inside a run fed its sums and its
+// trajectory window values after the loss, so the run's end writes NaN into its
+// window from the loss on and the outputs of its sums that reach past the loss
+// are NaN, as the turn-by-turn sums make them.
+// tune_birkhoff_construct's add of index idx to one turn sample T (weights
+// w[0..T)): idx < T into tune 1, T <= idx < 2T into tune 2 (sums x1, x2, y1,
+// y2). Both halves are written every time, so the four sums stay registers.
+__device__ inline void tune_birkhoff_add(double (&acc)[4], uint32_t idx, uint32_t T, const double *w, double dx,
+                                         double dy)
+{
+    if ((uint64_t)idx >= 2ull * T) return;
+    const bool lo = idx < T;
+    const double wt = w[lo ? idx : idx - T];
+    const double ax = dx * wt, ay = dy * wt;
+    acc[0] = lo ? acc[0] + ax : acc[0];
+    acc[2] = lo ? acc[2] + ay : acc[2];
+    acc[1] = lo ? acc[1] : acc[1] + ax;
+    acc[3] = lo ? acc[3] : acc[3] + ay;
+}
+
 template <int MODE, bool OCT, int V>
 __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
 {
     constexpr bool EXTRA = V == 2, BW = V >= 1;
+    constexpr int SL = ind_state_len(V);
     constexpr int MAP = map_mode(MODE), TAN = tangent_mode(MODE);
     const uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;  // position in the live list
     const uint32_t n_live = a.count_in ? *a.count_in : a.n;
@@ -206,14 +230,17 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
     for (int w = 0; w < kMaxBirk; w++) bacc[w][0] = bacc[w][1] = 0.0;
     double thx = 0.0, thy = 0.0;
     if (!a.first && in_range) {
-        const double *st = a.state + (size_t)i * kIndState;
+        const double *st = a.state + (size_t)i * SL;
 #pragma unroll
         for (int q = 0; q < 16; q++) m[q] = st[q];
         lscale = st[16]; l1 = st[17]; l2 = st[18]; f0 = st[19];
         if (BW && a.n_birk) {
-            thx = st[20]; thy = st[21];
+            thx = st[kIndBirkState]; thy = st[kIndBirkState + 1];
 #pragma unroll
-            for (int w = 0; w < kMaxBirk; w++) { bacc[w][0] = st[22 + 2 * w]; bacc[w][1] = st[23 + 2 * w]; }
+            for (int w = 0; w < kMaxBirk; w++) {
+                bacc[w][0] = st[kIndBirkState + 2 + 2 * w];
+                bacc[w][1] = st[kIndBirkState + 3 + 2 * w];
+            }
         }
     } else {
         if (a.matrix && in_range) {
@@ -228,6 +255,14 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
     }
     const double mu3 = a.mu * 3, mu6 = a.mu * 6, m3mu = -3.0 * a.mu;
     const bool alive0 = in_range && !any_nan(x, px, y, py);
+    // held: some but not all coordinates NaN. The particle is not tracked (its
+    // coordinates and steps stay), but its tangent matrix and sums go on at the
+    // held point with the call's table records (index steps + call turn), so it
+    // is carried between segments like a live particle; the straight runs,
+    // which move every lane's coordinates, are off for its wavefront.
+    const bool held = in_range && !alive0 && !(__builtin_isnan(x) && __builtin_isnan(px) &&
+                                               __builtin_isnan(y) && __builtin_isnan(py));
+    if (held) s += k0;
     bool alive = alive0;
     uint32_t done = k1;  // the turn a particle is lost in (k1: survived the launch)
     double v0[4];
@@ -250,11 +285,23 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
         tbacc[w][0] = tbacc[w][1] = tbacc[w][2] = tbacc[w][3] = 0.0;
     }
     double ctx = 0.0, cty = 0.0;
-    if (EXTRA && a.n_tb) {
-        ctx = atan2(px, x);
-        cty = atan2(py, y);
+    if (EXTRA && (a.n_mb || a.n_tb)) {
+        if (!a.first && in_range) {
+            const double *st = a.state + (size_t)i * SL + kIndConsState;
+#pragma unroll
+            for (int w = 0; w < kMaxBirk; w++) {
+                mbacc[w] = st[w];
+#pragma unroll
+                for (int c = 0; c < 4; c++) tbacc[w][c] = st[kMaxBirk + 4 * w + c];
+            }
+            ctx = st[5 * kMaxBirk];
+            cty = st[5 * kMaxBirk + 1];
+        } else if (a.n_tb) {
+            ctx = atan2(px, x);
+            cty = atan2(py, y);
+        }
     }
-    if (EXTRA && in_range && a.hist_len && a.hist_from == 0) {
+    if (EXTRA && a.first && in_range && a.hist_len && a.hist_from == 0) {
         const size_t n = a.n;
         a.hist[0 * n + i] = x; a.hist[1 * n + i] = px; a.hist[2 * n + i] = y; a.hist[3 * n + i] = py;
     }
@@ -262,7 +309,8 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
     const uint64_t live = __ballot(alive0);
     uint32_t s0 = 0;
     if (live) s0 = __builtin_amdgcn_readlane(s, __builtin_ctzll(live));
-    const bool uniform = live && __all(!alive0 || s == s0) && (uint64_t)s0 + (k1 - k0) <= (uint64_t)a.tab_len;
+    const bool uniform = live && __all(!alive0 || s == s0) && !__any(held) &&
+                         (uint64_t)s0 + (k1 - k0) <= (uint64_t)a.tab_len;
     // record of call turn k at tp + k (s0 = the shared index at turn k0)
     const ConstPtr<TabEntry> tp = as_const(a.tab + s0) - k0;
     const ConstPtr<double> inv_idx = as_const(a.inv_idx);
@@ -287,18 +335,20 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
     bool pending = false;
     double w_prev = 0.0, inv_prev = 0.0;
     uint32_t idx_prev = 0;
+    // megno_birkhoff_construct after the MEGNO sums of add index idx (turn
+    // idx - 1): L2_j += L1 w_j[idx-1] / idx for idx <= T_j
+    auto megno_birkhoff_turn = [&](uint32_t idx, double inv) {
+#pragma unroll
+        for (int w = 0; w < kMaxBirk; w++)
+            if (w < (int)a.n_mb && idx <= a.mb_T[w])
+                mbacc[w] = mbacc[w] + (l1 * a.birk_w[a.mb_off[w] + idx - 1]) * inv;
+    };
     auto megno_update = [&]() {
         const double fa2 = frob2_fused(m);
         l1 = l1 + (0.5 * log10(fa2 / fb2)) * w_prev;
         l2 = l2 + inv_prev * l1;
         fb2 = fa2;
-        if (EXTRA) {
-            // megno_birkhoff_construct: L2_j += L1 w_j[idx-1] / idx, idx <= T_j
-#pragma unroll
-            for (int w = 0; w < kMaxBirk; w++)
-                if (w < (int)a.n_mb && idx_prev <= a.mb_T[w])
-                    mbacc[w] = mbacc[w] + (l1 * a.birk_w[a.mb_off[w] + idx_prev - 1]) * inv_prev;
-        }
+        if (EXTRA && a.n_mb) megno_birkhoff_turn(idx_prev, inv_prev);
         if (idx_prev == next_megno) {
             if (in_range && a.megno) a.megno[(size_t)sample_m * a.n + i] = inv_prev * l2;
             ++sample_m;
@@ -323,8 +373,28 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
             }
         }
     };
+    // tune_birkhoff_construct's add after turn idx - 1 (add index idx)
+    auto tune_birkhoff_turn = [&](uint32_t idx) {
+        const double nx = atan2(px, x), ny = atan2(py, y);
+        double dx = nx - ctx, dy = ny - cty;
+        dx = dx > 0 ? dx : dx + 2 * M_PI;
+        dy = dy > 0 ? dy : dy + 2 * M_PI;
+        ctx = nx;
+        cty = ny;
+#pragma unroll
+        for (int w = 0; w < kMaxBirk; w++)
+            if (w < (int)a.n_tb) tune_birkhoff_add(tbacc[w], idx, a.tb_T[w], a.birk_w + a.tb_off[w], dx, dy);
+    };
+    // the trajectory window's sample idx (the state after turn idx - 1)
+    auto hist_store = [&](uint32_t idx, double vx, double vpx, double vy, double vpy) {
+        if (in_range && idx >= a.hist_from && idx < a.hist_from + a.hist_len) {
+            const size_t n = a.n, base = (size_t)(idx - a.hist_from) * 4 * n;
+            a.hist[base + 0 * n + i] = vx; a.hist[base + 1 * n + i] = vpx;
+            a.hist[base + 2 * n + i] = vy; a.hist[base + 3 * n + i] = vpy;
+        }
+    };
     for (; k < k1; ++k) {
-        if (!EXTRA && uniform && pending) {
+        if (uniform && pending) {
             // Plain turns up to the next event (an FLI sample at its turn, a
             // MEGNO sample at the top of the turn after it, a renormalisation
             // turn): no per-turn checks, so the turn is one straight block in
@@ -354,12 +424,15 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
                         l2 = l2 + inv_prev * l1;
                         fb2 = fa2;
                     }
+                    if (EXTRA && a.n_mb) megno_birkhoff_turn(idx_prev, inv_prev);
                     const Jac jc = jac_coeffs<TAN>(x, y, mu3, mu6);
                     mat_mul_structured<TAN>(jc, t, m);
                     const bool out = forward_turn_out<MAP, OCT>(x, px, y, py, xx, yy, a.mu, m3mu, t, a.b2, b2s);
                     done = (alive & out) ? kk : done;
                     alive = alive & !out;
                     if (BW && a.n_birk) birkhoff_turn(kk + 1);
+                    if (EXTRA && a.n_tb) tune_birkhoff_turn(kk + 1);
+                    if (EXTRA && a.hist_len && alive) hist_store(kk + 1, x, px, y, py);
                     w_prev = (double)(kk + 1);
                     inv_prev = inv_k;
                     idx_prev = kk + 1;
@@ -438,41 +511,14 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
             }
         }
         if (BW && a.n_birk) birkhoff_turn(idx);
-        if (EXTRA && a.n_tb) {
-            // tune_birkhoff_construct's add after this turn (add index idx)
-            const double nx = atan2(px, x), ny = atan2(py, y);
-            double dx = nx - ctx, dy = ny - cty;
-            dx = dx > 0 ? dx : dx + 2 * M_PI;
-            dy = dy > 0 ? dy : dy + 2 * M_PI;
-            ctx = nx;
-            cty = ny;
-#pragma unroll
-            for (int w = 0; w < kMaxBirk; w++) {
-                if (w < (int)a.n_tb) {
-                    const uint32_t T = a.tb_T[w];
-                    if (idx < T) {
-                        const double wt = a.birk_w[a.tb_off[w] + idx];
-                        tbacc[w][0] = tbacc[w][0] + dx * wt;
-                        tbacc[w][2] = tbacc[w][2] + dy * wt;
-                    } else if ((uint64_t)idx < 2ull * T) {
-                        const double wt = a.birk_w[a.tb_off[w] + idx - T];
-                        tbacc[w][1] = tbacc[w][1] + dx * wt;
-                        tbacc[w][3] = tbacc[w][3] + dy * wt;
-                    }
-                }
-            }
-        }
-        if (EXTRA && in_range && a.hist_len && idx >= a.hist_from && idx < a.hist_from + a.hist_len) {
-            const size_t n = a.n, base = (size_t)(idx - a.hist_from) * 4 * n;
-            a.hist[base + 0 * n + i] = x; a.hist[base + 1 * n + i] = px;
-            a.hist[base + 2 * n + i] = y; a.hist[base + 3 * n + i] = py;
-        }
-        if (!__any(alive) && !(EXTRA && (a.hist_len || a.n_birk || a.n_mb || a.n_tb))) break;
+        if (EXTRA && a.n_tb) tune_birkhoff_turn(idx);
+        if (EXTRA && a.hist_len && alive) hist_store(idx, x, px, y, py);
+        if (!__any(alive || held)) break;
     }
     // the last turn's MEGNO sums
     if (pending) megno_update();
     // survivors of a segment that is not the last carry on in the next launch
-    const bool carry = !a.last && alive;
+    const bool carry = !a.last && (alive || held);
     if (a.idx_out) {
         const uint64_t mlive = __ballot(in_range && carry);
         if (mlive) {
@@ -491,14 +537,28 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
         a.steps[i] = s + (done - k0);
     }
     if (carry) {
-        double *st = a.state + (size_t)i * kIndState;
+        double *st = a.state + (size_t)i * SL;
 #pragma unroll
         for (int q = 0; q < 16; q++) st[q] = m[q];
         st[16] = lscale; st[17] = l1; st[18] = l2; st[19] = fb2;
         if (BW && a.n_birk) {
-            st[20] = thx; st[21] = thy;
+            st[kIndBirkState] = thx; st[kIndBirkState + 1] = thy;
 #pragma unroll
-            for (int w = 0; w < kMaxBirk; w++) { st[22 + 2 * w] = bacc[w][0]; st[23 + 2 * w] = bacc[w][1]; }
+            for (int w = 0; w < kMaxBirk; w++) {
+                st[kIndBirkState + 2 + 2 * w] = bacc[w][0];
+                st[kIndBirkState + 3 + 2 * w] = bacc[w][1];
+            }
+        }
+        if (EXTRA && (a.n_mb || a.n_tb)) {
+            double *sc = st + kIndConsState;
+#pragma unroll
+            for (int w = 0; w < kMaxBirk; w++) {
+                sc[w] = mbacc[w];
+#pragma unroll
+                for (int c = 0; c < 4; c++) sc[kMaxBirk + 4 * w + c] = tbacc[w][c];
+            }
+            sc[5 * kMaxBirk] = ctx;
+            sc[5 * kMaxBirk + 1] = cty;
         }
         return;
     }
@@ -512,6 +572,37 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
         for (int q = 0; q < 16; q++) a.matrix[(size_t)i * 16 + q] = m[q];
     }
     if (a.log_scale) a.log_scale[i] = lscale;
+    if (EXTRA && !alive0) {
+        // a particle NaN from the start is not tracked: its coordinates stay
+        // as given, so every trajectory sample after the first is the start
+        // and every tune_birkhoff add takes the same angles (a straight run
+        // moved the registers' copies of an all-NaN particle, and its
+        // wavefront may have stopped early; the arrays hold the start)
+        x = a.x[i]; px = a.px[i]; y = a.y[i]; py = a.py[i];
+        if (a.n_tb) {
+            const double ax = atan2(px, x), ay = atan2(py, y);
+            double dx = ax - ax, dy = ay - ay;
+            dx = dx > 0 ? dx : dx + 2 * M_PI;
+            dy = dy > 0 ? dy : dy + 2 * M_PI;
+#pragma unroll
+            for (int w = 0; w < kMaxBirk; w++) {
+                if (w < (int)a.n_tb) {
+                    const uint64_t T = a.tb_T[w];
+                    tbacc[w][0] = tbacc[w][1] = tbacc[w][2] = tbacc[w][3] = 0.0;
+                    const uint64_t hi = min<uint64_t>(2 * T - 1, a.n_turns);
+                    for (uint64_t q = 1; q <= hi; ++q)
+                        tune_birkhoff_add(tbacc[w], (uint32_t)q, (uint32_t)T, a.birk_w + a.tb_off[w], dx, dy);
+                }
+            }
+        }
+    }
+    if (EXTRA && a.hist_len && (!alive0 || done < k1)) {
+        // samples from the loss on (lost particles: NaN) or after the start
+        // (particles NaN from the start: the start), which the turns above
+        // did not store
+        const uint32_t from = max(alive0 ? done + 1 : 1u, a.hist_from);
+        for (uint64_t q = from; q < (uint64_t)a.hist_from + a.hist_len; ++q) hist_store((uint32_t)q, x, px, y, py);
+    }
 #pragma unroll
     for (int w = 0; w < kMaxBirk; w++) {
         if (BW && w < (int)a.n_birk) {
@@ -522,10 +613,25 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
             a.birk_out[((size_t)w * 2 + 0) * a.n + i] = nan_w ? __builtin_nan("") : 1.0 - bacc[w][0] / (2 * M_PI);
             a.birk_out[((size_t)w * 2 + 1) * a.n + i] = nan_w ? __builtin_nan("") : 1.0 - bacc[w][1] / (2 * M_PI);
         }
-        if (EXTRA && w < (int)a.n_mb) a.mb_out[(size_t)w * a.n + i] = mbacc[w];
-        if (EXTRA && w < (int)a.n_tb)
+        // the construct sums of a particle lost at turn done: NaN once they take
+        // a sample from done + 1 on (megno_birkhoff: idx <= T; tune_birkhoff:
+        // 1 <= idx < T for the first half, T <= idx < 2T, idx <= n_turns for
+        // the second); a particle all NaN from the start has NaN MEGNO sums
+        // from the first turn on, a held one its own, and both the
+        // tune_birkhoff sums formed above
+        const bool lost = alive0 && done < k1;
+        if (EXTRA && w < (int)a.n_mb) {
+            const bool nan_m = (!alive0 && !held) || (lost && a.mb_T[w] >= done + 1);
+            a.mb_out[(size_t)w * a.n + i] = nan_m ? __builtin_nan("") : mbacc[w];
+        }
+        if (EXTRA && w < (int)a.n_tb) {
+            const uint64_t T = a.tb_T[w];
+            const bool nan1 = lost && (uint64_t)done + 1 < T;
+            const bool nan2 = lost && (uint64_t)done + 1 < 2 * T && T <= a.n_turns;
 #pragma unroll
-            for (int c = 0; c < 4; c++) a.tb_out[((size_t)w * 4 + c) * a.n + i] = tbacc[w][c];
+            for (int c = 0; c < 4; c++)
+                a.tb_out[((size_t)w * 4 + c) * a.n + i] = ((c % 2 == 0) ? nan1 : nan2) ? __builtin_nan("") : tbacc[w][c];
+        }
     }
 }
 
@@ -872,8 +978,7 @@ extern "C" int hm_track_indicators(double *x, double *px, double *y, double *py,
         LaunchTimer timer("k_indicators", s);
         // Birkhoff windows, and the trajectory and construct windows, are
         // compiled into their own variants, so the MEGNO/FLI-only kernel
-        // carries none of their code; only the last runs turn by turn in one
-        // launch
+        // carries none of their code
         const bool extra = a.hist_len || a.n_mb || a.n_tb;
         const int V = extra ? 2 : (a.n_birk ? 1 : 0);
         using K = void (*)(IndicatorArgs);
@@ -888,7 +993,7 @@ extern "C" int hm_track_indicators(double *x, double *px, double *y, double *py,
         a.first = a.last = 1;
         a.k0 = 0;
         a.k1 = n_turns;
-        if (extra || n_turns <= kIndFirstSegment || n > 0x7fffffffull) {
+        if (n_turns <= kIndFirstSegment || n > 0x7fffffffull) {
             hipLaunchKernelGGL(k, dim3(grid_of(n)), dim3(256), 0, s, a);
             launch_check("k_indicators");
             return;
@@ -906,7 +1011,7 @@ extern "C" int hm_track_indicators(double *x, double *px, double *y, double *py,
             len = std::min<uint32_t>(len * 2u, kIndCompactEvery);
         }
         const size_t S = seg.size(), N = n;
-        double *state = static_cast<double *>(workspace(N * kIndState * sizeof(double), 10));
+        double *state = static_cast<double *>(workspace(N * ind_state_len(V) * sizeof(double), 10));
         uint32_t *lists = static_cast<uint32_t *>(workspace(sizeof(uint32_t) * (2 * N + S + 64), 11));
         uint32_t *counts = lists + 2 * N;  // counts[i]: live entries entering segment i (i >= 1)
         uint32_t *host = indicator_counts(S + 1);

@@ -929,6 +929,10 @@ extern "C" int or_indicators(double *x, double *px, double *y, double *py, unsig
         double fb = std::sqrt(frob2(m));
         double l1 = 0.0, l2 = 0.0;
         bool alive = !check_nan(x[i], px[i], y[i], py[i]);
+        // a window holding a sample with any coordinate NaN is NaN in both
+        // planes (dynamic_indicator.cpp:284-287); a particle lost inside it
+        // is NaN in all four, one NaN from the start (not tracked) in some
+        const bool start_nan = !alive;
         std::vector<double> bacc(2 * n_birk, 0.0);
         double thx = std::atan2(px[i], x[i]), thy = std::atan2(py[i], y[i]);
         size_t sq = 0;
@@ -1001,8 +1005,9 @@ extern "C" int or_indicators(double *x, double *px, double *y, double *py, unsig
         if (matrix) std::memcpy(matrix + i * 16, m, sizeof(m));
         if (log_scale) log_scale[i] = lscale;
         for (size_t q = 0; q < n_birk; q++) {
-            birk_out[(2 * q) * n + i] = 1.0 - bacc[2 * q] / (2 * M_PI);
-            birk_out[(2 * q + 1) * n + i] = 1.0 - bacc[2 * q + 1] / (2 * M_PI);
+            const double qn = std::numeric_limits<double>::quiet_NaN();
+            birk_out[(2 * q) * n + i] = start_nan ? qn : 1.0 - bacc[2 * q] / (2 * M_PI);
+            birk_out[(2 * q + 1) * n + i] = start_nan ? qn : 1.0 - bacc[2 * q + 1] / (2 * M_PI);
         }
     });
     return bad;
