@@ -314,12 +314,13 @@ def tune_all_leg(n=90_000, n_turns=10_000, reps=2):
         dt = time.perf_counter() - t0
         ks = {k: C.profile_read(k)[0] for k in ("k_history_rows", "k_birkhoff_hist", "k_fft_tune_rows",
                                                 "k_cz_columns", "k_cz_rows", "k_cz_final", "k_cz_tune",
-                                                "k_mr_columns", "k_mr_rows", "k_mr_final", "k_mr_tune")}
+                                                "k_mr_columns", "k_mr_rows", "k_mr_final", "k_mr_tune",
+                                                "k_wh_tune")}
         C.profile_enable(False)
         if best is None or dt < best[0]:
             best = (dt, ks, np.asarray(rb), np.asarray(rf))
     dt, ks, rb, rf = best
-    fft_ms = sum(v for k, v in ks.items() if k.startswith(("k_cz", "k_mr")) or k == "k_fft_tune_rows")
+    fft_ms = sum(v for k, v in ks.items() if k.startswith(("k_cz", "k_mr", "k_wh")) or k == "k_fft_tune_rows")
     fft_bytes = 16.0 * n * 2 * (501 + 501 + 10001 + 10000)
     return {
         "workload": f"tune_all of tests/new_tune.ipynb: {n} particles at (0.1, 0, 0.1, 0), {n_turns} turns, "

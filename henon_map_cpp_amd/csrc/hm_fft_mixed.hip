@@ -410,6 +410,313 @@ __global__ void __launch_bounds__(kMrThreads) k_mr_tune(MrArgs a)
     }
 }
 
+// ---------------------------------------------------------------- whole-signal transforms
+// A window whose transform length fits one workgroup's LDS (about 10^4 points
+// in 160 KiB) runs in one kernel, one workgroup per signal: the samples are
+// read once, every pass runs in LDS and the tune is found there, with no
+// scratch in HBM and no second launch.
+//   direct : lengths whose prime factors are 2, 3, 5, 7 (the radix passes above)
+//            or other primes up to kWhPrimeMax (one prime-size pass each, below)
+//   chirp-z: other lengths N, over a smooth M >= 2N - 1 that fits: the chirped
+//            samples, FFT_M, times B = DFT_M(conj(c)), conjugated, FFT_M again;
+//            |X_k| = |V_k| / M for k < N (the scale does not change the tune)
+// The signal sits in LDS unpadded: the first pass takes an odd radix where the
+// length has an odd factor, so its stores step by an odd number of 16-byte
+// entries, and the later passes store runs of consecutive entries (a power of
+// two, here only as a chirp-z length, pays some bank conflicts in its first
+// pass).
+constexpr uint32_t kWhPrimeMax = 257;
+constexpr int kWhMaxPasses = 16;
+constexpr uint32_t kWhStaticLds = 256;
+
+// One pass: radix R after the radices of product Ns, B = L / R butterflies,
+// twiddle stride step = L / (Ns R); m_* are the multipliers of udiv_m for Ns,
+// H = (R - 1) / 2 and H + 1.
+struct WhPass {
+    uint32_t R, Ns, B, step, m_ns, m_h, m_h1;
+};
+struct WhPlan {
+    uint32_t L, n_pass;
+    WhPass pass[kWhMaxPasses];
+};
+
+// j / d for j < 2^16 and d <= 2^16 from m = floor(2^32 / d) + 1 (m = 0: d = 1):
+// j m / 2^32 = j / d + j (m - 2^32 / d) / 2^32 exceeds j / d by less than
+// 2^-16, and the fraction of j / d is at most 1 - 1 / d.
+__device__ __forceinline__ uint32_t udiv_m(uint32_t j, uint32_t m) { return m ? __umulhi(j, m) : j; }
+
+struct WhArgs {
+    const double *hist;
+    HistLayout lay;
+    const double *means;  // [4][n] or NULL
+    uint32_t n, p0, from, N, M;
+    WhPlan plan;          // passes of the M-point transform
+    const double *hann;   // [N]
+    const Cplx *tw;       // W_M^q, q < M
+    const Cplx *chirp;    // chirp-z: [N]
+    const Cplx *bhat;     // chirp-z: [M], B in natural order
+    double *out;          // [window][plane][n]
+    uint32_t window;
+};
+
+
+// One radix-R Stockham pass (as mr_pass) over the whole signal of length L:
+// butterfly j reads x[j + r L/R], twiddles by W_L^{k r L/(Ns R)} (k = j mod
+// Ns), takes the R-point DFT and writes element r to (j - k) R + k + r Ns.
+template <int R, uint32_t NT, uint32_t PMAX>
+__device__ __forceinline__ void wh_pass(Cplx *buf, const WhPass &ps, const Cplx *__restrict__ tw)
+{
+    constexpr uint32_t PER = (PMAX / R + NT - 1) / NT;
+    const uint32_t B = ps.B, step = ps.step, Ns = ps.Ns;
+    Cplx v[PER][R];
+    uint32_t base[PER];
+#pragma unroll
+    for (uint32_t q = 0; q < PER; ++q) {
+        const uint32_t j = threadIdx.x + q * NT;
+        base[q] = 0xffffffffu;
+        if (j < B) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = buf[j + r * B];
+            const uint32_t k = j - udiv_m(j, ps.m_ns) * Ns;
+            if (k != 0) {
+                const Cplx w1 = tw[k * step];
+                Cplx w = w1;
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    v[q][r] = cmul(v[q][r], w);
+                    if (r + 1 < R) w = cmul(w, w1);
+                }
+            }
+            dft_any<R>(v[q]);
+            base[q] = (j - k) * R + k;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t q = 0; q < PER; ++q)
+        if (base[q] != 0xffffffffu)
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[base[q] + r * Ns] = v[q][r];
+    __syncthreads();
+}
+
+// A pass of odd prime radix R (11 .. kWhPrimeMax), the same butterflies with
+// the R-point DFT from the pairs (r, R - r) as in dft_prime:
+//   A: each pair twiddled and replaced in place by a_r = x_r + x_{R-r} (at r)
+//      and b_r = x_r - x_{R-r} (at R - r)
+//   B: one thread per output pair (q, R - q) of a butterfly (q = 0: the sum),
+//      Y_q = x_0 + sum_r (a_r cos t - i b_r sin t), Y_{R-q} with + i, t = 2 pi r q / R;
+//      the lanes of a wavefront share the butterfly (the same a_r, b_r: LDS
+//      broadcasts) and step cos t, sin t by rotation through 2 pi q / R
+// R(R-1)/2 real multiply-adds per butterfly and plane pair instead of R^2
+// complex products; the rotation adds at most a few 1e-14 of relative error
+// over the (R-1)/2 steps.
+template <uint32_t NT, uint32_t PMAX>
+__device__ __forceinline__ void wh_pass_prime(Cplx *buf, const WhPass &ps, const Cplx *__restrict__ tw)
+{
+    // output pairs per thread: B (H + 1) <= L (R + 1) / (2R) <= 0.55 L for R >= 11
+    constexpr uint32_t GP = ((PMAX * 11 / 20 + NT - 1) / NT + 1) / 2 * 2;
+    const uint32_t R = ps.R, Ns = ps.Ns, B = ps.B, H = (R - 1) / 2, step = ps.step;
+    for (uint32_t e = threadIdx.x; e < B * H; e += NT) {
+        const uint32_t j = udiv_m(e, ps.m_h), r = e - j * H + 1, k = j - udiv_m(j, ps.m_ns) * Ns;
+        const uint32_t iu = j + r * B, iv = j + (R - r) * B;
+        Cplx u = buf[iu], v = buf[iv];
+        if (k != 0) {
+            // k r step < L: the exponents need no reduction
+            u = cmul(u, tw[k * r * step]);
+            v = cmul(v, tw[k * (R - r) * step]);
+        }
+        buf[iu] = cadd(u, v);
+        buf[iv] = csub(u, v);
+    }
+    __syncthreads();
+    // output positions (< 2^15) of Y_q in the low, of Y_{R-q} in the high half
+    static_assert(PMAX < 0xffffu, "output positions are packed in 16 bits");
+    static_assert(GP % 2 == 0, "output pairs are taken two at a time");
+    Cplx y0[GP], y1[GP];
+    uint32_t oo[GP];
+    const uint32_t items = B * (H + 1);
+#pragma unroll
+    for (uint32_t g = 0; g < GP; g += 2) {
+        // two output pairs at once (independent chains); q = 0 runs the same
+        // loop with cos = 1, sin = 0, which leaves Y_0 = x_0 + sum a_r
+        uint32_t ja[2], qa[2];
+        Cplx pa[2], sb[2];
+        double c[2], sn[2], c1[2], s1[2];
+#pragma unroll
+        for (uint32_t h = 0; h < 2; ++h) {
+            const uint32_t e = min(threadIdx.x + (g + h) * NT, items - 1);
+            ja[h] = udiv_m(e, ps.m_h1);
+            qa[h] = e - ja[h] * (H + 1);
+            const Cplx wq = tw[qa[h] * B];  // W_L^{q L / R} = exp(-2 pi i q / R)
+            c1[h] = wq.re;
+            s1[h] = -wq.im;
+            c[h] = c1[h];
+            sn[h] = s1[h];
+            pa[h] = buf[ja[h]];
+            sb[h] = Cplx{0.0, 0.0};
+        }
+        for (uint32_t r = 1; r <= H; ++r) {
+            Cplx av[2], bv[2];
+#pragma unroll
+            for (uint32_t h = 0; h < 2; ++h) {
+                av[h] = buf[ja[h] + r * B];
+                bv[h] = buf[ja[h] + (R - r) * B];
+            }
+#pragma unroll
+            for (uint32_t h = 0; h < 2; ++h) {
+                pa[h].re = fma(av[h].re, c[h], pa[h].re);
+                pa[h].im = fma(av[h].im, c[h], pa[h].im);
+                sb[h].re = fma(bv[h].re, sn[h], sb[h].re);
+                sb[h].im = fma(bv[h].im, sn[h], sb[h].im);
+                const double cn = fma(c[h], c1[h], -(sn[h] * s1[h])), sn2 = fma(sn[h], c1[h], c[h] * s1[h]);
+                c[h] = cn;
+                sn[h] = sn2;
+            }
+        }
+#pragma unroll
+        for (uint32_t h = 0; h < 2; ++h) {
+            const uint32_t e = threadIdx.x + (g + h) * NT;
+            const uint32_t k = ja[h] - udiv_m(ja[h], ps.m_ns) * Ns, ob = (ja[h] - k) * R + k, q = qa[h];
+            y0[g + h] = Cplx{pa[h].re + sb[h].im, pa[h].im - sb[h].re};  // Y_q
+            y1[g + h] = Cplx{pa[h].re - sb[h].im, pa[h].im + sb[h].re};  // Y_{R-q}
+            oo[g + h] = e >= items ? 0xffffffffu
+                        : q == 0   ? (0xffff0000u | ob)
+                                   : ((ob + q * Ns) | ((ob + (R - q) * Ns) << 16));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t g = 0; g < GP; ++g) {
+        const uint32_t lo = oo[g] & 0xffffu, hi = oo[g] >> 16;
+        if (lo != 0xffffu) buf[lo] = y0[g];
+        if (hi != 0xffffu) buf[hi] = y1[g];
+    }
+    __syncthreads();
+}
+
+// Forward DFT of the whole signal in LDS, in natural order. The plan lists its
+// radices grouped (3s, 5s, 7s, the other primes, then 2s, 4s, 8s: an odd or
+// small first radix keeps the first pass's strided stores off shared banks),
+// so each radix gets a loop of its own: one switch over the inlined passes
+// inside a single loop keeps more registers live than any pass needs.
+template <uint32_t NT, uint32_t PMAX>
+__device__ __forceinline__ void wh_fft(Cplx *buf, const WhPlan &p, const Cplx *__restrict__ tw)
+{
+    uint32_t s = 0;
+    for (; s < p.n_pass && p.pass[s].R == 3; ++s) wh_pass<3, NT, PMAX>(buf, p.pass[s], tw);
+    for (; s < p.n_pass && p.pass[s].R == 5; ++s) wh_pass<5, NT, PMAX>(buf, p.pass[s], tw);
+    for (; s < p.n_pass && p.pass[s].R == 7; ++s) wh_pass<7, NT, PMAX>(buf, p.pass[s], tw);
+    for (; s < p.n_pass && p.pass[s].R > 8; ++s) wh_pass_prime<NT, PMAX>(buf, p.pass[s], tw);
+    for (; s < p.n_pass && p.pass[s].R == 2; ++s) wh_pass<2, NT, PMAX>(buf, p.pass[s], tw);
+    for (; s < p.n_pass && p.pass[s].R == 4; ++s) wh_pass<4, NT, PMAX>(buf, p.pass[s], tw);
+    for (; s < p.n_pass && p.pass[s].R == 8; ++s) wh_pass<8, NT, PMAX>(buf, p.pass[s], tw);
+}
+
+// One workgroup per signal (2 per particle: plane x, plane y): load, mean
+// removal and Hanning, the transform(s), the first maximum of |X_k| over
+// k < N and the three-point interpolation (dynamic_indicator.cpp:139-196).
+template <bool CZ, uint32_t NT, uint32_t PMAX>
+__global__ void __launch_bounds__(NT) k_wh_tune(WhArgs a)
+{
+    extern __shared__ Cplx buf[];
+    __shared__ double sv[NT / 64];
+    __shared__ uint32_t si[NT / 64];
+    const uint32_t sig = blockIdx.x;
+    const uint32_t particle = a.p0 + sig / 2, plane = sig % 2;
+    const size_t nn = a.n;
+    double *o = a.out + ((size_t)a.window * 2 + plane) * nn + particle;
+    const uint32_t N = a.N, M = a.M;
+    double mc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (a.means)
+        for (int c = 0; c < 4; ++c) mc[c] = a.means[c * nn + particle];
+    if (any_nan(mc[0], mc[1], mc[2], mc[3])) {
+        // a NaN mean makes every sample of its coordinate NaN: nothing to read
+        if (threadIdx.x == 0) *o = __builtin_nan("");
+        return;
+    }
+    // four finite means prove that no sample of the particle is NaN or
+    // infinite: the other plane need not be read
+    const bool scan_other =
+        !a.means || !(isfinite(mc[0]) && isfinite(mc[1]) && isfinite(mc[2]) && isfinite(mc[3]));
+    const int cx = 2 * plane, cp = cx + 1, co0 = 2 * (1 - plane), co1 = co0 + 1;
+    int nan_seen = 0;
+    // every sample load of the thread issued before the first use (a loop
+    // that waits for each sample in turn leaves the workgroup idle)
+    constexpr uint32_t LP = (CZ ? PMAX / 2 : PMAX + NT - 1) / NT + (CZ ? 1 : 0);
+    double vx[LP], vp[LP], hh[LP];
+#pragma unroll
+    for (uint32_t u = 0; u < LP; ++u) {
+        const uint32_t m = threadIdx.x + u * NT;
+        vx[u] = vp[u] = hh[u] = 0.0;
+        if (m < N) {
+            const uint64_t sm = (uint64_t)a.from + m;
+            vx[u] = a.hist[hat(a.lay, sm, cx, particle)];
+            vp[u] = a.hist[hat(a.lay, sm, cp, particle)];
+            hh[u] = a.hann[m];
+        }
+    }
+    if (scan_other)
+        for (uint32_t m = threadIdx.x; m < N; m += NT) {
+            const uint64_t sm = (uint64_t)a.from + m;
+            nan_seen |= __builtin_isnan(a.hist[hat(a.lay, sm, co0, particle)] - mc[co0]) ||
+                        __builtin_isnan(a.hist[hat(a.lay, sm, co1, particle)] - mc[co1]);
+        }
+#pragma unroll
+    for (uint32_t u = 0; u < LP; ++u) {
+        const uint32_t m = threadIdx.x + u * NT;
+        if (m < N) {
+            const double x = vx[u] - mc[cx], xp = vp[u] - mc[cp];
+            nan_seen |= __builtin_isnan(x) || __builtin_isnan(xp);
+            Cplx z{x * hh[u], xp * hh[u]};
+            if constexpr (CZ) z = cmul(z, a.chirp[m]);
+            buf[m] = z;
+        }
+    }
+    if constexpr (CZ)
+        for (uint32_t m = N + threadIdx.x; m < M; m += NT) buf[m] = Cplx{0.0, 0.0};
+    if (__syncthreads_or(nan_seen)) {
+        if (threadIdx.x == 0) *o = __builtin_nan("");
+        return;
+    }
+    wh_fft<NT, PMAX>(buf, a.plan, a.tw);
+    if constexpr (CZ) {
+        for (uint32_t k = threadIdx.x; k < M; k += NT) {
+            const uint32_t l = k;
+            const Cplx z = cmul(buf[l], a.bhat[k]);
+            buf[l] = Cplx{z.re, -z.im};
+        }
+        __syncthreads();
+        wh_fft<NT, PMAX>(buf, a.plan, a.tw);
+    }
+    auto mag = [&](uint32_t k) {
+        const Cplx z = buf[k];
+        return sqrt(z.re * z.re + z.im * z.im);
+    };
+    // first maximum: each thread scans increasing k, ties go to the lower k
+    double v = -1.0;
+    uint32_t i = 0xffffffffu;
+    for (uint32_t k = threadIdx.x; k < N; k += NT) {
+        const double mg = mag(k);
+        if (mg > v || i == 0xffffffffu) { v = mg; i = k; }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(v, off);
+        const uint32_t oi = __shfl_xor(i, off);
+        if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+    }
+    if ((threadIdx.x & 63) == 0) { sv[threadIdx.x >> 6] = v; si[threadIdx.x >> 6] = i; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double bv = sv[0];
+        uint32_t bi = si[0];
+        for (uint32_t q = 1; q < NT / 64; ++q)
+            if (sv[q] > bv || (sv[q] == bv && si[q] < bi)) { bv = sv[q]; bi = si[q]; }
+        *o = interp_tune_at(mag, N, bi);
+    }
+}
+
 __global__ void k_mr_zero(uint32_t *__restrict__ p, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -603,10 +910,178 @@ static void set_lds(K kernel, size_t bytes)
               "hipFuncSetAttribute");
 }
 
+// ---------------------------------------------------------------- whole-signal plans
+// The two shapes of k_wh_tune: up to 2048 points with 256 threads (several
+// workgroups per CU), up to kWhPmax with 1024.
+constexpr uint32_t kWhSmallPmax = 2048, kWhSmallThreads = 256;
+constexpr uint32_t kWhPmax = 10240, kWhThreads = 1024;
+
+// Passes of the direct transform of L, grouped as wh_fft runs them: 3s, 5s,
+// 7s, the other primes in increasing order (each at most kWhPrimeMax), then the
+// power of two as 2, 4 and 8s (8 while it divides). `gen` = the sum of the
+// other primes (their passes cost about 2R operations per point against a few
+// tens for a small radix).
+static uint32_t udiv_magic(uint32_t d) { return d == 1 ? 0u : (uint32_t)((1ull << 32) / d + 1); }
+
+static bool wh_radix_plan(uint32_t L, WhPlan &p, uint32_t &gen)
+{
+    p = WhPlan{};
+    p.L = L;
+    gen = 0;
+    uint32_t r = L, Ns = 1;
+    auto push = [&](uint32_t R) {
+        if (p.n_pass == (uint32_t)kWhMaxPasses) return false;
+        WhPass &q = p.pass[p.n_pass++];
+        q.R = R; q.Ns = Ns; q.B = L / R; q.step = L / (Ns * R);
+        q.m_ns = udiv_magic(Ns); q.m_h = udiv_magic(std::max(1u, (R - 1) / 2)); q.m_h1 = udiv_magic((R - 1) / 2 + 1);
+        Ns *= R;
+        r /= R;
+        return true;
+    };
+    for (uint32_t R : {3u, 5u, 7u})
+        while (r % R == 0)
+            if (!push(R)) return false;
+    for (uint32_t q = 11; r > 1 && q <= kWhPrimeMax; q += 2)
+        while (r % q == 0) {
+            if (!push(q)) return false;
+            gen += q;
+        }
+    uint32_t twos = 0;
+    while (r % 2 == 0) { r /= 2; ++twos; }
+    if (r != 1) return false;
+    r = 1u << twos;  // the power of two, taken as 2^(twos mod 3 == 1), 4^(... == 2), then 8s
+    if (twos % 3 == 1 && !push(2)) return false;
+    if (twos % 3 == 2 && !push(4)) return false;
+    while (r > 1)
+        if (!push(8)) return false;
+    return true;
+}
+
+struct WhTables {
+    bool ok = false, cz = false;
+    uint32_t M = 0, nt = 0;
+    WhPlan plan{};
+    Cplx *tw = nullptr, *chirp = nullptr, *bhat = nullptr;
+};
+
+
+// Whole-signal plan of a window length (cached per device): the direct
+// transform when N is {2,3,5,7}-smooth or its other primes are cheap (sum at
+// most kWhCzGen), else chirp-z over the smallest smooth M >= 2N - 1 that
+// fits, else the direct transform with dearer prime passes; none when N does
+// not fit or has a prime above kWhPrimeMax and no chirp-z length fits.
+constexpr uint32_t kWhCzGen = 128;
+
+static const WhTables *wh_tables(uint32_t N, hipStream_t s)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, uint32_t>, WhTables> cache;
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find({dev, N});
+    if (it != cache.end()) return it->second.ok ? &it->second : nullptr;
+    int v = 0;
+    hip_check(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
+    const size_t lds = (size_t)v;
+    auto fits = [&](uint32_t M) { return M <= kWhPmax && (size_t)M * sizeof(Cplx) + kWhStaticLds <= lds; };
+    WhTables t;
+    WhPlan pd{};
+    uint32_t gen = 0;
+    const bool direct = fits(N) && wh_radix_plan(N, pd, gen);
+    uint32_t Mcz = 0;
+    WhPlan pc{};
+    if (!(direct && gen <= kWhCzGen)) {
+        for (uint64_t M = 2ull * N - 1; M <= kWhPmax && fits((uint32_t)M); ++M) {
+            uint32_t g = 0;
+            if (wh_radix_plan((uint32_t)M, pc, g) && g == 0) { Mcz = (uint32_t)M; break; }
+        }
+    }
+    if (direct && (gen <= kWhCzGen || !Mcz)) {
+        t.ok = true; t.cz = false; t.M = N; t.plan = pd;
+    } else if (Mcz) {
+        t.ok = true; t.cz = true; t.M = Mcz; t.plan = pc;
+    }
+    if (!t.ok) {
+        cache[{dev, N}] = WhTables{};
+        return nullptr;
+    }
+    const uint32_t M = t.M;
+    t.nt = M <= kWhSmallPmax ? kWhSmallThreads : kWhThreads;
+    const long double two_pi = 6.283185307179586476925286766559005768L, pi = two_pi / 2;
+    std::vector<Cplx> w(M);
+    for (uint32_t q = 0; q < M; ++q) {
+        const long double ang = two_pi * (long double)q / (long double)M;
+        w[q] = Cplx{(double)cosl(ang), (double)-sinl(ang)};
+    }
+    t.tw = upload(w, s, "hipMalloc(whole-signal twiddles)");
+    if (t.cz) {
+        // chirp c_m = exp(-i pi (m^2 mod 2N) / N), filter b_j = conj(c_j) for
+        // |j| < N (j mod M), B = DFT_M(b) in natural order
+        std::vector<Cplx> chirp(N), bhat(M);
+        std::vector<CplxL> b(M, CplxL(0, 0));
+        for (uint64_t q = 0; q < N; ++q) {
+            const long double ang = pi * (long double)((q * q) % (2ull * N)) / (long double)N;
+            const long double c = cosl(ang), sn = sinl(ang);
+            chirp[q] = Cplx{(double)c, (double)-sn};
+            b[q] = CplxL(c, sn);
+            if (q > 0) b[M - q] = b[q];
+        }
+        const std::vector<CplxL> B = host_dft(b);
+        for (uint32_t k = 0; k < M; ++k) bhat[k] = Cplx{(double)B[k].real(), (double)B[k].imag()};
+        t.chirp = upload(chirp, s, "hipMalloc(chirp)");
+        t.bhat = upload(bhat, s, "hipMalloc(chirp-z filter)");
+    }
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    cache[{dev, N}] = t;
+    return &cache[{dev, N}];
+}
+
+template <bool CZ, uint32_t NT, uint32_t PMAX>
+static void launch_wh(const WhArgs &a0, uint32_t n, size_t lds_bytes, hipStream_t s)
+{
+    auto k = k_wh_tune<CZ, NT, PMAX>;
+    set_lds(k, lds_bytes);
+    WhArgs a = a0;
+    // 2 signals per particle, one workgroup each: grids below 2^31 workgroups
+    const uint32_t chunk = 1u << 29;
+    for (uint32_t p0 = 0; p0 < n; p0 += chunk) {
+        const uint32_t nb = std::min(chunk, n - p0);
+        a.p0 = p0;
+        hipLaunchKernelGGL(k, dim3(2 * nb), dim3(NT), lds_bytes, s, a);
+        launch_check("k_wh_tune");
+    }
+}
+
+// The whole-signal transform of one window; false (nothing launched) when the
+// length has no whole-signal plan.
+static bool fft_tune_whole(const double *hist, HistLayout lay, uint32_t n, const double *means, uint32_t from,
+                           uint32_t N, const double *hann, double *out, uint32_t window, hipStream_t s)
+{
+    const WhTables *t = wh_tables(N, s);
+    if (!t) return false;
+    if (n == 0) return true;
+    WhArgs a{};
+    a.hist = hist; a.lay = lay; a.means = means; a.n = n; a.from = from; a.N = N; a.M = t->M;
+    a.plan = t->plan; a.hann = hann; a.tw = t->tw; a.chirp = t->chirp; a.bhat = t->bhat;
+    a.out = out; a.window = window;
+    const size_t lds_bytes = (size_t)t->M * sizeof(Cplx);
+    LaunchTimer timer("k_wh_tune", s);
+    if (t->nt == kWhSmallThreads) {
+        if (t->cz) launch_wh<true, kWhSmallThreads, kWhSmallPmax>(a, n, lds_bytes, s);
+        else launch_wh<false, kWhSmallThreads, kWhSmallPmax>(a, n, lds_bytes, s);
+    } else {
+        if (t->cz) launch_wh<true, kWhThreads, kWhPmax>(a, n, lds_bytes, s);
+        else launch_wh<false, kWhThreads, kWhPmax>(a, n, lds_bytes, s);
+    }
+    return true;
+}
+
 bool fft_tune_mixed(const double *hist, HistLayout lay, uint32_t n, const double *means, uint32_t from, uint32_t N,
                     const double *hann, double *out, uint32_t window, hipStream_t s)
 {
     if (N < 2) return false;
+    if (fft_tune_whole(hist, lay, n, means, from, N, hann, out, window, s)) return true;
     const MrTables *t = mr_tables(N, s);
     if (!t) return false;
     if (n == 0) return true;

@@ -22,7 +22,9 @@ __device__ __forceinline__ size_t hat(const HistLayout &h, uint64_t m, uint64_t 
     return (size_t)(m * h.ms + c * h.cs + p * h.ps);
 }
 
-struct Cplx {
+// 16-byte aligned, so that LDS and global accesses of one value are single
+// 128-bit operations (ds_read_b128, not two 64-bit halves a lane-stride apart)
+struct alignas(16) Cplx {
     double re, im;
 };
 __device__ __forceinline__ Cplx cadd(Cplx a, Cplx b) { return Cplx{a.re + b.re, a.im + b.im}; }
