@@ -1,6 +1,8 @@
-// FFT tunes of windows whose length has only the prime factors 2, 3, 5 and 7
-// (src/dynamic_indicator.cpp:32-61, 178-196, 255-331 for any N): a mixed-radix
-// four-step transform of the length itself, N = N1 x N2 (index n = N2 n1 + n2
+// FFT tunes of windows of any length (src/dynamic_indicator.cpp:32-61, 178-196,
+// 255-331 for any N). Windows of up to about 10^4 points run in one kernel per
+// signal with the whole signal in LDS (k_wh_tune, below the four-step
+// kernels). Longer windows whose length has only the prime factors 2, 3, 5 and
+// 7 run a mixed-radix four-step transform of the length itself, N = N1 x N2 (index n = N2 n1 + n2
 // in, k = k1 + N1 k2 out):
 //   X[k1 + N1 k2] = sum_{n2} W_{N2}^{n2 k2} W_N^{n2 k1} sum_{n1} x[N2 n1 + n2] W_{N1}^{n1 k1}
 // with both factors LDS-resident, so no padding to a power of two and no chirp:

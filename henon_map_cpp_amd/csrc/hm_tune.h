@@ -111,10 +111,13 @@ __device__ double interp_tune_at(Mag mag, uint32_t N, uint32_t index)
 void fft_tune_chirpz(const double *hist, HistLayout lay, uint32_t n, const double *means, uint32_t from,
                      uint32_t N, const double *hann, double *out, uint32_t window, hipStream_t s);
 
-// The same by mixed-radix four-step transforms (hm_fft_mixed.hip): of N itself
-// when its prime factors are 2, 3, 5 and 7 only and it splits into two factors
-// of at most 2048, else Bluestein's chirp-z over the smallest such length
-// M >= 2N - 1. Returns false (nothing launched) when no such M exists
+// The same by mixed-radix transforms (hm_fft_mixed.hip): windows that fit one
+// workgroup's LDS (about 10^4 points) in one kernel per signal (the length
+// itself, with prime passes up to 257, or chirp-z over a smooth length that
+// fits); longer ones as four-step transforms of N itself when its prime
+// factors are 2, 3, 5 and 7 only and it splits into two factors of at most
+// 2048, else Bluestein's chirp-z over the smallest such length M >= 2N - 1.
+// Returns false (nothing launched) when no such M exists
 // (N > 2^21 roughly).
 bool fft_tune_mixed(const double *hist, HistLayout lay, uint32_t n, const double *means, uint32_t from, uint32_t N,
                     const double *hann, double *out, uint32_t window, hipStream_t s);
