@@ -429,7 +429,6 @@ __global__ void __launch_bounds__(kMrThreads) k_mr_tune(MrArgs a)
 // pass).
 constexpr uint32_t kWhPrimeMax = 257;
 constexpr int kWhMaxPasses = 16;
-constexpr uint32_t kWhStaticLds = 256;
 
 // One pass: radix R after the radices of product Ns, B = L / R butterflies,
 // twiddle stride step = L / (Ns R); m_* are the multipliers of udiv_m for Ns,
@@ -985,8 +984,18 @@ static const WhTables *wh_tables(uint32_t N, hipStream_t s)
     if (it != cache.end()) return it->second.ok ? &it->second : nullptr;
     int v = 0;
     hip_check(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
-    const size_t lds = (size_t)v;
-    auto fits = [&](uint32_t M) { return M <= kWhPmax && (size_t)M * sizeof(Cplx) + kWhStaticLds <= lds; };
+    // the kernels' own LDS (reductions) comes on top of the signal
+    size_t fixed = 0;
+    for (const void *k : {(const void *)k_wh_tune<false, kWhSmallThreads, kWhSmallPmax>,
+                          (const void *)k_wh_tune<true, kWhSmallThreads, kWhSmallPmax>,
+                          (const void *)k_wh_tune<false, kWhThreads, kWhPmax>,
+                          (const void *)k_wh_tune<true, kWhThreads, kWhPmax>}) {
+        hipFuncAttributes fa{};
+        hip_check(hipFuncGetAttributes(&fa, k), "hipFuncGetAttributes");
+        fixed = std::max(fixed, (size_t)fa.sharedSizeBytes);
+    }
+    const size_t lds = (size_t)v > fixed ? (size_t)v - fixed : 0;
+    auto fits = [&](uint32_t M) { return M <= kWhPmax && (size_t)M * sizeof(Cplx) <= lds; };
     WhTables t;
     WhPlan pd{};
     uint32_t gen = 0;

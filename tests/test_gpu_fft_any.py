@@ -1,5 +1,6 @@
-"""FFT tunes of windows of any length (the mixed-radix transform, chirp-z
-over smooth and power-of-two lengths) through the C ABI and the drop-in API: against the oracle's long-double DFT where that finishes in
+"""FFT tunes of windows of any length (the whole-signal kernel for windows up
+to about 10^4 points, the four-step mixed-radix transform, chirp-z over smooth
+and power-of-two lengths) through the C ABI and the drop-in API: against the oracle's long-double DFT where that finishes in
 seconds (N up to 10001), against numpy's FFT magnitudes with the oracle's peak
 interpolation above that, against single tones, with NaN samples, in particle
 batches, and all_tunes over 1e5 turns (windows of 100000 and 100001 samples)."""
@@ -61,11 +62,30 @@ def test_mixed_radix_lengths_match_oracle_dft(N):
         np.testing.assert_allclose(got[0, :, i], want, rtol=0, atol=1e-12, err_msg=f"N={N} particle {i}")
 
 
-@pytest.mark.parametrize("N", [16384, 16411, 65537, 100000, 100001, 131072, 2_100_001])
+@pytest.mark.parametrize("N", [254, 10023, 10176])
+def test_whole_signal_prime_passes_match_oracle_dft(N):
+    """One-kernel transforms whose plans hold prime passes: 2 x 127 (256-thread
+    kernel), 3 x 13 x 257 (the largest prime pass, no chirp-z length fits),
+    2^6 x 3 x 53 (about the whole 160 KiB of LDS); a NaN sample makes its
+    particle's tunes NaN."""
+    n = 3
+    h = quasi_periodic(N, n, seed=N + 7)
+    h[N // 2, 1, 1] = np.nan
+    d = C.DeviceBuffer.from_array(np.ascontiguousarray(h))
+    got = C.history_tunes(d, n, N, [(0, N - 1)], "fft")
+    assert np.all(np.isnan(got[0, :, 1]))
+    for i in (0, 2):
+        want = [O.fft_tune(h[:, 0, i], h[:, 1, i]), O.fft_tune(h[:, 2, i], h[:, 3, i])]
+        np.testing.assert_allclose(got[0, :, i], want, rtol=0, atol=1e-12, err_msg=f"N={N} particle {i}")
+
+
+@pytest.mark.parametrize("N", [16384, 16411, 17640, 65537, 100000, 100001, 131072, 2_100_001])
 def test_chirpz_long_windows_match_numpy_fft(N):
-    """Up to 2^21 samples the chirp-z transform runs over a {2,3,5,7}-smooth
-    length; 2100001 samples (no smooth length of two 2048-point factors is
-    long enough) take the power-of-two chirp-z path."""
+    """Windows too long for one workgroup: {2,3,5,7}-smooth lengths (17640 =
+    2^3 3^2 5 7^2: every radix) as four-step transforms of the length, others up
+    to 2^21 samples by chirp-z over a smooth length; 2100001 samples (no smooth
+    length of two 2048-point factors is long enough) take the power-of-two
+    chirp-z path."""
     n = 4 if N < 1_000_000 else 2
     h = quasi_periodic(N, n, seed=N)
     d = C.DeviceBuffer.from_array(np.ascontiguousarray(h))
