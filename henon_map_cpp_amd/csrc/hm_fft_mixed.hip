@@ -508,10 +508,13 @@ __device__ __forceinline__ void wh_pass(Cplx *buf, const WhPass &ps, const Cplx 
 //   B: one thread per output pair (q, R - q) of a butterfly (q = 0: the sum),
 //      Y_q = x_0 + sum_r (a_r cos t - i b_r sin t), Y_{R-q} with + i, t = 2 pi r q / R;
 //      the lanes of a wavefront share the butterfly (the same a_r, b_r: LDS
-//      broadcasts) and step cos t, sin t by rotation through 2 pi q / R
+//      broadcasts) and step cos t, sin t by the three-term recurrence
+//      f(r+1) = 2 cos(2 pi q / R) f(r) - f(r-1) (two multiply-adds a step)
 // R(R-1)/2 real multiply-adds per butterfly and plane pair instead of R^2
-// complex products; the rotation adds at most a few 1e-14 of relative error
-// over the (R-1)/2 steps.
+// complex products. The recurrence's rounding errors grow at most like
+// r min(r, 1 / |sin(2 pi q / R)|) units in the last place: about 1e-12 of the
+// unit roots at most over the (R-1)/2 <= 128 steps, far under what moves a
+// tune (the tests hold the tunes to 1e-12 of the long-double DFT's).
 template <uint32_t NT, uint32_t PMAX>
 __device__ __forceinline__ void wh_pass_prime(Cplx *buf, const WhPass &ps, const Cplx *__restrict__ tw)
 {
@@ -543,17 +546,18 @@ __device__ __forceinline__ void wh_pass_prime(Cplx *buf, const WhPass &ps, const
         // loop with cos = 1, sin = 0, which leaves Y_0 = x_0 + sum a_r
         uint32_t ja[2], qa[2];
         Cplx pa[2], sb[2];
-        double c[2], sn[2], c1[2], s1[2];
+        double c[2], sn[2], cp[2], sp[2], t2[2];
 #pragma unroll
         for (uint32_t h = 0; h < 2; ++h) {
             const uint32_t e = min(threadIdx.x + (g + h) * NT, items - 1);
             ja[h] = udiv_m(e, ps.m_h1);
             qa[h] = e - ja[h] * (H + 1);
             const Cplx wq = tw[qa[h] * B];  // W_L^{q L / R} = exp(-2 pi i q / R)
-            c1[h] = wq.re;
-            s1[h] = -wq.im;
-            c[h] = c1[h];
-            sn[h] = s1[h];
+            c[h] = wq.re;
+            sn[h] = -wq.im;
+            cp[h] = 1.0;
+            sp[h] = 0.0;
+            t2[h] = 2.0 * wq.re;
             pa[h] = buf[ja[h]];
             sb[h] = Cplx{0.0, 0.0};
         }
@@ -570,7 +574,10 @@ __device__ __forceinline__ void wh_pass_prime(Cplx *buf, const WhPass &ps, const
                 pa[h].im = fma(av[h].im, c[h], pa[h].im);
                 sb[h].re = fma(bv[h].re, sn[h], sb[h].re);
                 sb[h].im = fma(bv[h].im, sn[h], sb[h].im);
-                const double cn = fma(c[h], c1[h], -(sn[h] * s1[h])), sn2 = fma(sn[h], c1[h], c[h] * s1[h]);
+                // cos, sin of (r + 1) 2 pi q / R: f(r+1) = 2 cos(2 pi q / R) f(r) - f(r-1)
+                const double cn = fma(t2[h], c[h], -cp[h]), sn2 = fma(t2[h], sn[h], -sp[h]);
+                cp[h] = c[h];
+                sp[h] = sn[h];
                 c[h] = cn;
                 sn[h] = sn2;
             }
@@ -618,7 +625,7 @@ __device__ __forceinline__ void wh_fft(Cplx *buf, const WhPlan &p, const Cplx *_
 // removal and Hanning, the transform(s), the first maximum of |X_k| over
 // k < N and the three-point interpolation (dynamic_indicator.cpp:139-196).
 template <bool CZ, uint32_t NT, uint32_t PMAX>
-__global__ void __launch_bounds__(NT) k_wh_tune(WhArgs a)
+__global__ void __launch_bounds__(NT, 4) k_wh_tune(WhArgs a)
 {
     extern __shared__ Cplx buf[];
     __shared__ double sv[NT / 64];
