@@ -262,7 +262,7 @@ def tunes_leg():
         birk = C.history_tunes(rows, n, T, [(0, T - 1)], "birkhoff", row_len=L, means=means)
         birk2, fft2 = C.tunes_rows(rows, n, T, [(0, T - 1)], L, means=means)
         h_ms = C.profile_read("k_history_rows")[0]
-        f_ms = sum(C.profile_read(k)[0] for k in ("k_signal_rows", "k_fft_tune_rows", "k_fft_tune_pow2"))
+        f_ms = sum(C.profile_read(k)[0] for k in ("k_signal_rows", "k_fft_tune_rows"))
         b_ms = C.profile_read("k_birkhoff_hist")[0]
         u_ms = C.profile_read("k_tunes_fused")[0]
         C.profile_enable(False)

@@ -3,8 +3,10 @@
 //  * k_history        : track n turns, writing every sample to HBM
 //  * k_means          : per-coordinate mean of each particle's samples
 //  * k_birkhoff_hist  : Birkhoff tunes of windows of the stored samples
-//  * k_fft_tune_pow2  : Hanning-windowed FFT tune, power-of-two length, radix-2 in LDS
-//  * other lengths    : mixed radix (hm_fft_mixed.hip) or chirp-z (hm_chirpz.hip)
+//  * k_fft_tune_stockham / k_fft_tune_2048 : Hanning-windowed FFT tune,
+//                       power-of-two lengths 8 .. 2048, one signal per workgroup
+//  * other lengths    : whole-signal and four-step mixed radix
+//                       (hm_fft_mixed.hip) or power-of-two chirp-z (hm_chirpz.hip)
 // Two history layouts, addressed through strides (HistLayout):
 //  * sample-major hist[sample][coord][particle] (coord = x, px, y, py): a
 //    tracking wavefront writes one contiguous row per turn (k_history);
@@ -296,132 +298,6 @@ __global__ void __launch_bounds__(256) k_birkhoff_rows(const double *__restrict_
 }
 
 // ---------------------------------------------------------------- FFT tune
-struct FftArgs {
-    const double *hist;
-    HistLayout lay;
-    uint32_t n;            // particles
-    const double *means;   // [4][n] or NULL
-    uint32_t from[kMaxWindows];
-    uint32_t count;        // windows, all of length N
-    uint32_t N;
-    const double *hann;    // [N], 2 sin^2(pi m / N) (host, glibc)
-    const double *tw;      // [N] complex: exp(-2 pi i q / N) as (cos, -sin) pairs
-    double *out;           // [window][plane][n]
-    uint32_t out_window0;  // first window index in `out`
-};
-
-__device__ double interp_tune(const double *mag, uint32_t N, uint32_t index)
-{
-    return interp_tune_at([&](uint32_t k) { return mag[k]; }, N, index);
-}
-
-// Signal of block b: XCD-aware order, so that workgroups dispatched to the same
-// XCD (b, b+8, b+16, ...) take neighbouring particles and share the L2 lines of
-// the particle-minor history rows.
-__device__ __forceinline__ void signal_of(uint32_t b, uint32_t n_signals_per_plane_window, uint32_t &particle,
-                                          uint32_t &plane, uint32_t &window, uint32_t n)
-{
-    const uint32_t total = n_signals_per_plane_window;  // n * 2 * windows
-    const uint32_t per_xcd = (total + 7) / 8;
-    uint32_t s = (b % 8) * per_xcd + b / 8;
-    if (s >= total) s = total;  // marks an idle block
-    particle = s % n;
-    plane = (s / n) % 2;
-    window = s / (2 * n);
-    if (s == total) window = 0xffffffffu;
-}
-
-__device__ __forceinline__ void block_argmax(double v, uint32_t idx, double *sv, uint32_t *si, double &best,
-                                             uint32_t &best_i)
-{
-    // first maximum (lowest index) over the block
-    const uint32_t t = threadIdx.x;
-    sv[t] = v;
-    si[t] = idx;
-    __syncthreads();
-    for (uint32_t st = blockDim.x / 2; st > 0; st >>= 1) {
-        if (t < st) {
-            const double ov = sv[t + st];
-            const uint32_t oi = si[t + st];
-            if (ov > sv[t] || (ov == sv[t] && oi < si[t]) || (sv[t] != sv[t] && ov == ov)) {
-                sv[t] = ov;
-                si[t] = oi;
-            }
-        }
-        __syncthreads();
-    }
-    best = sv[0];
-    best_i = si[0];
-    __syncthreads();
-}
-
-// Radix-2 decimation-in-time, bit-reversed load, in LDS; block of 256 threads.
-__global__ void __launch_bounds__(256) k_fft_tune_pow2(FftArgs a)
-{
-    extern __shared__ double lds[];
-    double *re = lds, *im = lds + a.N;
-    __shared__ double sv[256];
-    __shared__ uint32_t si[256];
-    __shared__ int any_nan_flag;
-    uint32_t particle, plane, window;
-    signal_of(blockIdx.x, a.n * 2 * a.count, particle, plane, window, a.n);
-    if (window == 0xffffffffu) return;
-    const uint32_t N = a.N;
-    const uint32_t logn = 31 - __builtin_clz(N);
-    const size_t nn = a.n;
-    const int cx = 2 * plane, cp = 2 * plane + 1;
-    double mc[4] = {0.0, 0.0, 0.0, 0.0};
-    if (a.means)
-        for (int c = 0; c < 4; c++) mc[c] = a.means[c * nn + particle];
-    const double mx = mc[cx], mp = mc[cp];
-    if (threadIdx.x == 0) any_nan_flag = 0;
-    __syncthreads();
-    bool nan_seen = false;
-    for (uint32_t m = threadIdx.x; m < N; m += blockDim.x) {
-        const uint64_t sm = a.from[window] + m;
-        // the NaN rule applies to the mean-removed samples of all four coordinates
-        for (int c = 0; c < 4; c++) nan_seen |= __builtin_isnan(a.hist[hat(a.lay, sm, c, particle)] - mc[c]);
-        const double h = a.hann[m];
-        const uint32_t r = __brev(m) >> (32 - logn);
-        re[r] = (a.hist[hat(a.lay, sm, cx, particle)] - mx) * h;
-        im[r] = (a.hist[hat(a.lay, sm, cp, particle)] - mp) * h;
-    }
-    if (nan_seen) atomicOr(&any_nan_flag, 1);
-    __syncthreads();
-    const size_t out_o = ((size_t)(a.out_window0 + window) * 2 + plane) * nn + particle;
-    if (any_nan_flag) {
-        if (threadIdx.x == 0) a.out[out_o] = __builtin_nan("");
-        return;
-    }
-    for (uint32_t half = 1; half < N; half <<= 1) {
-        const uint32_t stride = N / (2 * half);
-        for (uint32_t j = threadIdx.x; j < N / 2; j += blockDim.x) {
-            const uint32_t k = j & (half - 1);
-            const uint32_t p0 = ((j - k) << 1) + k, p1 = p0 + half;
-            const double wr = a.tw[2 * (k * stride)], wi = a.tw[2 * (k * stride) + 1];
-            const double br = re[p1], bi = im[p1];
-            const double tr = br * wr - bi * wi, ti = br * wi + bi * wr;
-            const double ar = re[p0], ai = im[p0];
-            re[p0] = ar + tr; im[p0] = ai + ti;
-            re[p1] = ar - tr; im[p1] = ai - ti;
-        }
-        __syncthreads();
-    }
-    // magnitudes into re[], then the first maximum
-    double best = -1.0;
-    uint32_t best_i = 0xffffffffu;
-    for (uint32_t k = threadIdx.x; k < N; k += blockDim.x) {
-        const double mg = sqrt(re[k] * re[k] + im[k] * im[k]);
-        re[k] = mg;
-        if (mg > best || best_i == 0xffffffffu) { best = mg; best_i = k; }
-    }
-    __syncthreads();
-    double bv;
-    uint32_t bi;
-    block_argmax(best, best_i, sv, si, bv, bi);
-    if (threadIdx.x == 0) a.out[out_o] = interp_tune(re, N, bi);
-}
-
 // Corner turn for the FFT: samples from..from+L-1 of the history rows
 // hist[sample][coord][particle] (particle-minor: what tracking writes) into
 // signal rows rows[particle - p0][coord][sample] minus the particle's means, so
@@ -463,87 +339,6 @@ struct FftRowArgs {
     const double *bw;      // Birkhoff weights [N - 1] (fused kernel only)
     double *bout;          // Birkhoff tunes [window][plane][n] (fused kernel only)
 };
-
-// Both planes of one particle per workgroup, radix-2 decimation-in-time in LDS
-// (re/im of x + i px and y + i py: 4 N doubles of dynamic LDS), signal rows read
-// contiguously.
-__global__ void __launch_bounds__(256) k_fft_tune_rows(FftRowArgs a)
-{
-    extern __shared__ double lds[];
-    const uint32_t N = a.N;
-    double *re0 = lds, *im0 = lds + N, *re1 = lds + 2 * N, *im1 = lds + 3 * N;
-    __shared__ double sv[256];
-    __shared__ uint32_t si[256];
-    __shared__ int any_nan_flag;
-    const uint32_t b = blockIdx.x;
-    if (b >= a.nb) return;
-    const uint32_t logn = 31 - __builtin_clz(N);
-    const double *rx = a.rows + (size_t)((a.row0 + b) * a.pstride + a.from), *rpx = rx + a.cstride,
-                 *ry = rx + 2 * a.cstride, *rpy = rx + 3 * a.cstride;
-    const size_t nn0 = a.n, pp = a.p0 + b;
-    const double m0 = a.means ? a.means[pp] : 0.0, m1 = a.means ? a.means[nn0 + pp] : 0.0;
-    const double m2 = a.means ? a.means[2 * nn0 + pp] : 0.0, m3 = a.means ? a.means[3 * nn0 + pp] : 0.0;
-    if (threadIdx.x == 0) any_nan_flag = 0;
-    __syncthreads();
-    bool nan_seen = false;
-    for (uint32_t m = threadIdx.x; m < N; m += blockDim.x) {
-        const double h = a.hann[m];
-        const double vx = rx[m] - m0, vpx = rpx[m] - m1, vy = ry[m] - m2, vpy = rpy[m] - m3;
-        nan_seen |= any_nan(vx, vpx, vy, vpy);
-        const uint32_t r = __brev(m) >> (32 - logn);
-        re0[r] = vx * h; im0[r] = vpx * h;
-        re1[r] = vy * h; im1[r] = vpy * h;
-    }
-    if (nan_seen) atomicOr(&any_nan_flag, 1);
-    __syncthreads();
-    const size_t particle = a.p0 + b, nn = a.n;
-    double *o0 = a.out + ((size_t)a.window * 2 + 0) * nn + particle;
-    double *o1 = a.out + ((size_t)a.window * 2 + 1) * nn + particle;
-    if (any_nan_flag) {
-        if (threadIdx.x == 0) { *o0 = __builtin_nan(""); *o1 = __builtin_nan(""); }
-        return;
-    }
-    for (uint32_t half = 1; half < N; half <<= 1) {
-        const uint32_t stride = N / (2 * half);
-        for (uint32_t j = threadIdx.x; j < N / 2; j += blockDim.x) {
-            const uint32_t k = j & (half - 1);
-            const uint32_t q0 = ((j - k) << 1) + k, q1 = q0 + half;
-            const double wr = a.tw[2 * (k * stride)], wi = a.tw[2 * (k * stride) + 1];
-            {
-                const double br = re0[q1], bi = im0[q1];
-                const double tr = br * wr - bi * wi, ti = br * wi + bi * wr;
-                const double ar = re0[q0], ai = im0[q0];
-                re0[q0] = ar + tr; im0[q0] = ai + ti;
-                re0[q1] = ar - tr; im0[q1] = ai - ti;
-            }
-            {
-                const double br = re1[q1], bi = im1[q1];
-                const double tr = br * wr - bi * wi, ti = br * wi + bi * wr;
-                const double ar = re1[q0], ai = im1[q0];
-                re1[q0] = ar + tr; im1[q0] = ai + ti;
-                re1[q1] = ar - tr; im1[q1] = ai - ti;
-            }
-        }
-        __syncthreads();
-    }
-    double best0 = -1.0, best1 = -1.0;
-    uint32_t bi0 = 0xffffffffu, bi1 = 0xffffffffu;
-    for (uint32_t k = threadIdx.x; k < N; k += blockDim.x) {
-        const double g0 = sqrt(re0[k] * re0[k] + im0[k] * im0[k]);
-        const double g1 = sqrt(re1[k] * re1[k] + im1[k] * im1[k]);
-        re0[k] = g0;
-        re1[k] = g1;
-        if (g0 > best0 || bi0 == 0xffffffffu) { best0 = g0; bi0 = k; }
-        if (g1 > best1 || bi1 == 0xffffffffu) { best1 = g1; bi1 = k; }
-    }
-    __syncthreads();
-    double bv;
-    uint32_t i0, i1;
-    block_argmax(best0, bi0, sv, si, bv, i0);
-    block_argmax(best1, bi1, sv, si, bv, i1);
-    if (threadIdx.x == 0) *o0 = interp_tune(re0, N, i0);
-    if (threadIdx.x == 1) *o1 = interp_tune(re1, N, i1);
-}
 
 // ---------------------------------------------------------------- Stockham FFT
 // Self-sorting (Stockham) FFT of one signal per workgroup, radix-8 passes with
@@ -981,25 +776,6 @@ static void host_birkhoff_weights(uint32_t n, double *out);
 
 // Signal rows staged per launch pair (bytes); particles are taken in batches that fit.
 static constexpr uint64_t kRowBudget = 4ull << 30;
-// Static LDS of the FFT kernels besides the signal (argmax scratch and flags).
-static constexpr size_t kRowsStaticLds = 256 * (sizeof(double) + sizeof(uint32_t)) + 64;
-
-// LDS a workgroup may use on the current device.
-static size_t lds_per_block()
-{
-    int dev = 0, v = 0;
-    hip_check(hipGetDevice(&dev), "hipGetDevice");
-    hip_check(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
-    return (size_t)v;
-}
-
-// Lets `fn` use `bytes` of dynamic LDS (needed above the default limit).
-static void allow_dynamic_lds(const void *fn, size_t bytes)
-{
-    hip_check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes),
-              "hipFuncSetAttribute(dynamic LDS)");
-}
-
 // HM_FFT_ROW_BYTES overrides the budget (small values force several batches).
 static uint64_t row_budget()
 {
@@ -1010,8 +786,6 @@ static uint64_t row_budget()
     }
     return kRowBudget;
 }
-// Longest window of the two-plane row kernel: 4 N doubles of LDS per workgroup.
-static constexpr uint32_t kRowsMaxN = 4096;
 
 static FftTables fft_tables(uint32_t N, hipStream_t s)
 {
@@ -1260,20 +1034,13 @@ static void launch_stockham(const FftRowArgs &r, uint32_t logn, hipStream_t s)
     for_row_parts(r, [&](const FftRowArgs &q) { launch_stockham_part<BIRK>(q, logn, s); });
 }
 
+// Power-of-two windows of 2 .. 2048 samples (the Stockham kernels; the timer
+// keeps the name the row kernels had).
 static void launch_row_fft(const FftRowArgs &r, uint32_t N, hipStream_t s)
 {
     const uint32_t logn = 31 - __builtin_clz(N);
     LaunchTimer timer("k_fft_tune_rows", s);
-    switch (logn) {
-    case 3: case 4: case 5: case 6: case 7: case 8: case 9: case 10: case 11:
-        launch_stockham<false>(r, logn, s);
-        break;
-    default:
-        allow_dynamic_lds((const void *)k_fft_tune_rows, 4 * N * sizeof(double));
-        for_row_parts(r, [&](const FftRowArgs &q) {
-            hipLaunchKernelGGL(k_fft_tune_rows, dim3(q.nb), dim3(256), 4 * N * sizeof(double), s, q);
-        });
-    }
+    launch_stockham<false>(r, logn, s);
     launch_check("k_fft_tune_rows");
 }
 
@@ -1296,67 +1063,43 @@ static void fft_impl(const double *hist, HistLayout lay, bool rows_input, size_t
     for (auto &kv : by_len) {
         const uint32_t N = kv.first;
         const FftTables t = fft_tables(N, s);
-        const bool pow2 = (N & (N - 1)) == 0 && N <= 8192;
+        // powers of two from 8 to 2048: the Stockham kernels; every other
+        // length (2, 4, 4096 and 8192 included) the whole-signal or four-step
+        // transforms of fft_tune_mixed, past 2^21 samples power-of-two chirp-z
+        const bool stockham = (N & (N - 1)) == 0 && N >= 8 && N <= 2048;
         const auto &ws = kv.second;
-        const bool lds_pow2 = pow2 && 2 * (size_t)N * sizeof(double) + kRowsStaticLds <= lds_per_block();
-        if (!lds_pow2) {
-            // any other length, one window at a time: the mixed-radix transform
-            // of N when it has only the factors 2, 3, 5, 7, else chirp-z over a
-            // smooth length, and the power-of-two chirp-z past 2^21 samples
+        if (!stockham) {
             for (uint32_t w : ws)
                 if (!fft_tune_mixed(hist, lay, (uint32_t)n, means, from[w], N, t.hann, out, w, s))
                     fft_tune_chirpz(hist, lay, (uint32_t)n, means, from[w], N, t.hann, out, w, s);
             continue;
         }
-        for (size_t b0 = 0; b0 < ws.size();) {
-            // consecutive output windows only (one launch writes a contiguous window range)
-            size_t b1 = b0 + 1;
-            while (b1 < ws.size() && b1 - b0 < (size_t)kMaxWindows && ws[b1] == ws[b1 - 1] + 1) ++b1;
-            FftArgs a{};
-            a.hist = hist; a.lay = lay; a.n = (uint32_t)n; a.means = means; a.count = (uint32_t)(b1 - b0); a.N = N;
-            a.hann = t.hann; a.tw = t.tw; a.out = out; a.out_window0 = ws[b0];
-            for (size_t q = b0; q < b1; q++) a.from[q - b0] = from[ws[q]];
-            const uint64_t signals = (uint64_t)n * 2 * a.count;
-            const uint32_t blocks = (uint32_t)(((signals + 7) / 8) * 8);
-            const size_t rows_lds = 4 * (size_t)N * sizeof(double) + kRowsStaticLds;
-            if (pow2 && N <= kRowsMaxN && rows_lds <= lds_per_block()) {
-                if (rows_input) {
-                    // one workgroup per particle, reading its rows in place
-                    for (size_t q = b0; q < b1; q++) {
-                        FftRowArgs r{hist, lay.ps, lay.cs, 0, from[ws[q]], means, (uint32_t)n, 0, (uint32_t)n, N,
-                                     t.hann, t.tw, out, ws[q]};
-                        launch_row_fft(r, N, s);
-                    }
-                } else {
-                    // per window: signal rows for a batch of particles (corner
-                    // turn), then one workgroup per particle reading them
-                    const uint64_t row_bytes = 4ull * N * sizeof(double);
-                    const uint32_t batch =
-                        (uint32_t)std::min<uint64_t>(n, std::max<uint64_t>(64, row_budget() / row_bytes));
-                    double *rows = static_cast<double *>(workspace((size_t)batch * row_bytes, 8));
-                    for (size_t q = b0; q < b1; q++) {
-                        for (uint32_t p0 = 0; p0 < (uint32_t)n; p0 += batch) {
-                            const uint32_t nb = std::min<uint32_t>(batch, (uint32_t)n - p0);
-                            {
-                                LaunchTimer timer("k_signal_rows", s);
-                                hipLaunchKernelGGL(k_signal_rows, dim3((nb + 63) / 64, (N + 63) / 64, 4), dim3(256), 0,
-                                                   s, hist, (uint32_t)n, means, from[ws[q]], N, p0, nb, rows);
-                                launch_check("k_signal_rows");
-                            }
-                            FftRowArgs r{rows, 4ull * N, N, 0, 0, nullptr, nb, p0, (uint32_t)n, N, t.hann, t.tw, out,
-                                         ws[q]};
-                            launch_row_fft(r, N, s);
-                        }
-                    }
-                }
-            } else {
-                // longer windows: one signal per workgroup (2 N doubles of LDS)
-                allow_dynamic_lds((const void *)k_fft_tune_pow2, 2 * N * sizeof(double));
-                LaunchTimer timer("k_fft_tune_pow2", s);
-                hipLaunchKernelGGL(k_fft_tune_pow2, dim3(blocks), dim3(256), 2 * N * sizeof(double), s, a);
-                launch_check("k_fft_tune_pow2");
+        if (rows_input) {
+            // one workgroup per particle and plane, reading its rows in place
+            for (uint32_t w : ws) {
+                FftRowArgs r{hist, lay.ps, lay.cs, 0, from[w], means, (uint32_t)n, 0, (uint32_t)n, N,
+                             t.hann, t.tw, out, w};
+                launch_row_fft(r, N, s);
             }
-            b0 = b1;
+            continue;
+        }
+        // sample-major history, per window: signal rows for a batch of
+        // particles (corner turn), then the row kernels reading them
+        const uint64_t row_bytes = 4ull * N * sizeof(double);
+        const uint32_t batch = (uint32_t)std::min<uint64_t>(n, std::max<uint64_t>(64, row_budget() / row_bytes));
+        double *rows = static_cast<double *>(workspace((size_t)batch * row_bytes, 8));
+        for (uint32_t w : ws) {
+            for (uint32_t p0 = 0; p0 < (uint32_t)n; p0 += batch) {
+                const uint32_t nb = std::min<uint32_t>(batch, (uint32_t)n - p0);
+                {
+                    LaunchTimer timer("k_signal_rows", s);
+                    hipLaunchKernelGGL(k_signal_rows, dim3((nb + 63) / 64, (N + 63) / 64, 4), dim3(256), 0, s, hist,
+                                       (uint32_t)n, means, from[w], N, p0, nb, rows);
+                    launch_check("k_signal_rows");
+                }
+                FftRowArgs r{rows, 4ull * N, N, 0, 0, nullptr, nb, p0, (uint32_t)n, N, t.hann, t.tw, out, w};
+                launch_row_fft(r, N, s);
+            }
         }
     }
 }

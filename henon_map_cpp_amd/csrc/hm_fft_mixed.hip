@@ -919,9 +919,11 @@ static void set_lds(K kernel, size_t bytes)
 }
 
 // ---------------------------------------------------------------- whole-signal plans
-// The two shapes of k_wh_tune: up to 2048 points with 256 threads (several
-// workgroups per CU), up to kWhPmax with 1024.
+// The three shapes of k_wh_tune: up to 2048 points with 256 threads, up to
+// 4096 with 512 (several workgroups per CU at four wavefronts per SIMD), up to
+// kWhPmax with 1024 (one).
 constexpr uint32_t kWhSmallPmax = 2048, kWhSmallThreads = 256;
+constexpr uint32_t kWhMidPmax = 4096, kWhMidThreads = 512;
 constexpr uint32_t kWhPmax = 10240, kWhThreads = 1024;
 
 // Passes of the direct transform of L, grouped as wh_fft runs them: 3s, 5s,
@@ -995,6 +997,8 @@ static const WhTables *wh_tables(uint32_t N, hipStream_t s)
     size_t fixed = 0;
     for (const void *k : {(const void *)k_wh_tune<false, kWhSmallThreads, kWhSmallPmax>,
                           (const void *)k_wh_tune<true, kWhSmallThreads, kWhSmallPmax>,
+                          (const void *)k_wh_tune<false, kWhMidThreads, kWhMidPmax>,
+                          (const void *)k_wh_tune<true, kWhMidThreads, kWhMidPmax>,
                           (const void *)k_wh_tune<false, kWhThreads, kWhPmax>,
                           (const void *)k_wh_tune<true, kWhThreads, kWhPmax>}) {
         hipFuncAttributes fa{};
@@ -1025,7 +1029,7 @@ static const WhTables *wh_tables(uint32_t N, hipStream_t s)
         return nullptr;
     }
     const uint32_t M = t.M;
-    t.nt = M <= kWhSmallPmax ? kWhSmallThreads : kWhThreads;
+    t.nt = M <= kWhSmallPmax ? kWhSmallThreads : M <= kWhMidPmax ? kWhMidThreads : kWhThreads;
     const long double two_pi = 6.283185307179586476925286766559005768L, pi = two_pi / 2;
     std::vector<Cplx> w(M);
     for (uint32_t q = 0; q < M; ++q) {
@@ -1088,6 +1092,9 @@ static bool fft_tune_whole(const double *hist, HistLayout lay, uint32_t n, const
     if (t->nt == kWhSmallThreads) {
         if (t->cz) launch_wh<true, kWhSmallThreads, kWhSmallPmax>(a, n, lds_bytes, s);
         else launch_wh<false, kWhSmallThreads, kWhSmallPmax>(a, n, lds_bytes, s);
+    } else if (t->nt == kWhMidThreads) {
+        if (t->cz) launch_wh<true, kWhMidThreads, kWhMidPmax>(a, n, lds_bytes, s);
+        else launch_wh<false, kWhMidThreads, kWhMidPmax>(a, n, lds_bytes, s);
     } else {
         if (t->cz) launch_wh<true, kWhThreads, kWhPmax>(a, n, lds_bytes, s);
         else launch_wh<false, kWhThreads, kWhPmax>(a, n, lds_bytes, s);

@@ -38,7 +38,7 @@ def numpy_fft_tune(x, px):
     return O.interpolation(mag)
 
 
-@pytest.mark.parametrize("N", [2, 3, 5, 31, 501, 777, 1101, 2049, 4095, 4097, 10000, 10001])
+@pytest.mark.parametrize("N", [2, 3, 4, 5, 31, 501, 777, 1101, 2049, 4095, 4096, 4097, 8192, 10000, 10001])
 def test_chirpz_matches_oracle_dft(N):
     n = 3 if N > 4000 else 8
     h = quasi_periodic(N, n, seed=N)
