@@ -229,7 +229,7 @@ __device__ __forceinline__ void megno_chunk(const BatchSeg &S, ConstPtr<JobDesc>
     double w_prev = 0.0, inv_prev = 0.0;
     auto megno_update = [&]() {
         const double fa2 = frob2_fused(m);
-        l1 = l1 + (0.5 * log10(fa2 / fb2)) * w_prev;
+        l1 = l1 + (0.5 * log10_fdl(fa2 / fb2)) * w_prev;
         l2 = l2 + inv_prev * l1;
         fb2 = fa2;
         pending = false;
@@ -258,7 +258,7 @@ __device__ __forceinline__ void megno_chunk(const BatchSeg &S, ConstPtr<JobDesc>
                     const double inv_n = inv_idx[S.seg_start + kn];
                     {
                         const double fa2 = frob2_fused(m);
-                        l1 = l1 + (0.5 * log10(fa2 / fb2)) * w_prev;
+                        l1 = l1 + (0.5 * log10_fdl(fa2 / fb2)) * w_prev;
                         l2 = l2 + inv_prev * l1;
                         fb2 = fa2;
                     }

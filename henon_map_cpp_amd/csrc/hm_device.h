@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "hm_log10.h"
+
 namespace hm {
 
 // One turn of the modulation table: sin/cos of omega_x(n), omega_y(n).

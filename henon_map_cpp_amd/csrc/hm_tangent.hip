@@ -174,8 +174,10 @@ constexpr int ind_state_len(int V) { return V == 2 ? kIndConsState + 5 * kMaxBir
 //   MEGNO (construct form): with idx = k+1, L1 += log10(|M_{k+1}|_F / |M_k|_F) idx,
 //     L2 += L1 / idx; recorded at sample t as L2 / t. The ratio is evaluated as
 //     0.5 log10(|M_{k+1}|^2 / |M_k|^2) with fused squared norms (one division,
-//     no square root) and 1/idx comes from a host table; the result agrees with
-//     the separate-rounding form to a few ulps per turn.
+//     no square root), the logarithm by log10_fdl (hm_log10.h: within 2 ulps
+//     of the library's, in about half its instructions) and 1/idx from a host
+//     table; the result agrees with the separate-rounding form to a few ulps
+//     per turn.
 //   FLI: log10(|M_t v0|) / t with v0 = fli_dir / |fli_dir|, including the
 //     log10 of the scale factored out by renormalisation.
 //   Renormalisation every renorm_every turns: M <- M / |M|_F, after the MEGNO
@@ -345,7 +347,7 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
     };
     auto megno_update = [&]() {
         const double fa2 = frob2_fused(m);
-        l1 = l1 + (0.5 * log10(fa2 / fb2)) * w_prev;
+        l1 = l1 + (0.5 * log10_fdl(fa2 / fb2)) * w_prev;
         l2 = l2 + inv_prev * l1;
         fb2 = fa2;
         if (EXTRA && a.n_mb) megno_birkhoff_turn(idx_prev, inv_prev);
@@ -420,7 +422,7 @@ __global__ void __launch_bounds__(256) k_indicators(IndicatorArgs a)
                     const double inv_n = inv_idx[kn];
                     {
                         const double fa2 = frob2_fused(m);
-                        l1 = l1 + (0.5 * log10(fa2 / fb2)) * w_prev;
+                        l1 = l1 + (0.5 * log10_fdl(fa2 / fb2)) * w_prev;
                         l2 = l2 + inv_prev * l1;
                         fb2 = fa2;
                     }
