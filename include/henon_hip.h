@@ -415,13 +415,17 @@ int hm_birkhoff_tunes_history(const double *hist, size_t n, uint32_t n_samples, 
  * 2 sin^2(pi i/N), forward DFT of (x-mean) + i (px-mean), magnitude, first
  * maximum, three-point interpolation, |1 - (k/N + asin(A sin(2pi/N))/2pi)|; 1.0 /
  * 0.0 at the spectrum's edges; NaN when a sample of the window is NaN. Windows
- * start at sample from[w] and have len[w] samples (host arrays). Power-of-two
- * lengths up to 2048 use a Stockham radix-8 FFT in LDS, 4096 and 8192 radix-2,
- * other lengths up to 16384 a direct DFT. Up to 4096 samples the windows are
- * first corner-turned into per-particle signal rows (library workspace of at most
- * 4 GiB, particles taken in batches that fit; the environment variable
- * HM_FFT_ROW_BYTES overrides the budget); hm_fft_tunes_rows reads particle rows
- * in place. out: device double[n_windows][2][n]. */
+ * start at sample from[w] and have len[w] samples (host arrays); any length of
+ * at least 2. Power-of-two lengths 8..2048 use a Stockham radix-8 FFT in LDS
+ * (from a sample-major history the windows are first corner-turned into
+ * per-particle signal rows: library workspace of at most 4 GiB, particles taken
+ * in batches that fit; the environment variable HM_FFT_ROW_BYTES overrides the
+ * budget); other lengths up to about 10^4 samples one LDS-resident transform per
+ * signal (mixed radix with prime passes up to 257, or chirp-z over a smooth
+ * length that fits); longer ones four-step mixed-radix or chirp-z transforms
+ * through HBM scratch (also sized by HM_FFT_ROW_BYTES, 8 GiB by default).
+ * hm_fft_tunes_rows reads particle rows in place. out: device
+ * double[n_windows][2][n]. */
 int hm_fft_tunes(const double *hist, size_t n, uint32_t n_samples, const double *means, const uint32_t *from,
                  const uint32_t *len, size_t n_windows, double *out, void *stream);
 
